@@ -1,0 +1,202 @@
+/* MI355X block-tensor contraction (libtensor btod_contract2 path) - C ABI.
+ *
+ * Plain C: opaque handles, pointers and sizes, no C++ or torch types.  Every
+ * function returns LT_OK (0) or a negative error code; lt_last_error() gives
+ * the message of the calling thread's last failure.  C++ exceptions never
+ * cross this boundary.
+ *
+ * Index conventions are libtensor's: row-major blocks, last index fastest;
+ * a permutation p maps position i of the result to position p[i] of the
+ * source (libtensor/core/permutation.h); block indexes are absolute indexes
+ * over the block-index dimensions.
+ *
+ * Each entry point names the libtensor interface it replaces.
+ */
+#ifndef LT_AMD_H
+#define LT_AMD_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define LT_OK             0
+#define LT_ERR_PARAM     -1   /* bad_parameter, out_of_bounds */
+#define LT_ERR_DIMS      -2   /* bad_dimensions */
+#define LT_ERR_SYMMETRY  -3   /* bad_symmetry */
+#define LT_ERR_DEVICE    -4   /* HIP runtime failure or no GPU */
+#define LT_ERR_NOMEM     -5   /* HBM budget exceeded (admission refused) */
+#define LT_ERR_STATE     -6
+#define LT_ERR_INTERNAL  -7
+
+#define LT_MODE_PER_BLOCK 0   /* one GEMM launch per C block */
+#define LT_MODE_GROUPED   1   /* one grouped GEMM launch for all C blocks */
+#define LT_MODE_OZAKI     2   /* grouped, split-precision INT8 products */
+
+#define LT_MAX_ORDER 8
+
+typedef struct lt_ctx lt_ctx;
+typedef struct lt_btensor lt_btensor;
+typedef struct lt_symel lt_symel;
+typedef struct lt_btod_contract2 lt_btod_contract2;
+typedef struct lt_server lt_server;
+
+typedef struct lt_stats {
+    double t_schedule_ms;   /* schedule build (constructor) */
+    double t_plan_ms;       /* work-list build (first perform) */
+    double t_gpu_ms;        /* device time of the last perform */
+    double t_gemm_ms;       /* device time of its contraction phase */
+    double flops;           /* sum over pairs of 2 |C block| K block */
+    double bytes_perm;      /* permute traffic of one perform */
+    uint64_t n_cblocks, n_pairs, n_tiles, n_unfold, n_launches;
+    int32_t tile_cfg;
+    int32_t ozaki_slices;
+    double ozaki_max_rel_err;
+} lt_stats;
+
+/* Message of the calling thread's last failed call. */
+const char *lt_last_error(void);
+int lt_version(void);
+
+/* ---- context (replaces the implicit host context of libtensor:
+ *      allocator<double>::init, libtensor/core/allocator.h:84-85, and
+ *      libutil::thread_pool, libutil/thread_pool/thread_pool.h:22) ---- */
+
+/* device >= 0: GPU context with an HBM budget (0 = most of free memory).
+ * device < 0: host-only context (schedules, symmetry, block storage in host
+ * memory; device operations fail with LT_ERR_DEVICE). */
+int lt_ctx_create(int device, size_t hbm_cap_bytes, lt_ctx **out);
+int lt_ctx_destroy(lt_ctx *ctx);
+int lt_ctx_sync(lt_ctx *ctx);
+int lt_ctx_mem(lt_ctx *ctx, size_t *used, size_t *cap);
+int lt_ctx_set_threads(lt_ctx *ctx, size_t nthreads);
+
+/* Device memory for the dense-tensor entry points. */
+int lt_dev_alloc(lt_ctx *ctx, size_t bytes, void **dptr);
+int lt_dev_free(lt_ctx *ctx, void *dptr);
+int lt_dev_upload(lt_ctx *ctx, void *dst, const void *host, size_t bytes);
+int lt_dev_download(lt_ctx *ctx, void *host, const void *src, size_t bytes);
+
+/* ---- dense tensors on the device (replace dense_tensor ops) ---- */
+
+/* tod_copy<N>(a, perm, c).perform(zero, b)
+ * (libtensor/dense_tensor/tod_copy.h:95, :112): b = c P(a) (zero != 0) or
+ * b += c P(a).  b has dimensions P(dims_a).  perm may be NULL (identity). */
+int lt_tod_copy(lt_ctx *ctx, size_t n, const size_t *dims_a, const double *a,
+    const size_t *perm, double c, int zero, double *b);
+
+/* tod_set<N>(v).perform(zero, a) (libtensor/dense_tensor/tod_set.h:37). */
+int lt_tod_set(lt_ctx *ctx, size_t nelem, double v, double *a);
+
+/* tod_contract2<N, M, K>(contr, a, b, d).perform(zero, c)
+ * (libtensor/dense_tensor/tod_contract2.h:114, :171).  The contraction is
+ * contraction2<N, M, K>(permc) with contract(ka[i], kb[i]) for i < K
+ * (libtensor/core/contraction2.h).  permc may be NULL. */
+int lt_tod_contract2(lt_ctx *ctx, size_t N, size_t M, size_t K,
+    const size_t *permc, const size_t *ka, const size_t *kb,
+    const size_t *dims_a, const double *a, const size_t *dims_b,
+    const double *b, double d, int zero, double *c);
+
+/* tod_dotprod<N>(a, perma, b, permb).calculate()
+ * (libtensor/dense_tensor/tod_dotprod.h:77, :98) times c. */
+int lt_tod_dotprod(lt_ctx *ctx, size_t n, const size_t *dims_a,
+    const double *a, const size_t *perma, const size_t *dims_b,
+    const double *b, const size_t *permb, double c, double *result);
+
+/* ---- point-group product tables (point_group_table +
+ *      product_table_container, libtensor/symmetry/point_group_table.h) ---- */
+int lt_product_table_create(const char *id, size_t nlabels);
+int lt_product_table_add_product(const char *id, size_t l1, size_t l2, size_t lr);
+
+/* ---- block tensors (block_tensor<N, double>,
+ *      libtensor/block_tensor/block_tensor.h:19; block_index_space
+ *      libtensor/core/block_index_space.h:144) ---- */
+int lt_btensor_create(lt_ctx *ctx, size_t n, const size_t *dims, lt_btensor **out);
+int lt_btensor_destroy(lt_btensor *bt);
+/* block_index_space::split(mask, pos); mask has n entries (0/1). */
+int lt_btensor_split(lt_btensor *bt, const int *mask, size_t pos);
+int lt_btensor_order(lt_btensor *bt, size_t *n);
+int lt_btensor_bidims(lt_btensor *bt, size_t *bidims);
+int lt_btensor_block_dims(lt_btensor *bt, size_t aidx, size_t *dims);
+int lt_btensor_block_start(lt_btensor *bt, size_t aidx, size_t *start);
+/* gen_block_tensor_ctrl::req_nonzero_blocks: sorted canonical non-zero blocks. */
+int lt_btensor_nonzero(lt_btensor *bt, size_t *out, size_t cap, size_t *count);
+/* Upload / download one canonical block (req_block / ret_block). */
+int lt_btensor_set_block(lt_btensor *bt, size_t aidx, const double *host);
+int lt_btensor_get_block(lt_btensor *bt, size_t aidx, double *host);
+int lt_btensor_zero_block(lt_btensor *bt, size_t aidx);
+int lt_btensor_zero_all(lt_btensor *bt);
+/* Orbit of a block: canonical index, transformation canonical -> block
+ * (perm[n], coefficient) and whether the orbit is allowed
+ * (libtensor/core/orbit.h:46). */
+int lt_btensor_orbit(lt_btensor *bt, size_t aidx, size_t *acidx,
+    size_t *perm, double *coeff, int *allowed);
+/* btod_random semantics with a fixed seed: every canonical allowed block is
+ * filled (dist 0: U[-1,1], 1: N(0,1)) times scale, optionally divided by
+ * 1 + |x_d0 - x_d1| (global element indexes; d0 < 0 disables), and
+ * symmetrized over the block's stabilizer. */
+int lt_btensor_random(lt_btensor *bt, uint64_t seed, int dist, double scale,
+    int d0, int d1);
+/* tod_btconv (libtensor/dense_tensor/tod_btconv.h:17): unfold into a dense
+ * row-major host array. */
+int lt_btensor_export(lt_btensor *bt, double *host);
+/* btod_import_raw: take the canonical allowed blocks of a dense host array
+ * (blocks that are exactly zero are left zero). */
+int lt_btensor_import(lt_btensor *bt, const double *host);
+/* Copy the symmetry of src (so_copy). */
+int lt_btensor_copy_symmetry(lt_btensor *dst, lt_btensor *src);
+
+/* ---- symmetry elements (libtensor/symmetry/se_perm.h:28, se_part.h:32,
+ *      se_label.h:37); build, then lt_btensor_insert (symmetry::insert) ---- */
+int lt_se_perm_create(size_t n, const size_t *perm, double coeff, lt_symel **out);
+int lt_se_part_create(lt_btensor *bt, const size_t *pdims, lt_symel **out);
+int lt_se_part_add_map(lt_symel *e, const size_t *idx1, const size_t *idx2, double coeff);
+int lt_se_part_mark_forbidden(lt_symel *e, const size_t *idx);
+int lt_se_label_create(lt_btensor *bt, const char *table_id, lt_symel **out);
+/* block_labeling::assign(mask, block, label) */
+int lt_se_label_assign(lt_symel *e, const int *mask, size_t blk, size_t label);
+/* se_label::set_rule(label_set) */
+int lt_se_label_set_rule(lt_symel *e, const size_t *intrinsic, size_t nintr);
+int lt_symel_destroy(lt_symel *e);
+int lt_btensor_insert(lt_btensor *bt, const lt_symel *e);
+
+/* ---- btod_contract2<N, M, K> (libtensor/block_tensor/btod_contract2.h:29;
+ *      constructor :68, perform(btc) :115, perform(btc, d) :121,
+ *      compute_block :124; called from
+ *      libtensor/expr/btensor/impl/eval_btensor_double_contract.C:218) ---- */
+int lt_btod_contract2_create(size_t N, size_t M, size_t K, const size_t *permc,
+    const size_t *ka_idx, const size_t *kb_idx, lt_btensor *a, double ka,
+    lt_btensor *b, double kb, double kc, lt_btod_contract2 **out);
+int lt_btod_contract2_destroy(lt_btod_contract2 *op);
+/* perform(btc): C := result.  mode: LT_MODE_*. */
+int lt_btod_contract2_perform(lt_btod_contract2 *op, lt_btensor *c, int mode);
+/* perform(btc, d): C += d * result (C with the result's symmetry). */
+int lt_btod_contract2_perform_add(lt_btod_contract2 *op, lt_btensor *c,
+    double d, int mode);
+/* get_schedule(): sorted non-zero canonical C blocks. */
+int lt_btod_contract2_schedule(lt_btod_contract2 *op, size_t *out, size_t cap,
+    size_t *count);
+/* Result symmetry: canonical index / allowedness of a C block. */
+int lt_btod_contract2_symc_orbit(lt_btod_contract2 *op, size_t aidx,
+    size_t *acidx, size_t *perm, double *coeff, int *allowed);
+/* Contraction list of a C block (gen_bto_contract2_clst_builder::build_list
+ * after coalescing), 2 + 2n + 2 values per pair:
+ * aia, acia, perm_a[NA], coeff_a, aib, acib, perm_b[NB], coeff_b. */
+int lt_btod_contract2_clst(lt_btod_contract2 *op, size_t aidx_c, double *out,
+    size_t cap_pairs, size_t *npairs);
+/* compute_block(zero = true, ic, identity, blk) on the host (FP64 loops). */
+int lt_btod_contract2_compute_block_host(lt_btod_contract2 *op, size_t aidx_c,
+    double *host);
+int lt_btod_contract2_stats(lt_btod_contract2 *op, lt_stats *out);
+/* Tile shape of the grouped GEMM (-1: chosen per schedule). */
+int lt_btod_contract2_set_tile(lt_btod_contract2 *op, int cfg);
+/* Split-precision mode: number of INT8 slices (0 = automatic). */
+int lt_btod_contract2_set_ozaki(lt_btod_contract2 *op, int slices);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* LT_AMD_H */

@@ -1,0 +1,446 @@
+"""MI355X block-tensor contraction: host-side mirror of libtensor's
+btod_contract2 path over the C ABI of liblt_amd.so (include/lt_amd.h).
+
+Names follow libtensor: block_index_space, block_tensor, se_perm, se_part,
+se_label, contraction2, btod_contract2, tod_copy, tod_contract2, tod_dotprod.
+Block indexes are tuples (or absolute indexes) over the block-index
+dimensions; blocks are row-major numpy float64 arrays.
+"""
+import ctypes
+
+import numpy as np
+
+from . import _capi
+from ._capi import (LT_MODE_GROUPED, LT_MODE_OZAKI, LT_MODE_PER_BLOCK, LtError,
+                    LtStats, call, dptr, ints, sizes)
+
+__all__ = ["Context", "product_table", "block_index_space", "block_tensor",
+           "se_perm", "se_part", "se_label", "contraction2", "btod_contract2",
+           "device_array", "tod_copy", "tod_set", "tod_contract2", "tod_dotprod",
+           "LtError", "LT_MODE_PER_BLOCK", "LT_MODE_GROUPED", "LT_MODE_OZAKI"]
+
+
+def _abs_index(idx, dims):
+    if isinstance(idx, (int, np.integer)):
+        return int(idx)
+    a = 0
+    for i, d in zip(idx, dims):
+        a = a * int(d) + int(i)
+    return a
+
+
+def _index(aidx, dims):
+    out = []
+    for d in reversed(dims):
+        out.append(aidx % d)
+        aidx //= d
+    return tuple(reversed(out))
+
+
+class Context(object):
+    """Device context (device >= 0) or host-only context (device < 0)."""
+
+    def __init__(self, device=0, hbm_cap=0):
+        h = ctypes.c_void_p()
+        call("lt_ctx_create", int(device), int(hbm_cap), ctypes.byref(h))
+        self._h = h
+        self.device = device
+
+    def close(self):
+        if self._h:
+            call("lt_ctx_destroy", self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def sync(self):
+        call("lt_ctx_sync", self._h)
+
+    def mem(self):
+        u, c = ctypes.c_size_t(), ctypes.c_size_t()
+        call("lt_ctx_mem", self._h, ctypes.byref(u), ctypes.byref(c))
+        return u.value, c.value
+
+    def set_threads(self, n):
+        call("lt_ctx_set_threads", self._h, int(n))
+
+
+class product_table(object):
+    """Point-group product table registered under an id
+    (libtensor/symmetry/point_group_table.h)."""
+
+    def __init__(self, table_id, nlabels):
+        self.id = table_id
+        call("lt_product_table_create", table_id.encode(), int(nlabels))
+
+    def add_product(self, l1, l2, lr):
+        call("lt_product_table_add_product", self.id.encode(), int(l1), int(l2), int(lr))
+
+
+class block_index_space(object):
+    """Dimensions plus split points (libtensor/core/block_index_space.h)."""
+
+    def __init__(self, dims):
+        self.dims = tuple(int(d) for d in dims)
+        self.splits = []
+
+    def split(self, mask, pos):
+        self.splits.append((tuple(1 if m else 0 for m in mask), int(pos)))
+        return self
+
+
+class se_perm(object):
+    """Permutational symmetry element (libtensor/symmetry/se_perm.h)."""
+
+    def __init__(self, perm, coeff=1.0):
+        self.perm = tuple(int(p) for p in perm)
+        self.coeff = float(coeff)
+
+    def _make(self, bt):
+        h = ctypes.c_void_p()
+        call("lt_se_perm_create", len(self.perm), sizes(self.perm), self.coeff,
+             ctypes.byref(h))
+        return h
+
+
+class se_part(object):
+    """Partition symmetry element (libtensor/symmetry/se_part.h); maps and
+    forbidden partitions are applied in call order."""
+
+    def __init__(self, pdims):
+        self.pdims = tuple(int(p) for p in pdims)
+        self.ops = []
+
+    def add_map(self, idx1, idx2, coeff=1.0):
+        self.ops.append(("map", tuple(idx1), tuple(idx2), float(coeff)))
+        return self
+
+    def mark_forbidden(self, idx):
+        self.ops.append(("forbid", tuple(idx)))
+        return self
+
+    def _make(self, bt):
+        h = ctypes.c_void_p()
+        call("lt_se_part_create", bt._h, sizes(self.pdims), ctypes.byref(h))
+        for op in self.ops:
+            if op[0] == "map":
+                call("lt_se_part_add_map", h, sizes(op[1]), sizes(op[2]), op[3])
+            else:
+                call("lt_se_part_mark_forbidden", h, sizes(op[1]))
+        return h
+
+
+class se_label(object):
+    """Label symmetry element (libtensor/symmetry/se_label.h)."""
+
+    def __init__(self, table_id):
+        self.table = table_id
+        self.assigns = []
+        self.rule = None
+
+    def assign(self, mask, blk, label):
+        self.assigns.append((tuple(1 if m else 0 for m in mask), int(blk), int(label)))
+        return self
+
+    def set_rule(self, labels):
+        if isinstance(labels, int):
+            labels = [labels]
+        self.rule = [int(l) for l in labels]
+        return self
+
+    def _make(self, bt):
+        h = ctypes.c_void_p()
+        call("lt_se_label_create", bt._h, self.table.encode(), ctypes.byref(h))
+        for m, b, l in self.assigns:
+            call("lt_se_label_assign", h, ints(m), b, l)
+        if self.rule is not None:
+            call("lt_se_label_set_rule", h, sizes(self.rule), len(self.rule))
+        return h
+
+
+class block_tensor(object):
+    """Block tensor whose canonical non-zero blocks live in HBM
+    (libtensor/block_tensor/block_tensor.h)."""
+
+    def __init__(self, ctx, bis):
+        if not isinstance(bis, block_index_space):
+            bis = block_index_space(bis)
+        h = ctypes.c_void_p()
+        call("lt_btensor_create", ctx._h, len(bis.dims), sizes(bis.dims), ctypes.byref(h))
+        self._h = h
+        self.ctx = ctx
+        self.dims = bis.dims
+        self.n = len(bis.dims)
+        for mask, pos in bis.splits:
+            call("lt_btensor_split", h, ints(mask), pos)
+        self._elems = []
+
+    def __del__(self):
+        try:
+            if self._h:
+                call("lt_btensor_destroy", self._h)
+                self._h = None
+        except Exception:
+            pass
+
+    @property
+    def bidims(self):
+        out = sizes([0] * self.n)
+        call("lt_btensor_bidims", self._h, out)
+        return tuple(out[i] for i in range(self.n))
+
+    def abs_index(self, idx):
+        return _abs_index(idx, self.bidims)
+
+    def index(self, aidx):
+        return _index(int(aidx), self.bidims)
+
+    def insert(self, el):
+        h = el._make(self)
+        try:
+            call("lt_btensor_insert", self._h, h)
+        finally:
+            call("lt_symel_destroy", h)
+        self._elems.append(el)
+
+    def block_dims(self, idx):
+        out = sizes([0] * self.n)
+        call("lt_btensor_block_dims", self._h, self.abs_index(idx), out)
+        return tuple(out[i] for i in range(self.n))
+
+    def block_start(self, idx):
+        out = sizes([0] * self.n)
+        call("lt_btensor_block_start", self._h, self.abs_index(idx), out)
+        return tuple(out[i] for i in range(self.n))
+
+    def req_nonzero_blocks(self):
+        cnt = ctypes.c_size_t()
+        call("lt_btensor_nonzero", self._h, None, 0, ctypes.byref(cnt))
+        out = (ctypes.c_size_t * max(1, cnt.value))()
+        call("lt_btensor_nonzero", self._h, out, cnt.value, ctypes.byref(cnt))
+        return [out[i] for i in range(cnt.value)]
+
+    def set_block(self, idx, data):
+        a = self.abs_index(idx)
+        arr = np.ascontiguousarray(data, dtype=np.float64)
+        if arr.shape != self.block_dims(a):
+            raise ValueError("block shape %s != %s" % (arr.shape, self.block_dims(a)))
+        call("lt_btensor_set_block", self._h, a, dptr(arr))
+
+    def get_block(self, idx):
+        a = self.abs_index(idx)
+        arr = np.empty(self.block_dims(a), dtype=np.float64)
+        call("lt_btensor_get_block", self._h, a, dptr(arr))
+        return arr
+
+    def zero_block(self, idx):
+        call("lt_btensor_zero_block", self._h, self.abs_index(idx))
+
+    def zero(self):
+        call("lt_btensor_zero_all", self._h)
+
+    def orbit(self, idx):
+        """(canonical abs index, perm canonical->block, coeff, allowed)."""
+        ac = ctypes.c_size_t()
+        perm = sizes([0] * self.n)
+        c = ctypes.c_double()
+        al = ctypes.c_int()
+        call("lt_btensor_orbit", self._h, self.abs_index(idx), ctypes.byref(ac), perm,
+             ctypes.byref(c), ctypes.byref(al))
+        return ac.value, tuple(perm[i] for i in range(self.n)), c.value, bool(al.value)
+
+    def random(self, seed, dist="uniform", scale=1.0, scale_dims=None):
+        d0, d1 = (-1, -1) if scale_dims is None else scale_dims
+        call("lt_btensor_random", self._h, int(seed), 1 if dist == "normal" else 0,
+             float(scale), int(d0), int(d1))
+
+    def to_dense(self):
+        arr = np.empty(self.dims, dtype=np.float64)
+        call("lt_btensor_export", self._h, dptr(arr))
+        return arr
+
+    def from_dense(self, arr):
+        arr = np.ascontiguousarray(arr, dtype=np.float64)
+        if arr.shape != self.dims:
+            raise ValueError("shape")
+        call("lt_btensor_import", self._h, dptr(arr))
+
+    def copy_symmetry_from(self, other):
+        call("lt_btensor_copy_symmetry", self._h, other._h)
+
+
+class contraction2(object):
+    """Connection of the indexes of A, B and C (libtensor/core/contraction2.h)."""
+
+    def __init__(self, N, M, K, permc=None):
+        self.N, self.M, self.K = int(N), int(M), int(K)
+        self.permc = tuple(range(N + M)) if permc is None else tuple(int(p) for p in permc)
+        self.pairs = []
+
+    def contract(self, ia, ib):
+        if len(self.pairs) >= self.K:
+            raise LtError(-1, "contraction is complete")
+        self.pairs.append((int(ia), int(ib)))
+        return self
+
+
+class btod_contract2(object):
+    """C = kc * contract(ka A, kb B) (libtensor/block_tensor/btod_contract2.h)."""
+
+    def __init__(self, contr, bta, btb, ka=1.0, kb=1.0, kc=1.0):
+        if len(contr.pairs) != contr.K:
+            raise LtError(-1, "contraction is incomplete")
+        h = ctypes.c_void_p()
+        ka_idx = [p[0] for p in contr.pairs]
+        kb_idx = [p[1] for p in contr.pairs]
+        call("lt_btod_contract2_create", contr.N, contr.M, contr.K, sizes(contr.permc),
+             sizes(ka_idx), sizes(kb_idx), bta._h, float(ka), btb._h, float(kb),
+             float(kc), ctypes.byref(h))
+        self._h = h
+        self.contr = contr
+        self.a, self.b = bta, btb
+
+    def __del__(self):
+        try:
+            if self._h:
+                call("lt_btod_contract2_destroy", self._h)
+                self._h = None
+        except Exception:
+            pass
+
+    def perform(self, btc, mode=LT_MODE_GROUPED):
+        call("lt_btod_contract2_perform", self._h, btc._h, int(mode))
+
+    def perform_add(self, btc, d=1.0, mode=LT_MODE_GROUPED):
+        call("lt_btod_contract2_perform_add", self._h, btc._h, float(d), int(mode))
+
+    def get_schedule(self):
+        cnt = ctypes.c_size_t()
+        call("lt_btod_contract2_schedule", self._h, None, 0, ctypes.byref(cnt))
+        out = (ctypes.c_size_t * max(1, cnt.value))()
+        call("lt_btod_contract2_schedule", self._h, out, cnt.value, ctypes.byref(cnt))
+        return [out[i] for i in range(cnt.value)]
+
+    def symc_orbit(self, aidx):
+        nc = self.contr.N + self.contr.M
+        ac = ctypes.c_size_t()
+        perm = sizes([0] * nc)
+        c = ctypes.c_double()
+        al = ctypes.c_int()
+        call("lt_btod_contract2_symc_orbit", self._h, int(aidx), ctypes.byref(ac), perm,
+             ctypes.byref(c), ctypes.byref(al))
+        return ac.value, tuple(perm[i] for i in range(nc)), c.value, bool(al.value)
+
+    def clst(self, aidx_c):
+        """Coalesced contraction list of a C block (original index space):
+        list of (aia, acia, perm_a, coeff_a, aib, acib, perm_b, coeff_b)."""
+        na = self.contr.N + self.contr.K
+        nb = self.contr.M + self.contr.K
+        w = 6 + na + nb
+        cnt = ctypes.c_size_t()
+        call("lt_btod_contract2_clst", self._h, int(aidx_c), None, 0, ctypes.byref(cnt))
+        buf = np.zeros(max(1, cnt.value) * w, dtype=np.float64)
+        call("lt_btod_contract2_clst", self._h, int(aidx_c), dptr(buf), cnt.value,
+             ctypes.byref(cnt))
+        out = []
+        for k in range(cnt.value):
+            r = buf[k * w:(k + 1) * w]
+            j = 0
+            aia, acia = int(r[0]), int(r[1])
+            pa = tuple(int(x) for x in r[2:2 + na])
+            ca = float(r[2 + na])
+            j = 3 + na
+            aib, acib = int(r[j]), int(r[j + 1])
+            pb = tuple(int(x) for x in r[j + 2:j + 2 + nb])
+            cb = float(r[j + 2 + nb])
+            out.append((aia, acia, pa, ca, aib, acib, pb, cb))
+        return out
+
+    def compute_block_host(self, aidx_c, dims):
+        arr = np.empty(dims, dtype=np.float64)
+        call("lt_btod_contract2_compute_block_host", self._h, int(aidx_c), dptr(arr))
+        return arr
+
+    def stats(self):
+        s = LtStats()
+        call("lt_btod_contract2_stats", self._h, ctypes.byref(s))
+        return {f: getattr(s, f) for f, _ in LtStats._fields_}
+
+    def set_tile(self, cfg):
+        call("lt_btod_contract2_set_tile", self._h, int(cfg))
+
+    def set_ozaki(self, slices):
+        call("lt_btod_contract2_set_ozaki", self._h, int(slices))
+
+
+# ---------------------------------------------------------------------------
+# Dense tensors in device memory
+
+class device_array(object):
+    """Dense float64 array in device memory (row-major)."""
+
+    def __init__(self, ctx, shape, data=None):
+        self.ctx = ctx
+        self.shape = tuple(int(s) for s in shape)
+        self.size = int(np.prod(self.shape)) if self.shape else 1
+        p = ctypes.c_void_p()
+        call("lt_dev_alloc", ctx._h, max(8, 8 * self.size), ctypes.byref(p))
+        self.ptr = p
+        if data is not None:
+            self.upload(data)
+
+    def upload(self, data):
+        a = np.ascontiguousarray(data, dtype=np.float64).reshape(self.shape)
+        call("lt_dev_upload", self.ctx._h, self.ptr, a.ctypes.data_as(ctypes.c_void_p),
+             8 * self.size)
+
+    def download(self):
+        a = np.empty(self.shape, dtype=np.float64)
+        call("lt_dev_download", self.ctx._h, a.ctypes.data_as(ctypes.c_void_p), self.ptr,
+             8 * self.size)
+        return a
+
+    def free(self):
+        if self.ptr:
+            call("lt_dev_free", self.ctx._h, self.ptr)
+            self.ptr = None
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass
+
+
+def tod_copy(a, b, perm=None, c=1.0, zero=True):
+    """b = c P(a) (zero) or b += c P(a) (libtensor/dense_tensor/tod_copy.h)."""
+    p = None if perm is None else sizes(perm)
+    call("lt_tod_copy", a.ctx._h, len(a.shape), sizes(a.shape), a.ptr, p, float(c),
+         1 if zero else 0, b.ptr)
+
+
+def tod_set(a, v=0.0):
+    call("lt_tod_set", a.ctx._h, a.size, float(v), a.ptr)
+
+
+def tod_contract2(contr, a, b, c, d=1.0, zero=True):
+    """c (=|+=) d contract(a, b) (libtensor/dense_tensor/tod_contract2.h)."""
+    ka = [p[0] for p in contr.pairs]
+    kb = [p[1] for p in contr.pairs]
+    call("lt_tod_contract2", a.ctx._h, contr.N, contr.M, contr.K, sizes(contr.permc),
+         sizes(ka), sizes(kb), sizes(a.shape), a.ptr, sizes(b.shape), b.ptr, float(d),
+         1 if zero else 0, c.ptr)
+
+
+def tod_dotprod(a, b, perma=None, permb=None, c=1.0):
+    """sum P_a(a) P_b(b) times c (libtensor/dense_tensor/tod_dotprod.h)."""
+    r = ctypes.c_double()
+    pa = None if perma is None else sizes(perma)
+    pb = None if permb is None else sizes(permb)
+    call("lt_tod_dotprod", a.ctx._h, len(a.shape), sizes(a.shape), a.ptr, pa,
+         sizes(b.shape), b.ptr, pb, float(c), ctypes.byref(r))
+    return r.value

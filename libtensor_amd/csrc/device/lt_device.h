@@ -1,0 +1,91 @@
+// Device work descriptors and kernel launchers for the block-contraction path
+// on CDNA4 (gfx950).  Plain structs only: these are copied to HBM as work
+// lists and read by the kernels.
+#pragma once
+
+#include <cstddef>
+#include <cstdint>
+#include <hip/hip_runtime_api.h>
+
+namespace lta {
+namespace dev {
+
+static const int kPermMaxOrder = 8;
+
+//! Fast unsigned division by a runtime constant for n < 2^31
+//! (Granlund & Montgomery, "Division by invariant integers using
+//! multiplication", PLDI 1994): q = (mulhi(n, mul) + n) >> shift.
+struct fastdiv {
+    uint32_t d, mul, shift, pad;
+};
+fastdiv make_fastdiv(uint32_t d);
+
+//! dst = alpha * P(src) (+ dst if accumulate).  dims are the destination
+//! dimensions after fusing; sstride[i] is the source stride of destination
+//! dimension i; the destination is contiguous.
+struct perm_desc {
+    const double *src;
+    double *dst;
+    uint32_t n;                       //!< fused order, 1..8
+    uint32_t total;                   //!< elements (< 2^31)
+    fastdiv div[kPermMaxOrder];       //!< destination dimensions
+    uint32_t sstride[kPermMaxOrder];  //!< source strides
+    double alpha;
+    int32_t accumulate;
+    int32_t pad;
+};
+
+//! One C block of a grouped GEMM: C(m x n, leading dimension ldc) is
+//! overwritten (zero != 0) or accumulated with the sum over its pairs.
+struct gemm_cblock {
+    double *c;
+    uint32_t m, n, ldc;
+    uint32_t pair_begin, pair_end;
+    int32_t zero;
+};
+
+//! One product d * op(A) op(B) contributing to a C block.  A is m x k
+//! ("ik", lda = row stride) or k x m ("ki"); B is k x n ("kj") or n x k
+//! ("jk"), as fixed by the launch's transposition flags.
+struct gemm_pair {
+    const double *a;
+    const double *b;
+    uint32_t k, lda, ldb, pad;
+    double d;
+};
+
+//! Work item of a grouped GEMM launch: one output tile of one C block.
+struct gemm_tile {
+    uint32_t cblock;
+    uint16_t tm, tn;
+};
+
+enum { TILE_64x64 = 0, TILE_32x64 = 1, TILE_64x32 = 2, TILE_32x32 = 3, TILE_COUNT = 4 };
+void tile_shape(int cfg, uint32_t &bm, uint32_t &bn);
+
+//! Batched permutation: every descriptor is split into chunks of `chunk`
+//! elements; chunk_desc[j] / chunk_off[j] name the descriptor and the first
+//! element of chunk j.
+hipError_t launch_permute(const perm_desc *descs, const uint32_t *chunk_desc,
+    const uint32_t *chunk_off, uint32_t nchunks, hipStream_t s);
+static const uint32_t kPermChunk = 2048;
+
+//! Grouped FP64 MFMA GEMM over a tile list.  ta: A stored k x m; tb: B
+//! stored n x k.  nwg workgroups loop over the tiles (nwg == ntiles gives
+//! one workgroup per tile).
+hipError_t launch_gemm(int cfg, bool ta, bool tb, const gemm_cblock *cb,
+    const gemm_pair *pairs, const gemm_tile *tiles, uint32_t ntiles,
+    uint32_t nwg, hipStream_t s);
+
+//! Fill n doubles with v.
+hipError_t launch_fill(double *p, size_t n, double v, hipStream_t s);
+
+//! Dot product of permuted operands: sum_i a[i] * b(P(i)) over descriptors
+//! whose `src` is b and `dst` is (const) a; partials[wg] receive one value
+//! per workgroup, then a final pass sums them in a fixed order into *out.
+hipError_t launch_dot(const perm_desc *descs, const uint32_t *chunk_desc,
+    const uint32_t *chunk_off, uint32_t nchunks, double *partials,
+    double *out, hipStream_t s);
+
+} // namespace dev
+} // namespace lta

@@ -1,0 +1,820 @@
+// C ABI of the block-contraction path (declarations in include/lt_amd.h).
+#include "../../../include/lt_amd.h"
+#include "lt_runtime.h"
+
+#include <cstring>
+#include <string>
+
+using namespace lta;
+
+struct lt_ctx { context *c; };
+struct lt_btensor { btensor *t; lt_ctx *ctx; };
+struct lt_symel { std::shared_ptr<sym_element> e; };
+struct lt_btod_contract2 { btod_contract2 *op; lt_btensor *a, *b; };
+
+namespace {
+
+thread_local std::string g_err;
+
+template<typename F>
+int guard(F f) {
+    try {
+        f();
+        return LT_OK;
+    } catch(const lt_error &e) {
+        g_err = e.what();
+        return e.code;
+    } catch(const std::bad_alloc &) {
+        g_err = "host out of memory";
+        return LT_ERR_NOMEM;
+    } catch(const std::exception &e) {
+        g_err = e.what();
+        return LT_ERR_INTERNAL;
+    } catch(...) {
+        g_err = "unknown error";
+        return LT_ERR_INTERNAL;
+    }
+}
+
+void need(const void *p, const char *what) {
+    if(!p) throw lt_error(LT_ERR_PARAM, std::string("null argument: ") + what);
+}
+
+perm_t perm_or_identity(size_t n, const size_t *p) {
+    perm_t r(n);
+    if(p) {
+        bool seen[kMaxOrder] = {false};
+        for(size_t i = 0; i < n; i++) {
+            lt_require(p[i] < n && !seen[p[i]], LT_ERR_PARAM, "invalid permutation");
+            seen[p[i]] = true;
+            r.p[i] = (uint8_t)p[i];
+        }
+    }
+    return r;
+}
+
+void fill_transf(const tensor_tr &tr, size_t n, size_t *perm, double *coeff) {
+    if(perm) for(size_t i = 0; i < n; i++) perm[i] = tr.perm[i];
+    if(coeff) *coeff = tr.st.c;
+}
+
+contraction2 make_contraction(size_t N, size_t M, size_t K, const size_t *permc,
+    const size_t *ka, const size_t *kb) {
+    lt_require(N + K <= LT_MAX_ORDER && M + K <= LT_MAX_ORDER &&
+        N + M <= LT_MAX_ORDER && N + M >= 1, LT_ERR_PARAM, "invalid orders");
+    perm_t pc = perm_or_identity(N + M, permc);
+    contraction2 c(N, M, K, &pc);
+    for(size_t i = 0; i < K; i++) c.contract(ka[i], kb[i]);
+    return c;
+}
+
+// Dense helper: dst = alpha * P(src) with the batched permute kernel.
+void dense_permute(context &ctx, const double *src, const dims_t &dsrc,
+    const tensor_tr &tr, double *dst, bool accumulate);
+
+} // unnamed namespace
+
+namespace {
+void dense_permute(context &ctx, const double *src, const dims_t &dsrc,
+    const tensor_tr &tr, double *dst, bool accumulate) {
+    dev::perm_desc d = make_perm_desc_public(src, dsrc, tr, dst, accumulate);
+    std::vector<uint32_t> cd, co;
+    for(uint32_t off = 0; off < d.total; off += dev::kPermChunk) {
+        cd.push_back(0);
+        co.push_back(off);
+    }
+    if(cd.empty()) return;
+    size_t bytes = sizeof(d) + 8 * cd.size() + 512;
+    char *mem = (char*)ctx.alloc(bytes);
+    std::vector<char> host(bytes, 0);
+    size_t ocd = (sizeof(d) + 255) & ~size_t(255);
+    size_t oco = ocd + ((4 * cd.size() + 255) & ~size_t(255));
+    std::memcpy(&host[0], &d, sizeof(d));
+    std::memcpy(&host[ocd], cd.data(), 4 * cd.size());
+    std::memcpy(&host[oco], co.data(), 4 * co.size());
+    try {
+        ctx.upload(mem, host.data(), bytes);
+        hip_check(dev::launch_permute((dev::perm_desc*)mem, (uint32_t*)(mem + ocd),
+            (uint32_t*)(mem + oco), (uint32_t)cd.size(), ctx.stream()), "permute");
+        ctx.sync();
+    } catch(...) {
+        ctx.free(mem);
+        throw;
+    }
+    ctx.free(mem);
+}
+}
+
+extern "C" {
+
+const char *lt_last_error(void) { return g_err.c_str(); }
+int lt_version(void) { return 10000; }
+
+// ---- context ----
+
+int lt_ctx_create(int device, size_t cap, lt_ctx **out) {
+    return guard([&]() {
+        need(out, "out");
+        lt_ctx *c = new lt_ctx;
+        try {
+            c->c = new context(device, cap);
+        } catch(...) {
+            delete c;
+            throw;
+        }
+        *out = c;
+    });
+}
+
+int lt_ctx_destroy(lt_ctx *ctx) {
+    return guard([&]() {
+        if(!ctx) return;
+        delete ctx->c;
+        delete ctx;
+    });
+}
+
+int lt_ctx_sync(lt_ctx *ctx) {
+    return guard([&]() { need(ctx, "ctx"); ctx->c->sync(); });
+}
+
+int lt_ctx_mem(lt_ctx *ctx, size_t *used, size_t *cap) {
+    return guard([&]() {
+        need(ctx, "ctx");
+        if(used) *used = ctx->c->used();
+        if(cap) *cap = ctx->c->cap();
+    });
+}
+
+int lt_ctx_set_threads(lt_ctx *ctx, size_t n) {
+    return guard([&]() { need(ctx, "ctx"); ctx->c->set_nthreads(n); });
+}
+
+int lt_dev_alloc(lt_ctx *ctx, size_t bytes, void **dptr) {
+    return guard([&]() {
+        need(ctx, "ctx"); need(dptr, "dptr");
+        ctx->c->require_device("lt_dev_alloc");
+        *dptr = ctx->c->alloc(bytes);
+    });
+}
+
+int lt_dev_free(lt_ctx *ctx, void *p) {
+    return guard([&]() { need(ctx, "ctx"); ctx->c->free(p); });
+}
+
+int lt_dev_upload(lt_ctx *ctx, void *dst, const void *host, size_t bytes) {
+    return guard([&]() {
+        need(ctx, "ctx"); need(dst, "dst"); need(host, "host");
+        ctx->c->upload(dst, host, bytes);
+    });
+}
+
+int lt_dev_download(lt_ctx *ctx, void *host, const void *src, size_t bytes) {
+    return guard([&]() {
+        need(ctx, "ctx"); need(src, "src"); need(host, "host");
+        ctx->c->download(host, src, bytes);
+    });
+}
+
+// ---- dense ----
+
+int lt_tod_copy(lt_ctx *ctx, size_t n, const size_t *dims_a, const double *a,
+    const size_t *perm, double c, int zero, double *b) {
+    return guard([&]() {
+        need(ctx, "ctx"); need(dims_a, "dims_a"); need(a, "a"); need(b, "b");
+        ctx->c->require_device("tod_copy");
+        lt_require(n >= 1 && n <= LT_MAX_ORDER, LT_ERR_PARAM, "order");
+        tensor_tr tr(perm_or_identity(n, perm), scalar_tr(c));
+        dense_permute(*ctx->c, a, dims_t::from(n, dims_a), tr, b, zero == 0);
+    });
+}
+
+int lt_tod_set(lt_ctx *ctx, size_t nelem, double v, double *a) {
+    return guard([&]() {
+        need(ctx, "ctx"); need(a, "a");
+        ctx->c->require_device("tod_set");
+        ctx->c->activate();
+        hip_check(dev::launch_fill(a, nelem, v, ctx->c->stream()), "fill");
+        ctx->c->sync();
+    });
+}
+
+int lt_tod_contract2(lt_ctx *ctx, size_t N, size_t M, size_t K,
+    const size_t *permc, const size_t *ka, const size_t *kb,
+    const size_t *dims_a, const double *a, const size_t *dims_b,
+    const double *b, double d, int zero, double *c) {
+    return guard([&]() {
+        need(ctx, "ctx"); need(dims_a, "dims_a"); need(dims_b, "dims_b");
+        need(a, "a"); need(b, "b"); need(c, "c");
+        if(K) { need(ka, "ka"); need(kb, "kb"); }
+        context &cx = *ctx->c;
+        cx.require_device("tod_contract2");
+        contraction2 contr = make_contraction(N, M, K, permc, ka, kb);
+        dims_t da = dims_t::from(N + K, dims_a), db = dims_t::from(M + K, dims_b);
+        dims_t dc = contract2_dims(contr, da, db);
+        if(d == 0.0) {
+            if(zero) {
+                cx.activate();
+                hip_check(dev::launch_fill(c, dc.size(), 0.0, cx.stream()), "fill");
+                cx.sync();
+            }
+            return;
+        }
+        // align as tod_contract2::perform (tod_contract2_impl.h:124-135),
+        // permute A / B into scratch if needed, one GEMM, permute C back
+        contraction2_align al(contr);
+        contraction2 ca(contr);
+        ca.permute_a(al.perma);
+        ca.permute_b(al.permb);
+        ca.permute_c(al.permc);
+        dims_t da2(da), db2(db), dc2(dc);
+        da2.permute(al.perma);
+        db2.permute(al.permb);
+        dc2.permute(al.permc);
+        std::vector<void*> tmp;
+        try {
+            const double *pa = a, *pb = b;
+            if(!al.perma.is_identity()) {
+                double *t = (double*)cx.alloc(da.size() * 8);
+                tmp.push_back(t);
+                dense_permute(cx, a, da, tensor_tr(al.perma, scalar_tr()), t, false);
+                pa = t;
+            }
+            if(!al.permb.is_identity()) {
+                double *t = (double*)cx.alloc(db.size() * 8);
+                tmp.push_back(t);
+                dense_permute(cx, b, db, tensor_tr(al.permb, scalar_tr()), t, false);
+                pb = t;
+            }
+            double *pc = c;
+            bool cperm = !al.permc.is_identity();
+            if(cperm) {
+                pc = (double*)cx.alloc(dc.size() * 8);
+                tmp.push_back(pc);
+            }
+            gemm_form g = aligned_gemm_form(ca, da2, db2);
+            dev::gemm_cblock cb;
+            dev::gemm_pair gp;
+            std::memset(&cb, 0, sizeof(cb));
+            std::memset(&gp, 0, sizeof(gp));
+            size_t lda = g.ta ? g.m : g.k, ldb = g.tb ? g.k : g.n;
+            bool ta = g.ta, tb = g.tb;
+            if(!g.tc) {
+                cb.m = (uint32_t)g.m; cb.n = (uint32_t)g.n;
+                gp.a = pa; gp.b = pb; gp.lda = (uint32_t)lda; gp.ldb = (uint32_t)ldb;
+            } else {
+                cb.m = (uint32_t)g.n; cb.n = (uint32_t)g.m;
+                gp.a = pb; gp.b = pa; gp.lda = (uint32_t)ldb; gp.ldb = (uint32_t)lda;
+                ta = !g.tb; tb = !g.ta;
+            }
+            cb.ldc = cb.n;
+            cb.c = pc;
+            cb.zero = (cperm || zero) ? 1 : 0;
+            cb.pair_begin = 0;
+            cb.pair_end = 1;
+            gp.k = (uint32_t)g.k;
+            gp.d = d;
+            uint32_t bm = 64, bn = 64;
+            dev::tile_shape(dev::TILE_64x64, bm, bn);
+            std::vector<dev::gemm_tile> tiles;
+            for(uint32_t i = 0; i < (cb.m + bm - 1) / bm; i++)
+                for(uint32_t j = 0; j < (cb.n + bn - 1) / bn; j++) {
+                    dev::gemm_tile t; t.cblock = 0; t.tm = (uint16_t)i; t.tn = (uint16_t)j;
+                    tiles.push_back(t);
+                }
+            size_t bytes = 512 + 512 + tiles.size() * sizeof(dev::gemm_tile);
+            char *mem = (char*)cx.alloc(bytes);
+            tmp.push_back(mem);
+            std::vector<char> host(bytes, 0);
+            std::memcpy(&host[0], &cb, sizeof(cb));
+            std::memcpy(&host[512], &gp, sizeof(gp));
+            std::memcpy(&host[1024], tiles.data(), tiles.size() * sizeof(dev::gemm_tile));
+            cx.upload(mem, host.data(), bytes);
+            cx.activate();
+            hip_check(dev::launch_gemm(dev::TILE_64x64, ta, tb, (dev::gemm_cblock*)mem,
+                (dev::gemm_pair*)(mem + 512), (dev::gemm_tile*)(mem + 1024),
+                (uint32_t)tiles.size(), (uint32_t)tiles.size(), cx.stream()), "gemm");
+            cx.sync();
+            if(cperm) {
+                dense_permute(cx, pc, dc2, tensor_tr(al.permc.inverse(), scalar_tr()),
+                    c, zero == 0);
+            }
+        } catch(...) {
+            for(size_t i = 0; i < tmp.size(); i++) cx.free(tmp[i]);
+            throw;
+        }
+        for(size_t i = 0; i < tmp.size(); i++) cx.free(tmp[i]);
+    });
+}
+
+int lt_tod_dotprod(lt_ctx *ctx, size_t n, const size_t *dims_a,
+    const double *a, const size_t *perma, const size_t *dims_b,
+    const double *b, const size_t *permb, double c, double *result) {
+    return guard([&]() {
+        need(ctx, "ctx"); need(dims_a, "dims_a"); need(dims_b, "dims_b");
+        need(a, "a"); need(b, "b"); need(result, "result");
+        context &cx = *ctx->c;
+        cx.require_device("tod_dotprod");
+        perm_t pa = perm_or_identity(n, perma), pb = perm_or_identity(n, permb);
+        dims_t da = dims_t::from(n, dims_a), db = dims_t::from(n, dims_b);
+        dims_t da2(da), db2(db);
+        da2.permute(pa);
+        db2.permute(pb);
+        lt_require(da2 == db2, LT_ERR_DIMS, "tod_dotprod: dimensions differ");
+        // B seen in the order of A: apply pb, then the inverse of pa
+        perm_t q(pb);
+        q.permute(pa.inverse());
+        dev::perm_desc d = make_perm_desc_public(b, db, tensor_tr(q, scalar_tr(c)),
+            const_cast<double*>(a), false);
+        std::vector<uint32_t> cd, co;
+        for(uint32_t off = 0; off < d.total; off += dev::kPermChunk) {
+            cd.push_back(0);
+            co.push_back(off);
+        }
+        size_t nch = cd.size();
+        size_t o_cd = 256, o_co = o_cd + ((4 * nch + 255) & ~size_t(255));
+        size_t o_part = o_co + ((4 * nch + 255) & ~size_t(255));
+        size_t o_out = o_part + ((8 * nch + 255) & ~size_t(255));
+        size_t bytes = o_out + 256;
+        char *mem = (char*)cx.alloc(bytes);
+        try {
+            std::vector<char> host(bytes, 0);
+            std::memcpy(&host[0], &d, sizeof(d));
+            if(nch) {
+                std::memcpy(&host[o_cd], cd.data(), 4 * nch);
+                std::memcpy(&host[o_co], co.data(), 4 * nch);
+            }
+            cx.upload(mem, host.data(), bytes);
+            cx.activate();
+            hip_check(dev::launch_dot((dev::perm_desc*)mem, (uint32_t*)(mem + o_cd),
+                (uint32_t*)(mem + o_co), (uint32_t)nch, (double*)(mem + o_part),
+                (double*)(mem + o_out), cx.stream()), "dot");
+            cx.download(result, mem + o_out, 8);
+        } catch(...) {
+            cx.free(mem);
+            throw;
+        }
+        cx.free(mem);
+    });
+}
+
+// ---- product tables ----
+
+int lt_product_table_create(const char *id, size_t nlabels) {
+    return guard([&]() {
+        need(id, "id");
+        product_table_registry::instance().add(product_table(id, nlabels));
+    });
+}
+
+int lt_product_table_add_product(const char *id, size_t l1, size_t l2, size_t lr) {
+    return guard([&]() {
+        need(id, "id");
+        product_table t = product_table_registry::instance().get(id);
+        t.add_product(l1, l2, lr);
+        product_table_registry::instance().add(t);
+    });
+}
+
+// ---- block tensors ----
+
+int lt_btensor_create(lt_ctx *ctx, size_t n, const size_t *dims, lt_btensor **out) {
+    return guard([&]() {
+        need(ctx, "ctx"); need(dims, "dims"); need(out, "out");
+        lt_require(n >= 1 && n <= LT_MAX_ORDER, LT_ERR_PARAM, "order must be 1..8");
+        for(size_t i = 0; i < n; i++) lt_require(dims[i] > 0, LT_ERR_PARAM, "zero dimension");
+        bis_t bis(dims_t::from(n, dims));
+        lt_btensor *b = new lt_btensor;
+        b->ctx = ctx;
+        try {
+            b->t = new btensor(*ctx->c, bis);
+        } catch(...) {
+            delete b;
+            throw;
+        }
+        *out = b;
+    });
+}
+
+int lt_btensor_destroy(lt_btensor *bt) {
+    return guard([&]() {
+        if(!bt) return;
+        delete bt->t;
+        delete bt;
+    });
+}
+
+int lt_btensor_split(lt_btensor *bt, const int *mask, size_t pos) {
+    return guard([&]() {
+        need(bt, "bt"); need(mask, "mask");
+        bool m[kMaxOrder] = {false};
+        for(size_t i = 0; i < bt->t->bis().order(); i++) m[i] = mask[i] != 0;
+        bt->t->split(m, pos);
+    });
+}
+
+int lt_btensor_order(lt_btensor *bt, size_t *n) {
+    return guard([&]() { need(bt, "bt"); need(n, "n"); *n = bt->t->bis().order(); });
+}
+
+int lt_btensor_bidims(lt_btensor *bt, size_t *bidims) {
+    return guard([&]() {
+        need(bt, "bt"); need(bidims, "bidims");
+        dims_t d = bt->t->bidims();
+        for(size_t i = 0; i < d.n; i++) bidims[i] = d[i];
+    });
+}
+
+int lt_btensor_block_dims(lt_btensor *bt, size_t aidx, size_t *dims) {
+    return guard([&]() {
+        need(bt, "bt"); need(dims, "dims");
+        lt_require(aidx < bt->t->bidims().size(), LT_ERR_PARAM, "block index out of range");
+        dims_t d = bt->t->bis().block_dims(get_index(aidx, bt->t->bidims()));
+        for(size_t i = 0; i < d.n; i++) dims[i] = d[i];
+    });
+}
+
+int lt_btensor_block_start(lt_btensor *bt, size_t aidx, size_t *start) {
+    return guard([&]() {
+        need(bt, "bt"); need(start, "start");
+        lt_require(aidx < bt->t->bidims().size(), LT_ERR_PARAM, "block index out of range");
+        index_t s = bt->t->bis().block_start(get_index(aidx, bt->t->bidims()));
+        for(size_t i = 0; i < s.n; i++) start[i] = s[i];
+    });
+}
+
+int lt_btensor_nonzero(lt_btensor *bt, size_t *out, size_t cap, size_t *count) {
+    return guard([&]() {
+        need(bt, "bt"); need(count, "count");
+        std::vector<size_t> v = bt->t->nonzero_blocks();
+        *count = v.size();
+        if(out) for(size_t i = 0; i < v.size() && i < cap; i++) out[i] = v[i];
+    });
+}
+
+int lt_btensor_set_block(lt_btensor *bt, size_t aidx, const double *host) {
+    return guard([&]() {
+        need(bt, "bt"); need(host, "host");
+        lt_require(aidx < bt->t->bidims().size(), LT_ERR_PARAM, "block index out of range");
+        bt->t->set_block(aidx, host);
+    });
+}
+
+int lt_btensor_get_block(lt_btensor *bt, size_t aidx, double *host) {
+    return guard([&]() {
+        need(bt, "bt"); need(host, "host");
+        lt_require(aidx < bt->t->bidims().size(), LT_ERR_PARAM, "block index out of range");
+        bt->t->get_block(aidx, host);
+    });
+}
+
+int lt_btensor_zero_block(lt_btensor *bt, size_t aidx) {
+    return guard([&]() { need(bt, "bt"); bt->t->zero_block(aidx); });
+}
+
+int lt_btensor_zero_all(lt_btensor *bt) {
+    return guard([&]() { need(bt, "bt"); bt->t->zero_all(); });
+}
+
+int lt_btensor_orbit(lt_btensor *bt, size_t aidx, size_t *acidx, size_t *perm,
+    double *coeff, int *allowed) {
+    return guard([&]() {
+        need(bt, "bt");
+        lt_require(aidx < bt->t->bidims().size(), LT_ERR_PARAM, "block index out of range");
+        orbit o(bt->t->sym(), aidx, true);
+        if(acidx) *acidx = o.acidx;
+        fill_transf(o.get_transf(aidx), bt->t->bis().order(), perm, coeff);
+        if(allowed) *allowed = o.allowed ? 1 : 0;
+    });
+}
+
+int lt_btensor_random(lt_btensor *bt, uint64_t seed, int dist, double scale, int d0, int d1) {
+    return guard([&]() {
+        need(bt, "bt");
+        int n = (int)bt->t->bis().order();
+        lt_require(d0 < n && d1 < n, LT_ERR_PARAM, "scaling dimension out of range");
+        bt->t->fill_random(seed, dist, scale, d0, d1);
+    });
+}
+
+int lt_btensor_export(lt_btensor *bt, double *host) {
+    return guard([&]() {
+        need(bt, "bt"); need(host, "host");
+        btensor &t = *bt->t;
+        const bis_t &bis = t.bis();
+        dims_t dims = bis.dims, bid = t.bidims();
+        const size_t n = dims.n;
+        size_t total = dims.size();
+        for(size_t i = 0; i < total; i++) host[i] = 0.0;
+        std::map<size_t, std::vector<double> > cache;
+        for(size_t a = 0; a < bid.size(); a++) {
+            orbit o(t.sym(), a, true);
+            if(!o.allowed || !t.is_nonzero(o.acidx)) continue;
+            std::vector<double> &src = cache[o.acidx];
+            if(src.empty()) {
+                src.resize(t.block_size(o.acidx));
+                t.get_block(o.acidx, src.data());
+            }
+            index_t bi = get_index(a, bid);
+            dims_t db = bis.block_dims(bi);
+            std::vector<double> blk(db.size());
+            host_transform(src.data(), bis.block_dims(get_index(o.acidx, bid)),
+                o.get_transf(a), blk.data(), false);
+            index_t st = bis.block_start(bi);
+            index_t e(n);
+            for(size_t k = 0; k < blk.size(); k++) {
+                size_t off = 0;
+                for(size_t i = 0; i < n; i++) off = off * dims[i] + st[i] + e[i];
+                host[off] = blk[k];
+                inc_index(e, db);
+            }
+        }
+    });
+}
+
+int lt_btensor_import(lt_btensor *bt, const double *host) {
+    return guard([&]() {
+        need(bt, "bt"); need(host, "host");
+        btensor &t = *bt->t;
+        const bis_t &bis = t.bis();
+        dims_t dims = bis.dims, bid = t.bidims();
+        const size_t n = dims.n;
+        t.zero_all();
+        for(size_t a = 0; a < bid.size(); a++) {
+            short_orbit so(t.sym(), a, true);
+            if(!so.allowed || so.acidx != a) continue;
+            index_t bi = get_index(a, bid);
+            dims_t db = bis.block_dims(bi);
+            index_t st = bis.block_start(bi);
+            std::vector<double> blk(db.size());
+            index_t e(n);
+            bool nz = false;
+            for(size_t k = 0; k < blk.size(); k++) {
+                size_t off = 0;
+                for(size_t i = 0; i < n; i++) off = off * dims[i] + st[i] + e[i];
+                blk[k] = host[off];
+                if(blk[k] != 0.0) nz = true;
+                inc_index(e, db);
+            }
+            if(nz) t.set_block(a, blk.data());
+        }
+    });
+}
+
+int lt_btensor_copy_symmetry(lt_btensor *dst, lt_btensor *src) {
+    return guard([&]() {
+        need(dst, "dst"); need(src, "src");
+        dst->t->set_symmetry(src->t->sym());
+    });
+}
+
+// ---- symmetry elements ----
+
+int lt_se_perm_create(size_t n, const size_t *perm, double coeff, lt_symel **out) {
+    return guard([&]() {
+        need(perm, "perm"); need(out, "out");
+        lt_require(n >= 1 && n <= LT_MAX_ORDER, LT_ERR_PARAM, "order");
+        lt_symel *e = new lt_symel;
+        e->e.reset(new se_perm(perm_or_identity(n, perm), scalar_tr(coeff)));
+        *out = e;
+    });
+}
+
+int lt_se_part_create(lt_btensor *bt, const size_t *pdims, lt_symel **out) {
+    return guard([&]() {
+        need(bt, "bt"); need(pdims, "pdims"); need(out, "out");
+        size_t n = bt->t->bis().order();
+        lt_symel *e = new lt_symel;
+        e->e.reset(new se_part(bt->t->bis(), dims_t::from(n, pdims)));
+        *out = e;
+    });
+}
+
+int lt_se_part_add_map(lt_symel *e, const size_t *idx1, const size_t *idx2, double coeff) {
+    return guard([&]() {
+        need(e, "e"); need(idx1, "idx1"); need(idx2, "idx2");
+        se_part *p = dynamic_cast<se_part*>(e->e.get());
+        lt_require(p != 0, LT_ERR_PARAM, "not an se_part");
+        size_t n = p->pdims.n;
+        index_t i1(n), i2(n);
+        for(size_t i = 0; i < n; i++) {
+            lt_require(idx1[i] < p->pdims[i] && idx2[i] < p->pdims[i], LT_ERR_PARAM,
+                "partition index out of range");
+            i1[i] = idx1[i];
+            i2[i] = idx2[i];
+        }
+        p->add_map(i1, i2, scalar_tr(coeff));
+    });
+}
+
+int lt_se_part_mark_forbidden(lt_symel *e, const size_t *idx) {
+    return guard([&]() {
+        need(e, "e"); need(idx, "idx");
+        se_part *p = dynamic_cast<se_part*>(e->e.get());
+        lt_require(p != 0, LT_ERR_PARAM, "not an se_part");
+        size_t n = p->pdims.n;
+        index_t i1(n);
+        for(size_t i = 0; i < n; i++) {
+            lt_require(idx[i] < p->pdims[i], LT_ERR_PARAM, "partition index out of range");
+            i1[i] = idx[i];
+        }
+        p->mark_forbidden(i1);
+    });
+}
+
+int lt_se_label_create(lt_btensor *bt, const char *table_id, lt_symel **out) {
+    return guard([&]() {
+        need(bt, "bt"); need(table_id, "table_id"); need(out, "out");
+        lt_symel *e = new lt_symel;
+        e->e.reset(new se_label(bt->t->bidims(), table_id));
+        *out = e;
+    });
+}
+
+int lt_se_label_assign(lt_symel *e, const int *mask, size_t blk, size_t label) {
+    return guard([&]() {
+        need(e, "e"); need(mask, "mask");
+        se_label *l = dynamic_cast<se_label*>(e->e.get());
+        lt_require(l != 0, LT_ERR_PARAM, "not an se_label");
+        for(size_t i = 0; i < l->bidims.n; i++)
+            if(mask[i]) l->set_label(i, blk, label);
+    });
+}
+
+int lt_se_label_set_rule(lt_symel *e, const size_t *intr, size_t nintr) {
+    return guard([&]() {
+        need(e, "e");
+        se_label *l = dynamic_cast<se_label*>(e->e.get());
+        lt_require(l != 0, LT_ERR_PARAM, "not an se_label");
+        std::vector<size_t> v;
+        for(size_t i = 0; i < nintr; i++) v.push_back(intr[i]);
+        l->set_rule_set(v);
+    });
+}
+
+int lt_symel_destroy(lt_symel *e) {
+    return guard([&]() { delete e; });
+}
+
+int lt_btensor_insert(lt_btensor *bt, const lt_symel *e) {
+    return guard([&]() {
+        need(bt, "bt"); need(e, "e");
+        if(const se_perm *p = dynamic_cast<const se_perm*>(e->e.get())) {
+            lt_require(p->tr.perm.n == bt->t->bis().order(), LT_ERR_SYMMETRY, "order");
+            bis_t b2(bt->t->bis());
+            b2.permute(p->tr.perm);
+            lt_require(b2 == bt->t->bis(), LT_ERR_SYMMETRY,
+                "se_perm does not preserve the block index space");
+        }
+        if(const se_part *p = dynamic_cast<const se_part*>(e->e.get())) {
+            lt_require(p->bis == bt->t->bis(), LT_ERR_SYMMETRY, "se_part: block index space");
+        }
+        if(const se_label *l = dynamic_cast<const se_label*>(e->e.get())) {
+            lt_require(l->bidims == bt->t->bidims(), LT_ERR_SYMMETRY, "se_label: block index space");
+        }
+        bt->t->sym_rw().insert(*e->e);
+    });
+}
+
+// ---- btod_contract2 ----
+
+int lt_btod_contract2_create(size_t N, size_t M, size_t K, const size_t *permc,
+    const size_t *ka_idx, const size_t *kb_idx, lt_btensor *a, double ka,
+    lt_btensor *b, double kb, double kc, lt_btod_contract2 **out) {
+    return guard([&]() {
+        need(a, "a"); need(b, "b"); need(out, "out");
+        if(K) { need(ka_idx, "ka_idx"); need(kb_idx, "kb_idx"); }
+        lt_require(&a->t->ctx() == &b->t->ctx(), LT_ERR_PARAM, "tensors of different contexts");
+        contraction2 c = make_contraction(N, M, K, permc, ka_idx, kb_idx);
+        lt_btod_contract2 *h = new lt_btod_contract2;
+        h->a = a;
+        h->b = b;
+        try {
+            h->op = new btod_contract2(c, *a->t, ka, *b->t, kb, kc);
+        } catch(...) {
+            delete h;
+            throw;
+        }
+        *out = h;
+    });
+}
+
+int lt_btod_contract2_destroy(lt_btod_contract2 *op) {
+    return guard([&]() {
+        if(!op) return;
+        delete op->op;
+        delete op;
+    });
+}
+
+int lt_btod_contract2_perform(lt_btod_contract2 *op, lt_btensor *c, int mode) {
+    return guard([&]() {
+        need(op, "op"); need(c, "c");
+        lt_require(mode >= 0 && mode <= 2, LT_ERR_PARAM, "mode");
+        op->op->perform(*c->t, mode);
+    });
+}
+
+int lt_btod_contract2_perform_add(lt_btod_contract2 *op, lt_btensor *c, double d, int mode) {
+    return guard([&]() {
+        need(op, "op"); need(c, "c");
+        lt_require(mode >= 0 && mode <= 2, LT_ERR_PARAM, "mode");
+        op->op->perform_add(*c->t, d, mode);
+    });
+}
+
+int lt_btod_contract2_schedule(lt_btod_contract2 *op, size_t *out, size_t cap, size_t *count) {
+    return guard([&]() {
+        need(op, "op"); need(count, "count");
+        const std::vector<size_t> &s = op->op->schedule();
+        *count = s.size();
+        if(out) for(size_t i = 0; i < s.size() && i < cap; i++) out[i] = s[i];
+    });
+}
+
+int lt_btod_contract2_symc_orbit(lt_btod_contract2 *op, size_t aidx, size_t *acidx,
+    size_t *perm, double *coeff, int *allowed) {
+    return guard([&]() {
+        need(op, "op");
+        const symmetry &s = op->op->symc();
+        lt_require(aidx < s.bis.block_index_dims().size(), LT_ERR_PARAM, "block index");
+        orbit o(s, aidx, true);
+        if(acidx) *acidx = o.acidx;
+        fill_transf(o.get_transf(aidx), s.bis.order(), perm, coeff);
+        if(allowed) *allowed = o.allowed ? 1 : 0;
+    });
+}
+
+int lt_btod_contract2_clst(lt_btod_contract2 *op, size_t aidx_c, double *out,
+    size_t cap, size_t *npairs) {
+    return guard([&]() {
+        need(op, "op"); need(npairs, "npairs");
+        const symmetry &s = op->op->symc();
+        index_t ic = get_index(aidx_c, s.bis.block_index_dims());
+        contr_list cl;
+        op->op->clst(ic, cl);
+        *npairs = cl.size();
+        if(!out) return;
+        size_t na = op->a->t->bis().order(), nb = op->b->t->bis().order();
+        size_t w = 6 + na + nb, k = 0;
+        for(contr_list::const_iterator p = cl.begin(); p != cl.end() && k < cap; ++p, ++k) {
+            double *o = out + k * w;
+            size_t j = 0;
+            o[j++] = (double)p->aia;
+            o[j++] = (double)p->acia;
+            for(size_t i = 0; i < na; i++) o[j++] = (double)p->tra.perm[i];
+            o[j++] = p->tra.st.c;
+            o[j++] = (double)p->aib;
+            o[j++] = (double)p->acib;
+            for(size_t i = 0; i < nb; i++) o[j++] = (double)p->trb.perm[i];
+            o[j++] = p->trb.st.c;
+        }
+    });
+}
+
+int lt_btod_contract2_compute_block_host(lt_btod_contract2 *op, size_t aidx_c, double *host) {
+    return guard([&]() {
+        need(op, "op"); need(host, "host");
+        const symmetry &s = op->op->symc();
+        op->op->compute_block_host(get_index(aidx_c, s.bis.block_index_dims()), host);
+    });
+}
+
+int lt_btod_contract2_stats(lt_btod_contract2 *op, lt_stats *out) {
+    return guard([&]() {
+        need(op, "op"); need(out, "out");
+        const exec_stats &s = op->op->stats();
+        out->t_schedule_ms = s.t_schedule_ms;
+        out->t_plan_ms = s.t_plan_ms;
+        out->t_gpu_ms = s.t_gpu_ms;
+        out->t_gemm_ms = s.t_gemm_ms;
+        out->flops = s.flops;
+        out->bytes_perm = s.bytes_perm;
+        out->n_cblocks = s.n_cblocks;
+        out->n_pairs = s.n_pairs;
+        out->n_tiles = s.n_tiles;
+        out->n_unfold = s.n_unfold;
+        out->n_launches = s.n_launches;
+        out->tile_cfg = s.tile_cfg;
+        out->ozaki_slices = s.ozaki_slices;
+        out->ozaki_max_rel_err = s.ozaki_max_rel_err;
+    });
+}
+
+int lt_btod_contract2_set_tile(lt_btod_contract2 *op, int cfg) {
+    return guard([&]() {
+        need(op, "op");
+        lt_require(cfg >= -1 && cfg < dev::TILE_COUNT, LT_ERR_PARAM, "tile configuration");
+        op->op->set_tile_cfg(cfg);
+    });
+}
+
+int lt_btod_contract2_set_ozaki(lt_btod_contract2 *op, int slices) {
+    return guard([&]() {
+        need(op, "op");
+        lt_require(slices >= 0 && slices <= 16, LT_ERR_PARAM, "slices");
+        op->op->set_ozaki(slices, 1);
+    });
+}
+
+} // extern "C"

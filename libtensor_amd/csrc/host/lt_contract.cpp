@@ -1,0 +1,703 @@
+// btod_contract2 on the device.
+//
+// Host side (as libtensor, libtensor/gen_block_tensor/impl/
+// gen_bto_contract2_impl.h:42-308 and gen_bto_contract2_batch_impl.h:177-291):
+// result symmetry and non-zero schedule in the constructor; on perform the
+// contraction is aligned to one matrix product per block pair
+// (contraction2_align), A and B symmetries are permuted, per-C-block
+// contraction lists are built and coalesced in the aligned space.
+//
+// Device side, one batch holding all blocks (HBM-resident tensors):
+//   1. prepare: every aligned A/B block a pair needs (canonical blocks of the
+//      permuted tensors and the non-canonical blocks of unfolded orbits) is
+//      produced from the stored canonical block by ONE permute+scale, or used
+//      in place when that transformation is the identity;
+//   2. contract: grouped FP64 MFMA GEMM, one workgroup per C tile running the
+//      whole contraction list of its C block (or one launch per C block);
+//   3. write back: aligned C blocks that differ from the stored layout are
+//      permuted into the canonical blocks of C.
+#include "lt_runtime.h"
+
+#include <chrono>
+#include <cmath>
+#include <numeric>
+#include <unordered_map>
+
+namespace lta {
+
+namespace {
+
+double now_ms() {
+    return std::chrono::duration<double, std::milli>(
+        std::chrono::steady_clock::now().time_since_epoch()).count();
+}
+
+//! Device permute descriptor for dst = c * P(src) with dims fusing.
+dev::perm_desc make_perm_desc(const double *src, const dims_t &dsrc,
+    const tensor_tr &tr, double *dst, bool accumulate) {
+
+    const size_t n = dsrc.n;
+    dims_t ddst(dsrc);
+    ddst.permute(tr.perm);
+    size_t sstr[kMaxOrder];
+    for(size_t i = 0; i < n; i++) sstr[i] = dsrc.inc(tr.perm[i]);
+    // fuse destination dims that are also adjacent, in order, in the source
+    size_t fd[kMaxOrder], fs[kMaxOrder];
+    size_t nf = 0;
+    for(size_t i = 0; i < n; i++) {
+        if(ddst[i] == 1) continue;
+        if(nf > 0 && fs[nf - 1] == sstr[i] * ddst[i]) {
+            fd[nf - 1] *= ddst[i];
+            fs[nf - 1] = sstr[i];
+        } else {
+            fd[nf] = ddst[i];
+            fs[nf] = sstr[i];
+            nf++;
+        }
+    }
+    if(nf == 0) { fd[0] = 1; fs[0] = 1; nf = 1; }
+    lt_require(nf <= (size_t)dev::kPermMaxOrder, LT_ERR_INTERNAL, "permute order");
+    size_t total = dsrc.size();
+    lt_require(total < (size_t(1) << 31), LT_ERR_PARAM, "block larger than 2^31 elements");
+    dev::perm_desc d;
+    std::memset(&d, 0, sizeof(d));
+    d.src = src;
+    d.dst = dst;
+    d.n = (uint32_t)nf;
+    d.total = (uint32_t)total;
+    for(size_t i = 0; i < (size_t)dev::kPermMaxOrder; i++) {
+        d.div[i] = dev::make_fastdiv(i < nf ? (uint32_t)fd[i] : 1);
+        d.sstride[i] = i < nf ? (uint32_t)fs[i] : 0;
+    }
+    d.alpha = tr.st.c;
+    d.accumulate = accumulate ? 1 : 0;
+    return d;
+}
+
+//! Chunk tables for a list of permute descriptors.
+void make_chunks(const std::vector<dev::perm_desc> &ds, std::vector<uint32_t> &cd,
+    std::vector<uint32_t> &co) {
+    cd.clear();
+    co.clear();
+    for(size_t i = 0; i < ds.size(); i++) {
+        for(uint32_t off = 0; off < ds[i].total; off += dev::kPermChunk) {
+            cd.push_back((uint32_t)i);
+            co.push_back(off);
+        }
+    }
+}
+
+//! Device-side copy of host vectors in one allocation.
+struct dev_lists {
+    context *ctx;
+    void *mem;
+    size_t bytes;
+    dev_lists() : ctx(0), mem(0), bytes(0) { }
+    ~dev_lists() { if(ctx && mem) ctx->free(mem); }
+};
+
+size_t align256(size_t x) { return (x + 255) & ~size_t(255); }
+
+} // unnamed namespace
+
+dev::perm_desc make_perm_desc_public(const double *src, const dims_t &dsrc,
+    const tensor_tr &tr, double *dst, bool accumulate) {
+    return make_perm_desc(src, dsrc, tr, dst, accumulate);
+}
+
+//! Execution plan cached by the operation (valid while the structure of A,
+//! B and C does not change).
+struct contract_plan {
+    uint64_t va, vb;
+    btensor *c;
+    bool add;
+    double d;
+    bool empty;
+
+    // host-side work lists
+    std::vector<dev::perm_desc> prep, wback;
+    std::vector<dev::gemm_cblock> cblocks;
+    std::vector<dev::gemm_pair> pairs;
+    std::vector<dev::gemm_tile> tiles_sorted, tiles_by_block;
+    std::vector<uint32_t> block_tile_begin;
+    std::vector<size_t> c_out;                        // canonical C blocks written
+    std::vector<std::pair<double*, size_t> > c_out_mem; // their storage
+    bool ta, tb;
+    int tile_cfg;
+
+    // device copies
+    dev_lists lists;
+    dev::perm_desc *d_prep, *d_wback;
+    uint32_t *d_prep_cd, *d_prep_co, *d_wb_cd, *d_wb_co;
+    uint32_t n_prep_chunks, n_wb_chunks;
+    dev::gemm_cblock *d_cb;
+    dev::gemm_pair *d_pairs;
+    dev::gemm_tile *d_tiles_sorted, *d_tiles_block;
+    double *scratch;
+    size_t scratch_elems;
+    context *ctx;
+
+    hipEvent_t ev[4];
+    bool events;
+
+    contract_plan() : va(0), vb(0), c(0), add(false), d(1.0), empty(false),
+        ta(false), tb(false), tile_cfg(0), d_prep(0), d_wback(0), d_prep_cd(0),
+        d_prep_co(0), d_wb_cd(0), d_wb_co(0), n_prep_chunks(0), n_wb_chunks(0),
+        d_cb(0), d_pairs(0), d_tiles_sorted(0), d_tiles_block(0), scratch(0),
+        scratch_elems(0), ctx(0), events(false) { }
+    ~contract_plan() {
+        if(ctx && scratch) ctx->free(scratch);
+        if(events) for(int i = 0; i < 4; i++) hipEventDestroy(ev[i]);
+    }
+};
+
+// ---------------------------------------------------------------------------
+
+btod_contract2::btod_contract2(const contraction2 &c, btensor &a, double ka,
+    btensor &b, double kb, double kc) :
+    m_contr(c), m_a(a), m_b(b), m_ka(ka), m_kb(kb), m_kc(kc),
+    m_tile_cfg(-1), m_oz_slices(0), m_oz_int8(1) {
+
+    lt_require(c.is_complete(), LT_ERR_PARAM, "contraction is incomplete");
+    lt_require(a.bis().order() == c.NA() && b.bis().order() == c.NB(),
+        LT_ERR_DIMS, "tensor orders do not match the contraction");
+    std::memset(&m_stats, 0, sizeof(m_stats));
+    double t0 = now_ms();
+    bool self = (&a == &b) && (c.N == c.M);
+    m_symc = contract2_symmetry(c, a.sym(), b.sym(), self);
+    m_sched = contract2_schedule(c, a.sym(), b.sym(), m_symc,
+        a.nonzero_blocks(), b.nonzero_blocks(), a.ctx().nthreads());
+    m_stats.t_schedule_ms = now_ms() - t0;
+}
+
+btod_contract2::~btod_contract2() { }
+
+void btod_contract2::clst(const index_t &ic, contr_list &out) {
+    dims_t bida = m_a.bidims(), bidb = m_b.bidims();
+    block_list bla(bida, m_a.nonzero_blocks()), blb(bidb, m_b.nonzero_blocks());
+    block_list blax(bida), blbx(bidb);
+    unfold_block_list(m_a.sym(), bla, blax);
+    unfold_block_list(m_b.sym(), blb, blbx);
+    contract2_block_list cbl(m_contr, blax, blbx);
+    orbit_cache oca(m_a.sym()), ocb(m_b.sym());
+    build_clst(m_contr, oca, ocb, blax, blbx, cbl, ic, out);
+}
+
+bool btod_contract2::plan_valid(btensor &c, bool add, double d, int) const {
+    if(!m_plan) return false;
+    return m_plan->va == m_a.version() && m_plan->vb == m_b.version() &&
+        m_plan->c == &c && m_plan->add == add && m_plan->d == d;
+}
+
+// ---------------------------------------------------------------------------
+
+void btod_contract2::build_plan(btensor &c, bool add, double dscale) {
+    double t0 = now_ms();
+    context &ctx = m_a.ctx();
+    m_plan.reset(new contract_plan());
+    contract_plan &pl = *m_plan;
+    pl.va = m_a.version();
+    pl.vb = m_b.version();
+    pl.c = &c;
+    pl.add = add;
+    pl.d = dscale;
+    pl.ctx = &ctx;
+
+    std::vector<size_t> nza = m_a.nonzero_blocks(), nzb = m_b.nonzero_blocks();
+    if(nza.empty() || nzb.empty() || m_sched.empty()) {
+        pl.empty = true;
+        return;
+    }
+
+    const contraction2_align al(m_contr);
+    const perm_t &perma = al.perma, &permb = al.permb, &permc = al.permc;
+    perm_t permcinv(permc.inverse());
+    contraction2 contr(m_contr);
+    contr.permute_a(perma);
+    contr.permute_b(permb);
+    contr.permute_c(permc);
+
+    symmetry syma2 = so_permute(m_a.sym(), perma);
+    symmetry symb2 = so_permute(m_b.sym(), permb);
+    symmetry symct = so_permute(m_symc, permc);
+    const bis_t &bisa2 = syma2.bis, &bisb2 = symb2.bis, &bisct = symct.bis;
+    dims_t bida = m_a.bidims(), bidb = m_b.bidims(), bidc = m_symc.bis.block_index_dims();
+    dims_t bida2 = bisa2.block_index_dims(), bidb2 = bisb2.block_index_dims();
+    dims_t bidct = bisct.block_index_dims();
+
+    // canonical non-zero blocks of the permuted arguments (the batches)
+    std::vector<size_t> bla2, blb2, batchc;
+    for(size_t i = 0; i < nza.size(); i++) {
+        if(perma.is_identity()) { bla2.push_back(nza[i]); continue; }
+        index_t ia = get_index(nza[i], bida);
+        perma.apply(ia);
+        bla2.push_back(short_orbit(syma2, abs_index(ia, bida2), true).acidx);
+    }
+    for(size_t i = 0; i < nzb.size(); i++) {
+        if(permb.is_identity()) { blb2.push_back(nzb[i]); continue; }
+        index_t ib = get_index(nzb[i], bidb);
+        permb.apply(ib);
+        blb2.push_back(short_orbit(symb2, abs_index(ib, bidb2), true).acidx);
+    }
+    for(size_t i = 0; i < m_sched.size(); i++) {
+        index_t ic = get_index(m_sched[i], bidc);
+        permc.apply(ic);
+        batchc.push_back(short_orbit(symct, abs_index(ic, bidct), true).acidx);
+    }
+    block_list bla(bida2, bla2), blb(bidb2, blb2);
+    block_list blax(bida2), blbx(bidb2);
+    unfold_block_list(syma2, bla, blax);
+    unfold_block_list(symb2, blb, blbx);
+    contract2_block_list cbl(contr, blax, blbx);
+    orbit_cache oca(syma2), ocb(symb2);
+
+    // contraction lists of all C blocks
+    std::vector<contr_list> clsts(batchc.size());
+    parallel_for(batchc.size(), ctx.nthreads(), [&](size_t t) {
+        build_clst(contr, oca, ocb, blax, blbx, cbl, get_index(batchc[t], bidct), clsts[t]);
+    });
+
+    // aligned operand blocks needed by the pairs
+    std::vector<size_t> usea, useb;
+    for(size_t t = 0; t < clsts.size(); t++) {
+        for(contr_list::const_iterator p = clsts[t].begin(); p != clsts[t].end(); ++p) {
+            usea.push_back(p->aia);
+            useb.push_back(p->aib);
+        }
+    }
+    std::sort(usea.begin(), usea.end());
+    usea.erase(std::unique(usea.begin(), usea.end()), usea.end());
+    std::sort(useb.begin(), useb.end());
+    useb.erase(std::unique(useb.begin(), useb.end()), useb.end());
+
+    // Where each aligned block comes from: stored canonical block Y and the
+    // transformation (A orbit of Y -> idx) then perma then (syma2 orbit).
+    struct src_t { size_t y; tensor_tr tr; };
+    auto source_of = [&](btensor &t, const symmetry &sym2, const perm_t &perm,
+        const dims_t &bid, const dims_t &bid2, size_t z) -> src_t {
+        orbit oz(sym2, z, false);
+        size_t x = oz.acidx;
+        tensor_tr tz = oz.get_transf(z);
+        index_t io = get_index(x, bid2);
+        perm.inverse().apply(io);
+        size_t ao = abs_index(io, bid);
+        orbit oa(t.sym(), ao, false);
+        src_t s;
+        s.y = oa.acidx;
+        s.tr = oa.get_transf(ao);
+        s.tr.transform(tensor_tr(perm, scalar_tr()));
+        s.tr.transform(tz);
+        return s;
+    };
+
+    size_t scratch = 0;
+    std::unordered_map<size_t, size_t> offa, offb;    // aligned block -> scratch offset
+    std::unordered_map<size_t, const double*> ptra, ptrb;
+    std::vector<std::pair<size_t, src_t> > prepa, prepb;
+    for(size_t i = 0; i < usea.size(); i++) {
+        src_t s = source_of(m_a, syma2, perma, bida, bida2, usea[i]);
+        lt_require(m_a.is_nonzero(s.y), LT_ERR_INTERNAL, "pair uses a zero block of A");
+        if(s.tr.is_identity()) {
+            ptra[usea[i]] = m_a.block_ptr(s.y);
+        } else {
+            offa[usea[i]] = scratch;
+            scratch += (bisa2.block_dims(get_index(usea[i], bida2)).size() + 31) & ~size_t(31);
+            prepa.push_back(std::make_pair(usea[i], s));
+        }
+    }
+    for(size_t i = 0; i < useb.size(); i++) {
+        src_t s = source_of(m_b, symb2, permb, bidb, bidb2, useb[i]);
+        lt_require(m_b.is_nonzero(s.y), LT_ERR_INTERNAL, "pair uses a zero block of B");
+        if(s.tr.is_identity()) {
+            ptrb[useb[i]] = m_b.block_ptr(s.y);
+        } else {
+            offb[useb[i]] = scratch;
+            scratch += (bisb2.block_dims(get_index(useb[i], bidb2)).size() + 31) & ~size_t(31);
+            prepb.push_back(std::make_pair(useb[i], s));
+        }
+    }
+
+    // C blocks: aligned canonical block -> stored canonical block of C
+    struct cdst_t { size_t yc; tensor_tr tr; bool direct; size_t off; };
+    std::vector<cdst_t> cdst(batchc.size());
+    for(size_t t = 0; t < batchc.size(); t++) {
+        index_t ib = get_index(batchc[t], bidct);
+        permcinv.apply(ib);
+        size_t aib = abs_index(ib, bidc);
+        orbit ob(m_symc, aib, false);
+        tensor_tr tr(permcinv, scalar_tr());
+        tr.transform(ob.get_transf(aib).inverse());
+        cdst[t].yc = ob.acidx;
+        cdst[t].tr = tr;
+        cdst[t].direct = tr.is_identity();
+        if(!cdst[t].direct) {
+            cdst[t].off = scratch;
+            scratch += (bisct.block_dims(get_index(batchc[t], bidct)).size() + 31) & ~size_t(31);
+        }
+    }
+
+    if(scratch > 0) {
+        pl.scratch = (double*)ctx.alloc(scratch * sizeof(double));
+        pl.scratch_elems = scratch;
+    }
+    for(std::unordered_map<size_t, size_t>::iterator i = offa.begin(); i != offa.end(); ++i)
+        ptra[i->first] = pl.scratch + i->second;
+    for(std::unordered_map<size_t, size_t>::iterator i = offb.begin(); i != offb.end(); ++i)
+        ptrb[i->first] = pl.scratch + i->second;
+
+    // 1. prepare descriptors
+    for(size_t i = 0; i < prepa.size(); i++) {
+        const src_t &s = prepa[i].second;
+        dims_t dy = m_a.bis().block_dims(get_index(s.y, bida));
+        pl.prep.push_back(make_perm_desc(m_a.block_ptr(s.y), dy, s.tr,
+            pl.scratch + offa[prepa[i].first], false));
+    }
+    for(size_t i = 0; i < prepb.size(); i++) {
+        const src_t &s = prepb[i].second;
+        dims_t dy = m_b.bis().block_dims(get_index(s.y, bidb));
+        pl.prep.push_back(make_perm_desc(m_b.block_ptr(s.y), dy, s.tr,
+            pl.scratch + offb[prepb[i].first], false));
+    }
+
+    // 2. GEMM work lists
+    gemm_form gf0 = aligned_gemm_form(contr,
+        bisa2.block_dims(get_index(usea[0], bida2)),
+        bisb2.block_dims(get_index(useb[0], bidb2)));
+    const bool tc = gf0.tc;
+    // with C stored as (j, i) the product is computed as C^T = B^T A^T
+    pl.ta = tc ? !gf0.tb : gf0.ta;
+    pl.tb = tc ? !gf0.ta : gf0.tb;
+
+    double flops = 0.0;
+    uint32_t bm = 64, bn = 64;
+    // tile shape: explicit choice, or the one that wastes least padding
+    int cfg = m_tile_cfg;
+    std::vector<std::pair<size_t, size_t> > mn(batchc.size());
+    for(size_t t = 0; t < batchc.size(); t++) {
+        index_t ic = get_index(batchc[t], bidct);
+        dims_t dc = bisct.block_dims(ic);
+        size_t m = 1, n = 1;
+        // C aligned: (A outer..., B outer...) or swapped when tc
+        size_t na = contr.N;
+        for(size_t i = 0; i < dc.n; i++) {
+            bool froma = (contr.conn[i] - contr.NC()) < contr.NA();
+            (void)na;
+            if(froma) m *= dc[i]; else n *= dc[i];
+        }
+        mn[t] = std::make_pair(m, n);
+    }
+    if(cfg < 0) {
+        double best = 1e300;
+        for(int k = 0; k < dev::TILE_COUNT; k++) {
+            uint32_t tbm, tbn;
+            dev::tile_shape(k, tbm, tbn);
+            double waste = 0.0, ntile = 0.0;
+            for(size_t t = 0; t < mn.size(); t++) {
+                size_t rows = tc ? mn[t].second : mn[t].first;
+                size_t cols = tc ? mn[t].first : mn[t].second;
+                size_t tm = (rows + tbm - 1) / tbm, tn = (cols + tbn - 1) / tbn;
+                waste += double(tm * tbm * tn * tbn);
+                ntile += double(tm * tn);
+            }
+            // padded work plus a per-tile overhead equivalent to 32x32 MACs
+            double cost = waste + ntile * 1024.0;
+            if(cost < best) { best = cost; cfg = k; }
+        }
+    }
+    dev::tile_shape(cfg, bm, bn);
+    pl.tile_cfg = cfg;
+
+    std::vector<double> tile_cost;
+    for(size_t t = 0; t < batchc.size(); t++) {
+        const contr_list &cl = clsts[t];
+        dev::gemm_cblock cb;
+        std::memset(&cb, 0, sizeof(cb));
+        size_t m = mn[t].first, n = mn[t].second;
+        size_t rows = tc ? n : m, cols = tc ? m : n;
+        cb.m = (uint32_t)rows;
+        cb.n = (uint32_t)cols;
+        cb.ldc = (uint32_t)cols;
+        double *cptr;
+        if(cdst[t].direct) {
+            cptr = c.block_ptr(cdst[t].yc);
+        } else {
+            cptr = pl.scratch + cdst[t].off;
+        }
+        cb.c = cptr;
+        cb.zero = (cdst[t].direct && add) ? 0 : 1;
+        cb.pair_begin = (uint32_t)pl.pairs.size();
+        double ksum = 0.0;
+        for(contr_list::const_iterator p = cl.begin(); p != cl.end(); ++p) {
+            // scalars as gen_bto_contract2_block::compute_block:
+            // inverse orbit transformation of the unfolded block, then the
+            // pair's transformation, then ka/kb/kc
+            orbit_info oa = oca.get(p->aia), ob = ocb.get(p->aib);
+            scalar_tr sa(oa.tr.st); sa.invert(); sa.transform(p->tra.st);
+            scalar_tr sb(ob.tr.st); sb.invert(); sb.transform(p->trb.st);
+            scalar_tr ka(sa), kb(sb), kc(m_kc);
+            ka.transform(scalar_tr(m_ka));
+            kb.transform(scalar_tr(m_kb));
+            double d = ka.c * kb.c * kc.c;
+            if(add) d *= dscale;
+            if(d == 0.0) continue;
+            dims_t da = bisa2.block_dims(get_index(p->aia, bida2));
+            dims_t db = bisb2.block_dims(get_index(p->aib, bidb2));
+            gemm_form g = aligned_gemm_form(contr, da, db);
+            dev::gemm_pair gp;
+            std::memset(&gp, 0, sizeof(gp));
+            const double *pa = ptra[p->aia], *pb = ptrb[p->aib];
+            gp.k = (uint32_t)g.k;
+            size_t lda = g.ta ? g.m : g.k;     // row stride of A as stored
+            size_t ldb = g.tb ? g.k : g.n;     // row stride of B as stored
+            if(!tc) {
+                gp.a = pa; gp.b = pb; gp.lda = (uint32_t)lda; gp.ldb = (uint32_t)ldb;
+            } else {
+                gp.a = pb; gp.b = pa; gp.lda = (uint32_t)ldb; gp.ldb = (uint32_t)lda;
+            }
+            gp.d = d;
+            pl.pairs.push_back(gp);
+            flops += 2.0 * double(g.m) * double(g.n) * double(g.k);
+            ksum += double(g.k);
+        }
+        cb.pair_end = (uint32_t)pl.pairs.size();
+        uint32_t cbi = (uint32_t)pl.cblocks.size();
+        pl.cblocks.push_back(cb);
+        pl.block_tile_begin.push_back((uint32_t)pl.tiles_by_block.size());
+        uint32_t tm = (uint32_t)((rows + bm - 1) / bm), tn = (uint32_t)((cols + bn - 1) / bn);
+        for(uint32_t i = 0; i < tm; i++) {
+            for(uint32_t j = 0; j < tn; j++) {
+                dev::gemm_tile tl;
+                tl.cblock = cbi;
+                tl.tm = (uint16_t)i;
+                tl.tn = (uint16_t)j;
+                pl.tiles_by_block.push_back(tl);
+                tile_cost.push_back(ksum);
+            }
+        }
+        pl.c_out.push_back(cdst[t].yc);
+        pl.c_out_mem.push_back(std::make_pair(c.block_ptr(cdst[t].yc), c.block_size(cdst[t].yc)));
+    }
+    pl.block_tile_begin.push_back((uint32_t)pl.tiles_by_block.size());
+    lt_require(pl.tiles_by_block.size() < (size_t(1) << 31), LT_ERR_PARAM, "too many tiles");
+
+    // cost-sorted tile order for the grouped launch (longest first)
+    std::vector<size_t> order(pl.tiles_by_block.size());
+    std::iota(order.begin(), order.end(), size_t(0));
+    std::stable_sort(order.begin(), order.end(), [&](size_t x, size_t y) {
+        return tile_cost[x] > tile_cost[y];
+    });
+    pl.tiles_sorted.resize(order.size());
+    for(size_t i = 0; i < order.size(); i++) pl.tiles_sorted[i] = pl.tiles_by_block[order[i]];
+
+    // 3. write-back descriptors
+    for(size_t t = 0; t < batchc.size(); t++) {
+        if(cdst[t].direct) continue;
+        dims_t dc = bisct.block_dims(get_index(batchc[t], bidct));
+        double *dst = c.block_ptr(cdst[t].yc);
+        bool acc = add;
+        pl.wback.push_back(make_perm_desc(pl.scratch + cdst[t].off, dc, cdst[t].tr, dst, acc));
+    }
+
+    // upload all lists in one allocation
+    std::vector<uint32_t> pcd, pco, wcd, wco;
+    make_chunks(pl.prep, pcd, pco);
+    make_chunks(pl.wback, wcd, wco);
+    pl.n_prep_chunks = (uint32_t)pcd.size();
+    pl.n_wb_chunks = (uint32_t)wcd.size();
+    size_t sz = 0;
+    size_t o_prep = sz; sz = align256(sz + pl.prep.size() * sizeof(dev::perm_desc));
+    size_t o_wb = sz; sz = align256(sz + pl.wback.size() * sizeof(dev::perm_desc));
+    size_t o_pcd = sz; sz = align256(sz + pcd.size() * 4);
+    size_t o_pco = sz; sz = align256(sz + pco.size() * 4);
+    size_t o_wcd = sz; sz = align256(sz + wcd.size() * 4);
+    size_t o_wco = sz; sz = align256(sz + wco.size() * 4);
+    size_t o_cb = sz; sz = align256(sz + pl.cblocks.size() * sizeof(dev::gemm_cblock));
+    size_t o_pr = sz; sz = align256(sz + pl.pairs.size() * sizeof(dev::gemm_pair));
+    size_t o_ts = sz; sz = align256(sz + pl.tiles_sorted.size() * sizeof(dev::gemm_tile));
+    size_t o_tb = sz; sz = align256(sz + pl.tiles_by_block.size() * sizeof(dev::gemm_tile));
+    std::vector<char> host(sz, 0);
+    auto put = [&](size_t off, const void *src, size_t n) { if(n) std::memcpy(&host[off], src, n); };
+    put(o_prep, pl.prep.data(), pl.prep.size() * sizeof(dev::perm_desc));
+    put(o_wb, pl.wback.data(), pl.wback.size() * sizeof(dev::perm_desc));
+    put(o_pcd, pcd.data(), pcd.size() * 4);
+    put(o_pco, pco.data(), pco.size() * 4);
+    put(o_wcd, wcd.data(), wcd.size() * 4);
+    put(o_wco, wco.data(), wco.size() * 4);
+    put(o_cb, pl.cblocks.data(), pl.cblocks.size() * sizeof(dev::gemm_cblock));
+    put(o_pr, pl.pairs.data(), pl.pairs.size() * sizeof(dev::gemm_pair));
+    put(o_ts, pl.tiles_sorted.data(), pl.tiles_sorted.size() * sizeof(dev::gemm_tile));
+    put(o_tb, pl.tiles_by_block.data(), pl.tiles_by_block.size() * sizeof(dev::gemm_tile));
+    pl.lists.ctx = &ctx;
+    pl.lists.mem = ctx.alloc(sz);
+    pl.lists.bytes = sz;
+    ctx.upload(pl.lists.mem, host.data(), sz);
+    char *base = (char*)pl.lists.mem;
+    pl.d_prep = (dev::perm_desc*)(base + o_prep);
+    pl.d_wback = (dev::perm_desc*)(base + o_wb);
+    pl.d_prep_cd = (uint32_t*)(base + o_pcd);
+    pl.d_prep_co = (uint32_t*)(base + o_pco);
+    pl.d_wb_cd = (uint32_t*)(base + o_wcd);
+    pl.d_wb_co = (uint32_t*)(base + o_wco);
+    pl.d_cb = (dev::gemm_cblock*)(base + o_cb);
+    pl.d_pairs = (dev::gemm_pair*)(base + o_pr);
+    pl.d_tiles_sorted = (dev::gemm_tile*)(base + o_ts);
+    pl.d_tiles_block = (dev::gemm_tile*)(base + o_tb);
+
+    ctx.activate();
+    for(int i = 0; i < 4; i++) hip_check(hipEventCreate(&pl.ev[i]), "hipEventCreate");
+    pl.events = true;
+
+    m_stats.n_cblocks = pl.cblocks.size();
+    m_stats.n_pairs = pl.pairs.size();
+    m_stats.n_tiles = pl.tiles_by_block.size();
+    m_stats.n_unfold = pl.prep.size();
+    m_stats.flops = flops;
+    m_stats.tile_cfg = cfg;
+    double bytes = 0.0;
+    for(size_t i = 0; i < pl.prep.size(); i++) bytes += 16.0 * pl.prep[i].total;
+    for(size_t i = 0; i < pl.wback.size(); i++)
+        bytes += (pl.wback[i].accumulate ? 24.0 : 16.0) * pl.wback[i].total;
+    m_stats.bytes_perm = bytes;
+    m_stats.t_plan_ms = now_ms() - t0;
+}
+
+void btod_contract2::run(btensor &c, int mode) {
+    contract_plan &pl = *m_plan;
+    context &ctx = m_a.ctx();
+    ctx.activate();
+    hipStream_t s = ctx.stream();
+    if(!pl.add) {
+        c.zero_all();
+        c.set_symmetry(m_symc);
+    }
+    if(pl.empty) {
+        m_stats.t_gpu_ms = m_stats.t_gemm_ms = 0.0;
+        return;
+    }
+    size_t launches = 0;
+    hip_check(hipEventRecord(pl.ev[0], s), "hipEventRecord");
+    if(pl.add) {
+        // blocks that become non-zero start from zero before accumulation
+        for(size_t i = 0; i < pl.c_out.size(); i++) {
+            if(c.is_nonzero(pl.c_out[i])) continue;
+            hip_check(dev::launch_fill(pl.c_out_mem[i].first, pl.c_out_mem[i].second,
+                0.0, s), "fill");
+            launches++;
+        }
+    }
+    if(pl.n_prep_chunks) {
+        hip_check(dev::launch_permute(pl.d_prep, pl.d_prep_cd, pl.d_prep_co,
+            pl.n_prep_chunks, s), "permute (prepare)");
+        launches++;
+    }
+    hip_check(hipEventRecord(pl.ev[1], s), "hipEventRecord");
+    if(mode == MODE_PER_BLOCK) {
+        for(size_t b = 0; b + 1 < pl.block_tile_begin.size(); b++) {
+            uint32_t t0 = pl.block_tile_begin[b], t1 = pl.block_tile_begin[b + 1];
+            if(t1 == t0) continue;
+            hip_check(dev::launch_gemm(pl.tile_cfg, pl.ta, pl.tb, pl.d_cb, pl.d_pairs,
+                pl.d_tiles_block + t0, t1 - t0, t1 - t0, s), "gemm (per block)");
+            launches++;
+        }
+    } else {
+        uint32_t nt = (uint32_t)pl.tiles_sorted.size();
+        hip_check(dev::launch_gemm(pl.tile_cfg, pl.ta, pl.tb, pl.d_cb, pl.d_pairs,
+            pl.d_tiles_sorted, nt, nt, s), "gemm (grouped)");
+        launches++;
+    }
+    hip_check(hipEventRecord(pl.ev[2], s), "hipEventRecord");
+    if(pl.n_wb_chunks) {
+        hip_check(dev::launch_permute(pl.d_wback, pl.d_wb_cd, pl.d_wb_co,
+            pl.n_wb_chunks, s), "permute (write back)");
+        launches++;
+    }
+    hip_check(hipEventRecord(pl.ev[3], s), "hipEventRecord");
+    hip_check(hipStreamSynchronize(s), "hipStreamSynchronize");
+    float ms = 0.f, msg = 0.f;
+    hip_check(hipEventElapsedTime(&ms, pl.ev[0], pl.ev[3]), "hipEventElapsedTime");
+    hip_check(hipEventElapsedTime(&msg, pl.ev[1], pl.ev[2]), "hipEventElapsedTime");
+    m_stats.t_gpu_ms = ms;
+    m_stats.t_gemm_ms = msg;
+    m_stats.n_launches = launches;
+    for(size_t i = 0; i < pl.c_out.size(); i++) c.mark_nonzero(pl.c_out[i]);
+}
+
+void btod_contract2::perform(btensor &c, int mode) {
+    m_a.ctx().require_device("btod_contract2::perform");
+    lt_require(c.bis() == m_symc.bis, LT_ERR_DIMS, "result tensor has a different block index space");
+    if(!plan_valid(c, false, 1.0, mode)) build_plan(c, false, 1.0);
+    run(c, mode);
+}
+
+void btod_contract2::perform_add(btensor &c, double d, int mode) {
+    m_a.ctx().require_device("btod_contract2::perform");
+    lt_require(c.bis() == m_symc.bis, LT_ERR_DIMS, "result tensor has a different block index space");
+    // the += path supports a result tensor with the same symmetry orbits
+    for(size_t i = 0; i < m_sched.size(); i++) {
+        short_orbit so(c.sym(), m_sched[i], false);
+        lt_require(so.acidx == m_sched[i], LT_ERR_SYMMETRY,
+            "perform_add: result symmetry differs from the contraction's");
+    }
+    if(!plan_valid(c, true, d, mode)) build_plan(c, true, d);
+    run(c, mode);
+}
+
+// ---------------------------------------------------------------------------
+
+void btod_contract2::compute_block_host(const index_t &ic, double *out) {
+    // Direct path of libtensor (gen_bto_contract2::compute_block): full
+    // contraction list in the original space, canonical blocks with their
+    // transformations, each pair contracted on the host in FP64.
+    dims_t bida = m_a.bidims(), bidb = m_b.bidims();
+    block_list bla(bida, m_a.nonzero_blocks()), blb(bidb, m_b.nonzero_blocks());
+    block_list blax(bida), blbx(bidb);
+    unfold_block_list(m_a.sym(), bla, blax);
+    unfold_block_list(m_b.sym(), blb, blbx);
+    contr_list cl;
+    build_clst_full(m_contr, m_a.sym(), m_b.sym(), blax, blbx,
+        m_symc.bis.block_index_dims(), ic, false, cl);
+    dims_t dc = m_symc.bis.block_dims(ic);
+    for(size_t i = 0; i < dc.size(); i++) out[i] = 0.0;
+    for(contr_list::const_iterator p = cl.begin(); p != cl.end(); ++p) {
+        dims_t da = m_a.bis().block_dims(get_index(p->acia, bida));
+        dims_t db = m_b.bis().block_dims(get_index(p->acib, bidb));
+        std::vector<double> a(da.size()), b(db.size());
+        m_a.get_block(p->acia, a.data());
+        m_b.get_block(p->acib, b.data());
+        contraction2 cc(m_contr);
+        cc.permute_a(p->tra.perm.inverse());
+        cc.permute_b(p->trb.perm.inverse());
+        double d = p->tra.st.c * m_ka * p->trb.st.c * m_kb * m_kc;
+        // brute-force contraction over the connection sequence
+        const size_t NC = cc.NC(), NA = cc.NA(), NB = cc.NB();
+        dims_t dk(cc.K);
+        for(size_t i = 0, k = 0; i < NA; i++)
+            if(cc.conn[NC + i] >= NC + NA) dk[k++] = da[i];
+        size_t nk = dk.size();
+        index_t ie(NC);
+        for(size_t e = 0; e < dc.size(); e++) {
+            double sum = 0.0;
+            index_t ik(cc.K);
+            for(size_t kk = 0; kk < nk; kk++) {
+                index_t xa(NA), xb(NB);
+                for(size_t i = 0; i < NC; i++) {
+                    size_t j = cc.conn[i] - NC;
+                    if(j < NA) xa[j] = ie[i]; else xb[j - NA] = ie[i];
+                }
+                for(size_t i = 0, k = 0; i < NA; i++) {
+                    if(cc.conn[NC + i] >= NC + NA) {
+                        xa[i] = ik[k];
+                        xb[cc.conn[NC + i] - NC - NA] = ik[k];
+                        k++;
+                    }
+                }
+                sum += a[abs_index(xa, da)] * b[abs_index(xb, db)];
+                inc_index(ik, dk);
+            }
+            out[e] += d * sum;
+            inc_index(ie, dc);
+        }
+    }
+}
+
+} // namespace lta

@@ -1,0 +1,198 @@
+// Device runtime of the block-contraction path: one context per GPU (stream,
+// HBM accounting with a hard cap used for admission), HBM-resident block
+// tensors, and the btod_contract2 operation with its cached execution plan.
+#pragma once
+
+#include <atomic>
+#include <map>
+#include <memory>
+#include <mutex>
+#include <string>
+#include <vector>
+#include <hip/hip_runtime_api.h>
+#include "lt_core.h"
+#include "lt_sym.h"
+#include "lt_sched.h"
+#include "../device/lt_device.h"
+
+namespace lta {
+
+void hip_check(hipError_t e, const char *what);
+
+//! Device context: one HIP device, one stream, an HBM budget.
+class context {
+public:
+    //! device < 0 creates a host-only context: blocks live in host memory,
+    //! schedules and symmetry can be computed, device operations refuse.
+    context(int device, size_t hbm_cap);
+    ~context();
+
+    int device() const { return m_device; }
+    bool is_device() const { return m_device >= 0; }
+    void require_device(const char *what) const;
+    hipStream_t stream() const { return m_stream; }
+
+    //! Allocates device memory within the budget; throws LT_ERR_NOMEM when
+    //! the request does not fit (callers use this for admission).
+    void *alloc(size_t bytes);
+    void free(void *p);
+    bool fits(size_t bytes) const { return m_used + bytes <= m_cap; }
+    size_t used() const { return m_used; }
+    size_t cap() const { return m_cap; }
+
+    void upload(void *dst, const void *src, size_t bytes);
+    void download(void *dst, const void *src, size_t bytes);
+    void sync();
+    //! Makes this context's device current on the calling thread.
+    void activate() const;
+
+    size_t nthreads() const { return m_nthreads; }
+    void set_nthreads(size_t n) { m_nthreads = n ? n : 1; }
+
+private:
+    int m_device;
+    hipStream_t m_stream;
+    size_t m_cap;
+    std::atomic<size_t> m_used;
+    std::mutex m_mtx;
+    std::map<void*, size_t> m_sizes;
+    size_t m_nthreads;
+};
+
+//! Slab allocator for the blocks of one tensor (few hipMallocs, stable
+//! block addresses).
+class slab {
+public:
+    explicit slab(context &ctx) : m_ctx(ctx), m_off(0), m_cap(0) { }
+    ~slab();
+    double *take(size_t n);
+    size_t bytes() const;
+    void release_all();
+private:
+    context &m_ctx;
+    std::vector<std::pair<double*, size_t> > m_chunks;
+    size_t m_off, m_cap;
+};
+
+//! Block tensor with HBM-resident canonical blocks
+//! (libtensor/block_tensor/block_tensor.h semantics: only canonical,
+//! allowed, non-zero blocks are stored).
+class btensor {
+public:
+    btensor(context &ctx, const bis_t &bis);
+
+    context &ctx() { return m_ctx; }
+    const bis_t &bis() const { return m_sym.bis; }
+    const symmetry &sym() const { return m_sym; }
+    symmetry &sym_rw();
+    dims_t bidims() const { return m_bidims; }
+
+    //! Device storage of a block (allocated on first use; stays at the same
+    //! address for the lifetime of the tensor).
+    double *block_ptr(size_t aidx);
+    bool is_nonzero(size_t aidx) const;
+    void mark_nonzero(size_t aidx);
+    void zero_block(size_t aidx);
+    void zero_all();
+    std::vector<size_t> nonzero_blocks() const;
+    size_t block_size(size_t aidx) const;
+
+    void set_block(size_t aidx, const double *host);
+    void get_block(size_t aidx, double *host);
+
+    //! Structure version: changes when the set of non-zero blocks, the
+    //! symmetry or the split points change.
+    uint64_t version() const { return m_version; }
+    void touch() { m_version++; }
+
+    void split(const bool *mask, size_t pos);
+    void set_symmetry(const symmetry &s);
+
+    //! Fills all canonical allowed blocks with random data symmetrized over
+    //! each block's stabilizer (btod_random semantics).
+    void fill_random(uint64_t seed, int dist, double scale, int sk_d0, int sk_d1);
+
+private:
+    context &m_ctx;
+    symmetry m_sym;
+    dims_t m_bidims;
+    slab m_slab;
+    std::map<size_t, double*> m_store;
+    std::vector<size_t> m_nonzero;   // sorted
+    uint64_t m_version;
+    static std::atomic<uint64_t> s_version;
+};
+
+//! Stabilizer of a block: transformations g (perm, scalar) with g(X) = X.
+std::vector<tensor_tr> block_stabilizer(const symmetry &sym, size_t aidx);
+
+//! Applies a tensor transformation to a dense block on the host
+//! (dst = c * P(src), dims of dst = P(dims of src)).
+void host_transform(const double *src, const dims_t &dsrc, const tensor_tr &tr,
+    double *dst, bool accumulate);
+
+//! Device permute descriptor for dst = c * P(src) (dimensions fused).
+dev::perm_desc make_perm_desc_public(const double *src, const dims_t &dsrc,
+    const tensor_tr &tr, double *dst, bool accumulate);
+
+enum exec_mode { MODE_PER_BLOCK = 0, MODE_GROUPED = 1, MODE_OZAKI = 2 };
+
+struct exec_stats {
+    double t_schedule_ms;   //!< make_schedule (constructor)
+    double t_plan_ms;       //!< work-list build on the first perform
+    double t_gpu_ms;        //!< device time of the last perform (events)
+    double t_gemm_ms;       //!< device time of the GEMM phase of the last perform
+    double flops;           //!< 2 |C_blk| K_blk summed over all pairs
+    double bytes_perm;      //!< permute traffic of the last perform (bytes)
+    size_t n_cblocks, n_pairs, n_tiles, n_unfold, n_launches;
+    int tile_cfg;
+    double ozaki_max_rel_err; //!< set when mode == MODE_OZAKI and checked
+    int ozaki_slices;
+};
+
+struct contract_plan;
+
+//! btod_contract2<N, M, K> (libtensor/block_tensor/btod_contract2.h:29):
+//! C = kc * contract(ka * A, kb * B).
+class btod_contract2 {
+public:
+    btod_contract2(const contraction2 &c, btensor &a, double ka, btensor &b,
+        double kb, double kc);
+    ~btod_contract2();
+
+    const symmetry &symc() const { return m_symc; }
+    const std::vector<size_t> &schedule() const { return m_sched; }
+
+    //! C := result (gen_bto_aux_copy): zero C, take the result symmetry,
+    //! write every scheduled block.
+    void perform(btensor &c, int mode);
+    //! C += d * result (gen_bto_aux_add) for C with the result's symmetry.
+    void perform_add(btensor &c, double d, int mode);
+    //! One block computed directly on the host from canonical blocks
+    //! (gen_bto_contract2::compute_block, reference-order numerics on CPU).
+    void compute_block_host(const index_t &ic, double *out);
+    //! Contraction list of a C block in the original index space.
+    void clst(const index_t &ic, contr_list &out);
+
+    const exec_stats &stats() const { return m_stats; }
+    void set_tile_cfg(int cfg) { m_tile_cfg = cfg; }
+    void set_ozaki(int slices, int use_int8) { m_oz_slices = slices; m_oz_int8 = use_int8; }
+    double flops() const { return m_stats.flops; }
+
+private:
+    contraction2 m_contr;
+    btensor &m_a, &m_b;
+    double m_ka, m_kb, m_kc;
+    symmetry m_symc;
+    std::vector<size_t> m_sched;
+    exec_stats m_stats;
+    int m_tile_cfg;
+    int m_oz_slices, m_oz_int8;
+    std::unique_ptr<contract_plan> m_plan;
+
+    void build_plan(btensor &c, bool add, double d);
+    bool plan_valid(btensor &c, bool add, double d, int mode) const;
+    void run(btensor &c, int mode);
+};
+
+} // namespace lta

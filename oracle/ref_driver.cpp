@@ -1,0 +1,372 @@
+// Test infrastructure (checker only, never the product).
+//
+// Driver of the REAL libtensor block-contraction path, built from the
+// reference sources by oracle/Makefile into oracle/_ref/ref_driver.  It reads
+// a problem description (spec_format.h), builds the block tensors with
+// libtensor's own classes, runs btod_contract2 and writes:
+//   <out>.txt         result symmetry (canonical index and allowedness of every
+//                     C block), the non-zero schedule, the contraction list of
+//                     every scheduled block (gen_bto_contract2_clst_builder,
+//                     same inputs as gen_bto_contract2_nzorb), timings
+//   <out>.result.bin  the result unfolded to a dense row-major array
+//                     (tod_btconv)
+// Options: --threads T (libutil::thread_pool of T workers), --repeat R
+// (timed performs after one warm-up), --no-symc, --no-clst, --no-result.
+#include <libtensor/libtensor.h>
+#include <libtensor/block_tensor/btod_contract2.h>
+#include <libtensor/block_tensor/btod_traits.h>
+#include <libtensor/core/orbit.h>
+#include <libtensor/core/orbit_list.h>
+#include <libtensor/core/batching_policy_base.h>
+#include <libtensor/dense_tensor/tod_btconv.h>
+#include <libtensor/gen_block_tensor/impl/gen_bto_contract2_block_list.h>
+#include <libtensor/gen_block_tensor/impl/gen_bto_contract2_clst_builder.h>
+#include <libtensor/gen_block_tensor/impl/gen_bto_unfold_block_list.h>
+#include <libtensor/symmetry/point_group_table.h>
+#include <libtensor/symmetry/product_table_container.h>
+#include <libtensor/symmetry/se_label.h>
+#include <libtensor/symmetry/se_part.h>
+#include <libtensor/symmetry/se_perm.h>
+#include <libutil/thread_pool/thread_pool.h>
+
+#include <algorithm>
+#include <cstdio>
+#include <cstring>
+#include <sys/time.h>
+#include <vector>
+
+#include "spec_format.h"
+
+using namespace libtensor;
+
+namespace {
+
+struct options {
+    size_t threads, repeat;
+    bool symc, clst, result;
+    options() : threads(1), repeat(0), symc(true), clst(true), result(true) { }
+};
+
+double wall_ms() {
+    struct timeval tv;
+    gettimeofday(&tv, 0);
+    return tv.tv_sec * 1e3 + tv.tv_usec * 1e-3;
+}
+
+template<size_t N>
+permutation<N> make_perm(const std::vector<size_t> &p) {
+    sequence<N, size_t> s1(0), s2(0);
+    for(size_t i = 0; i < N; i++) { s1[i] = p[i]; s2[i] = i; }
+    return permutation_builder<N>(s1, s2).get_perm();
+}
+
+template<size_t N>
+libtensor::index<N> make_index(const std::vector<size_t> &v) {
+    libtensor::index<N> i;
+    for(size_t k = 0; k < N; k++) i[k] = v[k];
+    return i;
+}
+
+template<size_t N>
+block_index_space<N> make_bis(const spec::tensor &t) {
+    libtensor::index<N> i1, i2;
+    for(size_t i = 0; i < N; i++) i2[i] = t.dims[i] - 1;
+    block_index_space<N> bis(dimensions<N>(index_range<N>(i1, i2)));
+    for(size_t s = 0; s < t.splits.size(); s++) {
+        mask<N> m;
+        for(size_t i = 0; i < N; i++) m[i] = t.splits[s].mask[i] != 0;
+        bis.split(m, t.splits[s].pos);
+    }
+    return bis;
+}
+
+template<size_t N>
+void fill_tensor(const spec::tensor &t, block_tensor<N, double, allocator<double> > &bt) {
+    block_tensor_ctrl<N, double> ctrl(bt);
+    const block_index_space<N> &bis = bt.get_bis();
+    symmetry<N, double> &sym = ctrl.req_symmetry();
+    for(size_t e = 0; e < t.elems.size(); e++) {
+        const spec::element &el = t.elems[e];
+        if(el.kind == 0) {
+            sym.insert(se_perm<N, double>(make_perm<N>(el.perm),
+                scalar_transf<double>(el.coeff)));
+        } else if(el.kind == 1) {
+            libtensor::index<N> i1, i2;
+            for(size_t i = 0; i < N; i++) i2[i] = el.part.pdims[i] - 1;
+            se_part<N, double> sp(bis, dimensions<N>(index_range<N>(i1, i2)));
+            size_t im = 0, ifb = 0;
+            for(size_t k = 0; k < el.part.order.size(); k++) {
+                if(el.part.order[k] == 0) {
+                    const spec::part_el::mp &m = el.part.maps[im++];
+                    sp.add_map(make_index<N>(m.i1), make_index<N>(m.i2),
+                        scalar_transf<double>(m.c));
+                } else {
+                    sp.mark_forbidden(make_index<N>(el.part.forbid[ifb++]));
+                }
+            }
+            sym.insert(sp);
+        } else {
+            se_label<N, double> sl(bis.get_block_index_dims(), el.label.table);
+            for(size_t a = 0; a < el.label.assigns.size(); a++) {
+                mask<N> m;
+                for(size_t i = 0; i < N; i++) m[i] = el.label.assigns[a].mask[i] != 0;
+                sl.get_labeling().assign(m, el.label.assigns[a].blk,
+                    el.label.assigns[a].label);
+            }
+            if(el.label.has_rule) {
+                product_table_i::label_set_t ls;
+                for(size_t r = 0; r < el.label.rule.size(); r++) ls.insert(el.label.rule[r]);
+                sl.set_rule(ls);
+            }
+            sym.insert(sl);
+        }
+    }
+
+    dimensions<N> bidims = bis.get_block_index_dims();
+    const dimensions<N> &dims = bis.get_dims();
+    if(!t.dense_file.empty()) {
+        std::vector<double> dense(dims.get_size());
+        FILE *f = fopen(t.dense_file.c_str(), "rb");
+        if(!f || fread(&dense[0], 8, dense.size(), f) != dense.size())
+            throw std::runtime_error("cannot read " + t.dense_file);
+        fclose(f);
+        orbit_list<N, double> ol(sym);
+        for(typename orbit_list<N, double>::iterator i = ol.begin(); i != ol.end(); ++i) {
+            libtensor::index<N> bi;
+            ol.get_index(i, bi);
+            libtensor::index<N> st = bis.get_block_start(bi);
+            dimensions<N> bd = bis.get_block_dims(bi);
+            std::vector<double> blk(bd.get_size());
+            bool nz = false;
+            abs_index<N> ai(bd);
+            do {
+                const libtensor::index<N> &e = ai.get_index();
+                size_t off = 0;
+                for(size_t k = 0; k < N; k++) off = off * dims[k] + st[k] + e[k];
+                blk[ai.get_abs_index()] = dense[off];
+                if(dense[off] != 0.0) nz = true;
+            } while(ai.inc());
+            if(!nz) continue;
+            dense_tensor_wr_i<N, double> &b = ctrl.req_block(bi);
+            {
+                dense_tensor_wr_ctrl<N, double> dc(b);
+                double *p = dc.req_dataptr();
+                std::memcpy(p, &blk[0], 8 * blk.size());
+                dc.ret_dataptr(p);
+            }
+            ctrl.ret_block(bi);
+        }
+    }
+    if(!t.blocks_file.empty()) {
+        FILE *f = fopen(t.blocks_file.c_str(), "rb");
+        if(!f) throw std::runtime_error("cannot read " + t.blocks_file);
+        unsigned long long cnt = 0;
+        if(fread(&cnt, 8, 1, f) != 1) throw std::runtime_error("blocks: header");
+        for(unsigned long long k = 0; k < cnt; k++) {
+            unsigned long long aidx = 0, ne = 0;
+            if(fread(&aidx, 8, 1, f) != 1 || fread(&ne, 8, 1, f) != 1)
+                throw std::runtime_error("blocks: record");
+            std::vector<double> blk(ne);
+            if(ne && fread(&blk[0], 8, ne, f) != ne) throw std::runtime_error("blocks: data");
+            libtensor::index<N> bi;
+            abs_index<N>::get_index(aidx, bidims, bi);
+            dense_tensor_wr_i<N, double> &b = ctrl.req_block(bi);
+            {
+                dense_tensor_wr_ctrl<N, double> dc(b);
+                double *p = dc.req_dataptr();
+                if(ne != b.get_dims().get_size() && ne) throw std::runtime_error("blocks: size");
+                std::memcpy(p, &blk[0], 8 * ne);
+                dc.ret_dataptr(p);
+            }
+            ctrl.ret_block(bi);
+        }
+        fclose(f);
+    }
+}
+
+template<size_t N>
+void print_tr(FILE *f, const tensor_transf<N, double> &tr) {
+    for(size_t i = 0; i < N; i++) fprintf(f, " %zu", tr.get_perm()[i]);
+    fprintf(f, " %.17g", tr.get_scalar_tr().get_coeff());
+}
+
+template<size_t N, size_t M, size_t K>
+int run(const spec::problem &p, const std::string &out, const options &opt) {
+    enum { NA = N + K, NB = M + K, NC = N + M };
+
+    block_index_space<NA> bisa = make_bis<NA>(p.a);
+    block_tensor<NA, double, allocator<double> > bta(bisa);
+    fill_tensor<NA>(p.a, bta);
+    block_tensor<NB, double, allocator<double> > *btbp = 0;
+    if(!p.self) {
+        block_index_space<NB> bisb = make_bis<NB>(p.b);
+        btbp = new block_tensor<NB, double, allocator<double> >(bisb);
+        fill_tensor<NB>(p.b, *btbp);
+    }
+    block_tensor<NB, double, allocator<double> > &btb = p.self ?
+        reinterpret_cast<block_tensor<NB, double, allocator<double> >&>(bta) : *btbp;
+
+    sequence<NC, size_t> pcs(0);
+    std::vector<size_t> pcv(p.permc);
+    contraction2<N, M, K> contr(make_perm<NC>(pcv));
+    for(size_t i = 0; i < K; i++) contr.contract(p.ka[i], p.kb[i]);
+
+    FILE *f = fopen((out + ".txt").c_str(), "w");
+    double t0 = wall_ms();
+    btod_contract2<N, M, K> op(contr, bta, p.sa, btb, p.sb, p.sc);
+    double t_sched = wall_ms() - t0;
+    fprintf(f, "schedule_ms %.6f\n", t_sched);
+
+    const symmetry<NC, double> &symc = op.get_symmetry();
+    dimensions<NC> bidc = symc.get_bis().get_block_index_dims();
+    if(opt.symc) {
+        fprintf(f, "symc %zu\n", bidc.get_size());
+        for(size_t a = 0; a < bidc.get_size(); a++) {
+            orbit<NC, double> o(symc, a);
+            fprintf(f, "%zu %zu %d\n", a, o.get_acindex(), o.is_allowed() ? 1 : 0);
+        }
+    }
+    const assignment_schedule<NC, double> &sch = op.get_schedule();
+    std::vector<size_t> sched;
+    for(typename assignment_schedule<NC, double>::iterator i = sch.begin(); i != sch.end(); ++i)
+        sched.push_back(sch.get_abs_index(i));
+    std::sort(sched.begin(), sched.end());
+    fprintf(f, "schedule %zu\n", sched.size());
+    for(size_t i = 0; i < sched.size(); i++) fprintf(f, "%zu\n", sched[i]);
+
+    if(opt.clst) {
+        block_tensor_ctrl<NA, double> ca(bta);
+        block_tensor_ctrl<NB, double> cb(btb);
+        const symmetry<NA, double> &syma = ca.req_const_symmetry();
+        const symmetry<NB, double> &symb = cb.req_const_symmetry();
+        dimensions<NA> bida = bta.get_bis().get_block_index_dims();
+        dimensions<NB> bidb = btb.get_bis().get_block_index_dims();
+        std::vector<size_t> nza, nzb;
+        gen_block_tensor_rd_ctrl<NA, block_tensor_i_traits<double> >(bta).req_nonzero_blocks(nza);
+        gen_block_tensor_rd_ctrl<NB, block_tensor_i_traits<double> >(btb).req_nonzero_blocks(nzb);
+        block_list<NA> bla(bida, nza), blax(bida);
+        block_list<NB> blb(bidb, nzb), blbx(bidb);
+        gen_bto_unfold_block_list<NA, btod_traits>(syma, bla).build(blax);
+        gen_bto_unfold_block_list<NB, btod_traits>(symb, blb).build(blbx);
+        gen_bto_contract2_block_list<N, M, K> cbl(contr, bida, blax, bidb, blbx);
+        for(size_t s = 0; s < sched.size(); s++) {
+            libtensor::index<NC> ic;
+            abs_index<NC>::get_index(sched[s], bidc, ic);
+            gen_bto_contract2_clst_builder<N, M, K, btod_traits> bld(contr, syma, symb,
+                blax, blbx, bidc, ic);
+            bld.build_list(false, cbl);
+            const typename gen_bto_contract2_clst<N, M, K, double>::list_type &cl =
+                bld.get_clst();
+            fprintf(f, "clst %zu %zu\n", sched[s], cl.size());
+            for(typename gen_bto_contract2_clst<N, M, K, double>::list_type::const_iterator
+                j = cl.begin(); j != cl.end(); ++j) {
+                fprintf(f, "%zu %zu", j->get_aindex_a(), j->get_acindex_a());
+                print_tr<NA>(f, j->get_transf_a());
+                fprintf(f, " %zu %zu", j->get_aindex_b(), j->get_acindex_b());
+                print_tr<NB>(f, j->get_transf_b());
+                fprintf(f, "\n");
+            }
+        }
+    }
+
+    block_tensor<NC, double, allocator<double> > btc(op.get_bis());
+    libutil::thread_pool *tp = 0;
+    if(opt.threads > 1) {
+        tp = new libutil::thread_pool(opt.threads, opt.threads);
+        tp->associate();
+    }
+    t0 = wall_ms();
+    op.perform(btc);
+    double t_first = wall_ms() - t0;
+    fprintf(f, "perform_ms %.6f\n", t_first);
+    std::vector<double> times;
+    for(size_t r = 0; r < opt.repeat; r++) {
+        t0 = wall_ms();
+        op.perform(btc);
+        times.push_back(wall_ms() - t0);
+    }
+    if(!times.empty()) {
+        std::sort(times.begin(), times.end());
+        fprintf(f, "perform_median_ms %.6f\n", times[times.size() / 2]);
+    }
+    if(tp) {
+        tp->dissociate();
+        delete tp;
+    }
+    fprintf(f, "threads %zu\n", opt.threads);
+
+    if(opt.result) {
+        dimensions<NC> dc = op.get_bis().get_dims();
+        dense_tensor<NC, double, allocator<double> > t(dc);
+        tod_btconv<NC>(btc).perform(t);
+        dense_tensor_rd_ctrl<NC, double> ctl(t);
+        const double *pd = ctl.req_const_dataptr();
+        FILE *g = fopen((out + ".result.bin").c_str(), "wb");
+        fwrite(pd, 8, dc.get_size(), g);
+        fclose(g);
+        ctl.ret_const_dataptr(pd);
+    }
+    fclose(f);
+    delete btbp;
+    return 0;
+}
+
+typedef int (*runner)(const spec::problem &, const std::string &, const options &);
+
+struct entry { size_t n, m, k; runner r; };
+
+#define E(n, m, k) { n, m, k, &run<n, m, k> }
+const entry k_table[] = {
+    E(1, 1, 0), E(2, 1, 0), E(1, 2, 0), E(2, 2, 0),
+    E(0, 1, 1), E(1, 0, 1), E(1, 1, 1), E(2, 1, 1), E(1, 2, 1), E(2, 2, 1),
+    E(3, 1, 1), E(1, 3, 1), E(3, 3, 1),
+    E(0, 2, 2), E(2, 0, 2), E(1, 1, 2), E(2, 1, 2), E(1, 2, 2), E(2, 2, 2),
+    E(1, 1, 3), E(2, 2, 3), E(1, 3, 3)
+};
+#undef E
+
+} // unnamed namespace
+
+int main(int argc, char **argv) {
+    if(argc < 3) {
+        fprintf(stderr, "usage: %s spec out_prefix [--threads T] [--repeat R] "
+            "[--no-symc] [--no-clst] [--no-result]\n", argv[0]);
+        return 2;
+    }
+    options opt;
+    for(int i = 3; i < argc; i++) {
+        std::string a(argv[i]);
+        if(a == "--threads" && i + 1 < argc) opt.threads = atoi(argv[++i]);
+        else if(a == "--repeat" && i + 1 < argc) opt.repeat = atoi(argv[++i]);
+        else if(a == "--no-symc") opt.symc = false;
+        else if(a == "--no-clst") opt.clst = false;
+        else if(a == "--no-result") opt.result = false;
+    }
+    try {
+        spec::problem p = spec::parse(argv[1]);
+        for(size_t t = 0; t < p.tables.size(); t++) {
+            const spec::table &tb = p.tables[t];
+            std::vector<std::string> irreps;
+            char buf[32];
+            for(size_t l = 0; l < tb.nlabels; l++) {
+                snprintf(buf, sizeof(buf), "L%zu", l);
+                irreps.push_back(buf);
+            }
+            point_group_table pg(tb.id, irreps, irreps[0]);
+            for(size_t k = 0; k < tb.products.size(); k++)
+                pg.add_product(tb.products[k].l1, tb.products[k].l2, tb.products[k].lr);
+            product_table_container::get_instance().add(pg);
+        }
+        allocator<double>::init(4, 16, 16777216, 16777216);
+        batching_policy_base::set_batch_size(size_t(1) << 40);
+        for(size_t i = 0; i < sizeof(k_table) / sizeof(k_table[0]); i++) {
+            const entry &e = k_table[i];
+            if(e.n == p.N && e.m == p.M && e.k == p.K) return e.r(p, argv[2], opt);
+        }
+        fprintf(stderr, "unsupported contraction order (%zu, %zu, %zu)\n", p.N, p.M, p.K);
+        return 3;
+    } catch(std::exception &e) {
+        fprintf(stderr, "error: %s\n", e.what());
+        return 1;
+    }
+}
