@@ -1,0 +1,157 @@
+"""Benchmark: btod_contract2 on one MI355X for the pp-ladder configuration.
+
+Workload: R[i,j,a,b] = sum_{c,d} T[i,j,c,d] V[a,b,c,d], o = 40, v = 200,
+alpha/beta spin partitions, 4 C2v irreps, uneven blocks (o: 7+5+4+4 per spin,
+v: 8..32), (ij)/(ab)/(cd) antisymmetry, V symmetric under (ab)<->(cd);
+T ~ N(0, 0.01^2)/(1+|c-i|), V ~ N(0,1), symmetrized, fixed seeds.  Inputs are
+resident in HBM before timing.  A step is one btod_contract2::perform of the
+whole contraction (all non-zero block pairs) with the cached execution plan.
+
+Prints one JSON line: effective FP64-accurate TFLOP/s (schedule FLOPs / wall
+time per step), checked against the FP64 reference (max|dR|/max|R| <= 1e-12,
+libtensor built from its sources, run on the host cores), the roofline of the
+grouped GEMM kernel (HIP events around its launch) and the host-CPU baseline.
+"""
+import argparse
+import json
+import os
+import statistics
+import sys
+import tempfile
+import time
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, HERE)
+
+import numpy as np  # noqa: E402
+
+METRIC = "btod_contract2 eff. TFLOP/s @≤1e-12 rel err vs FP64; p50/p99 request latency"
+FP64_MFMA_PEAK = 78.6   # TFLOP/s, MI355X dense FP64 matrix
+
+
+def log(*a):
+    print(*a, file=sys.stderr)
+    sys.stderr.flush()
+
+
+def cpu_baseline(prob, a, b, flops, shape, threads):
+    """The real libtensor (oracle/_ref/ref_driver) on the host cores: the FP64
+    reference result and the reference's own perform() time."""
+    sys.path.insert(0, os.path.join(HERE, "oracle"))
+    import ref_io
+    if not ref_io.ref_available():
+        return None, None
+    with tempfile.TemporaryDirectory() as d:
+        t0 = time.time()
+        out = ref_io.run_ref(prob, d, shape, tensors={"A": a, "B": b},
+                             extra=["--threads", str(threads), "--repeat", "2",
+                                    "--no-symc", "--no-clst"], timeout=1500)
+        log("cpu reference done in %.1fs" % (time.time() - t0))
+    ms = out.get("perform_median_ms", out.get("perform_ms"))
+    base = {"value": flops / (ms * 1e-3) / 1e12, "unit": "TFLOP/s", "cores": threads,
+            "kind": "reference",
+            "sample": "full pp-ladder contraction (libtensor btod_contract2, linalg_generic "
+                      "FP64 loops, thread_pool of %d workers), median of 2 after 1 warm-up; "
+                      "%.1f ms per perform" % (threads, ms)}
+    return base, out.get("result")
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--mode", default="grouped", choices=["grouped", "per_block"])
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--cpu-threads", type=int, default=0)
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py runs on one GPU only: this build measures the single-GPU "
+              "path (see DESIGN.md for how the contraction would shard)", file=sys.stderr)
+        sys.exit(2)
+
+    import libtensor_amd as lt
+    from libtensor_amd import workloads as W
+
+    ctx = lt.Context(0)
+    prob = W.pp_ladder_o40_v200()
+    t0 = time.time()
+    a, b, op = W.build_random(ctx, prob, 1002)
+    c = W.result_tensor(ctx, prob)
+    log("inputs ready in %.1fs: %d + %d canonical blocks" % (
+        time.time() - t0, len(a.req_nonzero_blocks()), len(b.req_nonzero_blocks())))
+    mode = lt.LT_MODE_GROUPED if args.mode == "grouped" else lt.LT_MODE_PER_BLOCK
+
+    for _ in range(max(1, args.warmup)):
+        op.perform(c, mode)
+    ctx.sync()
+    st = op.stats()
+    flops = st["flops"]
+    log("plan %.1f ms, schedule %.1f ms, %d C blocks, %d pairs, %d tiles, %.3e flops" % (
+        st["t_plan_ms"], st["t_schedule_ms"], st["n_cblocks"], st["n_pairs"],
+        st["n_tiles"], flops))
+
+    gemm_ms, gpu_ms = [], []
+    ctx.sync()
+    t0 = time.perf_counter()
+    for i in range(args.steps):
+        op.perform(c, mode)
+        s = op.stats()
+        gemm_ms.append(s["t_gemm_ms"])
+        gpu_ms.append(s["t_gpu_ms"])
+    ctx.sync()
+    t1 = time.perf_counter()
+    ms_per_step = (t1 - t0) * 1e3 / args.steps
+    value = flops / (ms_per_step * 1e-3) / 1e12
+    gemm_avg = statistics.mean(gemm_ms)
+    log("step %.3f ms (device %.3f ms, gemm %.3f ms)" % (
+        ms_per_step, statistics.mean(gpu_ms), gemm_avg))
+
+    # per-block path on the same operation, for comparison
+    op.perform(c, lt.LT_MODE_PER_BLOCK)
+    t2 = time.perf_counter()
+    nb = max(3, args.steps // 4)
+    for i in range(nb):
+        op.perform(c, lt.LT_MODE_PER_BLOCK)
+    ctx.sync()
+    per_block_ms = (time.perf_counter() - t2) * 1e3 / nb
+    op.perform(c, mode)
+    result = c.to_dense()
+
+    base, ref = None, None
+    if not args.no_cpu:
+        threads = args.cpu_threads or min(16, os.cpu_count() or 1)
+        base, ref = cpu_baseline(prob, a, b, flops, result.shape, threads)
+    rel = None
+    if ref is not None:
+        den = float(np.max(np.abs(ref)))
+        rel = float(np.max(np.abs(result - ref)) / den) if den > 0 else None
+        log("max rel err vs reference: %.3e" % rel)
+    if rel is not None and rel > 1e-12:
+        value = 0.0   # not FP64-accurate: no effective throughput
+
+    achieved = flops / (gemm_avg * 1e-3) / 1e12
+    line = {
+        "metric": METRIC, "value": value, "unit": "TFLOP/s", "n_gpus": 1,
+        "steps": args.steps, "warmup": args.warmup, "ms_per_step": ms_per_step,
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "f64", "data": "synthetic",
+        "config": {"workload": "pp-ladder R[ijab] = sum_cd T[ijcd] V[abcd], o=40 v=200, "
+                               "alpha/beta se_part, C2v se_label, (ij)(ab)(cd) antisymmetric, "
+                               "uneven blocks 4-24", "mode": args.mode,
+                   "c_blocks": int(st["n_cblocks"]), "block_pairs": int(st["n_pairs"]),
+                   "flops_per_step": flops},
+        "roofline": {"bound": "mfma", "achieved": achieved, "peak": FP64_MFMA_PEAK,
+                     "unit": "TFLOP/s", "frac": achieved / FP64_MFMA_PEAK, "traffic": None},
+        "cpu_baseline": base,
+        "max_rel_err": rel,
+        "per_block_ms_per_step": per_block_ms,
+        "gemm_ms": gemm_avg,
+        "plan_ms": st["t_plan_ms"], "schedule_ms": st["t_schedule_ms"],
+        "p50_ms": None, "p99_ms": None,
+    }
+    print(json.dumps(line))
+
+
+if __name__ == "__main__":
+    main()

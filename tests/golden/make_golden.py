@@ -1,0 +1,137 @@
+"""Writes the golden fixtures of tests/golden/ by running the REAL libtensor
+block-contraction path (oracle/_ref/ref_driver, built from the reference
+sources by oracle/Makefile) on small problems.
+
+Each fixture <name>.json holds the problem description, the non-zero
+schedule, the canonical index / allowedness of every result block and the
+coalesced contraction list of every scheduled block as printed by libtensor;
+<name>.npz holds the dense inputs and libtensor's dense result.
+
+The inputs are generated with libtensor_amd's block_tensor.random on a
+host-only context (no GPU needed).  Run:  python tests/golden/make_golden.py
+"""
+import os
+import random
+import shutil
+import sys
+import tempfile
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, os.path.dirname(HERE))
+sys.path.insert(0, os.path.dirname(os.path.dirname(HERE)))
+
+import problems as P  # noqa: E402
+import libtensor_amd as lt  # noqa: E402
+
+
+def random_problem(rng, N, M, K, sym=False):
+    """Random dimensions, splits, contracted pairs and result permutation."""
+    def tensor(n, dims_pool):
+        return {"dims": list(dims_pool[:n]), "splits": []}
+    sizes = [rng.choice([3, 4, 5, 6]) for _ in range(N + M + K)]
+    # index kinds: A outer N, B outer M, contracted K
+    da = [0] * (N + K)
+    db = [0] * (M + K)
+    pos_a = list(range(N + K))
+    pos_b = list(range(M + K))
+    rng.shuffle(pos_a)
+    rng.shuffle(pos_b)
+    pairs = []
+    for k in range(K):
+        ia, ib = pos_a[N + k], pos_b[M + k]
+        da[ia] = db[ib] = sizes[N + M + k]
+        pairs.append((ia, ib))
+    for i in range(N):
+        da[pos_a[i]] = sizes[i]
+    for j in range(M):
+        db[pos_b[j]] = sizes[N + j]
+    A = {"dims": da, "splits": []}
+    B = {"dims": db, "splits": []}
+
+    def add_splits(t, extra):
+        for i, d in enumerate(t["dims"]):
+            if d >= 4:
+                p = rng.randint(1, d - 1)
+                m = [1 if x == i else 0 for x in range(len(t["dims"]))]
+                t["splits"].append((m, p))
+                extra.setdefault(i, []).append(p)
+    ea, eb = {}, {}
+    add_splits(A, ea)
+    # contracted dims must be split identically in A and B
+    for ia, ib in pairs:
+        for (m, p) in A["splits"]:
+            if m[ia]:
+                mb = [1 if x == ib else 0 for x in range(len(db))]
+                B["splits"].append((mb, p))
+    kb = set(p[1] for p in pairs)
+    for j, d in enumerate(db):
+        if j in kb or d < 4:
+            continue
+        p = rng.randint(1, d - 1)
+        B["splits"].append(([1 if x == j else 0 for x in range(len(db))], p))
+    permc = list(range(N + M))
+    rng.shuffle(permc)
+    prob = {"A": A, "B": B, "contr": {"N": N, "M": M, "K": K, "permc": permc,
+                                       "pairs": pairs, "ka": 0.5, "kb": -2.0, "kc": 1.5}}
+    if sym:
+        # (01)-antisymmetry on A when its first two dims agree
+        if len(da) >= 2:
+            da[1] = da[0]
+            A["splits"] = [(m, p) for (m, p) in A["splits"] if not (m[0] or m[1])]
+            if da[0] >= 4:
+                A["splits"].append(([1, 1] + [0] * (len(da) - 2), 2))
+            # keep contracted partners consistent
+            B["splits"] = [s for s in B["splits"] if not any(s[0][ib] for ia, ib in pairs)]
+            for ia, ib in pairs:
+                db[ib] = da[ia]
+                for (m, p) in A["splits"]:
+                    if m[ia]:
+                        B["splits"].append(([1 if x == ib else 0 for x in range(len(db))], p))
+            A["elems"] = [("perm", [1, 0] + list(range(2, len(da))), -1.0)]
+    return prob
+
+
+def cases():
+    rng = random.Random(20261202)
+    out = [
+        ("ring_small", P.prob_ring(4, 6, [2], [2, 4])),
+        ("ring_single_block", P.prob_ring(3, 5)),
+        ("ladder_nosym", P.prob_ladder(spin=False, labels=False, antisym=False)),
+        ("ladder_perm", P.prob_ladder(spin=False, labels=False)),
+        ("ladder_spin", P.prob_ladder(labels=False)),
+        ("ladder_label", P.prob_ladder(spin=False)),
+        ("ladder_full", P.prob_ladder()),
+    ]
+    for (N, M, K) in [(1, 1, 1), (2, 1, 1), (1, 2, 1), (2, 2, 1), (1, 1, 2), (2, 1, 2),
+                      (1, 2, 2), (2, 2, 2), (1, 1, 0), (2, 1, 0), (0, 2, 2), (2, 0, 2),
+                      (3, 1, 1), (1, 1, 3)]:
+        out.append(("rand_%d%d%d" % (N, M, K), random_problem(rng, N, M, K)))
+    for (N, M, K) in [(1, 1, 1), (2, 1, 1), (1, 2, 2), (2, 2, 2)]:
+        out.append(("randsym_%d%d%d" % (N, M, K), random_problem(rng, N, M, K, sym=True)))
+    self_prob = P.prob_ring(4, 4, [2], [2])
+    self_prob["self"] = True
+    self_prob["contr"] = {"N": 2, "M": 2, "K": 2, "permc": None, "pairs": [(2, 2), (3, 3)]}
+    self_prob["A"]["elems"] = [("perm", [1, 0, 2, 3], -1.0), ("perm", [0, 1, 3, 2], -1.0)]
+    out.append(("self_ladder", self_prob))
+    return out
+
+
+def main():
+    if not P.ref_available():
+        print("oracle/_ref/ref_driver is not built (make -C oracle ref)")
+        return 1
+    ctx = lt.Context(-1)
+    work = tempfile.mkdtemp(prefix="lt_golden_")
+    try:
+        for i, (name, prob) in enumerate(cases()):
+            dense = P.make_data(ctx, prob, 1000 + i)
+            ref = P.run_ref(prob, dense, os.path.join(work, name))
+            P.save_case(os.path.join(HERE, name), prob, dense, ref)
+            print("%-22s schedule %4d" % (name, len(ref["schedule"])))
+    finally:
+        shutil.rmtree(work, ignore_errors=True)
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())
