@@ -390,6 +390,7 @@ class device_array(object):
         p = ctypes.c_void_p()
         call("lt_dev_alloc", ctx._h, max(8, 8 * self.size), ctypes.byref(p))
         self.ptr = p
+        self._ctxh = ctx._h
         if data is not None:
             self.upload(data)
 
@@ -406,7 +407,7 @@ class device_array(object):
 
     def free(self):
         if self.ptr:
-            call("lt_dev_free", self.ctx._h, self.ptr)
+            call("lt_dev_free", self._ctxh, self.ptr)
             self.ptr = None
 
     def __del__(self):

@@ -2,15 +2,29 @@
 #include "../../../include/lt_amd.h"
 #include "lt_runtime.h"
 
+#include <atomic>
 #include <cstring>
 #include <string>
 
 using namespace lta;
 
-struct lt_ctx { context *c; };
+// A context stays alive while tensors, device arrays or operations created
+// from it exist, whatever order the caller releases handles in.
+struct lt_ctx {
+    context *c;
+    std::atomic<long> refs;
+};
 struct lt_btensor { btensor *t; lt_ctx *ctx; };
 struct lt_symel { std::shared_ptr<sym_element> e; };
-struct lt_btod_contract2 { btod_contract2 *op; lt_btensor *a, *b; };
+struct lt_btod_contract2 { btod_contract2 *op; lt_btensor *a, *b; lt_ctx *ctx; };
+
+static void ctx_retain(lt_ctx *x) { x->refs++; }
+static void ctx_release(lt_ctx *x) {
+    if(--x->refs == 0) {
+        delete x->c;
+        delete x;
+    }
+}
 
 namespace {
 
@@ -116,6 +130,7 @@ int lt_ctx_create(int device, size_t cap, lt_ctx **out) {
     return guard([&]() {
         need(out, "out");
         lt_ctx *c = new lt_ctx;
+        c->refs = 1;
         try {
             c->c = new context(device, cap);
         } catch(...) {
@@ -129,8 +144,7 @@ int lt_ctx_create(int device, size_t cap, lt_ctx **out) {
 int lt_ctx_destroy(lt_ctx *ctx) {
     return guard([&]() {
         if(!ctx) return;
-        delete ctx->c;
-        delete ctx;
+        ctx_release(ctx);
     });
 }
 
@@ -155,11 +169,17 @@ int lt_dev_alloc(lt_ctx *ctx, size_t bytes, void **dptr) {
         need(ctx, "ctx"); need(dptr, "dptr");
         ctx->c->require_device("lt_dev_alloc");
         *dptr = ctx->c->alloc(bytes);
+        ctx_retain(ctx);
     });
 }
 
 int lt_dev_free(lt_ctx *ctx, void *p) {
-    return guard([&]() { need(ctx, "ctx"); ctx->c->free(p); });
+    return guard([&]() {
+        need(ctx, "ctx");
+        if(!p) return;
+        ctx->c->free(p);
+        ctx_release(ctx);
+    });
 }
 
 int lt_dev_upload(lt_ctx *ctx, void *dst, const void *host, size_t bytes) {
@@ -392,6 +412,7 @@ int lt_btensor_create(lt_ctx *ctx, size_t n, const size_t *dims, lt_btensor **ou
             delete b;
             throw;
         }
+        ctx_retain(ctx);
         *out = b;
     });
 }
@@ -399,8 +420,10 @@ int lt_btensor_create(lt_ctx *ctx, size_t n, const size_t *dims, lt_btensor **ou
 int lt_btensor_destroy(lt_btensor *bt) {
     return guard([&]() {
         if(!bt) return;
+        lt_ctx *ctx = bt->ctx;
         delete bt->t;
         delete bt;
+        ctx_release(ctx);
     });
 }
 
@@ -689,12 +712,14 @@ int lt_btod_contract2_create(size_t N, size_t M, size_t K, const size_t *permc,
         lt_btod_contract2 *h = new lt_btod_contract2;
         h->a = a;
         h->b = b;
+        h->ctx = a->ctx;
         try {
             h->op = new btod_contract2(c, *a->t, ka, *b->t, kb, kc);
         } catch(...) {
             delete h;
             throw;
         }
+        ctx_retain(h->ctx);
         *out = h;
     });
 }
@@ -702,8 +727,10 @@ int lt_btod_contract2_create(size_t N, size_t M, size_t K, const size_t *permc,
 int lt_btod_contract2_destroy(lt_btod_contract2 *op) {
     return guard([&]() {
         if(!op) return;
+        lt_ctx *ctx = op->ctx;
         delete op->op;
         delete op;
+        ctx_release(ctx);
     });
 }
 
