@@ -158,7 +158,7 @@ hipError_t launch_fill(double *p, size_t n, double v, hipStream_t s) {
 //
 // v_mfma_f64_16x16x4f64 operand layout (lane l = 0..63):
 //   A: A[m = l % 16][k = l / 16]      B: B[k = l / 16][n = l % 16]
-//   D: D[m = 4 * (l / 16) + v][n = l % 16], v = 0..3
+//   D: D[m = (l / 16) + 4 * v][n = l % 16], v = 0..3 (measured on gfx950)
 
 template<int BM, int BN, bool TA, bool TB>
 __global__ void __launch_bounds__(256)
@@ -270,7 +270,7 @@ gemm_kernel(const gemm_cblock *__restrict__ cbs,
                 const int col = wn0 + j * 16 + lrow;
 #pragma unroll
                 for(int v = 0; v < 4; v++) {
-                    const int row = wm0 + i * 16 + 4 * lk + v;
+                    const int row = wm0 + i * 16 + lk + 4 * v;
                     if(row < mrem && col < nrem) {
                         double *cp = cb.c + size_t(m0 + row) * cb.ldc + (n0 + col);
                         double x = acc[i][j][v];
