@@ -58,8 +58,8 @@ def test_mfma_fragment_layout(gpu_ctx):
 @pytest.mark.parametrize("case", [
     ((2, 2, 2), (1, 0), (3, 2), [0, 2, 1, 3], (5, 7, 6, 9), (7, 4, 9, 8)),
     ((2, 2, 2), (2, 2), (3, 3), None, (6, 5, 8, 9), (7, 4, 8, 9)),
-    ((2, 1, 1), (0, 1), None, [2, 0, 1], (13, 5, 11), (13, 17)),
-    ((1, 2, 2), (2, 0), (0, 2), [1, 0, 2], (9, 3, 10), (10, 6, 9)),
+    ((2, 1, 1), (0, 0), None, [2, 0, 1], (13, 5, 11), (13, 17)),
+    ((1, 1, 2), (2, 0), (0, 2), [1, 0], (9, 3, 10), (10, 6, 9)),
     ((3, 1, 1), (3, 0), None, [3, 1, 0, 2], (4, 5, 6, 7), (7, 9)),
     ((2, 2, 0), None, None, [3, 0, 2, 1], (4, 5), (6, 7)),
 ])
