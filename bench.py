@@ -148,6 +148,7 @@ def main():
         "per_block_ms_per_step": per_block_ms,
         "gemm_ms": gemm_avg,
         "plan_ms": st["t_plan_ms"], "schedule_ms": st["t_schedule_ms"],
+        "gemm_kernel_cfg": int(st["tile_cfg"]), "autotune_ms": st["tune_ms"],
         "p50_ms": None, "p99_ms": None,
     }
     print(json.dumps(line))

@@ -54,6 +54,8 @@ typedef struct lt_stats {
     int32_t tile_cfg;
     int32_t ozaki_slices;
     double ozaki_max_rel_err;
+    double tune_ms[8];      /* autotuner: grouped GEMM time per kernel
+                               configuration (0 = not measured) */
 } lt_stats;
 
 /* Message of the calling thread's last failed call. */
@@ -190,7 +192,9 @@ int lt_btod_contract2_clst(lt_btod_contract2 *op, size_t aidx_c, double *out,
 int lt_btod_contract2_compute_block_host(lt_btod_contract2 *op, size_t aidx_c,
     double *host);
 int lt_btod_contract2_stats(lt_btod_contract2 *op, lt_stats *out);
-/* Tile shape of the grouped GEMM (-1: chosen per schedule). */
+/* Kernel configuration of the contraction GEMM: 0..7 fixed, -1 chosen by a
+ * padding estimate, -2 (default) measured on the plan's own work list when
+ * it is built (autotuner). */
 int lt_btod_contract2_set_tile(lt_btod_contract2 *op, int cfg);
 /* Split-precision mode: number of INT8 slices (0 = automatic). */
 int lt_btod_contract2_set_ozaki(lt_btod_contract2 *op, int slices);

@@ -368,7 +368,9 @@ class btod_contract2(object):
     def stats(self):
         s = LtStats()
         call("lt_btod_contract2_stats", self._h, ctypes.byref(s))
-        return {f: getattr(s, f) for f, _ in LtStats._fields_}
+        out = {f: getattr(s, f) for f, _ in LtStats._fields_}
+        out["tune_ms"] = list(out["tune_ms"])
+        return out
 
     def set_tile(self, cfg):
         call("lt_btod_contract2_set_tile", self._h, int(cfg))

@@ -35,6 +35,7 @@ class LtStats(ctypes.Structure):
         ("tile_cfg", ctypes.c_int32),
         ("ozaki_slices", ctypes.c_int32),
         ("ozaki_max_rel_err", ctypes.c_double),
+        ("tune_ms", ctypes.c_double * 8),
     ]
 
 

@@ -60,7 +60,12 @@ struct gemm_tile {
     uint16_t tm, tn;
 };
 
-enum { TILE_64x64 = 0, TILE_32x64 = 1, TILE_64x32 = 2, TILE_32x32 = 3, TILE_COUNT = 4 };
+//! GEMM kernel configurations.  TILE_*: workgroup tiles with LDS-staged
+//! operands; DIRECT_*: one wave per tile, operands loaded straight into
+//! MFMA fragments (no LDS, no barriers).
+enum { TILE_64x64 = 0, TILE_32x64 = 1, TILE_64x32 = 2, TILE_32x32 = 3,
+       DIRECT_16x64 = 4, DIRECT_32x32 = 5, DIRECT_32x64 = 6, DIRECT_16x128 = 7,
+       TILE_COUNT = 8 };
 void tile_shape(int cfg, uint32_t &bm, uint32_t &bn);
 
 //! Batched permutation: every descriptor is split into chunks of `chunk`

@@ -36,6 +36,10 @@ void tile_shape(int cfg, uint32_t &bm, uint32_t &bn) {
     case TILE_32x64: bm = 32; bn = 64; break;
     case TILE_64x32: bm = 64; bn = 32; break;
     case TILE_32x32: bm = 32; bn = 32; break;
+    case DIRECT_16x64: bm = 16; bn = 64; break;
+    case DIRECT_32x32: bm = 32; bn = 32; break;
+    case DIRECT_32x64: bm = 32; bn = 64; break;
+    case DIRECT_16x128: bm = 16; bn = 128; break;
     default: bm = 64; bn = 64; break;
     }
 }
@@ -295,6 +299,130 @@ static hipError_t launch_gemm_t(bool ta, bool tb, const gemm_cblock *cb,
     return hipGetLastError();
 }
 
+// ---------------------------------------------------------------------------
+// Direct FP64 MFMA GEMM: one wave per (16 FM) x (16 FN) tile, four waves per
+// workgroup working on four tiles.  Operands go from memory straight into
+// MFMA fragments.  Per chunk of 16 k, lane l (r = l % 16, g = l / 16) loads
+// A[m0 + r][k0 + 4g .. 4g + 3] and B[k0 + 4g .. 4g + 3][n0 + r]; MFMA step j
+// then feeds element j, so fragment slot g of step j carries k = k0 + 4g + j
+// for both operands and the 16 k of the chunk are each summed once.
+
+template<bool T>
+__device__ __forceinline__ void load_frag(double (&x)[4], const double *base,
+    bool rowok, int kk, int krem_full, int ld) {
+    // !T: element (row, k) at base[k] (row-major, k contiguous);
+    //  T: element (row, k) at base[k * ld] (k-major).
+    if(krem_full >= 4) {
+#pragma unroll
+        for(int j = 0; j < 4; j++)
+            x[j] = rowok ? (T ? base[size_t(kk + j) * ld] : base[kk + j]) : 0.0;
+    } else {
+#pragma unroll
+        for(int j = 0; j < 4; j++)
+            x[j] = (rowok && j < krem_full) ? (T ? base[size_t(kk + j) * ld] : base[kk + j]) : 0.0;
+    }
+}
+
+template<int FM, int FN, bool TA, bool TB>
+__global__ void __launch_bounds__(256)
+gemm_direct_kernel(const gemm_cblock *__restrict__ cbs,
+    const gemm_pair *__restrict__ pairs,
+    const gemm_tile *__restrict__ tiles, uint32_t ntiles) {
+
+    constexpr int TM = 16 * FM, TN = 16 * FN;
+    const int lane = threadIdx.x & 63;
+    const int wave = threadIdx.x >> 6;
+    const int lr = lane & 15, lg = lane >> 4;
+
+    for(uint32_t t = blockIdx.x * 4 + wave; t < ntiles; t += gridDim.x * 4) {
+        const gemm_tile tile = tiles[t];
+        const gemm_cblock cb = cbs[tile.cblock];
+        const int m0 = int(tile.tm) * TM, n0 = int(tile.tn) * TN;
+        const int mrem = int(cb.m) - m0, nrem = int(cb.n) - n0;
+
+        double4_t acc[FM][FN];
+#pragma unroll
+        for(int i = 0; i < FM; i++)
+#pragma unroll
+            for(int j = 0; j < FN; j++) acc[i][j] = double4_t{0.0, 0.0, 0.0, 0.0};
+
+        bool aok[FM], bok[FN];
+#pragma unroll
+        for(int i = 0; i < FM; i++) aok[i] = (16 * i + lr) < mrem;
+#pragma unroll
+        for(int j = 0; j < FN; j++) bok[j] = (16 * j + lr) < nrem;
+
+        for(uint32_t p = cb.pair_begin; p < cb.pair_end; p++) {
+            const gemm_pair pr = pairs[p];
+            const int K = int(pr.k);
+            const double d = pr.d;
+            const double *ab[FM];
+            const double *bb[FN];
+#pragma unroll
+            for(int i = 0; i < FM; i++) {
+                const int m = aok[i] ? (m0 + 16 * i + lr) : 0;
+                ab[i] = TA ? (pr.a + m) : (pr.a + size_t(m) * pr.lda);
+            }
+#pragma unroll
+            for(int j = 0; j < FN; j++) {
+                const int n = bok[j] ? (n0 + 16 * j + lr) : 0;
+                bb[j] = TB ? (pr.b + size_t(n) * pr.ldb) : (pr.b + n);
+            }
+            for(int k0 = 0; k0 < K; k0 += 16) {
+                const int kk = k0 + 4 * lg;
+                const int kr = K - kk;
+                double af[FM][4], bf[FN][4];
+#pragma unroll
+                for(int i = 0; i < FM; i++) load_frag<TA>(af[i], ab[i], aok[i], kk, kr, pr.lda);
+#pragma unroll
+                for(int j = 0; j < FN; j++) load_frag<!TB>(bf[j], bb[j], bok[j], kk, kr, pr.ldb);
+#pragma unroll
+                for(int i = 0; i < FM; i++)
+#pragma unroll
+                    for(int q = 0; q < 4; q++) af[i][q] *= d;
+#pragma unroll
+                for(int q = 0; q < 4; q++)
+#pragma unroll
+                    for(int i = 0; i < FM; i++)
+#pragma unroll
+                        for(int j = 0; j < FN; j++)
+                            acc[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(
+                                af[i][q], bf[j][q], acc[i][j], 0, 0, 0);
+            }
+        }
+
+#pragma unroll
+        for(int i = 0; i < FM; i++) {
+#pragma unroll
+            for(int j = 0; j < FN; j++) {
+                const int col = 16 * j + lr;
+#pragma unroll
+                for(int v = 0; v < 4; v++) {
+                    const int row = 16 * i + lg + 4 * v;
+                    if(row < mrem && col < nrem) {
+                        double *cp = cb.c + size_t(m0 + row) * cb.ldc + (n0 + col);
+                        double x = acc[i][j][v];
+                        if(!cb.zero) x += *cp;
+                        *cp = x;
+                    }
+                }
+            }
+        }
+    }
+}
+
+template<int FM, int FN>
+static hipError_t launch_direct_t(bool ta, bool tb, const gemm_cblock *cb,
+    const gemm_pair *pairs, const gemm_tile *tiles, uint32_t ntiles,
+    uint32_t nwg, hipStream_t s) {
+    dim3 g((nwg + 3) / 4), b(256);
+    if(!ta && !tb) hipLaunchKernelGGL((gemm_direct_kernel<FM, FN, false, false>), g, b, 0, s, cb, pairs, tiles, ntiles);
+    else if(!ta && tb) hipLaunchKernelGGL((gemm_direct_kernel<FM, FN, false, true>), g, b, 0, s, cb, pairs, tiles, ntiles);
+    else if(ta && !tb) hipLaunchKernelGGL((gemm_direct_kernel<FM, FN, true, false>), g, b, 0, s, cb, pairs, tiles, ntiles);
+    else hipLaunchKernelGGL((gemm_direct_kernel<FM, FN, true, true>), g, b, 0, s, cb, pairs, tiles, ntiles);
+    return hipGetLastError();
+}
+
 hipError_t launch_gemm(int cfg, bool ta, bool tb, const gemm_cblock *cb,
     const gemm_pair *pairs, const gemm_tile *tiles, uint32_t ntiles,
     uint32_t nwg, hipStream_t s) {
@@ -304,6 +432,10 @@ hipError_t launch_gemm(int cfg, bool ta, bool tb, const gemm_cblock *cb,
     case TILE_32x64: return launch_gemm_t<32, 64>(ta, tb, cb, pairs, tiles, ntiles, nwg, s);
     case TILE_64x32: return launch_gemm_t<64, 32>(ta, tb, cb, pairs, tiles, ntiles, nwg, s);
     case TILE_32x32: return launch_gemm_t<32, 32>(ta, tb, cb, pairs, tiles, ntiles, nwg, s);
+    case DIRECT_16x64: return launch_direct_t<1, 4>(ta, tb, cb, pairs, tiles, ntiles, nwg, s);
+    case DIRECT_32x32: return launch_direct_t<2, 2>(ta, tb, cb, pairs, tiles, ntiles, nwg, s);
+    case DIRECT_32x64: return launch_direct_t<2, 4>(ta, tb, cb, pairs, tiles, ntiles, nwg, s);
+    case DIRECT_16x128: return launch_direct_t<1, 8>(ta, tb, cb, pairs, tiles, ntiles, nwg, s);
     default: return launch_gemm_t<64, 64>(ta, tb, cb, pairs, tiles, ntiles, nwg, s);
     }
 }

@@ -825,13 +825,14 @@ int lt_btod_contract2_stats(lt_btod_contract2 *op, lt_stats *out) {
         out->tile_cfg = s.tile_cfg;
         out->ozaki_slices = s.ozaki_slices;
         out->ozaki_max_rel_err = s.ozaki_max_rel_err;
+        for(int i = 0; i < 8; i++) out->tune_ms[i] = i < dev::TILE_COUNT ? s.tune_ms[i] : 0.0;
     });
 }
 
 int lt_btod_contract2_set_tile(lt_btod_contract2 *op, int cfg) {
     return guard([&]() {
         need(op, "op");
-        lt_require(cfg >= -1 && cfg < dev::TILE_COUNT, LT_ERR_PARAM, "tile configuration");
+        lt_require(cfg >= -2 && cfg < dev::TILE_COUNT, LT_ERR_PARAM, "tile configuration");
         op->op->set_tile_cfg(cfg);
     });
 }

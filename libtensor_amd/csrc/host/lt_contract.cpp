@@ -126,7 +126,7 @@ struct contract_plan {
     int tile_cfg;
 
     // device copies
-    dev_lists lists;
+    dev_lists lists, tiles;
     dev::perm_desc *d_prep, *d_wback;
     uint32_t *d_prep_cd, *d_prep_co, *d_wb_cd, *d_wb_co;
     uint32_t n_prep_chunks, n_wb_chunks;
@@ -156,7 +156,7 @@ struct contract_plan {
 btod_contract2::btod_contract2(const contraction2 &c, btensor &a, double ka,
     btensor &b, double kb, double kc) :
     m_contr(c), m_a(a), m_b(b), m_ka(ka), m_kb(kb), m_kc(kc),
-    m_tile_cfg(-1), m_oz_slices(0), m_oz_int8(1) {
+    m_tile_cfg(-2), m_tuned_cfg(-1), m_oz_slices(0), m_oz_int8(1) {
 
     lt_require(c.is_complete(), LT_ERR_PARAM, "contraction is incomplete");
     lt_require(a.bis().order() == c.NA() && b.bis().order() == c.NB(),
@@ -369,9 +369,6 @@ void btod_contract2::build_plan(btensor &c, bool add, double dscale) {
     pl.tb = tc ? !gf0.ta : gf0.tb;
 
     double flops = 0.0;
-    uint32_t bm = 64, bn = 64;
-    // tile shape: explicit choice, or the one that wastes least padding
-    int cfg = m_tile_cfg;
     std::vector<std::pair<size_t, size_t> > mn(batchc.size());
     for(size_t t = 0; t < batchc.size(); t++) {
         index_t ic = get_index(batchc[t], bidct);
@@ -386,28 +383,8 @@ void btod_contract2::build_plan(btensor &c, bool add, double dscale) {
         }
         mn[t] = std::make_pair(m, n);
     }
-    if(cfg < 0) {
-        double best = 1e300;
-        for(int k = 0; k < dev::TILE_COUNT; k++) {
-            uint32_t tbm, tbn;
-            dev::tile_shape(k, tbm, tbn);
-            double waste = 0.0, ntile = 0.0;
-            for(size_t t = 0; t < mn.size(); t++) {
-                size_t rows = tc ? mn[t].second : mn[t].first;
-                size_t cols = tc ? mn[t].first : mn[t].second;
-                size_t tm = (rows + tbm - 1) / tbm, tn = (cols + tbn - 1) / tbn;
-                waste += double(tm * tbm * tn * tbn);
-                ntile += double(tm * tn);
-            }
-            // padded work plus a per-tile overhead equivalent to 32x32 MACs
-            double cost = waste + ntile * 1024.0;
-            if(cost < best) { best = cost; cfg = k; }
-        }
-    }
-    dev::tile_shape(cfg, bm, bn);
-    pl.tile_cfg = cfg;
 
-    std::vector<double> tile_cost;
+    std::vector<double> cb_ksum;
     for(size_t t = 0; t < batchc.size(); t++) {
         const contr_list &cl = clsts[t];
         dev::gemm_cblock cb;
@@ -460,34 +437,79 @@ void btod_contract2::build_plan(btensor &c, bool add, double dscale) {
             ksum += double(g.k);
         }
         cb.pair_end = (uint32_t)pl.pairs.size();
-        uint32_t cbi = (uint32_t)pl.cblocks.size();
         pl.cblocks.push_back(cb);
-        pl.block_tile_begin.push_back((uint32_t)pl.tiles_by_block.size());
-        uint32_t tm = (uint32_t)((rows + bm - 1) / bm), tn = (uint32_t)((cols + bn - 1) / bn);
-        for(uint32_t i = 0; i < tm; i++) {
-            for(uint32_t j = 0; j < tn; j++) {
-                dev::gemm_tile tl;
-                tl.cblock = cbi;
-                tl.tm = (uint16_t)i;
-                tl.tn = (uint16_t)j;
-                pl.tiles_by_block.push_back(tl);
-                tile_cost.push_back(ksum);
-            }
-        }
+        cb_ksum.push_back(ksum);
         pl.c_out.push_back(cdst[t].yc);
         pl.c_out_mem.push_back(std::make_pair(c.block_ptr(cdst[t].yc), c.block_size(cdst[t].yc)));
     }
-    pl.block_tile_begin.push_back((uint32_t)pl.tiles_by_block.size());
-    lt_require(pl.tiles_by_block.size() < (size_t(1) << 31), LT_ERR_PARAM, "too many tiles");
 
-    // cost-sorted tile order for the grouped launch (longest first)
-    std::vector<size_t> order(pl.tiles_by_block.size());
-    std::iota(order.begin(), order.end(), size_t(0));
-    std::stable_sort(order.begin(), order.end(), [&](size_t x, size_t y) {
-        return tile_cost[x] > tile_cost[y];
-    });
-    pl.tiles_sorted.resize(order.size());
-    for(size_t i = 0; i < order.size(); i++) pl.tiles_sorted[i] = pl.tiles_by_block[order[i]];
+    // Tile lists of one kernel configuration: tiles grouped by C block (one
+    // launch per block) and the same tiles longest-first (grouped launch).
+    auto make_tiles = [&](int cfg, std::vector<dev::gemm_tile> &by_block,
+        std::vector<uint32_t> &begin, std::vector<dev::gemm_tile> &sorted) {
+        uint32_t bm = 64, bn = 64;
+        dev::tile_shape(cfg, bm, bn);
+        by_block.clear();
+        begin.clear();
+        std::vector<double> cost;
+        for(size_t b = 0; b < pl.cblocks.size(); b++) {
+            const dev::gemm_cblock &cb = pl.cblocks[b];
+            begin.push_back((uint32_t)by_block.size());
+            uint32_t tm = (cb.m + bm - 1) / bm, tn = (cb.n + bn - 1) / bn;
+            lt_require(tm < 65536 && tn < 65536, LT_ERR_PARAM, "C block too large");
+            for(uint32_t i = 0; i < tm; i++) {
+                for(uint32_t j = 0; j < tn; j++) {
+                    dev::gemm_tile tl;
+                    tl.cblock = (uint32_t)b;
+                    tl.tm = (uint16_t)i;
+                    tl.tn = (uint16_t)j;
+                    by_block.push_back(tl);
+                    cost.push_back(cb_ksum[b]);
+                }
+            }
+        }
+        begin.push_back((uint32_t)by_block.size());
+        lt_require(by_block.size() < (size_t(1) << 31), LT_ERR_PARAM, "too many tiles");
+        std::vector<size_t> order(by_block.size());
+        std::iota(order.begin(), order.end(), size_t(0));
+        std::stable_sort(order.begin(), order.end(), [&](size_t x, size_t y) {
+            return cost[x] > cost[y];
+        });
+        sorted.resize(order.size());
+        for(size_t i = 0; i < order.size(); i++) sorted[i] = by_block[order[i]];
+    };
+
+    // Kernel configuration: fixed, or the one wasting least padding, or
+    // measured on this plan (autotuner, below, after the lists are on the
+    // device).
+    int cfg = m_tile_cfg;
+    std::vector<int> candidates;
+    if(cfg >= 0) {
+        candidates.push_back(cfg);
+    } else {
+        std::vector<std::pair<double, int> > est;
+        for(int k = 0; k < dev::TILE_COUNT; k++) {
+            uint32_t tbm, tbn;
+            dev::tile_shape(k, tbm, tbn);
+            double waste = 0.0, ntile = 0.0;
+            for(size_t b = 0; b < pl.cblocks.size(); b++) {
+                size_t tm = (pl.cblocks[b].m + tbm - 1) / tbm;
+                size_t tn = (pl.cblocks[b].n + tbn - 1) / tbn;
+                waste += double(tm * tbm * tn * tbn) * cb_ksum[b];
+                ntile += double(tm * tn);
+            }
+            // padded work plus a per-tile overhead equivalent to 32x32 MACs
+            double kmean = 0.0;
+            for(size_t b = 0; b < cb_ksum.size(); b++) kmean += cb_ksum[b];
+            if(!cb_ksum.empty()) kmean /= double(cb_ksum.size());
+            est.push_back(std::make_pair(waste + ntile * 1024.0 * kmean, k));
+        }
+        std::sort(est.begin(), est.end());
+        for(size_t i = 0; i < est.size(); i++) candidates.push_back(est[i].second);
+        if(cfg == -1 || add) candidates.resize(1);
+        if(add && m_tuned_cfg >= 0) candidates.assign(1, m_tuned_cfg);
+    }
+    cfg = candidates[0];
 
     // 3. write-back descriptors
     for(size_t t = 0; t < batchc.size(); t++) {
@@ -513,8 +535,6 @@ void btod_contract2::build_plan(btensor &c, bool add, double dscale) {
     size_t o_wco = sz; sz = align256(sz + wco.size() * 4);
     size_t o_cb = sz; sz = align256(sz + pl.cblocks.size() * sizeof(dev::gemm_cblock));
     size_t o_pr = sz; sz = align256(sz + pl.pairs.size() * sizeof(dev::gemm_pair));
-    size_t o_ts = sz; sz = align256(sz + pl.tiles_sorted.size() * sizeof(dev::gemm_tile));
-    size_t o_tb = sz; sz = align256(sz + pl.tiles_by_block.size() * sizeof(dev::gemm_tile));
     std::vector<char> host(sz, 0);
     auto put = [&](size_t off, const void *src, size_t n) { if(n) std::memcpy(&host[off], src, n); };
     put(o_prep, pl.prep.data(), pl.prep.size() * sizeof(dev::perm_desc));
@@ -525,8 +545,6 @@ void btod_contract2::build_plan(btensor &c, bool add, double dscale) {
     put(o_wco, wco.data(), wco.size() * 4);
     put(o_cb, pl.cblocks.data(), pl.cblocks.size() * sizeof(dev::gemm_cblock));
     put(o_pr, pl.pairs.data(), pl.pairs.size() * sizeof(dev::gemm_pair));
-    put(o_ts, pl.tiles_sorted.data(), pl.tiles_sorted.size() * sizeof(dev::gemm_tile));
-    put(o_tb, pl.tiles_by_block.data(), pl.tiles_by_block.size() * sizeof(dev::gemm_tile));
     pl.lists.ctx = &ctx;
     pl.lists.mem = ctx.alloc(sz);
     pl.lists.bytes = sz;
@@ -540,12 +558,63 @@ void btod_contract2::build_plan(btensor &c, bool add, double dscale) {
     pl.d_wb_co = (uint32_t*)(base + o_wco);
     pl.d_cb = (dev::gemm_cblock*)(base + o_cb);
     pl.d_pairs = (dev::gemm_pair*)(base + o_pr);
-    pl.d_tiles_sorted = (dev::gemm_tile*)(base + o_ts);
-    pl.d_tiles_block = (dev::gemm_tile*)(base + o_tb);
+
+    // tile lists live in their own allocation (replaced by the autotuner)
+    auto upload_tiles = [&](int k) {
+        make_tiles(k, pl.tiles_by_block, pl.block_tile_begin, pl.tiles_sorted);
+        size_t nt = pl.tiles_sorted.size() * sizeof(dev::gemm_tile);
+        size_t tsz = 2 * align256(nt);
+        if(pl.tiles.mem) ctx.free(pl.tiles.mem);
+        pl.tiles.ctx = &ctx;
+        pl.tiles.mem = ctx.alloc(tsz);
+        pl.tiles.bytes = tsz;
+        std::vector<char> th(tsz, 0);
+        if(nt) {
+            std::memcpy(&th[0], pl.tiles_sorted.data(), nt);
+            std::memcpy(&th[align256(nt)], pl.tiles_by_block.data(), nt);
+        }
+        ctx.upload(pl.tiles.mem, th.data(), tsz);
+        pl.d_tiles_sorted = (dev::gemm_tile*)pl.tiles.mem;
+        pl.d_tiles_block = (dev::gemm_tile*)((char*)pl.tiles.mem + align256(nt));
+        pl.tile_cfg = k;
+    };
+    upload_tiles(cfg);
 
     ctx.activate();
     for(int i = 0; i < 4; i++) hip_check(hipEventCreate(&pl.ev[i]), "hipEventCreate");
     pl.events = true;
+
+    // Autotuner: time every candidate configuration on this plan's own work
+    // list (grouped launch, one warm-up), keep the fastest.  Results of these
+    // launches are overwritten by the perform that follows (never run for
+    // the accumulating path).
+    if(candidates.size() > 1) {
+        hipStream_t s = ctx.stream();
+        if(pl.n_prep_chunks)
+            hip_check(dev::launch_permute(pl.d_prep, pl.d_prep_cd, pl.d_prep_co,
+                pl.n_prep_chunks, s), "permute (prepare)");
+        float best = 1e30f;
+        int bestk = cfg;
+        for(size_t ci = 0; ci < candidates.size(); ci++) {
+            int k = candidates[ci];
+            upload_tiles(k);
+            uint32_t nt = (uint32_t)pl.tiles_sorted.size();
+            hip_check(dev::launch_gemm(k, pl.ta, pl.tb, pl.d_cb, pl.d_pairs,
+                pl.d_tiles_sorted, nt, nt, s), "autotune");
+            hip_check(hipEventRecord(pl.ev[0], s), "hipEventRecord");
+            hip_check(dev::launch_gemm(k, pl.ta, pl.tb, pl.d_cb, pl.d_pairs,
+                pl.d_tiles_sorted, nt, nt, s), "autotune");
+            hip_check(hipEventRecord(pl.ev[1], s), "hipEventRecord");
+            hip_check(hipStreamSynchronize(s), "hipStreamSynchronize");
+            float ms = 0.f;
+            hip_check(hipEventElapsedTime(&ms, pl.ev[0], pl.ev[1]), "hipEventElapsedTime");
+            m_stats.tune_ms[k] = ms;
+            if(ms < best) { best = ms; bestk = k; }
+        }
+        cfg = bestk;
+        upload_tiles(cfg);
+        m_tuned_cfg = cfg;
+    }
 
     m_stats.n_cblocks = pl.cblocks.size();
     m_stats.n_pairs = pl.pairs.size();

@@ -148,6 +148,7 @@ struct exec_stats {
     int tile_cfg;
     double ozaki_max_rel_err; //!< set when mode == MODE_OZAKI and checked
     int ozaki_slices;
+    double tune_ms[dev::TILE_COUNT]; //!< autotuner timings (0 = not measured)
 };
 
 struct contract_plan;
@@ -186,7 +187,8 @@ private:
     symmetry m_symc;
     std::vector<size_t> m_sched;
     exec_stats m_stats;
-    int m_tile_cfg;
+    int m_tile_cfg;         //!< >= 0 fixed, -1 estimate, -2 autotune
+    int m_tuned_cfg;        //!< configuration found by the autotuner
     int m_oz_slices, m_oz_int8;
     std::unique_ptr<contract_plan> m_plan;
 

@@ -83,11 +83,28 @@ def test_ladder_medium_against_dense_oracle(gpu_ctx):
     assert np.array_equal(c.to_dense(), first)
 
 
-@pytest.mark.parametrize("tile", [0, 1, 2, 3])
-def test_tile_shapes(gpu_ctx, tile):
-    prob, dense, meta, result = G.load("ladder_spin")
+@pytest.mark.parametrize("tile", list(range(8)))
+@pytest.mark.parametrize("name", G.names())
+def test_kernel_configurations(gpu_ctx, name, tile):
+    """Every GEMM kernel configuration (LDS-tiled and direct, all tile
+    shapes) on every golden case, in both launch modes."""
+    _say("config", name, tile)
+    prob, dense, meta, result = G.load(name)
     a, b, op = P.build(gpu_ctx, prob, dense)
     op.set_tile(tile)
+    for mode in (lt.LT_MODE_GROUPED, lt.LT_MODE_PER_BLOCK):
+        c = P.make_result_tensor(gpu_ctx, prob)
+        op.perform(c, mode)
+        assert op.stats()["tile_cfg"] == tile
+        assert O.rel_err(c.to_dense(), result) < TOL
+
+
+def test_autotuner_records_timings(gpu_ctx):
+    prob, dense, meta, result = G.load("ladder_full")
+    a, b, op = P.build(gpu_ctx, prob, dense)
     c = P.make_result_tensor(gpu_ctx, prob)
     op.perform(c)
+    st = op.stats()
+    assert all(t > 0 for t in st["tune_ms"])
+    assert st["tune_ms"][st["tile_cfg"]] == min(st["tune_ms"])
     assert O.rel_err(c.to_dense(), result) < TOL
