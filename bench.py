@@ -63,6 +63,8 @@ def main():
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--mode", default="grouped", choices=["grouped", "per_block"])
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-per-block", action="store_true",
+                    help="skip the per-block comparison run (kernel profiles)")
     ap.add_argument("--cpu-threads", type=int, default=0)
     args = ap.parse_args()
     if args.gpus != 1:
@@ -108,14 +110,16 @@ def main():
         ms_per_step, statistics.mean(gpu_ms), gemm_avg))
 
     # per-block path on the same operation, for comparison
-    op.perform(c, lt.LT_MODE_PER_BLOCK)
-    t2 = time.perf_counter()
-    nb = max(3, args.steps // 4)
-    for i in range(nb):
+    per_block_ms = None
+    if not args.no_per_block:
         op.perform(c, lt.LT_MODE_PER_BLOCK)
-    ctx.sync()
-    per_block_ms = (time.perf_counter() - t2) * 1e3 / nb
-    op.perform(c, mode)
+        t2 = time.perf_counter()
+        nb = max(3, args.steps // 4)
+        for i in range(nb):
+            op.perform(c, lt.LT_MODE_PER_BLOCK)
+        ctx.sync()
+        per_block_ms = (time.perf_counter() - t2) * 1e3 / nb
+        op.perform(c, mode)
     result = c.to_dense()
 
     base, ref = None, None

@@ -61,11 +61,14 @@ struct gemm_tile {
 };
 
 //! GEMM kernel configurations.  TILE_*: workgroup tiles with LDS-staged
-//! operands; DIRECT_*: one wave per tile, operands loaded straight into
-//! MFMA fragments (no LDS, no barriers).
+//! operands; SPLITK_*_Wn: one workgroup of n waves per tile, the tile's
+//! k chunks dealt to the waves, operands loaded straight into MFMA
+//! fragments, partial tiles summed in LDS in a fixed order.
 enum { TILE_64x64 = 0, TILE_32x64 = 1, TILE_64x32 = 2, TILE_32x32 = 3,
-       DIRECT_16x64 = 4, DIRECT_32x32 = 5, DIRECT_32x64 = 6, DIRECT_16x128 = 7,
-       TILE_COUNT = 8 };
+       SPLITK_16x64_W4 = 4, SPLITK_32x32_W4 = 5, SPLITK_32x64_W4 = 6,
+       SPLITK_64x32_W4 = 7, SPLITK_16x64_W2 = 8, SPLITK_32x32_W2 = 9,
+       SPLITK_16x32_W4 = 10, SPLITK_32x16_W4 = 11,
+       TILE_COUNT = 12 };
 void tile_shape(int cfg, uint32_t &bm, uint32_t &bn);
 
 //! Batched permutation: every descriptor is split into chunks of `chunk`

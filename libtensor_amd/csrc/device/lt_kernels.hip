@@ -36,10 +36,12 @@ void tile_shape(int cfg, uint32_t &bm, uint32_t &bn) {
     case TILE_32x64: bm = 32; bn = 64; break;
     case TILE_64x32: bm = 64; bn = 32; break;
     case TILE_32x32: bm = 32; bn = 32; break;
-    case DIRECT_16x64: bm = 16; bn = 64; break;
-    case DIRECT_32x32: bm = 32; bn = 32; break;
-    case DIRECT_32x64: bm = 32; bn = 64; break;
-    case DIRECT_16x128: bm = 16; bn = 128; break;
+    case SPLITK_16x64_W4: case SPLITK_16x64_W2: bm = 16; bn = 64; break;
+    case SPLITK_32x32_W4: case SPLITK_32x32_W2: bm = 32; bn = 32; break;
+    case SPLITK_32x64_W4: bm = 32; bn = 64; break;
+    case SPLITK_64x32_W4: bm = 64; bn = 32; break;
+    case SPLITK_16x32_W4: bm = 16; bn = 32; break;
+    case SPLITK_32x16_W4: bm = 32; bn = 16; break;
     default: bm = 64; bn = 64; break;
     }
 }
@@ -66,14 +68,32 @@ permute_kernel(const perm_desc *__restrict__ descs,
     const uint32_t *__restrict__ chunk_desc,
     const uint32_t *__restrict__ chunk_off) {
 
+    // each thread owns kPermChunk / 256 elements of the chunk; all source
+    // offsets are computed and all loads issued before the first store, so
+    // every thread has that many gathers in flight
+    constexpr int E = kPermChunk / 256;
     const uint32_t ch = blockIdx.x;
     const perm_desc d = descs[chunk_desc[ch]];
     const uint32_t base = chunk_off[ch];
     const uint32_t end = min(base + kPermChunk, d.total);
-    for(uint32_t e = base + threadIdx.x; e < end; e += blockDim.x) {
-        double v = d.alpha * d.src[src_offset(d, e)];
-        if(d.accumulate) v += d.dst[e];
-        d.dst[e] = v;
+    double v[E];
+#pragma unroll
+    for(int i = 0; i < E; i++) {
+        const uint32_t e = min(base + threadIdx.x + 256u * i, end - 1);
+        v[i] = d.src[src_offset(d, e)];
+    }
+    if(d.accumulate) {
+#pragma unroll
+        for(int i = 0; i < E; i++) {
+            const uint32_t e = base + threadIdx.x + 256u * i;
+            if(e < end) d.dst[e] = d.alpha * v[i] + d.dst[e];
+        }
+    } else {
+#pragma unroll
+        for(int i = 0; i < E; i++) {
+            const uint32_t e = base + threadIdx.x + 256u * i;
+            if(e < end) d.dst[e] = d.alpha * v[i];
+        }
     }
 }
 
@@ -300,41 +320,103 @@ static hipError_t launch_gemm_t(bool ta, bool tb, const gemm_cblock *cb,
 }
 
 // ---------------------------------------------------------------------------
-// Direct FP64 MFMA GEMM: one wave per (16 FM) x (16 FN) tile, four waves per
-// workgroup working on four tiles.  Operands go from memory straight into
-// MFMA fragments.  Per chunk of 16 k, lane l (r = l % 16, g = l / 16) loads
-// A[m0 + r][k0 + 4g .. 4g + 3] and B[k0 + 4g .. 4g + 3][n0 + r]; MFMA step j
-// then feeds element j, so fragment slot g of step j carries k = k0 + 4g + j
-// for both operands and the 16 k of the chunk are each summed once.
+// Direct split-K FP64 MFMA GEMM.  A workgroup of W waves owns one
+// (16 FM) x (16 FN) C tile; the k chunks (16 k each) of the tile's whole
+// pair list, taken in list order, are dealt round-robin to the W waves, so
+// a C block with a long contraction list keeps W waves busy instead of one.
+// Operands go from memory straight into MFMA fragments: per chunk, lane l
+// (r = l % 16, g = l / 16) loads A[m0 + r][k0 + 4g .. 4g + 3] and
+// B[k0 + 4g .. 4g + 3][n0 + r]; MFMA step j feeds element j, so fragment
+// slot g of step j carries k = k0 + 4g + j for both operands and each k is
+// summed once.  Loads are branch-free: rows or columns past the block edge
+// read row 0 (their outputs are never stored), k past the pair's extent
+// reads element K - 1 and is replaced by zero.  Each wave double-buffers
+// its chunks (the next chunk is in flight while the current one is on the
+// matrix cores).  The W partial tiles are summed in LDS in wave order
+// 0, 1, ..., W - 1, so results do not depend on timing.
 
-template<bool T>
-__device__ __forceinline__ void load_frag(double (&x)[4], const double *base,
-    bool rowok, int kk, int krem_full, int ld) {
-    // !T: element (row, k) at base[k] (row-major, k contiguous);
-    //  T: element (row, k) at base[k * ld] (k-major).
-    if(krem_full >= 4) {
-#pragma unroll
-        for(int j = 0; j < 4; j++)
-            x[j] = rowok ? (T ? base[size_t(kk + j) * ld] : base[kk + j]) : 0.0;
-    } else {
-#pragma unroll
-        for(int j = 0; j < 4; j++)
-            x[j] = (rowok && j < krem_full) ? (T ? base[size_t(kk + j) * ld] : base[kk + j]) : 0.0;
+typedef const double __attribute__((address_space(1))) gdouble;
+
+//! Advance (p, c) to the first chunk at or after chunk c of pair p.
+__device__ __forceinline__ void seek_chunk(const gemm_pair *__restrict__ pairs,
+    uint32_t end, uint32_t &p, int &c) {
+    while(p < end) {
+        const int nch = (int(pairs[p].k) + 15) >> 4;
+        if(c < nch) break;
+        c -= nch;
+        p++;
     }
 }
 
 template<int FM, int FN, bool TA, bool TB>
-__global__ void __launch_bounds__(256)
+__device__ __forceinline__ void load_chunk(double (&af)[FM][4], double (&bf)[FN][4],
+    const gemm_pair &pr, int c, const int (&arow)[FM], const int (&bcol)[FN], int lg) {
+    const int K = int(pr.k);
+    const int kk = 16 * c + 4 * lg;
+    gdouble *pa = (gdouble *)pr.a;
+    gdouble *pb = (gdouble *)pr.b;
+    int kc[4];
+    bool kv[4];
+#pragma unroll
+    for(int j = 0; j < 4; j++) {
+        kv[j] = kk + j < K;
+        kc[j] = kv[j] ? kk + j : K - 1;
+    }
+#pragma unroll
+    for(int i = 0; i < FM; i++) {
+#pragma unroll
+        for(int j = 0; j < 4; j++) {
+            const size_t off = TA ? size_t(kc[j]) * pr.lda + arow[i]
+                                  : size_t(arow[i]) * pr.lda + kc[j];
+            const double x = pa[off];
+            af[i][j] = kv[j] ? x : 0.0;
+        }
+    }
+#pragma unroll
+    for(int f = 0; f < FN; f++) {
+#pragma unroll
+        for(int j = 0; j < 4; j++) {
+            const size_t off = TB ? size_t(bcol[f]) * pr.ldb + kc[j]
+                                  : size_t(kc[j]) * pr.ldb + bcol[f];
+            const double x = pb[off];
+            bf[f][j] = kv[j] ? x : 0.0;
+        }
+    }
+}
+
+template<int FM, int FN>
+__device__ __forceinline__ void mma_chunk(double4_t (&acc)[FM][FN],
+    double (&af)[FM][4], const double (&bf)[FN][4], double d) {
+#pragma unroll
+    for(int i = 0; i < FM; i++)
+#pragma unroll
+        for(int q = 0; q < 4; q++) af[i][q] *= d;
+#pragma unroll
+    for(int q = 0; q < 4; q++)
+#pragma unroll
+        for(int i = 0; i < FM; i++)
+#pragma unroll
+            for(int j = 0; j < FN; j++)
+                acc[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(
+                    af[i][q], bf[j][q], acc[i][j], 0, 0, 0);
+}
+
+template<int FM, int FN, int W, bool TA, bool TB>
+__global__ void __launch_bounds__(64 * W)
 gemm_direct_kernel(const gemm_cblock *__restrict__ cbs,
     const gemm_pair *__restrict__ pairs,
     const gemm_tile *__restrict__ tiles, uint32_t ntiles) {
 
     constexpr int TM = 16 * FM, TN = 16 * FN;
+    // partial tiles of waves 1..W-1 in fragment order: [w][i][j][v][lane]
+    __shared__ double red[(W > 1 ? W - 1 : 1) * FM * FN * 4 * 64];
     const int lane = threadIdx.x & 63;
-    const int wave = threadIdx.x >> 6;
+    // wave-uniform: pair metadata and chunk bookkeeping stay in scalar
+    // registers and scalar loads
+    const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
     const int lr = lane & 15, lg = lane >> 4;
 
-    for(uint32_t t = blockIdx.x * 4 + wave; t < ntiles; t += gridDim.x * 4) {
+    for(uint32_t t = blockIdx.x; t < ntiles; t += gridDim.x) {
         const gemm_tile tile = tiles[t];
         const gemm_cblock cb = cbs[tile.cblock];
         const int m0 = int(tile.tm) * TM, n0 = int(tile.tn) * TN;
@@ -346,80 +428,102 @@ gemm_direct_kernel(const gemm_cblock *__restrict__ cbs,
 #pragma unroll
             for(int j = 0; j < FN; j++) acc[i][j] = double4_t{0.0, 0.0, 0.0, 0.0};
 
-        bool aok[FM], bok[FN];
+        int arow[FM], bcol[FN];
 #pragma unroll
-        for(int i = 0; i < FM; i++) aok[i] = (16 * i + lr) < mrem;
+        for(int i = 0; i < FM; i++) arow[i] = (16 * i + lr) < mrem ? m0 + 16 * i + lr : 0;
 #pragma unroll
-        for(int j = 0; j < FN; j++) bok[j] = (16 * j + lr) < nrem;
+        for(int j = 0; j < FN; j++) bcol[j] = (16 * j + lr) < nrem ? n0 + 16 * j + lr : 0;
 
-        for(uint32_t p = cb.pair_begin; p < cb.pair_end; p++) {
-            const gemm_pair pr = pairs[p];
-            const int K = int(pr.k);
-            const double d = pr.d;
-            const double *ab[FM];
-            const double *bb[FN];
-#pragma unroll
-            for(int i = 0; i < FM; i++) {
-                const int m = aok[i] ? (m0 + 16 * i + lr) : 0;
-                ab[i] = TA ? (pr.a + m) : (pr.a + size_t(m) * pr.lda);
+        const uint32_t pend = cb.pair_end;
+        uint32_t px = cb.pair_begin;
+        int cx = wave;
+        seek_chunk(pairs, pend, px, cx);
+        if(px < pend) {
+            double ax[FM][4], bx[FN][4], ay[FM][4], by[FN][4];
+            gemm_pair prx = pairs[px];
+            load_chunk<FM, FN, TA, TB>(ax, bx, prx, cx, arow, bcol, lg);
+            while(true) {
+                uint32_t py = px;
+                int cy = cx + W;
+                seek_chunk(pairs, pend, py, cy);
+                const bool hy = py < pend;
+                if(!hy) { py = px; cy = cx; }       // dummy in-bounds load
+                const gemm_pair pry = pairs[py];
+                load_chunk<FM, FN, TA, TB>(ay, by, pry, cy, arow, bcol, lg);
+                mma_chunk<FM, FN>(acc, ax, bx, prx.d);
+                if(!hy) break;
+                uint32_t pz = py;
+                int cz = cy + W;
+                seek_chunk(pairs, pend, pz, cz);
+                const bool hz = pz < pend;
+                if(!hz) { pz = py; cz = cy; }
+                prx = pairs[pz];
+                load_chunk<FM, FN, TA, TB>(ax, bx, prx, cz, arow, bcol, lg);
+                mma_chunk<FM, FN>(acc, ay, by, pry.d);
+                if(!hz) break;
+                px = pz;
+                cx = cz;
             }
-#pragma unroll
-            for(int j = 0; j < FN; j++) {
-                const int n = bok[j] ? (n0 + 16 * j + lr) : 0;
-                bb[j] = TB ? (pr.b + size_t(n) * pr.ldb) : (pr.b + n);
-            }
-            for(int k0 = 0; k0 < K; k0 += 16) {
-                const int kk = k0 + 4 * lg;
-                const int kr = K - kk;
-                double af[FM][4], bf[FN][4];
-#pragma unroll
-                for(int i = 0; i < FM; i++) load_frag<TA>(af[i], ab[i], aok[i], kk, kr, pr.lda);
-#pragma unroll
-                for(int j = 0; j < FN; j++) load_frag<!TB>(bf[j], bb[j], bok[j], kk, kr, pr.ldb);
+        }
+
+        if(W > 1) {
+            if(wave > 0) {
+                double *r = red + (wave - 1) * (FM * FN * 4 * 64);
 #pragma unroll
                 for(int i = 0; i < FM; i++)
 #pragma unroll
-                    for(int q = 0; q < 4; q++) af[i][q] *= d;
+                    for(int j = 0; j < FN; j++)
 #pragma unroll
-                for(int q = 0; q < 4; q++)
+                        for(int v = 0; v < 4; v++)
+                            r[((i * FN + j) * 4 + v) * 64 + lane] = acc[i][j][v];
+            }
+            __syncthreads();
+            if(wave == 0) {
+                for(int w = 1; w < W; w++) {
+                    const double *r = red + (w - 1) * (FM * FN * 4 * 64);
 #pragma unroll
                     for(int i = 0; i < FM; i++)
 #pragma unroll
                         for(int j = 0; j < FN; j++)
-                            acc[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(
-                                af[i][q], bf[j][q], acc[i][j], 0, 0, 0);
+#pragma unroll
+                            for(int v = 0; v < 4; v++)
+                                acc[i][j][v] += r[((i * FN + j) * 4 + v) * 64 + lane];
+                }
             }
         }
 
+        if(wave == 0) {
 #pragma unroll
-        for(int i = 0; i < FM; i++) {
+            for(int i = 0; i < FM; i++) {
 #pragma unroll
-            for(int j = 0; j < FN; j++) {
-                const int col = 16 * j + lr;
+                for(int j = 0; j < FN; j++) {
+                    const int col = 16 * j + lr;
 #pragma unroll
-                for(int v = 0; v < 4; v++) {
-                    const int row = 16 * i + lg + 4 * v;
-                    if(row < mrem && col < nrem) {
-                        double *cp = cb.c + size_t(m0 + row) * cb.ldc + (n0 + col);
-                        double x = acc[i][j][v];
-                        if(!cb.zero) x += *cp;
-                        *cp = x;
+                    for(int v = 0; v < 4; v++) {
+                        const int row = 16 * i + lg + 4 * v;
+                        if(row < mrem && col < nrem) {
+                            double *cp = cb.c + size_t(m0 + row) * cb.ldc + (n0 + col);
+                            double x = acc[i][j][v];
+                            if(!cb.zero) x += *cp;
+                            *cp = x;
+                        }
                     }
                 }
             }
         }
+        if(W > 1) __syncthreads();   // red is reused by the next tile
     }
 }
 
-template<int FM, int FN>
+template<int FM, int FN, int W>
 static hipError_t launch_direct_t(bool ta, bool tb, const gemm_cblock *cb,
     const gemm_pair *pairs, const gemm_tile *tiles, uint32_t ntiles,
     uint32_t nwg, hipStream_t s) {
-    dim3 g((nwg + 3) / 4), b(256);
-    if(!ta && !tb) hipLaunchKernelGGL((gemm_direct_kernel<FM, FN, false, false>), g, b, 0, s, cb, pairs, tiles, ntiles);
-    else if(!ta && tb) hipLaunchKernelGGL((gemm_direct_kernel<FM, FN, false, true>), g, b, 0, s, cb, pairs, tiles, ntiles);
-    else if(ta && !tb) hipLaunchKernelGGL((gemm_direct_kernel<FM, FN, true, false>), g, b, 0, s, cb, pairs, tiles, ntiles);
-    else hipLaunchKernelGGL((gemm_direct_kernel<FM, FN, true, true>), g, b, 0, s, cb, pairs, tiles, ntiles);
+    dim3 g(nwg), b(64 * W);
+    if(!ta && !tb) hipLaunchKernelGGL((gemm_direct_kernel<FM, FN, W, false, false>), g, b, 0, s, cb, pairs, tiles, ntiles);
+    else if(!ta && tb) hipLaunchKernelGGL((gemm_direct_kernel<FM, FN, W, false, true>), g, b, 0, s, cb, pairs, tiles, ntiles);
+    else if(ta && !tb) hipLaunchKernelGGL((gemm_direct_kernel<FM, FN, W, true, false>), g, b, 0, s, cb, pairs, tiles, ntiles);
+    else hipLaunchKernelGGL((gemm_direct_kernel<FM, FN, W, true, true>), g, b, 0, s, cb, pairs, tiles, ntiles);
     return hipGetLastError();
 }
 
@@ -432,10 +536,14 @@ hipError_t launch_gemm(int cfg, bool ta, bool tb, const gemm_cblock *cb,
     case TILE_32x64: return launch_gemm_t<32, 64>(ta, tb, cb, pairs, tiles, ntiles, nwg, s);
     case TILE_64x32: return launch_gemm_t<64, 32>(ta, tb, cb, pairs, tiles, ntiles, nwg, s);
     case TILE_32x32: return launch_gemm_t<32, 32>(ta, tb, cb, pairs, tiles, ntiles, nwg, s);
-    case DIRECT_16x64: return launch_direct_t<1, 4>(ta, tb, cb, pairs, tiles, ntiles, nwg, s);
-    case DIRECT_32x32: return launch_direct_t<2, 2>(ta, tb, cb, pairs, tiles, ntiles, nwg, s);
-    case DIRECT_32x64: return launch_direct_t<2, 4>(ta, tb, cb, pairs, tiles, ntiles, nwg, s);
-    case DIRECT_16x128: return launch_direct_t<1, 8>(ta, tb, cb, pairs, tiles, ntiles, nwg, s);
+    case SPLITK_16x64_W4: return launch_direct_t<1, 4, 4>(ta, tb, cb, pairs, tiles, ntiles, nwg, s);
+    case SPLITK_32x32_W4: return launch_direct_t<2, 2, 4>(ta, tb, cb, pairs, tiles, ntiles, nwg, s);
+    case SPLITK_32x64_W4: return launch_direct_t<2, 4, 4>(ta, tb, cb, pairs, tiles, ntiles, nwg, s);
+    case SPLITK_64x32_W4: return launch_direct_t<4, 2, 4>(ta, tb, cb, pairs, tiles, ntiles, nwg, s);
+    case SPLITK_16x64_W2: return launch_direct_t<1, 4, 2>(ta, tb, cb, pairs, tiles, ntiles, nwg, s);
+    case SPLITK_32x32_W2: return launch_direct_t<2, 2, 2>(ta, tb, cb, pairs, tiles, ntiles, nwg, s);
+    case SPLITK_16x32_W4: return launch_direct_t<1, 2, 4>(ta, tb, cb, pairs, tiles, ntiles, nwg, s);
+    case SPLITK_32x16_W4: return launch_direct_t<2, 1, 4>(ta, tb, cb, pairs, tiles, ntiles, nwg, s);
     default: return launch_gemm_t<64, 64>(ta, tb, cb, pairs, tiles, ntiles, nwg, s);
     }
 }

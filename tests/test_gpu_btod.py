@@ -83,7 +83,7 @@ def test_ladder_medium_against_dense_oracle(gpu_ctx):
     assert np.array_equal(c.to_dense(), first)
 
 
-@pytest.mark.parametrize("tile", list(range(8)))
+@pytest.mark.parametrize("tile", list(range(lt.GEMM_CONFIGS)))
 @pytest.mark.parametrize("name", G.names())
 def test_kernel_configurations(gpu_ctx, name, tile):
     """Every GEMM kernel configuration (LDS-tiled and direct, all tile
@@ -105,6 +105,7 @@ def test_autotuner_records_timings(gpu_ctx):
     c = P.make_result_tensor(gpu_ctx, prob)
     op.perform(c)
     st = op.stats()
-    assert all(t > 0 for t in st["tune_ms"])
-    assert st["tune_ms"][st["tile_cfg"]] == min(st["tune_ms"])
+    tune = st["tune_ms"][:lt.GEMM_CONFIGS]
+    assert all(t > 0 for t in tune)
+    assert tune[st["tile_cfg"]] == min(tune)
     assert O.rel_err(c.to_dense(), result) < TOL
