@@ -56,6 +56,9 @@ typedef struct lt_stats {
     double ozaki_max_rel_err;
     double tune_ms[16];      /* autotuner: grouped GEMM time per kernel
                                configuration (0 = not measured) */
+    int32_t graph;           /* 1: the last perform ran as a HIP graph
+                                (t_gemm_ms is then 0: not separately timed) */
+    int32_t pad;
 } lt_stats;
 
 /* Message of the calling thread's last failed call. */
@@ -196,6 +199,11 @@ int lt_btod_contract2_stats(lt_btod_contract2 *op, lt_stats *out);
  * padding estimate, -2 (default) measured on the plan's own work list when
  * it is built (autotuner). */
 int lt_btod_contract2_set_tile(lt_btod_contract2 *op, int cfg);
+/* Grouped performs (LT_MODE_GROUPED) run as a HIP graph captured once per
+ * execution plan (on != 0, the default) or as eager launches (on == 0).
+ * No counterpart in the reference (gen_bto_contract2_batch_impl.h:273-276
+ * runs the blocks as thread-pool tasks). */
+int lt_btod_contract2_set_graph(lt_btod_contract2 *op, int on);
 /* Split-precision mode: number of INT8 slices (0 = automatic). */
 int lt_btod_contract2_set_ozaki(lt_btod_contract2 *op, int slices);
 

@@ -375,6 +375,9 @@ class btod_contract2(object):
     def set_tile(self, cfg):
         call("lt_btod_contract2_set_tile", self._h, int(cfg))
 
+    def set_graph(self, on):
+        call("lt_btod_contract2_set_graph", self._h, 1 if on else 0)
+
     def set_ozaki(self, slices):
         call("lt_btod_contract2_set_ozaki", self._h, int(slices))
 

@@ -38,6 +38,8 @@ class LtStats(ctypes.Structure):
         ("ozaki_slices", ctypes.c_int32),
         ("ozaki_max_rel_err", ctypes.c_double),
         ("tune_ms", ctypes.c_double * 16),
+        ("graph", ctypes.c_int32),
+        ("pad", ctypes.c_int32),
     ]
 
 
@@ -105,6 +107,7 @@ _SIGS = {
     "lt_btod_contract2_compute_block_host": [vp, ctypes.c_size_t, c_double_p],
     "lt_btod_contract2_stats": [vp, ctypes.POINTER(LtStats)],
     "lt_btod_contract2_set_tile": [vp, ctypes.c_int],
+    "lt_btod_contract2_set_graph": [vp, ctypes.c_int],
     "lt_btod_contract2_set_ozaki": [vp, ctypes.c_int],
 }
 

@@ -826,6 +826,8 @@ int lt_btod_contract2_stats(lt_btod_contract2 *op, lt_stats *out) {
         out->ozaki_slices = s.ozaki_slices;
         out->ozaki_max_rel_err = s.ozaki_max_rel_err;
         for(int i = 0; i < 16; i++) out->tune_ms[i] = i < dev::TILE_COUNT ? s.tune_ms[i] : 0.0;
+        out->graph = s.graph;
+        out->pad = 0;
     });
 }
 
@@ -834,6 +836,13 @@ int lt_btod_contract2_set_tile(lt_btod_contract2 *op, int cfg) {
         need(op, "op");
         lt_require(cfg >= -2 && cfg < dev::TILE_COUNT, LT_ERR_PARAM, "tile configuration");
         op->op->set_tile_cfg(cfg);
+    });
+}
+
+int lt_btod_contract2_set_graph(lt_btod_contract2 *op, int on) {
+    return guard([&]() {
+        need(op, "op");
+        op->op->set_graph(on != 0);
     });
 }
 

@@ -139,13 +139,18 @@ struct contract_plan {
 
     hipEvent_t ev[4];
     bool events;
+    hipGraph_t graph;           // grouped perform, captured on first use
+    hipGraphExec_t gexec;
+    size_t graph_launches;
 
     contract_plan() : va(0), vb(0), c(0), add(false), d(1.0), empty(false),
         ta(false), tb(false), tile_cfg(0), d_prep(0), d_wback(0), d_prep_cd(0),
         d_prep_co(0), d_wb_cd(0), d_wb_co(0), n_prep_chunks(0), n_wb_chunks(0),
         d_cb(0), d_pairs(0), d_tiles_sorted(0), d_tiles_block(0), scratch(0),
-        scratch_elems(0), ctx(0), events(false) { }
+        scratch_elems(0), ctx(0), events(false), graph(0), gexec(0), graph_launches(0) { }
     ~contract_plan() {
+        if(gexec) hipGraphExecDestroy(gexec);
+        if(graph) hipGraphDestroy(graph);
         if(ctx && scratch) ctx->free(scratch);
         if(events) for(int i = 0; i < 4; i++) hipEventDestroy(ev[i]);
     }
@@ -156,7 +161,7 @@ struct contract_plan {
 btod_contract2::btod_contract2(const contraction2 &c, btensor &a, double ka,
     btensor &b, double kb, double kc) :
     m_contr(c), m_a(a), m_b(b), m_ka(ka), m_kb(kb), m_kc(kc),
-    m_tile_cfg(-2), m_tuned_cfg(-1), m_oz_slices(0), m_oz_int8(1) {
+    m_tile_cfg(-2), m_tuned_cfg(-1), m_use_graph(true), m_oz_slices(0), m_oz_int8(1) {
 
     lt_require(c.is_complete(), LT_ERR_PARAM, "contraction is incomplete");
     lt_require(a.bis().order() == c.NA() && b.bis().order() == c.NB(),
@@ -630,36 +635,14 @@ void btod_contract2::build_plan(btensor &c, bool add, double dscale) {
     m_stats.t_plan_ms = now_ms() - t0;
 }
 
-void btod_contract2::run(btensor &c, int mode) {
+void btod_contract2::enqueue(int mode, hipStream_t s, bool timed, size_t &launches) {
     contract_plan &pl = *m_plan;
-    context &ctx = m_a.ctx();
-    ctx.activate();
-    hipStream_t s = ctx.stream();
-    if(!pl.add) {
-        c.zero_all();
-        c.set_symmetry(m_symc);
-    }
-    if(pl.empty) {
-        m_stats.t_gpu_ms = m_stats.t_gemm_ms = 0.0;
-        return;
-    }
-    size_t launches = 0;
-    hip_check(hipEventRecord(pl.ev[0], s), "hipEventRecord");
-    if(pl.add) {
-        // blocks that become non-zero start from zero before accumulation
-        for(size_t i = 0; i < pl.c_out.size(); i++) {
-            if(c.is_nonzero(pl.c_out[i])) continue;
-            hip_check(dev::launch_fill(pl.c_out_mem[i].first, pl.c_out_mem[i].second,
-                0.0, s), "fill");
-            launches++;
-        }
-    }
     if(pl.n_prep_chunks) {
         hip_check(dev::launch_permute(pl.d_prep, pl.d_prep_cd, pl.d_prep_co,
             pl.n_prep_chunks, s), "permute (prepare)");
         launches++;
     }
-    hip_check(hipEventRecord(pl.ev[1], s), "hipEventRecord");
+    if(timed) hip_check(hipEventRecord(pl.ev[1], s), "hipEventRecord");
     if(mode == MODE_PER_BLOCK) {
         for(size_t b = 0; b + 1 < pl.block_tile_begin.size(); b++) {
             uint32_t t0 = pl.block_tile_begin[b], t1 = pl.block_tile_begin[b + 1];
@@ -674,20 +657,74 @@ void btod_contract2::run(btensor &c, int mode) {
             pl.d_tiles_sorted, nt, nt, s), "gemm (grouped)");
         launches++;
     }
-    hip_check(hipEventRecord(pl.ev[2], s), "hipEventRecord");
+    if(timed) hip_check(hipEventRecord(pl.ev[2], s), "hipEventRecord");
     if(pl.n_wb_chunks) {
         hip_check(dev::launch_permute(pl.d_wback, pl.d_wb_cd, pl.d_wb_co,
             pl.n_wb_chunks, s), "permute (write back)");
         launches++;
     }
+}
+
+void btod_contract2::run(btensor &c, int mode) {
+    contract_plan &pl = *m_plan;
+    context &ctx = m_a.ctx();
+    ctx.activate();
+    hipStream_t s = ctx.stream();
+    if(!pl.add) {
+        c.zero_all();
+        c.set_symmetry(m_symc);
+    }
+    if(pl.empty) {
+        m_stats.t_gpu_ms = m_stats.t_gemm_ms = 0.0;
+        m_stats.graph = 0;
+        return;
+    }
+    // The grouped perform runs as a HIP graph (prepare permute, grouped
+    // GEMM, write-back permute: kernel nodes only), captured once per plan.
+    const bool graph = m_use_graph && mode != MODE_PER_BLOCK;
+    size_t launches = 0;
+    if(graph && !pl.gexec) {
+        size_t n = 0;
+        hip_check(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal),
+            "hipStreamBeginCapture");
+        try {
+            enqueue(mode, s, false, n);
+        } catch(...) {
+            hipGraph_t g = 0;
+            hipStreamEndCapture(s, &g);
+            if(g) hipGraphDestroy(g);
+            throw;
+        }
+        hip_check(hipStreamEndCapture(s, &pl.graph), "hipStreamEndCapture");
+        hip_check(hipGraphInstantiate(&pl.gexec, pl.graph, 0, 0, 0), "hipGraphInstantiate");
+        pl.graph_launches = n;
+    }
+    hip_check(hipEventRecord(pl.ev[0], s), "hipEventRecord");
+    if(pl.add) {
+        // blocks that become non-zero start from zero before accumulation
+        for(size_t i = 0; i < pl.c_out.size(); i++) {
+            if(c.is_nonzero(pl.c_out[i])) continue;
+            hip_check(dev::launch_fill(pl.c_out_mem[i].first, pl.c_out_mem[i].second,
+                0.0, s), "fill");
+            launches++;
+        }
+    }
+    if(graph) {
+        hip_check(hipGraphLaunch(pl.gexec, s), "hipGraphLaunch");
+        launches += pl.graph_launches;
+    } else {
+        enqueue(mode, s, true, launches);
+    }
     hip_check(hipEventRecord(pl.ev[3], s), "hipEventRecord");
     hip_check(hipStreamSynchronize(s), "hipStreamSynchronize");
     float ms = 0.f, msg = 0.f;
     hip_check(hipEventElapsedTime(&ms, pl.ev[0], pl.ev[3]), "hipEventElapsedTime");
-    hip_check(hipEventElapsedTime(&msg, pl.ev[1], pl.ev[2]), "hipEventElapsedTime");
+    if(!graph)
+        hip_check(hipEventElapsedTime(&msg, pl.ev[1], pl.ev[2]), "hipEventElapsedTime");
     m_stats.t_gpu_ms = ms;
     m_stats.t_gemm_ms = msg;
     m_stats.n_launches = launches;
+    m_stats.graph = graph ? 1 : 0;
     for(size_t i = 0; i < pl.c_out.size(); i++) c.mark_nonzero(pl.c_out[i]);
 }
 

@@ -149,6 +149,7 @@ struct exec_stats {
     double ozaki_max_rel_err; //!< set when mode == MODE_OZAKI and checked
     int ozaki_slices;
     double tune_ms[dev::TILE_COUNT]; //!< autotuner timings (0 = not measured)
+    int graph;              //!< 1 if the last perform ran as a HIP graph
 };
 
 struct contract_plan;
@@ -177,6 +178,8 @@ public:
 
     const exec_stats &stats() const { return m_stats; }
     void set_tile_cfg(int cfg) { m_tile_cfg = cfg; }
+    //! Grouped performs run as a captured HIP graph (default) or eagerly.
+    void set_graph(bool on) { m_use_graph = on; }
     void set_ozaki(int slices, int use_int8) { m_oz_slices = slices; m_oz_int8 = use_int8; }
     double flops() const { return m_stats.flops; }
 
@@ -189,12 +192,14 @@ private:
     exec_stats m_stats;
     int m_tile_cfg;         //!< >= 0 fixed, -1 estimate, -2 autotune
     int m_tuned_cfg;        //!< configuration found by the autotuner
+    bool m_use_graph;
     int m_oz_slices, m_oz_int8;
     std::unique_ptr<contract_plan> m_plan;
 
     void build_plan(btensor &c, bool add, double d);
     bool plan_valid(btensor &c, bool add, double d, int mode) const;
     void run(btensor &c, int mode);
+    void enqueue(int mode, hipStream_t s, bool timed, size_t &launches);
 };
 
 } // namespace lta

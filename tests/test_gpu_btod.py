@@ -109,3 +109,39 @@ def test_autotuner_records_timings(gpu_ctx):
     assert all(t > 0 for t in tune)
     assert tune[st["tile_cfg"]] == min(tune)
     assert O.rel_err(c.to_dense(), result) < TOL
+
+
+@pytest.mark.parametrize("name", ["ladder_full", "rand_221", "randsym_222", "self_ladder"])
+def test_graph_replays_match_eager(gpu_ctx, name):
+    """The grouped perform runs as a captured HIP graph; every replay, with
+    eager work on the same stream in between, gives the eager result bit for
+    bit (same kernels, same configuration, fixed summation order)."""
+    _say("graph", name)
+    prob, dense, meta, result = G.load(name)
+    a, b, op = P.build(gpu_ctx, prob, dense)
+    c = P.make_result_tensor(gpu_ctx, prob)
+    op.set_graph(False)
+    op.perform(c, lt.LT_MODE_GROUPED)
+    assert op.stats()["graph"] == 0
+    eager = c.to_dense()
+    assert O.rel_err(eager, result) < TOL
+    op.set_graph(True)
+    for rep in range(4):
+        op.perform(c, lt.LT_MODE_GROUPED)
+        assert op.stats()["graph"] == 1
+        assert np.array_equal(c.to_dense(), eager), rep
+        op.perform(c, lt.LT_MODE_PER_BLOCK)        # eager launches in between
+        assert O.rel_err(c.to_dense(), result) < TOL
+    # accumulate through the graph: C = r, then C += 0.5 r twice
+    op.set_graph(False)
+    op.perform(c, lt.LT_MODE_GROUPED)
+    op.perform_add(c, 0.5, lt.LT_MODE_GROUPED)
+    op.perform_add(c, 0.5, lt.LT_MODE_GROUPED)
+    eager_add = c.to_dense()
+    op.set_graph(True)
+    op.perform(c, lt.LT_MODE_GROUPED)
+    op.perform_add(c, 0.5, lt.LT_MODE_GROUPED)
+    op.perform_add(c, 0.5, lt.LT_MODE_GROUPED)
+    assert op.stats()["graph"] == 1
+    assert np.array_equal(c.to_dense(), eager_add)
+    assert O.rel_err(eager_add, 2.0 * result) < TOL
