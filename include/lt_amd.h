@@ -204,7 +204,9 @@ int lt_btod_contract2_set_tile(lt_btod_contract2 *op, int cfg);
  * No counterpart in the reference (gen_bto_contract2_batch_impl.h:273-276
  * runs the blocks as thread-pool tasks). */
 int lt_btod_contract2_set_graph(lt_btod_contract2 *op, int on);
-/* Split-precision mode: number of INT8 slices (0 = automatic). */
+/* Split-precision mode (LT_MODE_OZAKI): number of int8 slices per operand,
+ * 1..8 (0 = 7).  No counterpart in the reference (all of its arithmetic is
+ * FP64 BLAS, linalg_cblas_level3.C:10-60). */
 int lt_btod_contract2_set_ozaki(lt_btod_contract2 *op, int slices);
 
 #ifdef __cplusplus

@@ -85,6 +85,43 @@ hipError_t launch_gemm(int cfg, bool ta, bool tb, const gemm_cblock *cb,
     const gemm_pair *pairs, const gemm_tile *tiles, uint32_t ntiles,
     uint32_t nwg, hipStream_t s);
 
+// ---- split-precision (Ozaki scheme) FP64 emulation on INT8 MFMA ----
+
+//! Splitting of one operand matrix into int8 slices with one power-of-two
+//! scale per row: row r = 2^exps[r] * sum_t slice_t[r][.] * 2^(-7 t),
+//! t = 1..slices.  Element (r, k) of the source is src[r * rs + k * ks];
+//! slice t - 1 is stored at dst + (t - 1) * sstride as rows x kp bytes
+//! (k contiguous, zero beyond k).
+struct oz_split_desc {
+    const double *src;
+    int8_t *dst;
+    int32_t *exps;
+    uint32_t rows, k, kp, slices;
+    uint32_t rs, ks, sstride, pad;
+};
+
+//! One product of a C block in split form: A slices (rows of C) and B
+//! slices (columns of C), both k-contiguous with kp (multiple of 32) bytes
+//! per row.
+struct oz_pair {
+    const int8_t *a;
+    const int8_t *b;
+    const int32_t *ea;
+    const int32_t *eb;
+    uint32_t astride, bstride, kp, pad;
+    double d;
+};
+
+static const uint32_t kOzTileM = 16, kOzTileN = 64;
+
+//! items: nitems pairs (descriptor, row).
+hipError_t launch_oz_split(const oz_split_desc *descs, const uint32_t *items,
+    uint32_t nitems, hipStream_t s);
+//! C tiles of kOzTileM x kOzTileN; products of slices t, u with
+//! t + u <= slices + 1, exact in int32, recombined in FP64.
+hipError_t launch_oz_gemm(const gemm_cblock *cb, const oz_pair *pairs,
+    const gemm_tile *tiles, uint32_t ntiles, int slices, hipStream_t s);
+
 //! Fill n doubles with v.
 hipError_t launch_fill(double *p, size_t n, double v, hipStream_t s);
 

@@ -849,7 +849,7 @@ int lt_btod_contract2_set_graph(lt_btod_contract2 *op, int on) {
 int lt_btod_contract2_set_ozaki(lt_btod_contract2 *op, int slices) {
     return guard([&]() {
         need(op, "op");
-        lt_require(slices >= 0 && slices <= 16, LT_ERR_PARAM, "slices");
+        lt_require(slices >= 0 && slices <= 8, LT_ERR_PARAM, "slices");
         op->op->set_ozaki(slices, 1);
     });
 }

@@ -18,6 +18,8 @@
 //      permuted into the canonical blocks of C.
 #include "lt_runtime.h"
 
+#include <map>
+
 #include <chrono>
 #include <cmath>
 #include <numeric>
@@ -143,11 +145,23 @@ struct contract_plan {
     hipGraphExec_t gexec;
     size_t graph_launches;
 
+    // split-precision form (built on the first perform in that mode)
+    int oz_slices;
+    dev_lists oz_lists, oz_mem;
+    dev::oz_split_desc *d_oz_split;
+    uint32_t *d_oz_items;
+    uint32_t n_oz_items;
+    dev::oz_pair *d_oz_pairs;
+    dev::gemm_tile *d_oz_tiles;
+    uint32_t n_oz_tiles;
+
     contract_plan() : va(0), vb(0), c(0), add(false), d(1.0), empty(false),
         ta(false), tb(false), tile_cfg(0), d_prep(0), d_wback(0), d_prep_cd(0),
         d_prep_co(0), d_wb_cd(0), d_wb_co(0), n_prep_chunks(0), n_wb_chunks(0),
         d_cb(0), d_pairs(0), d_tiles_sorted(0), d_tiles_block(0), scratch(0),
-        scratch_elems(0), ctx(0), events(false), graph(0), gexec(0), graph_launches(0) { }
+        scratch_elems(0), ctx(0), events(false), graph(0), gexec(0), graph_launches(0),
+        oz_slices(0), d_oz_split(0), d_oz_items(0), n_oz_items(0), d_oz_pairs(0),
+        d_oz_tiles(0), n_oz_tiles(0) { }
     ~contract_plan() {
         if(gexec) hipGraphExecDestroy(gexec);
         if(graph) hipGraphDestroy(graph);
@@ -635,6 +649,123 @@ void btod_contract2::build_plan(btensor &c, bool add, double dscale) {
     m_stats.t_plan_ms = now_ms() - t0;
 }
 
+void btod_contract2::build_ozaki(int slices) {
+    contract_plan &pl = *m_plan;
+    context &ctx = m_a.ctx();
+    pl.oz_lists = dev_lists();
+    pl.oz_mem = dev_lists();
+    pl.oz_slices = slices;
+
+    // one split per distinct operand matrix: A-role operands scaled per row
+    // of C, B-role operands per column of C
+    struct op_info { const double *src; uint32_t rows, k, kp, rs, ks; size_t off, eoff; };
+    std::vector<op_info> ops;
+    std::map<const double*, size_t> ida, idb;
+    std::vector<size_t> pa(pl.pairs.size()), pb(pl.pairs.size());
+    size_t bytes = 0, nexp = 0;
+    auto add_op = [&](std::map<const double*, size_t> &idx, const double *src,
+        uint32_t rows, uint32_t k, uint32_t ld, bool kmajor) {
+        std::map<const double*, size_t>::iterator i = idx.find(src);
+        if(i != idx.end()) {
+            lt_require(ops[i->second].rows == rows && ops[i->second].k == k, LT_ERR_INTERNAL,
+                "operand used with two shapes");
+            return i->second;
+        }
+        op_info o;
+        o.src = src; o.rows = rows; o.k = k; o.kp = (k + 31) & ~31u;
+        o.rs = kmajor ? 1 : ld;
+        o.ks = kmajor ? ld : 1;
+        o.off = bytes; o.eoff = nexp;
+        bytes += size_t(slices) * o.rows * o.kp;
+        bytes = (bytes + 255) & ~size_t(255);
+        nexp += rows;
+        ops.push_back(o);
+        idx[src] = ops.size() - 1;
+        return ops.size() - 1;
+    };
+    for(size_t b = 0; b < pl.cblocks.size(); b++) {
+        const dev::gemm_cblock &cb = pl.cblocks[b];
+        for(uint32_t p = cb.pair_begin; p < cb.pair_end; p++) {
+            const dev::gemm_pair &gp = pl.pairs[p];
+            // int32 accumulation of up to `slices` products of K terms of
+            // magnitude <= 127 * 127 must not overflow
+            lt_require(double(gp.k) * 16129.0 * slices < 2147483647.0, LT_ERR_PARAM,
+                "contraction extent too large for the split-precision mode");
+            pa[p] = add_op(ida, gp.a, cb.m, gp.k, gp.lda, pl.ta);
+            pb[p] = add_op(idb, gp.b, cb.n, gp.k, gp.ldb, !pl.tb);
+        }
+    }
+    size_t exp_off = bytes;
+    bytes += nexp * sizeof(int32_t);
+    pl.oz_mem.ctx = &ctx;
+    pl.oz_mem.mem = ctx.alloc(bytes ? bytes : 256);
+    pl.oz_mem.bytes = bytes;
+    int8_t *sl = (int8_t*)pl.oz_mem.mem;
+    int32_t *ex = (int32_t*)(sl + exp_off);
+
+    std::vector<dev::oz_split_desc> descs(ops.size());
+    std::vector<uint32_t> items;
+    for(size_t i = 0; i < ops.size(); i++) {
+        dev::oz_split_desc &d = descs[i];
+        std::memset(&d, 0, sizeof(d));
+        d.src = ops[i].src;
+        d.dst = sl + ops[i].off;
+        d.exps = ex + ops[i].eoff;
+        d.rows = ops[i].rows; d.k = ops[i].k; d.kp = ops[i].kp;
+        d.slices = (uint32_t)slices;
+        d.rs = ops[i].rs; d.ks = ops[i].ks;
+        d.sstride = ops[i].rows * ops[i].kp;
+        for(uint32_t r = 0; r < ops[i].rows; r++) {
+            items.push_back((uint32_t)i);
+            items.push_back(r);
+        }
+    }
+    std::vector<dev::oz_pair> ozp(pl.pairs.size());
+    for(size_t p = 0; p < pl.pairs.size(); p++) {
+        const op_info &a = ops[pa[p]], &b = ops[pb[p]];
+        dev::oz_pair &o = ozp[p];
+        std::memset(&o, 0, sizeof(o));
+        o.a = sl + a.off; o.b = sl + b.off;
+        o.ea = ex + a.eoff; o.eb = ex + b.eoff;
+        o.astride = a.rows * a.kp; o.bstride = b.rows * b.kp;
+        o.kp = a.kp;
+        o.d = pl.pairs[p].d;
+    }
+    std::vector<dev::gemm_tile> tiles;
+    for(size_t b = 0; b < pl.cblocks.size(); b++) {
+        const dev::gemm_cblock &cb = pl.cblocks[b];
+        uint32_t tm = (cb.m + dev::kOzTileM - 1) / dev::kOzTileM;
+        uint32_t tn = (cb.n + dev::kOzTileN - 1) / dev::kOzTileN;
+        lt_require(tm < 65536 && tn < 65536, LT_ERR_PARAM, "C block too large");
+        for(uint32_t i = 0; i < tm; i++) for(uint32_t j = 0; j < tn; j++) {
+            dev::gemm_tile t;
+            t.cblock = (uint32_t)b; t.tm = (uint16_t)i; t.tn = (uint16_t)j;
+            tiles.push_back(t);
+        }
+    }
+    size_t sz = 0;
+    size_t o_d = sz; sz = align256(sz + descs.size() * sizeof(dev::oz_split_desc));
+    size_t o_i = sz; sz = align256(sz + items.size() * 4);
+    size_t o_p = sz; sz = align256(sz + ozp.size() * sizeof(dev::oz_pair));
+    size_t o_t = sz; sz = align256(sz + tiles.size() * sizeof(dev::gemm_tile));
+    std::vector<char> host(sz ? sz : 256, 0);
+    if(!descs.empty()) std::memcpy(&host[o_d], descs.data(), descs.size() * sizeof(dev::oz_split_desc));
+    if(!items.empty()) std::memcpy(&host[o_i], items.data(), items.size() * 4);
+    if(!ozp.empty()) std::memcpy(&host[o_p], ozp.data(), ozp.size() * sizeof(dev::oz_pair));
+    if(!tiles.empty()) std::memcpy(&host[o_t], tiles.data(), tiles.size() * sizeof(dev::gemm_tile));
+    pl.oz_lists.ctx = &ctx;
+    pl.oz_lists.mem = ctx.alloc(host.size());
+    pl.oz_lists.bytes = host.size();
+    ctx.upload(pl.oz_lists.mem, host.data(), host.size());
+    char *base = (char*)pl.oz_lists.mem;
+    pl.d_oz_split = (dev::oz_split_desc*)(base + o_d);
+    pl.d_oz_items = (uint32_t*)(base + o_i);
+    pl.n_oz_items = (uint32_t)(items.size() / 2);
+    pl.d_oz_pairs = (dev::oz_pair*)(base + o_p);
+    pl.d_oz_tiles = (dev::gemm_tile*)(base + o_t);
+    pl.n_oz_tiles = (uint32_t)tiles.size();
+}
+
 void btod_contract2::enqueue(int mode, hipStream_t s, bool timed, size_t &launches) {
     contract_plan &pl = *m_plan;
     if(pl.n_prep_chunks) {
@@ -643,7 +774,13 @@ void btod_contract2::enqueue(int mode, hipStream_t s, bool timed, size_t &launch
         launches++;
     }
     if(timed) hip_check(hipEventRecord(pl.ev[1], s), "hipEventRecord");
-    if(mode == MODE_PER_BLOCK) {
+    if(mode == MODE_OZAKI) {
+        hip_check(dev::launch_oz_split(pl.d_oz_split, pl.d_oz_items, pl.n_oz_items, s),
+            "split (ozaki)");
+        hip_check(dev::launch_oz_gemm(pl.d_cb, pl.d_oz_pairs, pl.d_oz_tiles, pl.n_oz_tiles,
+            pl.oz_slices, s), "gemm (ozaki)");
+        launches += 2;
+    } else if(mode == MODE_PER_BLOCK) {
         for(size_t b = 0; b + 1 < pl.block_tile_begin.size(); b++) {
             uint32_t t0 = pl.block_tile_begin[b], t1 = pl.block_tile_begin[b + 1];
             if(t1 == t0) continue;
@@ -679,9 +816,14 @@ void btod_contract2::run(btensor &c, int mode) {
         m_stats.graph = 0;
         return;
     }
+    if(mode == MODE_OZAKI) {
+        int sl = m_oz_slices > 0 ? m_oz_slices : 7;
+        if(pl.oz_slices != sl) build_ozaki(sl);
+        m_stats.ozaki_slices = sl;
+    }
     // The grouped perform runs as a HIP graph (prepare permute, grouped
     // GEMM, write-back permute: kernel nodes only), captured once per plan.
-    const bool graph = m_use_graph && mode != MODE_PER_BLOCK;
+    const bool graph = m_use_graph && mode == MODE_GROUPED;
     size_t launches = 0;
     if(graph && !pl.gexec) {
         size_t n = 0;

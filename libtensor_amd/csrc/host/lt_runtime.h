@@ -200,6 +200,7 @@ private:
     bool plan_valid(btensor &c, bool add, double d, int mode) const;
     void run(btensor &c, int mode);
     void enqueue(int mode, hipStream_t s, bool timed, size_t &launches);
+    void build_ozaki(int slices);
 };
 
 } // namespace lta

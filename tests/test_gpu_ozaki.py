@@ -1,0 +1,73 @@
+"""Split-precision mode (LT_MODE_OZAKI): FP64 emulated with int8 slices on
+INT8 MFMA, checked against the FP64 reference results.
+
+* exactness where the slices hold the inputs exactly (small integers),
+* error decreasing with the slice count and <= 1e-12 (max|dC|/max|C|) at
+  7 slices on every golden case,
+* per-block errors reported for the pp-ladder shape at reduced size."""
+import sys
+
+import numpy as np
+import pytest
+
+import dense_oracle as O
+import golden_cases as G
+import libtensor_amd as lt
+import problems as P
+
+pytestmark = pytest.mark.gpu
+
+TOL = 1e-12
+
+
+def _say(*a):
+    print(*a)
+    sys.stdout.flush()
+
+
+def test_small_integers_exact(gpu_ctx):
+    """Integers |x| <= 60 are 7-bit multiples of their row's power-of-two
+    scale, so one slice holds them exactly: every slice count gives the
+    exact integer product (this also pins the MFMA operand and result
+    lane layouts)."""
+    prob = P.prob_ring(4, 12, so=(2,), sv=(5,))
+    rng = np.random.default_rng(3)
+    da = tuple(prob["A"]["dims"])
+    db = tuple(prob["B"]["dims"])
+    dense = {"A": rng.integers(-60, 61, size=da).astype(float),
+             "B": rng.integers(-60, 61, size=db).astype(float)}
+    a, b, op = P.build(gpu_ctx, prob, dense)
+    c = P.make_result_tensor(gpu_ctx, prob)
+    op.perform(c, lt.LT_MODE_GROUPED)
+    exact = c.to_dense()
+    for s in (1, 2, 7):
+        op.set_ozaki(s)
+        op.perform(c, lt.LT_MODE_OZAKI)
+        assert op.stats()["ozaki_slices"] == s
+        assert np.array_equal(c.to_dense(), exact), s
+
+
+@pytest.mark.parametrize("name", G.names())
+def test_golden_cases_error_vs_slices(gpu_ctx, name):
+    prob, dense, meta, result = G.load(name)
+    a, b, op = P.build(gpu_ctx, prob, dense)
+    errs = {}
+    for s in (3, 5, 7):
+        op.set_ozaki(s)
+        c = P.make_result_tensor(gpu_ctx, prob)
+        op.perform(c, lt.LT_MODE_OZAKI)
+        assert c.req_nonzero_blocks() == meta["schedule"]
+        errs[s] = O.rel_err(c.to_dense(), result)
+    _say("ozaki", name, errs)
+    assert errs[7] < TOL
+    assert errs[5] <= max(errs[3], 1e-15) and errs[7] <= max(errs[5], 1e-15)
+
+
+def test_perform_add(gpu_ctx):
+    prob, dense, meta, result = G.load("ladder_full")
+    a, b, op = P.build(gpu_ctx, prob, dense)
+    op.set_ozaki(7)
+    c = P.make_result_tensor(gpu_ctx, prob)
+    op.perform(c, lt.LT_MODE_OZAKI)
+    op.perform_add(c, -0.5, lt.LT_MODE_OZAKI)
+    assert O.rel_err(c.to_dense(), 0.5 * result) < TOL
