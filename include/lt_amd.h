@@ -209,6 +209,38 @@ int lt_btod_contract2_set_graph(lt_btod_contract2 *op, int on);
  * FP64 BLAS, linalg_cblas_level3.C:10-60). */
 int lt_btod_contract2_set_ozaki(lt_btod_contract2 *op, int slices);
 
+/* ---- request loop: many calculations sharing one GPU.  No counterpart in
+ *      the reference (its operations run synchronously in the caller's
+ *      thread, libutil/thread_pool/thread_pool.h:22).  Block tensors stay
+ *      resident; requests are admitted by free device memory (estimate of
+ *      a plan's footprint, hbm_reserve kept free), planned on a host thread
+ *      while the GPU runs the previous batch, and all ready requests run as
+ *      one batch (one grouped GEMM launch over the union of their work
+ *      lists per operand layout and kernel configuration).  Requests that
+ *      arrive meanwhile join the next batch. ---- */
+typedef struct lt_server lt_server;
+typedef struct lt_request_stats {
+    double t_queue_ms;      /* submit -> start of its batch */
+    double t_run_ms;        /* start -> end of its batch */
+    double t_total_ms;      /* submit -> result ready (latency) */
+    uint64_t batch;         /* batch number, 1-based */
+    uint64_t batch_size;    /* requests in that batch */
+    int32_t status;         /* LT_OK or the request's error code */
+    int32_t pad;
+} lt_request_stats;
+int lt_server_create(lt_ctx *ctx, size_t max_batch, size_t hbm_reserve, lt_server **out);
+/* Drains every submitted request, then stops. */
+int lt_server_destroy(lt_server *srv);
+/* perform(btc) (add == 0) or perform(btc, d) (add != 0), grouped mode.  One
+ * request in flight per operation and per result tensor; neither may be
+ * used by the caller until lt_server_wait returns. */
+int lt_server_submit(lt_server *srv, lt_btod_contract2 *op, lt_btensor *c, int add,
+    double d, uint64_t *id);
+/* Blocks until the request is done; returns its status. */
+int lt_server_wait(lt_server *srv, uint64_t id, lt_request_stats *st);
+int lt_server_stats(lt_server *srv, uint64_t *batches, uint64_t *requests,
+    double *mean_batch, uint64_t *max_batch);
+
 #ifdef __cplusplus
 }
 #endif

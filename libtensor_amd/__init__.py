@@ -12,11 +12,11 @@ import numpy as np
 
 from . import _capi
 from ._capi import (GEMM_CONFIGS, LT_MODE_GROUPED, LT_MODE_OZAKI, LT_MODE_PER_BLOCK, LtError,
-                    LtStats, call, dptr, ints, sizes)
+                    LtRequestStats, LtStats, call, dptr, ints, sizes)
 
 __all__ = ["Context", "product_table", "block_index_space", "block_tensor",
            "se_perm", "se_part", "se_label", "contraction2", "btod_contract2",
-           "device_array", "tod_copy", "tod_set", "tod_contract2", "tod_dotprod",
+           "device_array", "tod_copy", "tod_set", "tod_contract2", "tod_dotprod", "server",
            "LtError", "GEMM_CONFIGS", "LT_MODE_PER_BLOCK", "LT_MODE_GROUPED", "LT_MODE_OZAKI"]
 
 
@@ -450,3 +450,45 @@ def tod_dotprod(a, b, perma=None, permb=None, c=1.0):
     call("lt_tod_dotprod", a.ctx._h, len(a.shape), sizes(a.shape), a.ptr, pa,
          sizes(b.shape), b.ptr, pb, float(c), ctypes.byref(r))
     return r.value
+
+
+class server(object):
+    """Request loop on one context: submit(op, btc[, d]) returns a ticket,
+    wait(ticket) the request's latency figures (continuous batching of
+    btod_contract2 performs from many calculations)."""
+
+    def __init__(self, ctx, max_batch=64, hbm_reserve=0):
+        h = ctypes.c_void_p()
+        call("lt_server_create", ctx._h, int(max_batch), int(hbm_reserve), ctypes.byref(h))
+        self._h = h
+        self.ctx = ctx
+
+    def close(self):
+        if self._h:
+            call("lt_server_destroy", self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def submit(self, op, btc, add=False, d=1.0):
+        t = ctypes.c_uint64()
+        call("lt_server_submit", self._h, op._h, btc._h, 1 if add else 0, float(d),
+             ctypes.byref(t))
+        return t.value
+
+    def wait(self, ticket):
+        st = LtRequestStats()
+        call("lt_server_wait", self._h, ctypes.c_uint64(ticket), ctypes.byref(st))
+        return {f: getattr(st, f) for f, _ in LtRequestStats._fields_}
+
+    def stats(self):
+        b, r, mb = ctypes.c_uint64(), ctypes.c_uint64(), ctypes.c_uint64()
+        mean = ctypes.c_double()
+        call("lt_server_stats", self._h, ctypes.byref(b), ctypes.byref(r), ctypes.byref(mean),
+             ctypes.byref(mb))
+        return {"batches": b.value, "requests": r.value, "mean_batch": mean.value,
+                "max_batch": mb.value}

@@ -43,6 +43,18 @@ class LtStats(ctypes.Structure):
     ]
 
 
+class LtRequestStats(ctypes.Structure):
+    _fields_ = [
+        ("t_queue_ms", ctypes.c_double),
+        ("t_run_ms", ctypes.c_double),
+        ("t_total_ms", ctypes.c_double),
+        ("batch", ctypes.c_uint64),
+        ("batch_size", ctypes.c_uint64),
+        ("status", ctypes.c_int32),
+        ("pad", ctypes.c_int32),
+    ]
+
+
 # name -> (argtypes)
 _SIGS = {
     "lt_version": [],
@@ -109,6 +121,13 @@ _SIGS = {
     "lt_btod_contract2_set_tile": [vp, ctypes.c_int],
     "lt_btod_contract2_set_graph": [vp, ctypes.c_int],
     "lt_btod_contract2_set_ozaki": [vp, ctypes.c_int],
+    "lt_server_create": [vp, ctypes.c_size_t, ctypes.c_size_t, ctypes.POINTER(vp)],
+    "lt_server_destroy": [vp],
+    "lt_server_submit": [vp, vp, vp, ctypes.c_int, ctypes.c_double,
+                         ctypes.POINTER(ctypes.c_uint64)],
+    "lt_server_wait": [vp, ctypes.c_uint64, ctypes.POINTER(LtRequestStats)],
+    "lt_server_stats": [vp, ctypes.POINTER(ctypes.c_uint64), ctypes.POINTER(ctypes.c_uint64),
+                        c_double_p, ctypes.POINTER(ctypes.c_uint64)],
 }
 
 _OPTIONAL = {}

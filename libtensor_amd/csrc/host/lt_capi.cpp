@@ -1,6 +1,7 @@
 // C ABI of the block-contraction path (declarations in include/lt_amd.h).
 #include "../../../include/lt_amd.h"
 #include "lt_runtime.h"
+#include "lt_server.h"
 
 #include <atomic>
 #include <cstring>
@@ -17,6 +18,7 @@ struct lt_ctx {
 struct lt_btensor { btensor *t; lt_ctx *ctx; };
 struct lt_symel { std::shared_ptr<sym_element> e; };
 struct lt_btod_contract2 { btod_contract2 *op; lt_btensor *a, *b; lt_ctx *ctx; };
+struct lt_server { server *s; lt_ctx *ctx; };
 
 static void ctx_retain(lt_ctx *x) { x->refs++; }
 static void ctx_release(lt_ctx *x) {
@@ -851,6 +853,78 @@ int lt_btod_contract2_set_ozaki(lt_btod_contract2 *op, int slices) {
         need(op, "op");
         lt_require(slices >= 0 && slices <= 8, LT_ERR_PARAM, "slices");
         op->op->set_ozaki(slices, 1);
+    });
+}
+
+// ---- request loop ----
+
+int lt_server_create(lt_ctx *ctx, size_t max_batch, size_t hbm_reserve, lt_server **out) {
+    return guard([&]() {
+        need(ctx, "ctx"); need(out, "out");
+        lt_server *x = new lt_server;
+        try {
+            x->s = new server(*ctx->c, max_batch, hbm_reserve);
+        } catch(...) {
+            delete x;
+            throw;
+        }
+        x->ctx = ctx;
+        ctx_retain(ctx);
+        *out = x;
+    });
+}
+
+int lt_server_destroy(lt_server *srv) {
+    return guard([&]() {
+        if(!srv) return;
+        lt_ctx *ctx = srv->ctx;
+        delete srv->s;
+        delete srv;
+        ctx_release(ctx);
+    });
+}
+
+int lt_server_submit(lt_server *srv, lt_btod_contract2 *op, lt_btensor *c, int add,
+    double d, uint64_t *id) {
+    return guard([&]() {
+        need(srv, "srv"); need(op, "op"); need(c, "c"); need(id, "id");
+        *id = srv->s->submit(op->op, c->t, add != 0, d);
+    });
+}
+
+int lt_server_wait(lt_server *srv, uint64_t id, lt_request_stats *st) {
+    request_stats r;
+    int rc = guard([&]() {
+        need(srv, "srv");
+        srv->s->wait(id, r);
+        if(st) {
+            st->t_queue_ms = r.t_queue_ms;
+            st->t_run_ms = r.t_run_ms;
+            st->t_total_ms = r.t_total_ms;
+            st->batch = r.batch;
+            st->batch_size = r.batch_size;
+            st->status = r.status;
+            st->pad = 0;
+        }
+    });
+    if(rc != LT_OK) return rc;
+    if(r.status != LT_OK) {
+        g_err = r.message;
+        return r.status;
+    }
+    return LT_OK;
+}
+
+int lt_server_stats(lt_server *srv, uint64_t *batches, uint64_t *requests,
+    double *mean_batch, uint64_t *max_batch) {
+    return guard([&]() {
+        need(srv, "srv");
+        server_stats t;
+        srv->s->totals(t);
+        if(batches) *batches = t.batches;
+        if(requests) *requests = t.requests;
+        if(mean_batch) *mean_batch = t.mean_batch;
+        if(max_batch) *max_batch = t.max_batch;
     });
 }
 

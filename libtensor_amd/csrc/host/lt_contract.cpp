@@ -122,6 +122,7 @@ struct contract_plan {
     std::vector<dev::gemm_pair> pairs;
     std::vector<dev::gemm_tile> tiles_sorted, tiles_by_block;
     std::vector<uint32_t> block_tile_begin;
+    std::vector<double> cb_ksum;                      // sum of k over a C block's pairs
     std::vector<size_t> c_out;                        // canonical C blocks written
     std::vector<std::pair<double*, size_t> > c_out_mem; // their storage
     bool ta, tb;
@@ -461,6 +462,8 @@ void btod_contract2::build_plan(btensor &c, bool add, double dscale) {
         pl.c_out.push_back(cdst[t].yc);
         pl.c_out_mem.push_back(std::make_pair(c.block_ptr(cdst[t].yc), c.block_size(cdst[t].yc)));
     }
+
+    pl.cb_ksum = cb_ksum;
 
     // Tile lists of one kernel configuration: tiles grouped by C block (one
     // launch per block) and the same tiles longest-first (grouped launch).
@@ -868,6 +871,174 @@ void btod_contract2::run(btensor &c, int mode) {
     m_stats.n_launches = launches;
     m_stats.graph = graph ? 1 : 0;
     for(size_t i = 0; i < pl.c_out.size(); i++) c.mark_nonzero(pl.c_out[i]);
+}
+
+void btod_contract2::perform_batch(std::vector<batch_item> &items, dev_buffer &merged) {
+    if(items.empty()) return;
+    context &ctx = items[0].op->m_a.ctx();
+    ctx.require_device("btod_contract2::perform_batch");
+    ctx.activate();
+    hipStream_t s = ctx.stream();
+
+    // plans (cached per operation), result preparation, prepare permutes
+    std::vector<contract_plan*> plans(items.size(), (contract_plan*)0);
+    for(size_t i = 0; i < items.size(); i++) {
+        batch_item &it = items[i];
+        btod_contract2 &op = *it.op;
+        lt_require(&op.m_a.ctx() == &ctx, LT_ERR_PARAM, "batch spans contexts");
+        lt_require(it.c->bis() == op.m_symc.bis, LT_ERR_DIMS,
+            "result tensor has a different block index space");
+        if(it.add) {
+            for(size_t j = 0; j < op.m_sched.size(); j++) {
+                short_orbit so(it.c->sym(), op.m_sched[j], false);
+                lt_require(so.acidx == op.m_sched[j], LT_ERR_SYMMETRY,
+                    "perform_add: result symmetry differs from the contraction's");
+            }
+        }
+        if(!op.plan_valid(*it.c, it.add, it.d, MODE_GROUPED))
+            op.build_plan(*it.c, it.add, it.d);
+        plans[i] = op.m_plan.get();
+    }
+    hip_check(hipEventRecord(plans[0]->ev[0], s), "hipEventRecord");
+    for(size_t i = 0; i < items.size(); i++) {
+        contract_plan &pl = *plans[i];
+        btensor &c = *items[i].c;
+        if(!pl.add) {
+            c.zero_all();
+            c.set_symmetry(items[i].op->m_symc);
+        } else {
+            for(size_t j = 0; j < pl.c_out.size(); j++) {
+                if(c.is_nonzero(pl.c_out[j])) continue;
+                hip_check(dev::launch_fill(pl.c_out_mem[j].first, pl.c_out_mem[j].second,
+                    0.0, s), "fill");
+            }
+        }
+        if(!pl.empty && pl.n_prep_chunks)
+            hip_check(dev::launch_permute(pl.d_prep, pl.d_prep_cd, pl.d_prep_co,
+                pl.n_prep_chunks, s), "permute (prepare)");
+    }
+
+    // one grouped GEMM per (layout, kernel configuration): the work lists of
+    // the plans are concatenated (C block and pair indexes shifted) and the
+    // tiles ordered longest first across all requests
+    std::map<std::pair<int, int>, std::vector<size_t> > groups;
+    for(size_t i = 0; i < items.size(); i++) {
+        if(plans[i]->empty) continue;
+        int key = (plans[i]->ta ? 2 : 0) + (plans[i]->tb ? 1 : 0);
+        groups[std::make_pair(key, plans[i]->tile_cfg)].push_back(i);
+    }
+    std::vector<dev::gemm_cblock> cbs;
+    std::vector<dev::gemm_pair> prs;
+    std::vector<std::pair<double, dev::gemm_tile> > tl;
+    std::vector<std::pair<size_t, size_t> > spans;     // (cblock offset, tile offset) per group
+    std::vector<char> host;
+    struct launch { int cfg; bool ta, tb; size_t ocb, opr, otl; uint32_t nt; };
+    std::vector<launch> launches;
+    size_t sz = 0;
+    std::vector<std::vector<char> > parts;
+    for(std::map<std::pair<int, int>, std::vector<size_t> >::iterator g = groups.begin();
+        g != groups.end(); ++g) {
+        cbs.clear(); prs.clear(); tl.clear();
+        for(size_t q = 0; q < g->second.size(); q++) {
+            contract_plan &pl = *plans[g->second[q]];
+            uint32_t cb0 = (uint32_t)cbs.size(), pr0 = (uint32_t)prs.size();
+            for(size_t b = 0; b < pl.cblocks.size(); b++) {
+                dev::gemm_cblock cb = pl.cblocks[b];
+                cb.pair_begin += pr0;
+                cb.pair_end += pr0;
+                cbs.push_back(cb);
+            }
+            prs.insert(prs.end(), pl.pairs.begin(), pl.pairs.end());
+            for(size_t t = 0; t < pl.tiles_sorted.size(); t++) {
+                dev::gemm_tile x = pl.tiles_sorted[t];
+                double cost = pl.cb_ksum[x.cblock];
+                x.cblock += cb0;
+                tl.push_back(std::make_pair(cost, x));
+            }
+        }
+        std::stable_sort(tl.begin(), tl.end(),
+            [](const std::pair<double, dev::gemm_tile> &x, const std::pair<double, dev::gemm_tile> &y) {
+                return x.first > y.first; });
+        launch L;
+        L.cfg = g->first.second;
+        L.ta = (g->first.first & 2) != 0;
+        L.tb = (g->first.first & 1) != 0;
+        L.ocb = sz; sz = align256(sz + cbs.size() * sizeof(dev::gemm_cblock));
+        L.opr = sz; sz = align256(sz + prs.size() * sizeof(dev::gemm_pair));
+        L.otl = sz; sz = align256(sz + tl.size() * sizeof(dev::gemm_tile));
+        L.nt = (uint32_t)tl.size();
+        host.resize(sz, 0);
+        if(!cbs.empty()) std::memcpy(&host[L.ocb], cbs.data(), cbs.size() * sizeof(dev::gemm_cblock));
+        if(!prs.empty()) std::memcpy(&host[L.opr], prs.data(), prs.size() * sizeof(dev::gemm_pair));
+        for(size_t t = 0; t < tl.size(); t++)
+            std::memcpy(&host[L.otl + t * sizeof(dev::gemm_tile)], &tl[t].second, sizeof(dev::gemm_tile));
+        launches.push_back(L);
+    }
+    if(sz) {
+        if(merged.bytes < sz) {
+            merged.release();
+            merged.ctx = &ctx;
+            merged.mem = ctx.alloc(sz);
+            merged.bytes = sz;
+        }
+        hip_check(hipMemcpyAsync(merged.mem, host.data(), sz, hipMemcpyHostToDevice, s),
+            "hipMemcpy H2D");
+    }
+    char *base = (char*)merged.mem;
+    hip_check(hipEventRecord(plans[0]->ev[1], s), "hipEventRecord");
+    for(size_t l = 0; l < launches.size(); l++) {
+        const launch &L = launches[l];
+        hip_check(dev::launch_gemm(L.cfg, L.ta, L.tb, (dev::gemm_cblock*)(base + L.ocb),
+            (dev::gemm_pair*)(base + L.opr), (dev::gemm_tile*)(base + L.otl), L.nt, L.nt, s),
+            "gemm (batch)");
+    }
+    hip_check(hipEventRecord(plans[0]->ev[2], s), "hipEventRecord");
+    for(size_t i = 0; i < items.size(); i++) {
+        contract_plan &pl = *plans[i];
+        if(!pl.empty && pl.n_wb_chunks)
+            hip_check(dev::launch_permute(pl.d_wback, pl.d_wb_cd, pl.d_wb_co,
+                pl.n_wb_chunks, s), "permute (write back)");
+    }
+    hip_check(hipEventRecord(plans[0]->ev[3], s), "hipEventRecord");
+    hip_check(hipStreamSynchronize(s), "hipStreamSynchronize");
+    float ms = 0.f, msg = 0.f;
+    hip_check(hipEventElapsedTime(&ms, plans[0]->ev[0], plans[0]->ev[3]), "hipEventElapsedTime");
+    hip_check(hipEventElapsedTime(&msg, plans[0]->ev[1], plans[0]->ev[2]), "hipEventElapsedTime");
+    for(size_t i = 0; i < items.size(); i++) {
+        contract_plan &pl = *plans[i];
+        for(size_t j = 0; j < pl.c_out.size(); j++) items[i].c->mark_nonzero(pl.c_out[j]);
+        exec_stats &st = items[i].op->m_stats;
+        st.t_gpu_ms = ms;
+        st.t_gemm_ms = msg;
+        st.n_launches = launches.size();
+        st.graph = 0;
+        items[i].batch_gpu_ms = ms;
+        items[i].batch_gemm_launches = launches.size();
+    }
+}
+
+void btod_contract2::prepare(btensor &c, bool add, double d) {
+    m_a.ctx().require_device("btod_contract2::prepare");
+    lt_require(c.bis() == m_symc.bis, LT_ERR_DIMS, "result tensor has a different block index space");
+    if(add) {
+        for(size_t i = 0; i < m_sched.size(); i++) {
+            short_orbit so(c.sym(), m_sched[i], false);
+            lt_require(so.acidx == m_sched[i], LT_ERR_SYMMETRY,
+                "perform_add: result symmetry differs from the contraction's");
+        }
+    }
+    if(!plan_valid(c, add, d, MODE_GROUPED)) build_plan(c, add, d);
+}
+
+size_t btod_contract2::footprint_estimate() {
+    size_t n = 0;
+    std::vector<size_t> nza = m_a.nonzero_blocks(), nzb = m_b.nonzero_blocks();
+    for(size_t i = 0; i < nza.size(); i++) n += 4 * m_a.block_size(nza[i]);
+    for(size_t i = 0; i < nzb.size(); i++) n += 4 * m_b.block_size(nzb[i]);
+    dims_t bidc = m_symc.bis.block_index_dims();
+    for(size_t i = 0; i < m_sched.size(); i++)
+        n += m_symc.bis.block_dims(get_index(m_sched[i], bidc)).size();
+    return n * sizeof(double) + (size_t(1) << 20);
 }
 
 void btod_contract2::perform(btensor &c, int mode) {

@@ -153,6 +153,27 @@ struct exec_stats {
 };
 
 struct contract_plan;
+class btod_contract2;
+
+//! Device buffer owned by a caller and reused across calls.
+struct dev_buffer {
+    context *ctx;
+    void *mem;
+    size_t bytes;
+    dev_buffer() : ctx(0), mem(0), bytes(0) { }
+    ~dev_buffer() { release(); }
+    void release() { if(ctx && mem) ctx->free(mem); mem = 0; bytes = 0; }
+};
+
+//! One perform of a batch (btod_contract2::perform_batch).
+struct batch_item {
+    btod_contract2 *op;
+    btensor *c;
+    bool add;                 //!< false: C := result; true: C += d * result
+    double d;
+    double batch_gpu_ms;      //!< out: device time of the whole batch
+    size_t batch_gemm_launches;
+};
 
 //! btod_contract2<N, M, K> (libtensor/block_tensor/btod_contract2.h:29):
 //! C = kc * contract(ka * A, kb * B).
@@ -175,6 +196,20 @@ public:
     void compute_block_host(const index_t &ic, double *out);
     //! Contraction list of a C block in the original index space.
     void clst(const index_t &ic, contr_list &out);
+    //! Runs several performs (grouped mode) as one batch on their common
+    //! context: every prepare permute, then one grouped GEMM over the union
+    //! of the work lists per operand layout and kernel configuration, then
+    //! every write-back.  `merged` holds the concatenated lists.
+    static void perform_batch(std::vector<batch_item> &items, dev_buffer &merged);
+    //! Builds the execution plan for C unless it is cached (device memory
+    //! for work lists and operand scratch, kernel autotuning).
+    void prepare(btensor &c, bool add, double d);
+    bool plan_resident(btensor &c, bool add, double d) const {
+        return plan_valid(c, add, d, MODE_GROUPED);
+    }
+    //! Device memory a plan may take: four orientations of every stored A
+    //! and B block plus the scheduled C blocks (admission estimate).
+    size_t footprint_estimate();
 
     const exec_stats &stats() const { return m_stats; }
     void set_tile_cfg(int cfg) { m_tile_cfg = cfg; }

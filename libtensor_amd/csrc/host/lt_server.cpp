@@ -1,0 +1,225 @@
+// Request loop for many concurrent calculations on one GPU: block tensors stay
+// resident in HBM, contractions are admitted by free device memory, planned
+// on a host thread while the GPU runs the previous batch, and every batch
+// runs all ready requests together (btod_contract2::perform_batch: one
+// grouped GEMM launch over the union of their work lists).  Requests that
+// arrive while a batch is on the GPU join the next batch (continuous
+// batching at perform granularity).
+#include "lt_server.h"
+
+#include <algorithm>
+#include <chrono>
+
+namespace lta {
+
+namespace {
+
+double now_ms() {
+    using namespace std::chrono;
+    return duration<double, std::milli>(steady_clock::now().time_since_epoch()).count();
+}
+
+} // unnamed namespace
+
+server::server(context &ctx, size_t max_batch, size_t hbm_reserve) :
+    m_ctx(ctx), m_max_batch(max_batch ? max_batch : 64), m_reserve(hbm_reserve),
+    m_stop(false), m_next(1), m_batches(0), m_done(0), m_batched_requests(0),
+    m_max_seen(0), m_rejected(0) {
+    m_ctx.require_device("server");
+    m_planner = std::thread([this]() { planner_loop(); });
+    m_executor = std::thread([this]() { executor_loop(); });
+}
+
+server::~server() {
+    {
+        std::lock_guard<std::mutex> lk(m_mtx);
+        m_stop = true;
+    }
+    m_cv.notify_all();
+    if(m_planner.joinable()) m_planner.join();
+    if(m_executor.joinable()) m_executor.join();
+}
+
+uint64_t server::submit(btod_contract2 *op, btensor *c, bool add, double d) {
+    lt_require(op && c, LT_ERR_PARAM, "null request");
+    std::lock_guard<std::mutex> lk(m_mtx);
+    lt_require(!m_stop, LT_ERR_PARAM, "server is stopping");
+    for(std::map<uint64_t, request>::iterator i = m_reqs.begin(); i != m_reqs.end(); ++i) {
+        if(i->second.state == DONE) continue;
+        lt_require(i->second.op != op && i->second.c != c, LT_ERR_PARAM,
+            "operation or result tensor already has a request in flight");
+    }
+    request r;
+    r.op = op; r.c = c; r.add = add; r.d = d;
+    r.state = QUEUED;
+    r.status = LT_OK;
+    r.t_submit = now_ms();
+    r.t_start = r.t_end = 0.0;
+    r.batch = 0; r.batch_size = 0;
+    uint64_t id = m_next++;
+    m_reqs[id] = r;
+    m_queue.push_back(id);
+    m_cv.notify_all();
+    return id;
+}
+
+void server::wait(uint64_t id, request_stats &out) {
+    std::unique_lock<std::mutex> lk(m_mtx);
+    std::map<uint64_t, request>::iterator i = m_reqs.find(id);
+    lt_require(i != m_reqs.end(), LT_ERR_PARAM, "unknown request");
+    m_cv.wait(lk, [&]() { return i->second.state == DONE; });
+    const request &r = i->second;
+    out.t_queue_ms = r.t_start - r.t_submit;
+    out.t_run_ms = r.t_end - r.t_start;
+    out.t_total_ms = r.t_end - r.t_submit;
+    out.batch = r.batch;
+    out.batch_size = r.batch_size;
+    out.status = r.status;
+    out.message = r.message;
+    m_reqs.erase(i);
+}
+
+void server::totals(server_stats &out) {
+    std::lock_guard<std::mutex> lk(m_mtx);
+    out.batches = m_batches;
+    out.requests = m_done;
+    out.mean_batch = m_batches ? double(m_batched_requests) / double(m_batches) : 0.0;
+    out.max_batch = m_max_seen;
+    out.rejected = m_rejected;
+}
+
+// Planner: admission by device memory, then the execution plan on the host
+// (schedule-derived work lists, device upload, kernel autotuning on the
+// first use of an operation).
+void server::planner_loop() {
+    while(true) {
+        uint64_t id = 0;
+        {
+            std::unique_lock<std::mutex> lk(m_mtx);
+            m_cv.wait(lk, [&]() { return m_stop || !m_queue.empty(); });
+            if(m_queue.empty()) {
+                if(m_stop) break;
+                continue;
+            }
+            id = m_queue.front();
+            request &r = m_reqs[id];
+            // admission: the operation's plan is resident already, or its
+            // estimated footprint fits in the free budget; otherwise wait
+            // for running batches to finish, and reject what can never fit
+            size_t need = r.op->plan_resident(*r.c, r.add, r.d) ? 0 :
+                r.op->footprint_estimate();
+            size_t used = m_ctx.used(), cap = m_ctx.cap();
+            bool fits = used + need + m_reserve <= cap;
+            if(!fits && in_flight()) {
+                m_cv.wait_for(lk, std::chrono::milliseconds(1));
+                continue;
+            }
+            m_queue.pop_front();
+            r.state = PLANNING;
+        }
+        request snapshot;
+        {
+            std::lock_guard<std::mutex> lk(m_mtx);
+            snapshot = m_reqs[id];
+        }
+        int status = LT_OK;
+        std::string msg;
+        try {
+            snapshot.op->prepare(*snapshot.c, snapshot.add, snapshot.d);
+        } catch(const lt_error &e) {
+            status = e.code;
+            msg = e.what();
+        } catch(const std::exception &e) {
+            status = LT_ERR_INTERNAL;
+            msg = e.what();
+        }
+        {
+            std::lock_guard<std::mutex> lk(m_mtx);
+            request &r = m_reqs[id];
+            if(status != LT_OK) {
+                r.status = status;
+                r.message = msg;
+                r.t_start = r.t_end = now_ms();
+                r.state = DONE;
+                m_rejected++;
+            } else {
+                r.state = READY;
+                m_ready.push_back(id);
+            }
+        }
+        m_cv.notify_all();
+    }
+}
+
+// Executor: every ready request joins the next batch (up to max_batch).
+void server::executor_loop() {
+    dev_buffer merged;
+    while(true) {
+        std::vector<uint64_t> ids;
+        std::vector<batch_item> items;
+        {
+            std::unique_lock<std::mutex> lk(m_mtx);
+            m_cv.wait(lk, [&]() {
+                return !m_ready.empty() || (m_stop && m_queue.empty() && !planning());
+            });
+            if(m_ready.empty()) break;
+            double t = now_ms();
+            while(!m_ready.empty() && ids.size() < m_max_batch) {
+                uint64_t id = m_ready.front();
+                m_ready.pop_front();
+                request &r = m_reqs[id];
+                r.state = RUNNING;
+                r.t_start = t;
+                ids.push_back(id);
+                batch_item it;
+                it.op = r.op; it.c = r.c; it.add = r.add; it.d = r.d;
+                it.batch_gpu_ms = 0.0; it.batch_gemm_launches = 0;
+                items.push_back(it);
+            }
+        }
+        int status = LT_OK;
+        std::string msg;
+        try {
+            btod_contract2::perform_batch(items, merged);
+        } catch(const lt_error &e) {
+            status = e.code;
+            msg = e.what();
+        } catch(const std::exception &e) {
+            status = LT_ERR_INTERNAL;
+            msg = e.what();
+        }
+        {
+            std::lock_guard<std::mutex> lk(m_mtx);
+            double t = now_ms();
+            m_batches++;
+            m_batched_requests += ids.size();
+            m_max_seen = std::max(m_max_seen, ids.size());
+            for(size_t i = 0; i < ids.size(); i++) {
+                request &r = m_reqs[ids[i]];
+                r.t_end = t;
+                r.batch = m_batches;
+                r.batch_size = ids.size();
+                r.status = status;
+                r.message = msg;
+                r.state = DONE;
+                m_done++;
+            }
+        }
+        m_cv.notify_all();
+    }
+}
+
+bool server::in_flight() const {
+    if(!m_ready.empty()) return true;
+    for(std::map<uint64_t, request>::const_iterator i = m_reqs.begin(); i != m_reqs.end(); ++i)
+        if(i->second.state == RUNNING) return true;
+    return false;
+}
+
+bool server::planning() const {
+    for(std::map<uint64_t, request>::const_iterator i = m_reqs.begin(); i != m_reqs.end(); ++i)
+        if(i->second.state == PLANNING) return true;
+    return false;
+}
+
+} // namespace lta

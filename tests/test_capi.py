@@ -70,6 +70,9 @@ def test_device_ops_refuse_on_host_context(host_ctx):
     assert e.value.code == -4
     with pytest.raises(lt.LtError):
         lt.device_array(host_ctx, (4,))
+    with pytest.raises(lt.LtError) as e:
+        lt.server(host_ctx)
+    assert e.value.code == -4
 
 
 def test_block_access_and_orbits(host_ctx):
