@@ -607,7 +607,7 @@ void btod_contract2::build_plan(btensor &c, bool add, double dscale) {
     pl.events = true;
 
     // Autotuner: time every candidate configuration on this plan's own work
-    // list (grouped launch, one warm-up), keep the fastest.  Results of these
+    // list (grouped launch), keep the fastest.  Results of these
     // launches are overwritten by the perform that follows (never run for
     // the accumulating path).
     if(candidates.size() > 1) {
@@ -621,15 +621,18 @@ void btod_contract2::build_plan(btensor &c, bool add, double dscale) {
             int k = candidates[ci];
             upload_tiles(k);
             uint32_t nt = (uint32_t)pl.tiles_sorted.size();
-            hip_check(dev::launch_gemm(k, pl.ta, pl.tb, pl.d_cb, pl.d_pairs,
-                pl.d_tiles_sorted, nt, nt, s), "autotune");
-            hip_check(hipEventRecord(pl.ev[0], s), "hipEventRecord");
-            hip_check(dev::launch_gemm(k, pl.ta, pl.tb, pl.d_cb, pl.d_pairs,
-                pl.d_tiles_sorted, nt, nt, s), "autotune");
-            hip_check(hipEventRecord(pl.ev[1], s), "hipEventRecord");
-            hip_check(hipStreamSynchronize(s), "hipStreamSynchronize");
+            // one timed launch; a short one (first-launch overheads are
+            // then not negligible) is timed again
             float ms = 0.f;
-            hip_check(hipEventElapsedTime(&ms, pl.ev[0], pl.ev[1]), "hipEventElapsedTime");
+            for(int rep = 0; rep < 2; rep++) {
+                hip_check(hipEventRecord(pl.ev[0], s), "hipEventRecord");
+                hip_check(dev::launch_gemm(k, pl.ta, pl.tb, pl.d_cb, pl.d_pairs,
+                    pl.d_tiles_sorted, nt, nt, s), "autotune");
+                hip_check(hipEventRecord(pl.ev[1], s), "hipEventRecord");
+                hip_check(hipStreamSynchronize(s), "hipStreamSynchronize");
+                hip_check(hipEventElapsedTime(&ms, pl.ev[0], pl.ev[1]), "hipEventElapsedTime");
+                if(ms > 20.f) break;
+            }
             m_stats.tune_ms[k] = ms;
             if(ms < best) { best = ms; bestk = k; }
         }

@@ -11,6 +11,7 @@ pairs, ka, kb, kc); "tables" lists point-group product tables.
                      (se_label), (ij)/(ab)/(cd) antisymmetry and the
                      (ab)<->(cd) pair symmetry of V (se_perm)
   pp_ladder_o40_v200()  the benchmark configuration (o = 40, v = 200)
+  ring_batched_o60_v300()  ring, o = 60, v = 300, block sizes U{8..96}
 """
 import itertools
 
@@ -102,6 +103,27 @@ def pp_ladder_o40_v200():
     C2v irrep) and virtual blocks 18+16 (A1), 16 (A2), 14+12 (B1), 24 (B2)."""
     return ladder(o_blocks=(7, 5, 4, 4), v_blocks=(18, 16, 16, 14, 12, 24),
                   irreps_o=(0, 1, 2, 3), irreps_v=(0, 0, 1, 2, 2, 3))
+
+
+def uneven_splits(n, lo, hi, rng):
+    """Split points of n into block sizes drawn from U{lo..hi} (the last block
+    takes the remainder; a remainder below lo joins the previous block)."""
+    sizes = []
+    while sum(sizes) < n:
+        sizes.append(int(rng.integers(lo, hi + 1)))
+    sizes[-1] -= sum(sizes) - n
+    if sizes[-1] < lo and len(sizes) > 1:
+        sizes[-2] += sizes.pop()
+    return [int(x) for x in np.cumsum(sizes)[:-1]]
+
+
+def ring_batched_o60_v300(seed=1003):
+    """C[i,j,a,b] = sum_{k,c} A[i,k,a,c] B[k,j,c,b], o = 60, v = 300, no
+    symmetry, block sizes drawn from U{8..96} (numpy default_rng(seed))."""
+    rng = np.random.default_rng(seed)
+    so = uneven_splits(60, 8, 96, rng)
+    sv = uneven_splits(300, 8, 96, rng)
+    return ring(60, 300, so, sv)
 
 
 # ---------------------------------------------------------------------------

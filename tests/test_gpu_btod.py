@@ -66,6 +66,44 @@ def test_config1_full_size(gpu_ctx):
     assert st["flops"] == 2 * 160000 * 400
 
 
+def test_config3_batched_full_size(gpu_ctx):
+    """Ring o=60, v=300, block sizes U{8..96} (1728 block pairs, 1.17e13
+    FLOPs): grouped (graph) == eager grouped == per-block launches bit for
+    bit with one kernel configuration; the autotuned configuration within
+    1e-12; three C blocks against the dense oracle on slices of A and B."""
+    from libtensor_amd import workloads as W
+    prob = W.ring_batched_o60_v300()
+    a, b, op = W.build_random(gpu_ctx, prob, 1003)
+    c = W.result_tensor(gpu_ctx, prob)
+    _say("config3 built")
+    op.set_tile(0)
+    op.perform(c, lt.LT_MODE_GROUPED)
+    st = op.stats()
+    assert st["n_pairs"] == 1728 and st["flops"] == 2.0 * 60**3 * 300**3
+    first = c.to_dense()
+    op.perform(c, lt.LT_MODE_PER_BLOCK)
+    assert np.array_equal(c.to_dense(), first)
+    op.set_graph(False)
+    op.perform(c, lt.LT_MODE_GROUPED)
+    assert np.array_equal(c.to_dense(), first)
+    op.set_graph(True)
+    op2 = lt.btod_contract2(W.make_contraction(prob), a, b)
+    c2 = W.result_tensor(gpu_ctx, prob)
+    op2.perform(c2, lt.LT_MODE_GROUPED)
+    assert O.rel_err(c2.to_dense(), first) < TOL
+    _say("config3 autotuned cfg", op2.stats()["tile_cfg"])
+    A, B = a.to_dense(), b.to_dense()
+    sched = op.get_schedule()
+    for bi in (sched[0], sched[len(sched) // 2], sched[-1]):
+        idx = c.index(bi)
+        st0, dims = c.block_start(idx), c.block_dims(idx)
+        (i0, j0, a0, b0), (ni, nj, na, nb) = st0, dims
+        ref = np.einsum("ikac,kjcb->ijab", A[i0:i0 + ni, :, a0:a0 + na, :],
+                        B[:, j0:j0 + nj, :, b0:b0 + nb], optimize=True)
+        got = first[i0:i0 + ni, j0:j0 + nj, a0:a0 + na, b0:b0 + nb]
+        assert O.rel_err(got, ref) < TOL, bi
+
+
 def test_ladder_medium_against_dense_oracle(gpu_ctx):
     prob = P.prob_ladder(o_blocks=(3, 2, 2, 1), v_blocks=(5, 4, 3, 6, 2),
                          irreps_o=(0, 1, 2, 3), irreps_v=(0, 0, 1, 2, 3))
