@@ -330,19 +330,31 @@ void btensor::fill_random(uint64_t seed, int dist, double scale, int sk_d0, int 
         short_orbit so(m_sym, a, true);
         if(so.allowed && so.acidx == a) canon.push_back(a);
     }
+    // Values are drawn in chunks of 2^20 elements, each chunk with its own
+    // generator seeded from (seed, block, chunk), so large blocks fill in
+    // parallel and the values do not depend on the thread count.
+    const size_t kChunk = size_t(1) << 20;
     std::vector<std::vector<double> > data(canon.size());
-    parallel_for(canon.size(), m_ctx.nthreads(), [&](size_t t) {
+    std::vector<std::pair<size_t, size_t> > tasks;
+    for(size_t t = 0; t < canon.size(); t++) {
+        size_t sz = m_sym.bis.block_dims(get_index(canon[t], m_bidims)).size();
+        data[t].resize(sz);
+        for(size_t q = 0; q * kChunk < sz; q++) tasks.push_back(std::make_pair(t, q));
+    }
+    parallel_for(tasks.size(), m_ctx.nthreads(), [&](size_t w) {
+        size_t t = tasks[w].first, q = tasks[w].second;
         size_t a = canon[t];
         index_t bidx = get_index(a, m_bidims);
         dims_t bd = m_sym.bis.block_dims(bidx);
         index_t start = m_sym.bis.block_start(bidx);
-        size_t sz = bd.size();
-        std::vector<double> r(sz), s(sz, 0.0);
-        std::mt19937_64 rng(seed * 0x9E3779B97F4A7C15ull + a * 0xBF58476D1CE4E5B9ull + 1);
+        size_t sz = bd.size(), i0 = q * kChunk, i1 = std::min(sz, i0 + kChunk);
+        std::mt19937_64 rng(seed * 0x9E3779B97F4A7C15ull + a * 0xBF58476D1CE4E5B9ull + 1 +
+            q * 0x94D049BB133111EBull);
         std::uniform_real_distribution<double> u(-1.0, 1.0);
         std::normal_distribution<double> g(0.0, 1.0);
-        index_t e(n);
-        for(size_t i = 0; i < sz; i++) {
+        index_t e = get_index(i0, bd);
+        double *r = data[t].data();
+        for(size_t i = i0; i < i1; i++) {
             double v = (dist == 1) ? g(rng) : u(rng);
             v *= scale;
             if(sk_d0 >= 0 && sk_d1 >= 0) {
@@ -353,15 +365,25 @@ void btensor::fill_random(uint64_t seed, int dist, double scale, int sk_d0, int 
             r[i] = v;
             inc_index(e, bd);
         }
-        std::vector<tensor_tr> stab = block_stabilizer(m_sym, a);
-        for(size_t i = 0; i < stab.size(); i++) host_transform(r.data(), bd, stab[i], s.data(), true);
-        double inv = 1.0 / double(stab.size());
-        for(size_t i = 0; i < sz; i++) s[i] *= inv;
-        data[t].swap(s);
     });
+    // symmetrization over each block's stabilizer
+    parallel_for(canon.size(), m_ctx.nthreads(), [&](size_t t) {
+        size_t a = canon[t];
+        std::vector<tensor_tr> stab = block_stabilizer(m_sym, a);
+        if(stab.size() <= 1) return;
+        dims_t bd = m_sym.bis.block_dims(get_index(a, m_bidims));
+        std::vector<double> sym(data[t].size(), 0.0);
+        for(size_t i = 0; i < stab.size(); i++)
+            host_transform(data[t].data(), bd, stab[i], sym.data(), true);
+        double inv = 1.0 / double(stab.size());
+        for(size_t i = 0; i < sym.size(); i++) sym[i] *= inv;
+        data[t].swap(sym);
+    });
+    (void)n;
     for(size_t t = 0; t < canon.size(); t++) {
         double *p = block_ptr(canon[t]);
         m_ctx.upload(p, data[t].data(), data[t].size() * sizeof(double));
+        std::vector<double>().swap(data[t]);
         mark_nonzero(canon[t]);
     }
 }

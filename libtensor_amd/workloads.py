@@ -12,6 +12,8 @@ pairs, ka, kb, kc); "tables" lists point-group product tables.
                      (ab)<->(cd) pair symmetry of V (se_perm)
   pp_ladder_o40_v200()  the benchmark configuration (o = 40, v = 200)
   ring_batched_o60_v300()  ring, o = 60, v = 300, block sizes U{8..96}
+  serving_clients()  the serving mix: 64 calculations, o ~ U{8..48},
+                     v ~ U{40..240}, ring and ladder patterns
 """
 import itertools
 
@@ -113,7 +115,8 @@ def uneven_splits(n, lo, hi, rng):
         sizes.append(int(rng.integers(lo, hi + 1)))
     sizes[-1] -= sum(sizes) - n
     if sizes[-1] < lo and len(sizes) > 1:
-        sizes[-2] += sizes.pop()
+        last = sizes.pop()
+        sizes[-1] += last
     return [int(x) for x in np.cumsum(sizes)[:-1]]
 
 
@@ -124,6 +127,45 @@ def ring_batched_o60_v300(seed=1003):
     so = uneven_splits(60, 8, 96, rng)
     sv = uneven_splits(300, 8, 96, rng)
     return ring(60, 300, so, sv)
+
+
+def _parts(n, k, rng):
+    """n split into k positive parts of nearly equal size (random order)."""
+    base = [n // k + (1 if i < n % k else 0) for i in range(k)]
+    rng.shuffle(base)
+    return [int(x) for x in base]
+
+
+def serving_clients(n=64, seed=1005):
+    """Problems of the serving mix: client i draws o ~ U{8..48} and
+    v ~ U{40..240} (numpy default_rng(seed + i)); even clients run a ring
+    (one block per dimension, no symmetry), odd clients a ladder with spin
+    and C2v symmetry (occupied: one block per irrep and spin; virtual:
+    blocks U{8..32} per spin, irreps assigned in turn)."""
+    out = []
+    for i in range(n):
+        rng = np.random.default_rng(seed + i)
+        o = int(rng.integers(8, 49))
+        v = int(rng.integers(40, 241))
+        if i % 2 == 0:
+            p = ring(o, v)
+            p["name"] = "ring o=%d v=%d" % (o, v)
+        else:
+            oh, vh = max(4, o // 2), max(8, v // 2)
+            ob = [x for x in _parts(oh, 4, rng) if x > 0]
+            vb = []
+            while sum(vb) < vh:
+                vb.append(int(rng.integers(8, 33)))
+            vb[-1] -= sum(vb) - vh
+            if vb[-1] < 4 and len(vb) > 1:
+                last = vb.pop()
+                vb[-1] += last
+            p = ladder(o_blocks=tuple(ob), v_blocks=tuple(vb),
+                       irreps_o=tuple(range(len(ob))),
+                       irreps_v=tuple(j % 4 for j in range(len(vb))))
+            p["name"] = "ladder o=%d v=%d" % (2 * oh, 2 * vh)
+        out.append(p)
+    return out
 
 
 # ---------------------------------------------------------------------------
