@@ -13,11 +13,14 @@ libtensor built from its sources, run on the host cores), the roofline of the
 grouped GEMM kernel (HIP events around its launch) and the host-CPU baseline.
 """
 import argparse
+import hashlib
 import json
 import os
+import random
 import statistics
 import sys
 import tempfile
+import threading
 import time
 
 HERE = os.path.dirname(os.path.abspath(__file__))
@@ -56,6 +59,58 @@ def cpu_baseline(prob, a, b, flops, shape, threads):
     return base, out.get("result")
 
 
+def serving_mix(ctx, lt, W, n_clients, rounds, rate, seed=1005):
+    """The serving mix: n_clients calculations (o ~ U{8..48}, v ~ U{40..240},
+    ring / ladder), each submitting `rounds` contractions with Poisson
+    arrivals (aggregate `rate` per second) to one request loop.  Every result
+    is compared bit for bit with the same contraction run alone."""
+    probs = W.serving_clients(n_clients, seed)
+    clients = []
+    t0 = time.time()
+    for i, p in enumerate(probs):
+        a, b, op = W.build_random(ctx, p, 2000 + i)
+        op.set_tile(-1)      # configuration from the padding estimate (no per-client tuning)
+        c = W.result_tensor(ctx, p)
+        op.perform(c, lt.LT_MODE_GROUPED)
+        digest = hashlib.sha1(c.to_dense().tobytes()).hexdigest()
+        clients.append((p["name"], a, b, op, c, digest, op.stats()["flops"]))
+    log("serving: %d clients ready in %.1fs" % (len(clients), time.time() - t0))
+    srv = lt.server(ctx, max_batch=64)
+    lat, bad = [], []
+    lock = threading.Lock()
+
+    def client(i):
+        rng = random.Random(seed * 1000 + i)
+        name, a, b, op, c, digest, fl = clients[i]
+        for r in range(rounds):
+            time.sleep(rng.expovariate(rate / len(clients)))
+            st = srv.wait(srv.submit(op, c))
+            with lock:
+                lat.append(st["t_total_ms"])
+
+    t1 = time.perf_counter()
+    th = [threading.Thread(target=client, args=(i,)) for i in range(len(clients))]
+    for t in th:
+        t.start()
+    for t in th:
+        t.join()
+    wall = time.perf_counter() - t1
+    st = srv.stats()
+    srv.close()
+    for name, a, b, op, c, digest, fl in clients:
+        if hashlib.sha1(c.to_dense().tobytes()).hexdigest() != digest:
+            bad.append(name)
+    flops = rounds * sum(x[6] for x in clients)
+    out = {"clients": len(clients), "requests": int(st["requests"]),
+           "batches": int(st["batches"]), "mean_batch": st["mean_batch"],
+           "max_batch": int(st["max_batch"]), "arrival_rate_per_s": rate,
+           "wall_s": wall, "tflops": flops / wall / 1e12,
+           "p50_ms": float(np.percentile(lat, 50)), "p99_ms": float(np.percentile(lat, 99)),
+           "results_same_as_alone": not bad, "differing": bad}
+    log("serving: %s" % json.dumps(out))
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -66,6 +121,11 @@ def main():
     ap.add_argument("--no-per-block", action="store_true",
                     help="skip the per-block comparison run (kernel profiles)")
     ap.add_argument("--cpu-threads", type=int, default=0)
+    ap.add_argument("--no-serve", action="store_true", help="skip the serving mix")
+    ap.add_argument("--serve-clients", type=int, default=64)
+    ap.add_argument("--serve-rounds", type=int, default=3)
+    ap.add_argument("--serve-rate", type=float, default=50.0,
+                    help="aggregate Poisson arrival rate of the serving mix (1/s)")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py runs on one GPU only: this build measures the single-GPU "
@@ -122,6 +182,10 @@ def main():
         op.perform(c, mode)
     result = c.to_dense()
 
+    serve = None
+    if not args.no_serve:
+        serve = serving_mix(ctx, lt, W, args.serve_clients, args.serve_rounds, args.serve_rate)
+
     base, ref = None, None
     if not args.no_cpu:
         threads = args.cpu_threads or min(16, os.cpu_count() or 1)
@@ -153,7 +217,9 @@ def main():
         "gemm_ms": gemm_avg,
         "plan_ms": st["t_plan_ms"], "schedule_ms": st["t_schedule_ms"],
         "gemm_kernel_cfg": int(st["tile_cfg"]), "autotune_ms": st["tune_ms"],
-        "p50_ms": None, "p99_ms": None,
+        "p50_ms": serve["p50_ms"] if serve else None,
+        "p99_ms": serve["p99_ms"] if serve else None,
+        "serving": serve,
     }
     print(json.dumps(line))
 
