@@ -44,15 +44,17 @@ struct gemm_cblock {
     int32_t zero;
 };
 
-//! One product d * op(A) op(B) contributing to a C block.  A is m x k
-//! ("ik", lda = row stride) or k x m ("ki"); B is k x n ("kj") or n x k
-//! ("jk"), as fixed by the launch's transposition flags.
+//! One product d * A B contributing to a C block.  A is stored m x k
+//! (A(m, k) at a[m * lda + k]) or, with kPairAKMajor, k x m (at
+//! a[k * lda + m]); B is stored k x n (B(k, n) at b[k * ldb + n]) or, with
+//! kPairBNMajor, n x k (at b[n * ldb + k]).
 struct gemm_pair {
     const double *a;
     const double *b;
-    uint32_t k, lda, ldb, pad;
+    uint32_t k, lda, ldb, flags;
     double d;
 };
+static const uint32_t kPairAKMajor = 1, kPairBNMajor = 2;
 
 //! Work item of a grouped GEMM launch: one output tile of one C block.
 struct gemm_tile {
@@ -78,10 +80,10 @@ hipError_t launch_permute(const perm_desc *descs, const uint32_t *chunk_desc,
     const uint32_t *chunk_off, uint32_t nchunks, hipStream_t s);
 static const uint32_t kPermChunk = 2048;
 
-//! Grouped FP64 MFMA GEMM over a tile list.  ta: A stored k x m; tb: B
-//! stored n x k.  nwg workgroups loop over the tiles (nwg == ntiles gives
-//! one workgroup per tile).
-hipError_t launch_gemm(int cfg, bool ta, bool tb, const gemm_cblock *cb,
+//! Grouped FP64 MFMA GEMM over a tile list (operand layouts per pair).
+//! nwg workgroups loop over the tiles (nwg == ntiles gives one workgroup
+//! per tile).
+hipError_t launch_gemm(int cfg, const gemm_cblock *cb,
     const gemm_pair *pairs, const gemm_tile *tiles, uint32_t ntiles,
     uint32_t nwg, hipStream_t s);
 

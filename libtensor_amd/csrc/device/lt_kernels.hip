@@ -184,7 +184,7 @@ hipError_t launch_fill(double *p, size_t n, double v, hipStream_t s) {
 //   A: A[m = l % 16][k = l / 16]      B: B[k = l / 16][n = l % 16]
 //   D: D[m = (l / 16) + 4 * v][n = l % 16], v = 0..3 (measured on gfx950)
 
-template<int BM, int BN, bool TA, bool TB>
+template<int BM, int BN>
 __global__ void __launch_bounds__(256)
 gemm_kernel(const gemm_cblock *__restrict__ cbs,
     const gemm_pair *__restrict__ pairs,
@@ -221,6 +221,8 @@ gemm_kernel(const gemm_cblock *__restrict__ cbs,
             const gemm_pair pr = pairs[p];
             const int K = int(pr.k);
             const double d = pr.d;
+            const bool TA = (pr.flags & kPairAKMajor) != 0;
+            const bool TB = (pr.flags & kPairBNMajor) != 0;
             for(int k0 = 0; k0 < K; k0 += BK) {
                 const int krem = K - k0;
                 // A tile -> As[k][m]
@@ -308,14 +310,10 @@ gemm_kernel(const gemm_cblock *__restrict__ cbs,
 }
 
 template<int BM, int BN>
-static hipError_t launch_gemm_t(bool ta, bool tb, const gemm_cblock *cb,
+static hipError_t launch_gemm_t(const gemm_cblock *cb,
     const gemm_pair *pairs, const gemm_tile *tiles, uint32_t ntiles,
     uint32_t nwg, hipStream_t s) {
-    dim3 g(nwg), b(256);
-    if(!ta && !tb) hipLaunchKernelGGL((gemm_kernel<BM, BN, false, false>), g, b, 0, s, cb, pairs, tiles, ntiles);
-    else if(!ta && tb) hipLaunchKernelGGL((gemm_kernel<BM, BN, false, true>), g, b, 0, s, cb, pairs, tiles, ntiles);
-    else if(ta && !tb) hipLaunchKernelGGL((gemm_kernel<BM, BN, true, false>), g, b, 0, s, cb, pairs, tiles, ntiles);
-    else hipLaunchKernelGGL((gemm_kernel<BM, BN, true, true>), g, b, 0, s, cb, pairs, tiles, ntiles);
+    hipLaunchKernelGGL((gemm_kernel<BM, BN>), dim3(nwg), dim3(256), 0, s, cb, pairs, tiles, ntiles);
     return hipGetLastError();
 }
 
@@ -401,7 +399,19 @@ __device__ __forceinline__ void mma_chunk(double4_t (&acc)[FM][FN],
                     af[i][q], bf[j][q], acc[i][j], 0, 0, 0);
 }
 
-template<int FM, int FN, int W, bool TA, bool TB>
+//! Loads of one chunk in the pair's operand layout (wave-uniform branch).
+template<int FM, int FN>
+__device__ __forceinline__ void load_chunk_any(double (&af)[FM][4], double (&bf)[FN][4],
+    const gemm_pair &pr, int c, const int (&arow)[FM], const int (&bcol)[FN], int lg) {
+    switch(pr.flags & (kPairAKMajor | kPairBNMajor)) {
+    case 0: load_chunk<FM, FN, false, false>(af, bf, pr, c, arow, bcol, lg); break;
+    case kPairAKMajor: load_chunk<FM, FN, true, false>(af, bf, pr, c, arow, bcol, lg); break;
+    case kPairBNMajor: load_chunk<FM, FN, false, true>(af, bf, pr, c, arow, bcol, lg); break;
+    default: load_chunk<FM, FN, true, true>(af, bf, pr, c, arow, bcol, lg); break;
+    }
+}
+
+template<int FM, int FN, int W>
 __global__ void __launch_bounds__(64 * W)
 gemm_direct_kernel(const gemm_cblock *__restrict__ cbs,
     const gemm_pair *__restrict__ pairs,
@@ -441,7 +451,7 @@ gemm_direct_kernel(const gemm_cblock *__restrict__ cbs,
         if(px < pend) {
             double ax[FM][4], bx[FN][4], ay[FM][4], by[FN][4];
             gemm_pair prx = pairs[px];
-            load_chunk<FM, FN, TA, TB>(ax, bx, prx, cx, arow, bcol, lg);
+            load_chunk_any<FM, FN>(ax, bx, prx, cx, arow, bcol, lg);
             while(true) {
                 uint32_t py = px;
                 int cy = cx + W;
@@ -449,7 +459,7 @@ gemm_direct_kernel(const gemm_cblock *__restrict__ cbs,
                 const bool hy = py < pend;
                 if(!hy) { py = px; cy = cx; }       // dummy in-bounds load
                 const gemm_pair pry = pairs[py];
-                load_chunk<FM, FN, TA, TB>(ay, by, pry, cy, arow, bcol, lg);
+                load_chunk_any<FM, FN>(ay, by, pry, cy, arow, bcol, lg);
                 mma_chunk<FM, FN>(acc, ax, bx, prx.d);
                 if(!hy) break;
                 uint32_t pz = py;
@@ -458,7 +468,7 @@ gemm_direct_kernel(const gemm_cblock *__restrict__ cbs,
                 const bool hz = pz < pend;
                 if(!hz) { pz = py; cz = cy; }
                 prx = pairs[pz];
-                load_chunk<FM, FN, TA, TB>(ax, bx, prx, cz, arow, bcol, lg);
+                load_chunk_any<FM, FN>(ax, bx, prx, cz, arow, bcol, lg);
                 mma_chunk<FM, FN>(acc, ay, by, pry.d);
                 if(!hz) break;
                 px = pz;
@@ -516,35 +526,32 @@ gemm_direct_kernel(const gemm_cblock *__restrict__ cbs,
 }
 
 template<int FM, int FN, int W>
-static hipError_t launch_direct_t(bool ta, bool tb, const gemm_cblock *cb,
+static hipError_t launch_direct_t(const gemm_cblock *cb,
     const gemm_pair *pairs, const gemm_tile *tiles, uint32_t ntiles,
     uint32_t nwg, hipStream_t s) {
-    dim3 g(nwg), b(64 * W);
-    if(!ta && !tb) hipLaunchKernelGGL((gemm_direct_kernel<FM, FN, W, false, false>), g, b, 0, s, cb, pairs, tiles, ntiles);
-    else if(!ta && tb) hipLaunchKernelGGL((gemm_direct_kernel<FM, FN, W, false, true>), g, b, 0, s, cb, pairs, tiles, ntiles);
-    else if(ta && !tb) hipLaunchKernelGGL((gemm_direct_kernel<FM, FN, W, true, false>), g, b, 0, s, cb, pairs, tiles, ntiles);
-    else hipLaunchKernelGGL((gemm_direct_kernel<FM, FN, W, true, true>), g, b, 0, s, cb, pairs, tiles, ntiles);
+    hipLaunchKernelGGL((gemm_direct_kernel<FM, FN, W>), dim3(nwg), dim3(64 * W), 0, s,
+        cb, pairs, tiles, ntiles);
     return hipGetLastError();
 }
 
-hipError_t launch_gemm(int cfg, bool ta, bool tb, const gemm_cblock *cb,
+hipError_t launch_gemm(int cfg, const gemm_cblock *cb,
     const gemm_pair *pairs, const gemm_tile *tiles, uint32_t ntiles,
     uint32_t nwg, hipStream_t s) {
     if(ntiles == 0) return hipSuccess;
     if(nwg == 0 || nwg > ntiles) nwg = ntiles;
     switch(cfg) {
-    case TILE_32x64: return launch_gemm_t<32, 64>(ta, tb, cb, pairs, tiles, ntiles, nwg, s);
-    case TILE_64x32: return launch_gemm_t<64, 32>(ta, tb, cb, pairs, tiles, ntiles, nwg, s);
-    case TILE_32x32: return launch_gemm_t<32, 32>(ta, tb, cb, pairs, tiles, ntiles, nwg, s);
-    case SPLITK_16x64_W4: return launch_direct_t<1, 4, 4>(ta, tb, cb, pairs, tiles, ntiles, nwg, s);
-    case SPLITK_32x32_W4: return launch_direct_t<2, 2, 4>(ta, tb, cb, pairs, tiles, ntiles, nwg, s);
-    case SPLITK_32x64_W4: return launch_direct_t<2, 4, 4>(ta, tb, cb, pairs, tiles, ntiles, nwg, s);
-    case SPLITK_64x32_W4: return launch_direct_t<4, 2, 4>(ta, tb, cb, pairs, tiles, ntiles, nwg, s);
-    case SPLITK_16x64_W2: return launch_direct_t<1, 4, 2>(ta, tb, cb, pairs, tiles, ntiles, nwg, s);
-    case SPLITK_32x32_W2: return launch_direct_t<2, 2, 2>(ta, tb, cb, pairs, tiles, ntiles, nwg, s);
-    case SPLITK_16x32_W4: return launch_direct_t<1, 2, 4>(ta, tb, cb, pairs, tiles, ntiles, nwg, s);
-    case SPLITK_32x16_W4: return launch_direct_t<2, 1, 4>(ta, tb, cb, pairs, tiles, ntiles, nwg, s);
-    default: return launch_gemm_t<64, 64>(ta, tb, cb, pairs, tiles, ntiles, nwg, s);
+    case TILE_32x64: return launch_gemm_t<32, 64>(cb, pairs, tiles, ntiles, nwg, s);
+    case TILE_64x32: return launch_gemm_t<64, 32>(cb, pairs, tiles, ntiles, nwg, s);
+    case TILE_32x32: return launch_gemm_t<32, 32>(cb, pairs, tiles, ntiles, nwg, s);
+    case SPLITK_16x64_W4: return launch_direct_t<1, 4, 4>(cb, pairs, tiles, ntiles, nwg, s);
+    case SPLITK_32x32_W4: return launch_direct_t<2, 2, 4>(cb, pairs, tiles, ntiles, nwg, s);
+    case SPLITK_32x64_W4: return launch_direct_t<2, 4, 4>(cb, pairs, tiles, ntiles, nwg, s);
+    case SPLITK_64x32_W4: return launch_direct_t<4, 2, 4>(cb, pairs, tiles, ntiles, nwg, s);
+    case SPLITK_16x64_W2: return launch_direct_t<1, 4, 2>(cb, pairs, tiles, ntiles, nwg, s);
+    case SPLITK_32x32_W2: return launch_direct_t<2, 2, 2>(cb, pairs, tiles, ntiles, nwg, s);
+    case SPLITK_16x32_W4: return launch_direct_t<1, 2, 4>(cb, pairs, tiles, ntiles, nwg, s);
+    case SPLITK_32x16_W4: return launch_direct_t<2, 1, 4>(cb, pairs, tiles, ntiles, nwg, s);
+    default: return launch_gemm_t<64, 64>(cb, pairs, tiles, ntiles, nwg, s);
     }
 }
 

@@ -296,6 +296,7 @@ int lt_tod_contract2(lt_ctx *ctx, size_t N, size_t M, size_t K,
             cb.pair_end = 1;
             gp.k = (uint32_t)g.k;
             gp.d = d;
+            gp.flags = (ta ? dev::kPairAKMajor : 0) | (tb ? dev::kPairBNMajor : 0);
             uint32_t bm = 64, bn = 64;
             dev::tile_shape(dev::TILE_64x64, bm, bn);
             std::vector<dev::gemm_tile> tiles;
@@ -313,7 +314,7 @@ int lt_tod_contract2(lt_ctx *ctx, size_t N, size_t M, size_t K,
             std::memcpy(&host[1024], tiles.data(), tiles.size() * sizeof(dev::gemm_tile));
             cx.upload(mem, host.data(), bytes);
             cx.activate();
-            hip_check(dev::launch_gemm(dev::TILE_64x64, ta, tb, (dev::gemm_cblock*)mem,
+            hip_check(dev::launch_gemm(dev::TILE_64x64, (dev::gemm_cblock*)mem,
                 (dev::gemm_pair*)(mem + 512), (dev::gemm_tile*)(mem + 1024),
                 (uint32_t)tiles.size(), (uint32_t)tiles.size(), cx.stream()), "gemm");
             cx.sync();

@@ -310,31 +310,70 @@ void btod_contract2::build_plan(btensor &c, bool add, double dscale) {
         return s;
     };
 
+    // Operand layouts of the aligned product (the same for every pair):
+    // A stored m x k (or k x m when ta), B stored k x n (or n x k when tb).
+    gemm_form gf0 = aligned_gemm_form(contr,
+        bisa2.block_dims(get_index(usea[0], bida2)),
+        bisb2.block_dims(get_index(useb[0], bidb2)));
+
+    // How the kernels read an aligned operand block: straight from the
+    // stored canonical block when the transformation is a scalar, or a
+    // scalar and the exchange of the two matrix index groups (the kernel
+    // then reads the operand in the other layout); from a prepared copy in
+    // scratch otherwise.
+    struct opref { const double *p; double scale; bool flip; uint32_t ldf; };
+    auto fold = [&](btensor &t, const dims_t &bid, const src_t &s, const dims_t &dz,
+        size_t nfirst, opref &r) -> bool {
+        r.p = t.block_ptr(s.y);
+        r.scale = 1.0; r.flip = false; r.ldf = 0;
+        if(s.tr.is_identity()) return true;
+        dims_t dy = t.bis().block_dims(get_index(s.y, bid));
+        dev::perm_desc pd = make_perm_desc(r.p, dy, s.tr, 0, false);
+        size_t first = 1;
+        for(size_t i = 0; i < nfirst; i++) first *= dz[i];
+        if(pd.n == 1 && pd.sstride[0] == 1) {
+            r.scale = pd.alpha;
+            return true;
+        }
+        if(pd.n == 2 && pd.sstride[0] == 1 && pd.sstride[1] == pd.div[0].d &&
+            pd.div[0].d == first) {
+            r.scale = pd.alpha;
+            r.flip = true;
+            r.ldf = (uint32_t)first;
+            return true;
+        }
+        return false;
+    };
+
     size_t scratch = 0;
     std::unordered_map<size_t, size_t> offa, offb;    // aligned block -> scratch offset
-    std::unordered_map<size_t, const double*> ptra, ptrb;
+    std::unordered_map<size_t, opref> refa, refb;
     std::vector<std::pair<size_t, src_t> > prepa, prepb;
     for(size_t i = 0; i < usea.size(); i++) {
         src_t s = source_of(m_a, syma2, perma, bida, bida2, usea[i]);
         lt_require(m_a.is_nonzero(s.y), LT_ERR_INTERNAL, "pair uses a zero block of A");
-        if(s.tr.is_identity()) {
-            ptra[usea[i]] = m_a.block_ptr(s.y);
-        } else {
+        dims_t dz = bisa2.block_dims(get_index(usea[i], bida2));
+        opref r;
+        if(!fold(m_a, bida, s, dz, gf0.ta ? contr.K : contr.N, r)) {
             offa[usea[i]] = scratch;
-            scratch += (bisa2.block_dims(get_index(usea[i], bida2)).size() + 31) & ~size_t(31);
+            scratch += (dz.size() + 31) & ~size_t(31);
             prepa.push_back(std::make_pair(usea[i], s));
+            r.p = 0; r.scale = 1.0; r.flip = false; r.ldf = 0;
         }
+        refa[usea[i]] = r;
     }
     for(size_t i = 0; i < useb.size(); i++) {
         src_t s = source_of(m_b, symb2, permb, bidb, bidb2, useb[i]);
         lt_require(m_b.is_nonzero(s.y), LT_ERR_INTERNAL, "pair uses a zero block of B");
-        if(s.tr.is_identity()) {
-            ptrb[useb[i]] = m_b.block_ptr(s.y);
-        } else {
+        dims_t dz = bisb2.block_dims(get_index(useb[i], bidb2));
+        opref r;
+        if(!fold(m_b, bidb, s, dz, gf0.tb ? contr.M : contr.K, r)) {
             offb[useb[i]] = scratch;
-            scratch += (bisb2.block_dims(get_index(useb[i], bidb2)).size() + 31) & ~size_t(31);
+            scratch += (dz.size() + 31) & ~size_t(31);
             prepb.push_back(std::make_pair(useb[i], s));
+            r.p = 0; r.scale = 1.0; r.flip = false; r.ldf = 0;
         }
+        refb[useb[i]] = r;
     }
 
     // C blocks: aligned canonical block -> stored canonical block of C
@@ -361,9 +400,9 @@ void btod_contract2::build_plan(btensor &c, bool add, double dscale) {
         pl.scratch_elems = scratch;
     }
     for(std::unordered_map<size_t, size_t>::iterator i = offa.begin(); i != offa.end(); ++i)
-        ptra[i->first] = pl.scratch + i->second;
+        refa[i->first].p = pl.scratch + i->second;
     for(std::unordered_map<size_t, size_t>::iterator i = offb.begin(); i != offb.end(); ++i)
-        ptrb[i->first] = pl.scratch + i->second;
+        refb[i->first].p = pl.scratch + i->second;
 
     // 1. prepare descriptors
     for(size_t i = 0; i < prepa.size(); i++) {
@@ -380,9 +419,6 @@ void btod_contract2::build_plan(btensor &c, bool add, double dscale) {
     }
 
     // 2. GEMM work lists
-    gemm_form gf0 = aligned_gemm_form(contr,
-        bisa2.block_dims(get_index(usea[0], bida2)),
-        bisb2.block_dims(get_index(useb[0], bidb2)));
     const bool tc = gf0.tc;
     // with C stored as (j, i) the product is computed as C^T = B^T A^T
     pl.ta = tc ? !gf0.tb : gf0.ta;
@@ -442,16 +478,22 @@ void btod_contract2::build_plan(btensor &c, bool add, double dscale) {
             gemm_form g = aligned_gemm_form(contr, da, db);
             dev::gemm_pair gp;
             std::memset(&gp, 0, sizeof(gp));
-            const double *pa = ptra[p->aia], *pb = ptrb[p->aib];
+            const opref &ra = refa[p->aia], &rb = refb[p->aib];
             gp.k = (uint32_t)g.k;
-            size_t lda = g.ta ? g.m : g.k;     // row stride of A as stored
-            size_t ldb = g.tb ? g.k : g.n;     // row stride of B as stored
+            // layout and row stride of A and B as read
+            bool ta = g.ta != ra.flip, tb = g.tb != rb.flip;
+            size_t lda = ra.flip ? ra.ldf : (g.ta ? g.m : g.k);
+            size_t ldb = rb.flip ? rb.ldf : (g.tb ? g.k : g.n);
             if(!tc) {
-                gp.a = pa; gp.b = pb; gp.lda = (uint32_t)lda; gp.ldb = (uint32_t)ldb;
+                gp.a = ra.p; gp.b = rb.p; gp.lda = (uint32_t)lda; gp.ldb = (uint32_t)ldb;
+                gp.flags = (ta ? dev::kPairAKMajor : 0) | (tb ? dev::kPairBNMajor : 0);
             } else {
-                gp.a = pb; gp.b = pa; gp.lda = (uint32_t)ldb; gp.ldb = (uint32_t)lda;
+                // C^T = B^T A^T
+                gp.a = rb.p; gp.b = ra.p; gp.lda = (uint32_t)ldb; gp.ldb = (uint32_t)lda;
+                gp.flags = (!tb ? dev::kPairAKMajor : 0) | (!ta ? dev::kPairBNMajor : 0);
             }
-            gp.d = d;
+            gp.d = d * ra.scale * rb.scale;
+            if(gp.d == 0.0) continue;
             pl.pairs.push_back(gp);
             flops += 2.0 * double(g.m) * double(g.n) * double(g.k);
             ksum += double(g.k);
@@ -626,7 +668,7 @@ void btod_contract2::build_plan(btensor &c, bool add, double dscale) {
             float ms = 0.f;
             for(int rep = 0; rep < 2; rep++) {
                 hip_check(hipEventRecord(pl.ev[0], s), "hipEventRecord");
-                hip_check(dev::launch_gemm(k, pl.ta, pl.tb, pl.d_cb, pl.d_pairs,
+                hip_check(dev::launch_gemm(k, pl.d_cb, pl.d_pairs,
                     pl.d_tiles_sorted, nt, nt, s), "autotune");
                 hip_check(hipEventRecord(pl.ev[1], s), "hipEventRecord");
                 hip_check(hipStreamSynchronize(s), "hipStreamSynchronize");
@@ -666,12 +708,13 @@ void btod_contract2::build_ozaki(int slices) {
     // of C, B-role operands per column of C
     struct op_info { const double *src; uint32_t rows, k, kp, rs, ks; size_t off, eoff; };
     std::vector<op_info> ops;
-    std::map<const double*, size_t> ida, idb;
+    typedef std::map<std::pair<const double*, bool>, size_t> op_index;
+    op_index ida, idb;
     std::vector<size_t> pa(pl.pairs.size()), pb(pl.pairs.size());
     size_t bytes = 0, nexp = 0;
-    auto add_op = [&](std::map<const double*, size_t> &idx, const double *src,
+    auto add_op = [&](op_index &idx, const double *src,
         uint32_t rows, uint32_t k, uint32_t ld, bool kmajor) {
-        std::map<const double*, size_t>::iterator i = idx.find(src);
+        op_index::iterator i = idx.find(std::make_pair(src, kmajor));
         if(i != idx.end()) {
             lt_require(ops[i->second].rows == rows && ops[i->second].k == k, LT_ERR_INTERNAL,
                 "operand used with two shapes");
@@ -686,7 +729,7 @@ void btod_contract2::build_ozaki(int slices) {
         bytes = (bytes + 255) & ~size_t(255);
         nexp += rows;
         ops.push_back(o);
-        idx[src] = ops.size() - 1;
+        idx[std::make_pair(src, kmajor)] = ops.size() - 1;
         return ops.size() - 1;
     };
     for(size_t b = 0; b < pl.cblocks.size(); b++) {
@@ -697,8 +740,8 @@ void btod_contract2::build_ozaki(int slices) {
             // magnitude <= 127 * 127 must not overflow
             lt_require(double(gp.k) * 16129.0 * slices < 2147483647.0, LT_ERR_PARAM,
                 "contraction extent too large for the split-precision mode");
-            pa[p] = add_op(ida, gp.a, cb.m, gp.k, gp.lda, pl.ta);
-            pb[p] = add_op(idb, gp.b, cb.n, gp.k, gp.ldb, !pl.tb);
+            pa[p] = add_op(ida, gp.a, cb.m, gp.k, gp.lda, (gp.flags & dev::kPairAKMajor) != 0);
+            pb[p] = add_op(idb, gp.b, cb.n, gp.k, gp.ldb, (gp.flags & dev::kPairBNMajor) == 0);
         }
     }
     size_t exp_off = bytes;
@@ -790,13 +833,13 @@ void btod_contract2::enqueue(int mode, hipStream_t s, bool timed, size_t &launch
         for(size_t b = 0; b + 1 < pl.block_tile_begin.size(); b++) {
             uint32_t t0 = pl.block_tile_begin[b], t1 = pl.block_tile_begin[b + 1];
             if(t1 == t0) continue;
-            hip_check(dev::launch_gemm(pl.tile_cfg, pl.ta, pl.tb, pl.d_cb, pl.d_pairs,
+            hip_check(dev::launch_gemm(pl.tile_cfg, pl.d_cb, pl.d_pairs,
                 pl.d_tiles_block + t0, t1 - t0, t1 - t0, s), "gemm (per block)");
             launches++;
         }
     } else {
         uint32_t nt = (uint32_t)pl.tiles_sorted.size();
-        hip_check(dev::launch_gemm(pl.tile_cfg, pl.ta, pl.tb, pl.d_cb, pl.d_pairs,
+        hip_check(dev::launch_gemm(pl.tile_cfg, pl.d_cb, pl.d_pairs,
             pl.d_tiles_sorted, nt, nt, s), "gemm (grouped)");
         launches++;
     }
@@ -921,21 +964,20 @@ void btod_contract2::perform_batch(std::vector<batch_item> &items, dev_buffer &m
                 pl.n_prep_chunks, s), "permute (prepare)");
     }
 
-    // one grouped GEMM per (layout, kernel configuration): the work lists of
+    // one grouped GEMM per kernel configuration: the work lists of
     // the plans are concatenated (C block and pair indexes shifted) and the
     // tiles ordered longest first across all requests
     std::map<std::pair<int, int>, std::vector<size_t> > groups;
     for(size_t i = 0; i < items.size(); i++) {
         if(plans[i]->empty) continue;
-        int key = (plans[i]->ta ? 2 : 0) + (plans[i]->tb ? 1 : 0);
-        groups[std::make_pair(key, plans[i]->tile_cfg)].push_back(i);
+        groups[std::make_pair(0, plans[i]->tile_cfg)].push_back(i);
     }
     std::vector<dev::gemm_cblock> cbs;
     std::vector<dev::gemm_pair> prs;
     std::vector<std::pair<double, dev::gemm_tile> > tl;
     std::vector<std::pair<size_t, size_t> > spans;     // (cblock offset, tile offset) per group
     std::vector<char> host;
-    struct launch { int cfg; bool ta, tb; size_t ocb, opr, otl; uint32_t nt; };
+    struct launch { int cfg; size_t ocb, opr, otl; uint32_t nt; };
     std::vector<launch> launches;
     size_t sz = 0;
     std::vector<std::vector<char> > parts;
@@ -964,8 +1006,6 @@ void btod_contract2::perform_batch(std::vector<batch_item> &items, dev_buffer &m
                 return x.first > y.first; });
         launch L;
         L.cfg = g->first.second;
-        L.ta = (g->first.first & 2) != 0;
-        L.tb = (g->first.first & 1) != 0;
         L.ocb = sz; sz = align256(sz + cbs.size() * sizeof(dev::gemm_cblock));
         L.opr = sz; sz = align256(sz + prs.size() * sizeof(dev::gemm_pair));
         L.otl = sz; sz = align256(sz + tl.size() * sizeof(dev::gemm_tile));
@@ -991,7 +1031,7 @@ void btod_contract2::perform_batch(std::vector<batch_item> &items, dev_buffer &m
     hip_check(hipEventRecord(plans[0]->ev[1], s), "hipEventRecord");
     for(size_t l = 0; l < launches.size(); l++) {
         const launch &L = launches[l];
-        hip_check(dev::launch_gemm(L.cfg, L.ta, L.tb, (dev::gemm_cblock*)(base + L.ocb),
+        hip_check(dev::launch_gemm(L.cfg, (dev::gemm_cblock*)(base + L.ocb),
             (dev::gemm_pair*)(base + L.opr), (dev::gemm_tile*)(base + L.otl), L.nt, L.nt, s),
             "gemm (batch)");
     }
