@@ -110,7 +110,7 @@ dev::perm_desc make_perm_desc_public(const double *src, const dims_t &dsrc,
 //! Execution plan cached by the operation (valid while the structure of A,
 //! B and C does not change).
 struct contract_plan {
-    uint64_t va, vb;
+    uint64_t va, vb, cuid;
     btensor *c;
     bool add;
     double d;
@@ -156,7 +156,7 @@ struct contract_plan {
     dev::gemm_tile *d_oz_tiles;
     uint32_t n_oz_tiles;
 
-    contract_plan() : va(0), vb(0), c(0), add(false), d(1.0), empty(false),
+    contract_plan() : va(0), vb(0), cuid(0), c(0), add(false), d(1.0), empty(false),
         ta(false), tb(false), tile_cfg(0), d_prep(0), d_wback(0), d_prep_cd(0),
         d_prep_co(0), d_wb_cd(0), d_wb_co(0), n_prep_chunks(0), n_wb_chunks(0),
         d_cb(0), d_pairs(0), d_tiles_sorted(0), d_tiles_block(0), scratch(0),
@@ -206,7 +206,7 @@ void btod_contract2::clst(const index_t &ic, contr_list &out) {
 bool btod_contract2::plan_valid(btensor &c, bool add, double d, int) const {
     if(!m_plan) return false;
     return m_plan->va == m_a.version() && m_plan->vb == m_b.version() &&
-        m_plan->c == &c && m_plan->add == add && m_plan->d == d;
+        m_plan->cuid == c.uid() && m_plan->add == add && m_plan->d == d;
 }
 
 // ---------------------------------------------------------------------------
@@ -219,6 +219,7 @@ void btod_contract2::build_plan(btensor &c, bool add, double dscale) {
     pl.va = m_a.version();
     pl.vb = m_b.version();
     pl.c = &c;
+    pl.cuid = c.uid();
     pl.add = add;
     pl.d = dscale;
     pl.ctx = &ctx;

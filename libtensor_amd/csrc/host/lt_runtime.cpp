@@ -164,7 +164,7 @@ std::atomic<uint64_t> btensor::s_version(1);
 
 btensor::btensor(context &ctx, const bis_t &bis) :
     m_ctx(ctx), m_sym(bis), m_bidims(bis.block_index_dims()), m_slab(ctx),
-    m_version(s_version++) {
+    m_version(s_version++), m_uid(s_version++) {
 
     lt_require(bis.order() >= 1 && bis.order() <= LT_MAX_ORDER, LT_ERR_PARAM,
         "block tensor order must be 1..8");

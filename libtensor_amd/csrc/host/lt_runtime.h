@@ -103,7 +103,10 @@ public:
     //! Structure version: changes when the set of non-zero blocks, the
     //! symmetry or the split points change.
     uint64_t version() const { return m_version; }
-    void touch() { m_version++; }
+    void touch() { m_version = s_version++; }
+    //! Identity of the tensor object, never reused within the process
+    //! (execution plans are keyed on it rather than on the address).
+    uint64_t uid() const { return m_uid; }
 
     void split(const bool *mask, size_t pos);
     void set_symmetry(const symmetry &s);
@@ -119,7 +122,7 @@ private:
     slab m_slab;
     std::map<size_t, double*> m_store;
     std::vector<size_t> m_nonzero;   // sorted
-    uint64_t m_version;
+    uint64_t m_version, m_uid;
     static std::atomic<uint64_t> s_version;
 };
 

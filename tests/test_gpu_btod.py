@@ -183,3 +183,17 @@ def test_graph_replays_match_eager(gpu_ctx, name):
     assert op.stats()["graph"] == 1
     assert np.array_equal(c.to_dense(), eager_add)
     assert O.rel_err(eager_add, 2.0 * result) < TOL
+
+
+def test_plan_not_reused_for_a_new_result_tensor(gpu_ctx):
+    """A result tensor created after another was released may occupy the
+    same host address; the cached plan must not be taken for it."""
+    import gc
+    prob, dense, meta, result = G.load("ladder_full")
+    a, b, op = P.build(gpu_ctx, prob, dense)
+    for rep in range(4):
+        c = P.make_result_tensor(gpu_ctx, prob)
+        op.perform(c, lt.LT_MODE_GROUPED)
+        assert O.rel_err(c.to_dense(), result) < TOL, rep
+        del c
+        gc.collect()
