@@ -162,6 +162,9 @@ def main():
     ap.add_argument("--cpu-threads", type=int, default=0)
     ap.add_argument("--no-serve", action="store_true", help="skip the serving mix")
     ap.add_argument("--no-ozaki", action="store_true", help="skip the split-precision sweep")
+    ap.add_argument("--copy-probe", action="store_true",
+                    help="only run a plain 256 MiB device copy (tod_copy, identity) three "
+                         "times: the reference for scaling HBM counters")
     ap.add_argument("--serve-clients", type=int, default=64)
     ap.add_argument("--serve-rounds", type=int, default=3)
     ap.add_argument("--serve-rate", type=float, default=50.0,
@@ -176,6 +179,15 @@ def main():
     from libtensor_amd import workloads as W
 
     ctx = lt.Context(0)
+    if args.copy_probe:
+        n = 1 << 25
+        src = lt.device_array(ctx, (n,), np.ones(n))
+        dst = lt.device_array(ctx, (n,))
+        for i in range(3):
+            lt.tod_copy(src, dst, (0,), 1.0, zero=True)
+        print(json.dumps({"copy_probe_bytes_read": n * 8, "copy_probe_bytes_written": n * 8,
+                          "launches": 3}))
+        return
     prob = W.pp_ladder_o40_v200()
     t0 = time.time()
     a, b, op = W.build_random(ctx, prob, 1002)
