@@ -269,6 +269,15 @@ def main():
         serve = serving_mix(ctx, lt, W, args.serve_clients, args.serve_rounds, args.serve_rate)
 
     achieved = flops / (gemm_avg * 1e-3) / 1e12
+    # HBM bytes per GEMM launch from the committed rocprofv3 counter runs
+    # (valid for the kernel configuration they were taken with)
+    traffic = None
+    tpath = os.path.join(HERE, "profiles", "pp_ladder_o40_v200_traffic.json")
+    if os.path.exists(tpath):
+        with open(tpath) as f:
+            tr = json.load(f)
+        if tr.get("gemm_kernel_cfg") == int(st["tile_cfg"]):
+            traffic = tr["traffic_bytes_per_launch"]
     line = {
         "metric": METRIC, "value": value, "unit": "TFLOP/s", "n_gpus": 1,
         "steps": args.steps, "warmup": args.warmup, "ms_per_step": ms_per_step,
@@ -280,7 +289,7 @@ def main():
                    "c_blocks": int(st["n_cblocks"]), "block_pairs": int(st["n_pairs"]),
                    "flops_per_step": flops},
         "roofline": {"bound": "mfma", "achieved": achieved, "peak": FP64_MFMA_PEAK,
-                     "unit": "TFLOP/s", "frac": achieved / FP64_MFMA_PEAK, "traffic": None},
+                     "unit": "TFLOP/s", "frac": achieved / FP64_MFMA_PEAK, "traffic": traffic},
         "cpu_baseline": base,
         "max_rel_err": rel,
         "per_block_ms_per_step": per_block_ms,
