@@ -32,8 +32,15 @@ struct perm_desc {
     uint32_t sstride[kPermMaxOrder];  //!< source strides
     double alpha;
     int32_t accumulate;
-    int32_t pad;
+    //! Transposing descriptors (the source's unit-stride dimension is
+    //! destination dimension tile_p < n - 1): 32 x 32 tiles over
+    //! (tile_p, n - 1) staged in LDS, read and written contiguously.
+    //! -1: element-wise gather.
+    int32_t tile_p;
+    fastdiv dps;                      //!< destination stride of tile_p
 };
+static const uint32_t kPermTile = 32;
+static const uint32_t kPermTileItem = 0x80000000u;
 
 //! One C block of a grouped GEMM: C(m x n, leading dimension ldc) is
 //! overwritten (zero != 0) or accumulated with the sum over its pairs.
@@ -73,9 +80,10 @@ enum { TILE_64x64 = 0, TILE_32x64 = 1, TILE_64x32 = 2, TILE_32x32 = 3,
        TILE_COUNT = 12 };
 void tile_shape(int cfg, uint32_t &bm, uint32_t &bn);
 
-//! Batched permutation: every descriptor is split into chunks of `chunk`
-//! elements; chunk_desc[j] / chunk_off[j] name the descriptor and the first
-//! element of chunk j.
+//! Batched permutation: work item j is chunk_desc[j] / chunk_off[j]: a
+//! descriptor and the first destination element of a chunk of kPermChunk
+//! elements, or (chunk_desc[j] | kPermTileItem) and the destination index
+//! of a tile's corner for transposing descriptors.
 hipError_t launch_permute(const perm_desc *descs, const uint32_t *chunk_desc,
     const uint32_t *chunk_off, uint32_t nchunks, hipStream_t s);
 static const uint32_t kPermChunk = 2048;

@@ -68,13 +68,45 @@ permute_kernel(const perm_desc *__restrict__ descs,
     const uint32_t *__restrict__ chunk_desc,
     const uint32_t *__restrict__ chunk_off) {
 
-    // each thread owns kPermChunk / 256 elements of the chunk; all source
-    // offsets are computed and all loads issued before the first store, so
-    // every thread has that many gathers in flight
-    constexpr int E = kPermChunk / 256;
+    __shared__ double tile[kPermTile][kPermTile + 1];
     const uint32_t ch = blockIdx.x;
-    const perm_desc d = descs[chunk_desc[ch]];
+    const uint32_t cd = chunk_desc[ch];
+    const perm_desc d = descs[cd & ~kPermTileItem];
     const uint32_t base = chunk_off[ch];
+
+    if(cd & kPermTileItem) {
+        // 32 x 32 tile over destination dimensions (p, q = n - 1); the
+        // source is contiguous along p, the destination along q
+        const uint32_t p = uint32_t(d.tile_p), q = d.n - 1;
+        const uint32_t dq = d.div[q].d, dp = d.div[p].d;
+        const uint32_t q0 = base - fdiv(base, d.div[q]) * dq;
+        const uint32_t e1 = fdiv(base, d.dps);
+        const uint32_t p0 = e1 - fdiv(e1, d.div[p]) * dp;
+        const uint32_t np = min(kPermTile, dp - p0), nq = min(kPermTile, dq - q0);
+        const uint32_t s0 = src_offset(d, base), sq = d.sstride[q], dps = d.dps.d;
+        const uint32_t ti = threadIdx.x & 31, tj = threadIdx.x >> 5;
+#pragma unroll
+        for(uint32_t r = 0; r < kPermTile; r += 8) {
+            const uint32_t iq = tj + r;
+            if(ti < np && iq < nq) tile[iq][ti] = d.src[s0 + ti + iq * sq];
+        }
+        __syncthreads();
+#pragma unroll
+        for(uint32_t r = 0; r < kPermTile; r += 8) {
+            const uint32_t ip = tj + r;
+            if(ti < nq && ip < np) {
+                double *o = d.dst + base + ip * dps + ti;
+                const double v = d.alpha * tile[ti][ip];
+                *o = d.accumulate ? v + *o : v;
+            }
+        }
+        return;
+    }
+
+    // element-wise gather: each thread owns kPermChunk / 256 elements of the
+    // chunk; all source offsets are computed and all loads issued before the
+    // first store, so every thread has that many gathers in flight
+    constexpr int E = kPermChunk / 256;
     const uint32_t end = min(base + kPermChunk, d.total);
     double v[E];
 #pragma unroll

@@ -95,10 +95,7 @@ void dense_permute(context &ctx, const double *src, const dims_t &dsrc,
     const tensor_tr &tr, double *dst, bool accumulate) {
     dev::perm_desc d = make_perm_desc_public(src, dsrc, tr, dst, accumulate);
     std::vector<uint32_t> cd, co;
-    for(uint32_t off = 0; off < d.total; off += dev::kPermChunk) {
-        cd.push_back(0);
-        co.push_back(off);
-    }
+    make_chunks(std::vector<dev::perm_desc>(1, d), cd, co);
     if(cd.empty()) return;
     size_t bytes = sizeof(d) + 8 * cd.size() + 512;
     char *mem = (char*)ctx.alloc(bytes);

@@ -73,20 +73,21 @@ dev::perm_desc make_perm_desc(const double *src, const dims_t &dsrc,
     }
     d.alpha = tr.st.c;
     d.accumulate = accumulate ? 1 : 0;
-    return d;
-}
-
-//! Chunk tables for a list of permute descriptors.
-void make_chunks(const std::vector<dev::perm_desc> &ds, std::vector<uint32_t> &cd,
-    std::vector<uint32_t> &co) {
-    cd.clear();
-    co.clear();
-    for(size_t i = 0; i < ds.size(); i++) {
-        for(uint32_t off = 0; off < ds[i].total; off += dev::kPermChunk) {
-            cd.push_back((uint32_t)i);
-            co.push_back(off);
+    // transposing: the source's unit-stride dimension is not the
+    // destination's fastest one
+    d.tile_p = -1;
+    d.dps = dev::make_fastdiv(1);
+    if(nf >= 2 && fs[nf - 1] != 1) {
+        for(size_t i = 0; i + 1 < nf; i++) {
+            if(fs[i] != 1) continue;
+            size_t st = 1;
+            for(size_t j = i + 1; j < nf; j++) st *= fd[j];
+            d.tile_p = (int32_t)i;
+            d.dps = dev::make_fastdiv((uint32_t)st);
+            break;
         }
     }
+    return d;
 }
 
 //! Device-side copy of host vectors in one allocation.
@@ -101,6 +102,51 @@ struct dev_lists {
 size_t align256(size_t x) { return (x + 255) & ~size_t(255); }
 
 } // unnamed namespace
+
+//! Work items of a list of permute descriptors: chunks of kPermChunk
+//! destination elements, or the corners of 32 x 32 tiles for transposing
+//! descriptors.
+void make_chunks(const std::vector<dev::perm_desc> &ds, std::vector<uint32_t> &cd,
+    std::vector<uint32_t> &co) {
+    cd.clear();
+    co.clear();
+    for(size_t i = 0; i < ds.size(); i++) {
+        const dev::perm_desc &d = ds[i];
+        if(d.tile_p < 0) {
+            for(uint32_t off = 0; off < d.total; off += dev::kPermChunk) {
+                cd.push_back((uint32_t)i);
+                co.push_back(off);
+            }
+            continue;
+        }
+        // corners: every index of the other dimensions, p and q in steps of
+        // the tile size
+        const uint32_t n = d.n, p = (uint32_t)d.tile_p, q = n - 1;
+        uint32_t dims[dev::kPermMaxOrder], step[dev::kPermMaxOrder], idx[dev::kPermMaxOrder];
+        uint32_t str[dev::kPermMaxOrder];
+        uint32_t acc = 1;
+        for(int j = (int)n - 1; j >= 0; j--) {
+            dims[j] = d.div[j].d;
+            str[j] = acc;
+            acc *= dims[j];
+            step[j] = (uint32_t(j) == p || uint32_t(j) == q) ? dev::kPermTile : 1;
+            idx[j] = 0;
+        }
+        while(true) {
+            uint32_t e = 0;
+            for(uint32_t j = 0; j < n; j++) e += idx[j] * str[j];
+            cd.push_back((uint32_t)i | dev::kPermTileItem);
+            co.push_back(e);
+            int j = (int)n - 1;
+            for(; j >= 0; j--) {
+                idx[j] += step[j];
+                if(idx[j] < dims[j]) break;
+                idx[j] = 0;
+            }
+            if(j < 0) break;
+        }
+    }
+}
 
 dev::perm_desc make_perm_desc_public(const double *src, const dims_t &dsrc,
     const tensor_tr &tr, double *dst, bool accumulate) {
@@ -510,14 +556,55 @@ void btod_contract2::build_plan(btensor &c, bool add, double dscale) {
 
     // Tile lists of one kernel configuration: tiles grouped by C block (one
     // launch per block) and the same tiles longest-first (grouped launch).
+    // Grouped launch order.  Workgroups are dealt to the XCDs round-robin
+    // (workgroup w runs on XCD w % nx), and each XCD has its own L2: C
+    // blocks whose pairs read the same B blocks form one group, groups are
+    // assigned to XCDs longest-processing-time first, and each XCD's tiles
+    // (groups longest first, a group's tiles together so that they share A
+    // rows and B columns in that XCD's L2) go to positions x, x + nx, ...
+    // Positions left over at the end hold empty tiles (a C block with no
+    // rows).
+    int nx = 8;
+    {
+        int v = 0;
+        ctx.activate();
+        if(hipDeviceGetAttribute(&v, hipDeviceAttributeNumberOfXccs, ctx.device()) == hipSuccess &&
+            v >= 1 && v <= 64) nx = v;
+        else hipGetLastError();
+    }
+    std::vector<size_t> cb_group(pl.cblocks.size());
+    size_t ngroups = 0;
+    {
+        std::map<std::vector<std::pair<const double*, uint32_t> >, size_t> gid;
+        for(size_t b = 0; b < pl.cblocks.size(); b++) {
+            std::vector<std::pair<const double*, uint32_t> > key;
+            for(uint32_t p = pl.cblocks[b].pair_begin; p < pl.cblocks[b].pair_end; p++)
+                key.push_back(std::make_pair(pl.pairs[p].b, pl.pairs[p].flags));
+            std::sort(key.begin(), key.end());
+            std::map<std::vector<std::pair<const double*, uint32_t> >, size_t>::iterator i =
+                gid.find(key);
+            if(i == gid.end()) i = gid.insert(std::make_pair(key, ngroups++)).first;
+            cb_group[b] = i->second;
+        }
+    }
+    const uint32_t null_cb = (uint32_t)pl.cblocks.size();
+    {
+        dev::gemm_cblock z;
+        std::memset(&z, 0, sizeof(z));
+        z.zero = 1;
+        pl.cblocks.push_back(z);     // empty C block of the padding tiles
+    }
+
+    // Tile lists of one kernel configuration: tiles grouped by C block (one
+    // launch per block) and the grouped-launch order above.
     auto make_tiles = [&](int cfg, std::vector<dev::gemm_tile> &by_block,
         std::vector<uint32_t> &begin, std::vector<dev::gemm_tile> &sorted) {
         uint32_t bm = 64, bn = 64;
         dev::tile_shape(cfg, bm, bn);
         by_block.clear();
         begin.clear();
-        std::vector<double> cost;
-        for(size_t b = 0; b < pl.cblocks.size(); b++) {
+        std::vector<double> gcost(ngroups, 0.0);
+        for(size_t b = 0; b < null_cb; b++) {
             const dev::gemm_cblock &cb = pl.cblocks[b];
             begin.push_back((uint32_t)by_block.size());
             uint32_t tm = (cb.m + bm - 1) / bm, tn = (cb.n + bn - 1) / bn;
@@ -529,19 +616,43 @@ void btod_contract2::build_plan(btensor &c, bool add, double dscale) {
                     tl.tm = (uint16_t)i;
                     tl.tn = (uint16_t)j;
                     by_block.push_back(tl);
-                    cost.push_back(cb_ksum[b]);
+                    gcost[cb_group[b]] += cb_ksum[b];
                 }
             }
         }
         begin.push_back((uint32_t)by_block.size());
-        lt_require(by_block.size() < (size_t(1) << 31), LT_ERR_PARAM, "too many tiles");
-        std::vector<size_t> order(by_block.size());
-        std::iota(order.begin(), order.end(), size_t(0));
-        std::stable_sort(order.begin(), order.end(), [&](size_t x, size_t y) {
-            return cost[x] > cost[y];
-        });
-        sorted.resize(order.size());
-        for(size_t i = 0; i < order.size(); i++) sorted[i] = by_block[order[i]];
+        begin.push_back((uint32_t)by_block.size());     // the empty C block
+        lt_require(by_block.size() < (size_t(1) << 30), LT_ERR_PARAM, "too many tiles");
+        // groups to XCDs, longest first onto the least loaded
+        std::vector<size_t> gorder(ngroups);
+        std::iota(gorder.begin(), gorder.end(), size_t(0));
+        std::stable_sort(gorder.begin(), gorder.end(), [&](size_t x, size_t y) {
+            return gcost[x] > gcost[y]; });
+        std::vector<double> load(nx, 0.0);
+        std::vector<std::vector<size_t> > xg(nx);
+        for(size_t i = 0; i < gorder.size(); i++) {
+            size_t x = std::min_element(load.begin(), load.end()) - load.begin();
+            xg[x].push_back(gorder[i]);
+            load[x] += gcost[gorder[i]];
+        }
+        std::vector<std::vector<size_t> > gblocks(ngroups);
+        for(size_t b = 0; b < null_cb; b++) gblocks[cb_group[b]].push_back(b);
+        std::vector<std::vector<dev::gemm_tile> > q(nx);
+        size_t qmax = 0;
+        for(int x = 0; x < nx; x++) {
+            for(size_t i = 0; i < xg[x].size(); i++) {
+                const std::vector<size_t> &bl = gblocks[xg[x][i]];
+                for(size_t j = 0; j < bl.size(); j++)
+                    for(uint32_t t = begin[bl[j]]; t < begin[bl[j] + 1]; t++)
+                        q[x].push_back(by_block[t]);
+            }
+            qmax = std::max(qmax, q[x].size());
+        }
+        dev::gemm_tile empty;
+        empty.cblock = null_cb; empty.tm = 0; empty.tn = 0;
+        sorted.assign(qmax * nx, empty);
+        for(int x = 0; x < nx; x++)
+            for(size_t l = 0; l < q[x].size(); l++) sorted[l * nx + x] = q[x][l];
     };
 
     // Kernel configuration: fixed, or the one wasting least padding, or
@@ -557,7 +668,7 @@ void btod_contract2::build_plan(btensor &c, bool add, double dscale) {
             uint32_t tbm, tbn;
             dev::tile_shape(k, tbm, tbn);
             double waste = 0.0, ntile = 0.0;
-            for(size_t b = 0; b < pl.cblocks.size(); b++) {
+            for(size_t b = 0; b < null_cb; b++) {
                 size_t tm = (pl.cblocks[b].m + tbm - 1) / tbm;
                 size_t tn = (pl.cblocks[b].n + tbn - 1) / tbn;
                 waste += double(tm * tbm * tn * tbn) * cb_ksum[b];
@@ -684,7 +795,7 @@ void btod_contract2::build_plan(btensor &c, bool add, double dscale) {
         m_tuned_cfg = cfg;
     }
 
-    m_stats.n_cblocks = pl.cblocks.size();
+    m_stats.n_cblocks = null_cb;
     m_stats.n_pairs = pl.pairs.size();
     m_stats.n_tiles = pl.tiles_by_block.size();
     m_stats.n_unfold = pl.prep.size();
@@ -997,6 +1108,7 @@ void btod_contract2::perform_batch(std::vector<batch_item> &items, dev_buffer &m
             prs.insert(prs.end(), pl.pairs.begin(), pl.pairs.end());
             for(size_t t = 0; t < pl.tiles_sorted.size(); t++) {
                 dev::gemm_tile x = pl.tiles_sorted[t];
+                if(x.cblock >= pl.cb_ksum.size()) continue;     // padding tile
                 double cost = pl.cb_ksum[x.cblock];
                 x.cblock += cb0;
                 tl.push_back(std::make_pair(cost, x));

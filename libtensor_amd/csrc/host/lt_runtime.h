@@ -134,6 +134,11 @@ std::vector<tensor_tr> block_stabilizer(const symmetry &sym, size_t aidx);
 void host_transform(const double *src, const dims_t &dsrc, const tensor_tr &tr,
     double *dst, bool accumulate);
 
+//! Work items (chunks, or tiles of transposing descriptors) of permute
+//! descriptors for dev::launch_permute.
+void make_chunks(const std::vector<dev::perm_desc> &ds, std::vector<uint32_t> &cd,
+    std::vector<uint32_t> &co);
+
 //! Device permute descriptor for dst = c * P(src) (dimensions fused).
 dev::perm_desc make_perm_desc_public(const double *src, const dims_t &dsrc,
     const tensor_tr &tr, double *dst, bool accumulate);
