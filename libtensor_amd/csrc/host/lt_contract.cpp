@@ -222,7 +222,7 @@ struct contract_plan {
 btod_contract2::btod_contract2(const contraction2 &c, btensor &a, double ka,
     btensor &b, double kb, double kc) :
     m_contr(c), m_a(a), m_b(b), m_ka(ka), m_kb(kb), m_kc(kc),
-    m_tile_cfg(-2), m_tuned_cfg(-1), m_use_graph(true), m_oz_slices(0), m_oz_int8(1) {
+    m_tile_cfg(-2), m_tuned_cfg(-1), m_tuned_va(0), m_tuned_vb(0), m_use_graph(true), m_oz_slices(0), m_oz_int8(1) {
 
     lt_require(c.is_complete(), LT_ERR_PARAM, "contraction is incomplete");
     lt_require(a.bis().order() == c.NA() && b.bis().order() == c.NB(),
@@ -683,7 +683,11 @@ void btod_contract2::build_plan(btensor &c, bool add, double dscale) {
         std::sort(est.begin(), est.end());
         for(size_t i = 0; i < est.size(); i++) candidates.push_back(est[i].second);
         if(cfg == -1 || add) candidates.resize(1);
-        if(add && m_tuned_cfg >= 0) candidates.assign(1, m_tuned_cfg);
+        // the configuration measured for these operands (same pairs, same
+        // work list for any result tensor) is reused without measuring again
+        if(cfg == -2 && m_tuned_cfg >= 0 && m_tuned_va == pl.va && m_tuned_vb == pl.vb)
+            candidates.assign(1, m_tuned_cfg);
+        else if(add && m_tuned_cfg >= 0) candidates.assign(1, m_tuned_cfg);
     }
     cfg = candidates[0];
 
@@ -793,6 +797,8 @@ void btod_contract2::build_plan(btensor &c, bool add, double dscale) {
         cfg = bestk;
         upload_tiles(cfg);
         m_tuned_cfg = cfg;
+        m_tuned_va = pl.va;
+        m_tuned_vb = pl.vb;
     }
 
     m_stats.n_cblocks = null_cb;
