@@ -122,7 +122,7 @@ struct oz_pair {
     double d;
 };
 
-static const uint32_t kOzTileM = 16, kOzTileN = 64;
+static const uint32_t kOzTileM = 16, kOzTileN = 32;
 
 //! items: nitems pairs (descriptor, row).
 hipError_t launch_oz_split(const oz_split_desc *descs, const uint32_t *items,

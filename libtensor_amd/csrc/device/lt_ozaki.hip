@@ -61,12 +61,16 @@ hipError_t launch_oz_split(const oz_split_desc *descs, const uint32_t *items,
 // One wave per 16 x 16 FN C tile, four waves (tiles) per workgroup.
 // v_mfma_i32_16x16x32_i8: lane l supplies A[l % 16][8 (l / 16) + j] and
 // B[8 (l / 16) + j][l % 16], j = 0..7, as 8 packed bytes; it receives
-// D[4 (l / 16) + v][l % 16], v = 0..3.
-template<int FN>
+// D[4 (l / 16) + v][l % 16], v = 0..3.  Per chunk of 32 k the S slices of A
+// and of B are loaded once and every product t + u = L <= S + 1 runs from
+// registers into the int32 accumulator of its weight L; at the end of a
+// pair each weight is scaled by 2^(e_i + f_j - 7 L) and added in FP64 in
+// the order L = 2, 3, ..., S + 1.
+template<int S, int FN>
 __global__ void __launch_bounds__(256)
 oz_gemm_kernel(const gemm_cblock *__restrict__ cbs,
     const oz_pair *__restrict__ pairs,
-    const gemm_tile *__restrict__ tiles, uint32_t ntiles, int slices) {
+    const gemm_tile *__restrict__ tiles, uint32_t ntiles) {
 
     const uint32_t t = blockIdx.x * 4 + (threadIdx.x >> 6);
     if(t >= ntiles) return;           // whole wave: no barrier in this kernel
@@ -93,39 +97,45 @@ oz_gemm_kernel(const gemm_cblock *__restrict__ cbs,
 
     for(uint32_t p = cb.pair_begin; p < cb.pair_end; p++) {
         const oz_pair pr = pairs[p];
+        int4_t acc[S][FN];
+#pragma unroll
+        for(int l = 0; l < S; l++)
+#pragma unroll
+            for(int f = 0; f < FN; f++) acc[l][f] = int4_t{0, 0, 0, 0};
+        const int8_t *pa = pr.a + size_t(arow) * pr.kp + 8 * lg;
+        const int8_t *pb[FN];
+#pragma unroll
+        for(int f = 0; f < FN; f++) pb[f] = pr.b + size_t(bcol[f]) * pr.kp + 8 * lg;
+        for(uint32_t kc = 0; kc < pr.kp; kc += 32) {
+            long av[S], bv[S][FN];
+#pragma unroll
+            for(int ts = 0; ts < S; ts++) av[ts] = *(const long *)(pa + size_t(ts) * pr.astride + kc);
+#pragma unroll
+            for(int us = 0; us < S; us++)
+#pragma unroll
+                for(int f = 0; f < FN; f++)
+                    bv[us][f] = *(const long *)(pb[f] + size_t(us) * pr.bstride + kc);
+#pragma unroll
+            for(int ts = 0; ts < S; ts++)
+#pragma unroll
+                for(int us = 0; us + ts < S; us++)       // t + u <= S + 1 (1-based)
+#pragma unroll
+                    for(int f = 0; f < FN; f++)
+                        acc[ts + us][f] = __builtin_amdgcn_mfma_i32_16x16x32_i8(
+                            av[ts], bv[us][f], acc[ts + us][f], 0, 0, 0);
+        }
         int ea[4], eb[FN];
 #pragma unroll
         for(int v = 0; v < 4; v++) ea[v] = pr.ea[orow[v]];
 #pragma unroll
         for(int f = 0; f < FN; f++) eb[f] = pr.eb[bcol[f]];
-        for(int L = 2; L <= slices + 1; L++) {
-            int4_t acc[FN];
 #pragma unroll
-            for(int f = 0; f < FN; f++) acc[f] = int4_t{0, 0, 0, 0};
-            const int t0 = L - slices > 1 ? L - slices : 1;
-            const int t1 = L - 1 < slices ? L - 1 : slices;
-            for(int ts = t0; ts <= t1; ts++) {
-                const int us = L - ts;
-                const int8_t *pa = pr.a + size_t(ts - 1) * pr.astride + size_t(arow) * pr.kp + 8 * lg;
-                const int8_t *pb[FN];
-#pragma unroll
-                for(int f = 0; f < FN; f++)
-                    pb[f] = pr.b + size_t(us - 1) * pr.bstride + size_t(bcol[f]) * pr.kp + 8 * lg;
-                for(uint32_t kc = 0; kc < pr.kp; kc += 32) {
-                    const long av = *(const long *)(pa + kc);
-#pragma unroll
-                    for(int f = 0; f < FN; f++) {
-                        const long bv = *(const long *)(pb[f] + kc);
-                        acc[f] = __builtin_amdgcn_mfma_i32_16x16x32_i8(av, bv, acc[f], 0, 0, 0);
-                    }
-                }
-            }
+        for(int l = 0; l < S; l++)               // weight L = l + 2
 #pragma unroll
             for(int f = 0; f < FN; f++)
 #pragma unroll
                 for(int v = 0; v < 4; v++)
-                    cacc[f][v] += pr.d * ldexp(double(acc[f][v]), ea[v] + eb[f] - 7 * L);
-        }
+                    cacc[f][v] += pr.d * ldexp(double(acc[l][f][v]), ea[v] + eb[f] - 7 * (l + 2));
     }
 
 #pragma unroll
@@ -144,12 +154,28 @@ oz_gemm_kernel(const gemm_cblock *__restrict__ cbs,
     }
 }
 
+template<int S>
+static void launch_oz_s(const gemm_cblock *cb, const oz_pair *pairs,
+    const gemm_tile *tiles, uint32_t ntiles, hipStream_t s) {
+    hipLaunchKernelGGL((oz_gemm_kernel<S, kOzTileN / 16>), dim3((ntiles + 3) / 4), dim3(256), 0, s,
+        cb, pairs, tiles, ntiles);
+}
+
 hipError_t launch_oz_gemm(const gemm_cblock *cb, const oz_pair *pairs,
     const gemm_tile *tiles, uint32_t ntiles, int slices, hipStream_t s) {
     if(ntiles == 0) return hipSuccess;
-    static_assert(kOzTileN == 64 && kOzTileM == 16, "tile shape");
-    hipLaunchKernelGGL((oz_gemm_kernel<4>), dim3((ntiles + 3) / 4), dim3(256), 0, s,
-        cb, pairs, tiles, ntiles, slices);
+    static_assert(kOzTileM == 16 && kOzTileN % 16 == 0, "tile shape");
+    switch(slices) {
+    case 1: launch_oz_s<1>(cb, pairs, tiles, ntiles, s); break;
+    case 2: launch_oz_s<2>(cb, pairs, tiles, ntiles, s); break;
+    case 3: launch_oz_s<3>(cb, pairs, tiles, ntiles, s); break;
+    case 4: launch_oz_s<4>(cb, pairs, tiles, ntiles, s); break;
+    case 5: launch_oz_s<5>(cb, pairs, tiles, ntiles, s); break;
+    case 6: launch_oz_s<6>(cb, pairs, tiles, ntiles, s); break;
+    case 7: launch_oz_s<7>(cb, pairs, tiles, ntiles, s); break;
+    case 8: launch_oz_s<8>(cb, pairs, tiles, ntiles, s); break;
+    default: return hipErrorInvalidValue;
+    }
     return hipGetLastError();
 }
 
