@@ -209,6 +209,13 @@ int lt_btod_contract2_set_graph(lt_btod_contract2 *op, int on);
  * FP64 BLAS, linalg_cblas_level3.C:10-60). */
 int lt_btod_contract2_set_ozaki(lt_btod_contract2 *op, int slices);
 
+/* ---- btod_dotprod (libtensor/block_tensor/btod_dotprod.h:35-66:
+ *      btod_dotprod(bta, tra, btb, trb).calculate()): sum over all elements
+ *      of ca P_a(A) * cb P_b(B) for resident block tensors whose permuted
+ *      block index spaces agree; perma / permb may be NULL. ---- */
+int lt_btod_dotprod(lt_btensor *a, const size_t *perma, double ca, lt_btensor *b,
+    const size_t *permb, double cb, double *result);
+
 /* ---- request loop: many calculations sharing one GPU.  No counterpart in
  *      the reference (its operations run synchronously in the caller's
  *      thread, libutil/thread_pool/thread_pool.h:22).  Block tensors stay

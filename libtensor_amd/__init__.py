@@ -16,7 +16,7 @@ from ._capi import (GEMM_CONFIGS, LT_MODE_GROUPED, LT_MODE_OZAKI, LT_MODE_PER_BL
 
 __all__ = ["Context", "product_table", "block_index_space", "block_tensor",
            "se_perm", "se_part", "se_label", "contraction2", "btod_contract2",
-           "device_array", "tod_copy", "tod_set", "tod_contract2", "tod_dotprod", "server",
+           "device_array", "tod_copy", "tod_set", "tod_contract2", "tod_dotprod", "btod_dotprod", "server",
            "LtError", "GEMM_CONFIGS", "LT_MODE_PER_BLOCK", "LT_MODE_GROUPED", "LT_MODE_OZAKI"]
 
 
@@ -449,6 +449,15 @@ def tod_dotprod(a, b, perma=None, permb=None, c=1.0):
     pb = None if permb is None else sizes(permb)
     call("lt_tod_dotprod", a.ctx._h, len(a.shape), sizes(a.shape), a.ptr, pa,
          sizes(b.shape), b.ptr, pb, float(c), ctypes.byref(r))
+    return r.value
+
+
+def btod_dotprod(bta, btb, perma=None, permb=None, ca=1.0, cb=1.0):
+    """sum of ca P_a(A) * cb P_b(B) over all elements
+    (libtensor/block_tensor/btod_dotprod.h)."""
+    r = ctypes.c_double()
+    call("lt_btod_dotprod", bta._h, None if perma is None else sizes(perma), float(ca),
+         btb._h, None if permb is None else sizes(permb), float(cb), ctypes.byref(r))
     return r.value
 
 

@@ -854,6 +854,20 @@ int lt_btod_contract2_set_ozaki(lt_btod_contract2 *op, int slices) {
     });
 }
 
+// ---- btod_dotprod ----
+
+int lt_btod_dotprod(lt_btensor *a, const size_t *perma, double ca, lt_btensor *b,
+    const size_t *permb, double cb, double *result) {
+    return guard([&]() {
+        need(a, "a"); need(b, "b"); need(result, "result");
+        size_t n = a->t->bis().order();
+        lt_require(b->t->bis().order() == n, LT_ERR_DIMS, "btod_dotprod: orders differ");
+        tensor_tr tra(perm_or_identity(n, perma), scalar_tr(ca));
+        tensor_tr trb(perm_or_identity(n, permb), scalar_tr(cb));
+        *result = btod_dotprod(*a->t, tra, *b->t, trb);
+    });
+}
+
 // ---- request loop ----
 
 int lt_server_create(lt_ctx *ctx, size_t max_batch, size_t hbm_reserve, lt_server **out) {

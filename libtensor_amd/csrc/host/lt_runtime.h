@@ -143,6 +143,9 @@ void make_chunks(const std::vector<dev::perm_desc> &ds, std::vector<uint32_t> &c
 dev::perm_desc make_perm_desc_public(const double *src, const dims_t &dsrc,
     const tensor_tr &tr, double *dst, bool accumulate);
 
+//! btod_dotprod(A, tra, B, trb).calculate() on resident block tensors.
+double btod_dotprod(btensor &a, const tensor_tr &tra, btensor &b, const tensor_tr &trb);
+
 enum exec_mode { MODE_PER_BLOCK = 0, MODE_GROUPED = 1, MODE_OZAKI = 2 };
 
 struct exec_stats {

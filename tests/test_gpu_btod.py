@@ -197,3 +197,30 @@ def test_plan_not_reused_for_a_new_result_tensor(gpu_ctx):
         assert O.rel_err(c.to_dense(), result) < TOL, rep
         del c
         gc.collect()
+
+
+@pytest.mark.parametrize("name", ["ladder_full", "ladder_spin", "rand_222", "randsym_222",
+                                  "self_ladder", "ring_small"])
+def test_btod_dotprod(gpu_ctx, name):
+    """btod_dotprod on resident tensors against the dense oracle: the result
+    with itself and with permutations of itself (index pairs exchanged), and
+    an input with itself."""
+    import itertools
+    prob, dense, meta, result = G.load(name)
+    a, b, op = P.build(gpu_ctx, prob, dense)
+    c = P.make_result_tensor(gpu_ctx, prob)
+    op.perform(c)
+    R = c.to_dense()
+    norm = float(np.sum(R * R))
+    for perm in list(itertools.permutations(range(R.ndim)))[:8]:
+        if np.transpose(R, perm).shape != R.shape:
+            continue
+        try:
+            got = lt.btod_dotprod(c, c, None, perm, 1.0, -0.5)
+        except lt.LtError:
+            continue            # block index spaces differ under this permutation
+        exp = O.dotprod(R, R, None, perm, -0.5)
+        assert abs(got - exp) <= 1e-12 * norm, perm
+    A = a.to_dense()
+    got = lt.btod_dotprod(a, a)
+    assert abs(got - float(np.sum(A * A))) <= 1e-12 * float(np.sum(A * A))
