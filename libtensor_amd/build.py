@@ -45,14 +45,32 @@ def _run(cmd):
     return out
 
 
+def _check_no_scratch(src, out):
+    """Kernels must not spill to scratch memory (a register-indexed
+    descriptor or a spill turns into per-lane memory traffic)."""
+    name = None
+    bad = []
+    for line in out.splitlines():
+        if "Function Name:" in line:
+            name = line.split("Function Name:")[1].split("[")[0].strip()
+        elif "ScratchSize [bytes/lane]:" in line:
+            n = int(line.split("ScratchSize [bytes/lane]:")[1].split("[")[0].strip())
+            if n:
+                bad.append("%s (%d bytes/lane)" % (name, n))
+    if bad:
+        raise RuntimeError("%s: kernels use scratch memory: %s" % (src, ", ".join(bad)))
+
+
 def _compile(src):
     path = os.path.join(CSRC, src)
     obj = os.path.join(BUILD, src.replace("/", "_") + ".o")
     if src.endswith(".hip"):
-        cmd = [HIPCC] + DEV_FLAGS + ["-c", path, "-o", obj]
+        cmd = [HIPCC] + DEV_FLAGS + ["-Rpass-analysis=kernel-resource-usage",
+                                     "-c", path, "-o", obj]
+        _check_no_scratch(src, _run(cmd))
     else:
         cmd = ["g++"] + HOST_FLAGS + ["-c", path, "-o", obj]
-    _run(cmd)
+        _run(cmd)
     return obj
 
 

@@ -37,7 +37,9 @@ struct perm_desc {
     //! (tile_p, n - 1) staged in LDS, read and written contiguously.
     //! -1: element-wise gather.
     int32_t tile_p;
+    uint32_t tsq;                     //!< source stride of dimension n - 1
     fastdiv dps;                      //!< destination stride of tile_p
+    fastdiv tdp, tdq;                 //!< extents of tile_p and n - 1
 };
 static const uint32_t kPermTile = 32;
 static const uint32_t kPermTileItem = 0x80000000u;

@@ -77,13 +77,15 @@ permute_kernel(const perm_desc *__restrict__ descs,
     if(cd & kPermTileItem) {
         // 32 x 32 tile over destination dimensions (p, q = n - 1); the
         // source is contiguous along p, the destination along q
-        const uint32_t p = uint32_t(d.tile_p), q = d.n - 1;
-        const uint32_t dq = d.div[q].d, dp = d.div[p].d;
-        const uint32_t q0 = base - fdiv(base, d.div[q]) * dq;
+        // (extents and strides from dedicated fields: indexing the
+        // descriptor's arrays with a runtime dimension would put it in
+        // scratch memory)
+        const uint32_t dq = d.tdq.d, dp = d.tdp.d;
+        const uint32_t q0 = base - fdiv(base, d.tdq) * dq;
         const uint32_t e1 = fdiv(base, d.dps);
-        const uint32_t p0 = e1 - fdiv(e1, d.div[p]) * dp;
+        const uint32_t p0 = e1 - fdiv(e1, d.tdp) * dp;
         const uint32_t np = min(kPermTile, dp - p0), nq = min(kPermTile, dq - q0);
-        const uint32_t s0 = src_offset(d, base), sq = d.sstride[q], dps = d.dps.d;
+        const uint32_t s0 = src_offset(d, base), sq = d.tsq, dps = d.dps.d;
         const uint32_t ti = threadIdx.x & 31, tj = threadIdx.x >> 5;
 #pragma unroll
         for(uint32_t r = 0; r < kPermTile; r += 8) {

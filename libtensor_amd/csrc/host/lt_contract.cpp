@@ -76,14 +76,18 @@ dev::perm_desc make_perm_desc(const double *src, const dims_t &dsrc,
     // transposing: the source's unit-stride dimension is not the
     // destination's fastest one
     d.tile_p = -1;
-    d.dps = dev::make_fastdiv(1);
+    d.tsq = 0;
+    d.dps = d.tdp = d.tdq = dev::make_fastdiv(1);
     if(nf >= 2 && fs[nf - 1] != 1) {
         for(size_t i = 0; i + 1 < nf; i++) {
             if(fs[i] != 1) continue;
             size_t st = 1;
             for(size_t j = i + 1; j < nf; j++) st *= fd[j];
             d.tile_p = (int32_t)i;
+            d.tsq = (uint32_t)fs[nf - 1];
             d.dps = dev::make_fastdiv((uint32_t)st);
+            d.tdp = dev::make_fastdiv((uint32_t)fd[i]);
+            d.tdq = dev::make_fastdiv((uint32_t)fd[nf - 1]);
             break;
         }
     }
@@ -172,7 +176,7 @@ struct contract_plan {
     std::vector<size_t> c_out;                        // canonical C blocks written
     std::vector<std::pair<double*, size_t> > c_out_mem; // their storage
     bool ta, tb;
-    int tile_cfg;
+    int tile_cfg, tile_order;
 
     // device copies
     dev_lists lists, tiles;
@@ -203,7 +207,7 @@ struct contract_plan {
     uint32_t n_oz_tiles;
 
     contract_plan() : va(0), vb(0), cuid(0), c(0), add(false), d(1.0), empty(false),
-        ta(false), tb(false), tile_cfg(0), d_prep(0), d_wback(0), d_prep_cd(0),
+        ta(false), tb(false), tile_cfg(0), tile_order(0), d_prep(0), d_wback(0), d_prep_cd(0),
         d_prep_co(0), d_wb_cd(0), d_wb_co(0), n_prep_chunks(0), n_wb_chunks(0),
         d_cb(0), d_pairs(0), d_tiles_sorted(0), d_tiles_block(0), scratch(0),
         scratch_elems(0), ctx(0), events(false), graph(0), gexec(0), graph_launches(0),
@@ -222,7 +226,7 @@ struct contract_plan {
 btod_contract2::btod_contract2(const contraction2 &c, btensor &a, double ka,
     btensor &b, double kb, double kc) :
     m_contr(c), m_a(a), m_b(b), m_ka(ka), m_kb(kb), m_kc(kc),
-    m_tile_cfg(-2), m_tuned_cfg(-1), m_tuned_va(0), m_tuned_vb(0), m_use_graph(true), m_oz_slices(0), m_oz_int8(1) {
+    m_tile_cfg(-2), m_tuned_cfg(-1), m_tuned_order(-1), m_tuned_va(0), m_tuned_vb(0), m_use_graph(true), m_oz_slices(0), m_oz_int8(1) {
 
     lt_require(c.is_complete(), LT_ERR_PARAM, "contraction is incomplete");
     lt_require(a.bis().order() == c.NA() && b.bis().order() == c.NB(),
@@ -556,7 +560,8 @@ void btod_contract2::build_plan(btensor &c, bool add, double dscale) {
 
     // Tile lists of one kernel configuration: tiles grouped by C block (one
     // launch per block) and the same tiles longest-first (grouped launch).
-    // Grouped launch order.  Workgroups are dealt to the XCDs round-robin
+    // Grouped launch orders (the autotuner measures both): 0, tiles longest
+    // first; 1, XCD-aware.  Workgroups are dealt to the XCDs round-robin
     // (workgroup w runs on XCD w % nx), and each XCD has its own L2: C
     // blocks whose pairs read the same B blocks form one group, groups are
     // assigned to XCDs longest-processing-time first, and each XCD's tiles
@@ -597,7 +602,7 @@ void btod_contract2::build_plan(btensor &c, bool add, double dscale) {
 
     // Tile lists of one kernel configuration: tiles grouped by C block (one
     // launch per block) and the grouped-launch order above.
-    auto make_tiles = [&](int cfg, std::vector<dev::gemm_tile> &by_block,
+    auto make_tiles = [&](int cfg, int order, std::vector<dev::gemm_tile> &by_block,
         std::vector<uint32_t> &begin, std::vector<dev::gemm_tile> &sorted) {
         uint32_t bm = 64, bn = 64;
         dev::tile_shape(cfg, bm, bn);
@@ -623,6 +628,16 @@ void btod_contract2::build_plan(btensor &c, bool add, double dscale) {
         begin.push_back((uint32_t)by_block.size());
         begin.push_back((uint32_t)by_block.size());     // the empty C block
         lt_require(by_block.size() < (size_t(1) << 30), LT_ERR_PARAM, "too many tiles");
+        if(order == 0) {
+            // longest (largest k sum) first over the whole device
+            std::vector<size_t> idx(by_block.size());
+            std::iota(idx.begin(), idx.end(), size_t(0));
+            std::stable_sort(idx.begin(), idx.end(), [&](size_t x, size_t y) {
+                return cb_ksum[by_block[x].cblock] > cb_ksum[by_block[y].cblock]; });
+            sorted.resize(idx.size());
+            for(size_t i = 0; i < idx.size(); i++) sorted[i] = by_block[idx[i]];
+            return;
+        }
         // groups to XCDs, longest first onto the least loaded
         std::vector<size_t> gorder(ngroups);
         std::iota(gorder.begin(), gorder.end(), size_t(0));
@@ -740,8 +755,8 @@ void btod_contract2::build_plan(btensor &c, bool add, double dscale) {
     pl.d_pairs = (dev::gemm_pair*)(base + o_pr);
 
     // tile lists live in their own allocation (replaced by the autotuner)
-    auto upload_tiles = [&](int k) {
-        make_tiles(k, pl.tiles_by_block, pl.block_tile_begin, pl.tiles_sorted);
+    auto upload_tiles = [&](int k, int order) {
+        make_tiles(k, order, pl.tiles_by_block, pl.block_tile_begin, pl.tiles_sorted);
         size_t nt = pl.tiles_sorted.size() * sizeof(dev::gemm_tile);
         size_t tsz = 2 * align256(nt);
         if(pl.tiles.mem) ctx.free(pl.tiles.mem);
@@ -757,8 +772,10 @@ void btod_contract2::build_plan(btensor &c, bool add, double dscale) {
         pl.d_tiles_sorted = (dev::gemm_tile*)pl.tiles.mem;
         pl.d_tiles_block = (dev::gemm_tile*)((char*)pl.tiles.mem + align256(nt));
         pl.tile_cfg = k;
+        pl.tile_order = order;
     };
-    upload_tiles(cfg);
+    int order = (cfg == m_tuned_cfg && m_tuned_order >= 0) ? m_tuned_order : 0;
+    upload_tiles(cfg, order);
 
     ctx.activate();
     for(int i = 0; i < 4; i++) hip_check(hipEventCreate(&pl.ev[i]), "hipEventCreate");
@@ -774,10 +791,10 @@ void btod_contract2::build_plan(btensor &c, bool add, double dscale) {
             hip_check(dev::launch_permute(pl.d_prep, pl.d_prep_cd, pl.d_prep_co,
                 pl.n_prep_chunks, s), "permute (prepare)");
         float best = 1e30f;
-        int bestk = cfg;
-        for(size_t ci = 0; ci < candidates.size(); ci++) {
-            int k = candidates[ci];
-            upload_tiles(k);
+        int bestk = cfg, besto = 0;
+        for(size_t ci = 0; ci < 2 * candidates.size(); ci++) {
+            int k = candidates[ci / 2], o = int(ci % 2);
+            upload_tiles(k, o);
             uint32_t nt = (uint32_t)pl.tiles_sorted.size();
             // one timed launch; a short one (first-launch overheads are
             // then not negligible) is timed again
@@ -791,12 +808,13 @@ void btod_contract2::build_plan(btensor &c, bool add, double dscale) {
                 hip_check(hipEventElapsedTime(&ms, pl.ev[0], pl.ev[1]), "hipEventElapsedTime");
                 if(ms > 20.f) break;
             }
-            m_stats.tune_ms[k] = ms;
-            if(ms < best) { best = ms; bestk = k; }
+            if(o == 0 || ms < m_stats.tune_ms[k]) m_stats.tune_ms[k] = ms;
+            if(ms < best) { best = ms; bestk = k; besto = o; }
         }
         cfg = bestk;
-        upload_tiles(cfg);
+        upload_tiles(cfg, besto);
         m_tuned_cfg = cfg;
+        m_tuned_order = besto;
         m_tuned_va = pl.va;
         m_tuned_vb = pl.vb;
     }

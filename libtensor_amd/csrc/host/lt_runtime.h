@@ -238,6 +238,7 @@ private:
     exec_stats m_stats;
     int m_tile_cfg;         //!< >= 0 fixed, -1 estimate, -2 autotune
     int m_tuned_cfg;        //!< configuration found by the autotuner
+    int m_tuned_order;      //!< and its grouped-launch tile order
     uint64_t m_tuned_va, m_tuned_vb;   //!< operand versions it was found for
     bool m_use_graph;
     int m_oz_slices, m_oz_int8;
