@@ -75,12 +75,49 @@ struct gemm_tile {
 //! operands; SPLITK_*_Wn: one workgroup of n waves per tile, the tile's
 //! k chunks dealt to the waves, operands loaded straight into MFMA
 //! fragments, partial tiles summed in LDS in a fixed order.
+//! STACK_*: the split-K kernels over stacks (C blocks sharing their B
+//! operands, rows concatenated); used by grouped launches, per-block
+//! launches run the SPLITK configuration of the same shape.
 enum { TILE_64x64 = 0, TILE_32x64 = 1, TILE_64x32 = 2, TILE_32x32 = 3,
        SPLITK_16x64_W4 = 4, SPLITK_32x32_W4 = 5, SPLITK_32x64_W4 = 6,
        SPLITK_64x32_W4 = 7, SPLITK_16x64_W2 = 8, SPLITK_32x32_W2 = 9,
        SPLITK_16x32_W4 = 10, SPLITK_32x16_W4 = 11,
-       TILE_COUNT = 12 };
+       STACK_16x64_W4 = 12, STACK_32x32_W4 = 13, STACK_32x64_W4 = 14,
+       STACK_64x32_W4 = 15,
+       TILE_COUNT = 16 };
 void tile_shape(int cfg, uint32_t &bm, uint32_t &bn);
+inline bool is_stacked(int cfg) { return cfg >= STACK_16x64_W4 && cfg <= STACK_64x32_W4; }
+//! The per-C-block configuration with the same tile and summation order.
+inline int unstacked(int cfg) { return is_stacked(cfg) ? cfg - STACK_16x64_W4 + SPLITK_16x64_W4 : cfg; }
+
+//! A stack: C blocks whose contraction lists read the same B operands in the
+//! same order (same layouts and extents); their rows are concatenated
+//! (rows in all), so a tile reads each B chunk once for rows of several C
+//! blocks.  Pair j of the stack is stack_pair[pair_begin + j]; row r of the
+//! stack reads A through arow[arow_begin + j * rows + r] and writes through
+//! crow[crow_begin + r].
+struct stack_desc {
+    uint32_t rows, n, npairs, pair_begin;
+    uint32_t arow_begin, crow_begin, pad0, pad1;
+};
+struct stack_pair {
+    const double *b;
+    uint32_t k, ldb, flags, pad;      //!< flags: kPairAKMajor / kPairBNMajor
+};
+//! A operand of one row for one pair: row r of A at p[k] (or p[k * lda]
+//! with kPairAKMajor), coefficient d.
+struct stack_arow {
+    const double *p;
+    double d;
+    uint32_t lda, pad;
+};
+struct stack_crow {
+    double *c;                        //!< start of the row in its C block
+    uint32_t zero, pad;
+};
+hipError_t launch_gemm_stacked(int cfg, const stack_desc *st, const stack_pair *sp,
+    const stack_arow *ar, const stack_crow *cr, const gemm_tile *tiles, uint32_t ntiles,
+    hipStream_t s);
 
 //! Batched permutation: work item j is chunk_desc[j] / chunk_off[j]: a
 //! descriptor and the first destination element of a chunk of kPermChunk

@@ -36,10 +36,10 @@ void tile_shape(int cfg, uint32_t &bm, uint32_t &bn) {
     case TILE_32x64: bm = 32; bn = 64; break;
     case TILE_64x32: bm = 64; bn = 32; break;
     case TILE_32x32: bm = 32; bn = 32; break;
-    case SPLITK_16x64_W4: case SPLITK_16x64_W2: bm = 16; bn = 64; break;
-    case SPLITK_32x32_W4: case SPLITK_32x32_W2: bm = 32; bn = 32; break;
-    case SPLITK_32x64_W4: bm = 32; bn = 64; break;
-    case SPLITK_64x32_W4: bm = 64; bn = 32; break;
+    case SPLITK_16x64_W4: case SPLITK_16x64_W2: case STACK_16x64_W4: bm = 16; bn = 64; break;
+    case SPLITK_32x32_W4: case SPLITK_32x32_W2: case STACK_32x32_W4: bm = 32; bn = 32; break;
+    case SPLITK_32x64_W4: case STACK_32x64_W4: bm = 32; bn = 64; break;
+    case SPLITK_64x32_W4: case STACK_64x32_W4: bm = 64; bn = 32; break;
     case SPLITK_16x32_W4: bm = 16; bn = 32; break;
     case SPLITK_32x16_W4: bm = 32; bn = 16; break;
     default: bm = 64; bn = 64; break;
@@ -568,11 +568,255 @@ static hipError_t launch_direct_t(const gemm_cblock *cb,
     return hipGetLastError();
 }
 
+// ---------------------------------------------------------------------------
+// Stacked split-K FP64 MFMA GEMM: the direct kernel's scheme over a stack of
+// C blocks (rows of several blocks in one tile, one read of each B chunk for
+// all of them).  Per pair, each lane takes the A row of its fragment rows
+// from the stack's row table (row start, coefficient, stride) and the C row
+// from the stack's output-row table.  The chunks of the stack's pair list
+// are dealt to the W waves as in the direct kernel, so a C element gets the
+// same sums in the same order as there.
+
+template<int FM>
+struct st_rowsrc {
+    const double *p[FM];
+    double d[FM];
+    uint32_t lda[FM];
+};
+
+template<int FM, int FN, bool TA, bool TB>
+__device__ __forceinline__ void load_chunk_st(double (&af)[FM][4], double (&bf)[FN][4],
+    const st_rowsrc<FM> &ar, const stack_pair &sp, int c, const int (&bcol)[FN], int lg) {
+    const int K = int(sp.k);
+    const int kk = 16 * c + 4 * lg;
+    gdouble *pb = (gdouble *)sp.b;
+    int kc[4];
+    bool kv[4];
+#pragma unroll
+    for(int j = 0; j < 4; j++) {
+        kv[j] = kk + j < K;
+        kc[j] = kv[j] ? kk + j : K - 1;
+    }
+#pragma unroll
+    for(int i = 0; i < FM; i++) {
+        gdouble *pa = (gdouble *)ar.p[i];
+#pragma unroll
+        for(int j = 0; j < 4; j++) {
+            const size_t off = TA ? size_t(kc[j]) * ar.lda[i] : size_t(kc[j]);
+            const double x = pa[off];
+            af[i][j] = kv[j] ? x : 0.0;
+        }
+    }
+#pragma unroll
+    for(int f = 0; f < FN; f++) {
+#pragma unroll
+        for(int j = 0; j < 4; j++) {
+            const size_t off = TB ? size_t(bcol[f]) * sp.ldb + kc[j]
+                                  : size_t(kc[j]) * sp.ldb + bcol[f];
+            const double x = pb[off];
+            bf[f][j] = kv[j] ? x : 0.0;
+        }
+    }
+}
+
+template<int FM, int FN>
+__device__ __forceinline__ void load_chunk_st_any(double (&af)[FM][4], double (&bf)[FN][4],
+    const st_rowsrc<FM> &ar, const stack_pair &sp, int c, const int (&bcol)[FN], int lg) {
+    switch(sp.flags & (kPairAKMajor | kPairBNMajor)) {
+    case 0: load_chunk_st<FM, FN, false, false>(af, bf, ar, sp, c, bcol, lg); break;
+    case kPairAKMajor: load_chunk_st<FM, FN, true, false>(af, bf, ar, sp, c, bcol, lg); break;
+    case kPairBNMajor: load_chunk_st<FM, FN, false, true>(af, bf, ar, sp, c, bcol, lg); break;
+    default: load_chunk_st<FM, FN, true, true>(af, bf, ar, sp, c, bcol, lg); break;
+    }
+}
+
+template<int FM>
+__device__ __forceinline__ void st_rows(st_rowsrc<FM> &r, const stack_arow *__restrict__ arow,
+    uint32_t base, const int (&srow)[FM]) {
+#pragma unroll
+    for(int i = 0; i < FM; i++) {
+        const stack_arow e = arow[base + srow[i]];
+        r.p[i] = e.p;
+        r.d[i] = e.d;
+        r.lda[i] = e.lda;
+    }
+}
+
+template<int FM, int FN>
+__device__ __forceinline__ void mma_chunk_st(double4_t (&acc)[FM][FN],
+    double (&af)[FM][4], const double (&bf)[FN][4], const st_rowsrc<FM> &r) {
+#pragma unroll
+    for(int i = 0; i < FM; i++)
+#pragma unroll
+        for(int q = 0; q < 4; q++) af[i][q] *= r.d[i];
+#pragma unroll
+    for(int q = 0; q < 4; q++)
+#pragma unroll
+        for(int i = 0; i < FM; i++)
+#pragma unroll
+            for(int j = 0; j < FN; j++)
+                acc[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(
+                    af[i][q], bf[j][q], acc[i][j], 0, 0, 0);
+}
+
+__device__ __forceinline__ void seek_chunk_st(const stack_pair *__restrict__ sp,
+    uint32_t end, uint32_t &p, int &c) {
+    while(p < end) {
+        const int nch = (int(sp[p].k) + 15) >> 4;
+        if(c < nch) break;
+        c -= nch;
+        p++;
+    }
+}
+
+template<int FM, int FN, int W>
+__global__ void __launch_bounds__(64 * W)
+gemm_stacked_kernel(const stack_desc *__restrict__ stacks,
+    const stack_pair *__restrict__ spairs, const stack_arow *__restrict__ arow,
+    const stack_crow *__restrict__ crow, const gemm_tile *__restrict__ tiles,
+    uint32_t ntiles) {
+
+    constexpr int TM = 16 * FM, TN = 16 * FN;
+    __shared__ double red[(W > 1 ? W - 1 : 1) * FM * FN * 4 * 64];
+    const int lane = threadIdx.x & 63;
+    const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    const int lr = lane & 15, lg = lane >> 4;
+
+    for(uint32_t t = blockIdx.x; t < ntiles; t += gridDim.x) {
+        const gemm_tile tile = tiles[t];
+        const stack_desc st = stacks[tile.cblock];
+        const int m0 = int(tile.tm) * TM, n0 = int(tile.tn) * TN;
+        const int mrem = int(st.rows) - m0, nrem = int(st.n) - n0;
+
+        double4_t acc[FM][FN];
+#pragma unroll
+        for(int i = 0; i < FM; i++)
+#pragma unroll
+            for(int j = 0; j < FN; j++) acc[i][j] = double4_t{0.0, 0.0, 0.0, 0.0};
+
+        int srow[FM], bcol[FN];
+#pragma unroll
+        for(int i = 0; i < FM; i++) srow[i] = (16 * i + lr) < mrem ? m0 + 16 * i + lr : 0;
+#pragma unroll
+        for(int j = 0; j < FN; j++) bcol[j] = (16 * j + lr) < nrem ? n0 + 16 * j + lr : 0;
+
+        const uint32_t pbeg = st.pair_begin, pend = st.pair_begin + st.npairs;
+        uint32_t px = pbeg;
+        int cx = wave;
+        seek_chunk_st(spairs, pend, px, cx);
+        if(px < pend) {
+            double ax[FM][4], bx[FN][4], ay[FM][4], by[FN][4];
+            st_rowsrc<FM> rx, ry;
+            stack_pair sx = spairs[px];
+            st_rows<FM>(rx, arow, st.arow_begin + (px - pbeg) * st.rows, srow);
+            load_chunk_st_any<FM, FN>(ax, bx, rx, sx, cx, bcol, lg);
+            while(true) {
+                uint32_t py = px;
+                int cy = cx + W;
+                seek_chunk_st(spairs, pend, py, cy);
+                const bool hy = py < pend;
+                if(!hy) { py = px; cy = cx; }       // dummy in-bounds load
+                const stack_pair sy = spairs[py];
+                if(py == px) ry = rx;
+                else st_rows<FM>(ry, arow, st.arow_begin + (py - pbeg) * st.rows, srow);
+                load_chunk_st_any<FM, FN>(ay, by, ry, sy, cy, bcol, lg);
+                mma_chunk_st<FM, FN>(acc, ax, bx, rx);
+                if(!hy) break;
+                uint32_t pz = py;
+                int cz = cy + W;
+                seek_chunk_st(spairs, pend, pz, cz);
+                const bool hz = pz < pend;
+                if(!hz) { pz = py; cz = cy; }
+                sx = spairs[pz];
+                if(pz == py) rx = ry;
+                else st_rows<FM>(rx, arow, st.arow_begin + (pz - pbeg) * st.rows, srow);
+                load_chunk_st_any<FM, FN>(ax, bx, rx, sx, cz, bcol, lg);
+                mma_chunk_st<FM, FN>(acc, ay, by, ry);
+                if(!hz) break;
+                px = pz;
+                cx = cz;
+            }
+        }
+
+        if(W > 1) {
+            if(wave > 0) {
+                double *r = red + (wave - 1) * (FM * FN * 4 * 64);
+#pragma unroll
+                for(int i = 0; i < FM; i++)
+#pragma unroll
+                    for(int j = 0; j < FN; j++)
+#pragma unroll
+                        for(int v = 0; v < 4; v++)
+                            r[((i * FN + j) * 4 + v) * 64 + lane] = acc[i][j][v];
+            }
+            __syncthreads();
+            if(wave == 0) {
+                for(int w = 1; w < W; w++) {
+                    const double *r = red + (w - 1) * (FM * FN * 4 * 64);
+#pragma unroll
+                    for(int i = 0; i < FM; i++)
+#pragma unroll
+                        for(int j = 0; j < FN; j++)
+#pragma unroll
+                            for(int v = 0; v < 4; v++)
+                                acc[i][j][v] += r[((i * FN + j) * 4 + v) * 64 + lane];
+                }
+            }
+        }
+
+        if(wave == 0) {
+#pragma unroll
+            for(int i = 0; i < FM; i++) {
+#pragma unroll
+                for(int v = 0; v < 4; v++) {
+                    const int row = 16 * i + lg + 4 * v;
+                    if(row >= mrem) continue;
+                    const stack_crow cr = crow[st.crow_begin + m0 + row];
+#pragma unroll
+                    for(int j = 0; j < FN; j++) {
+                        const int col = 16 * j + lr;
+                        if(col < nrem) {
+                            double *cp = cr.c + (n0 + col);
+                            double x = acc[i][j][v];
+                            if(!cr.zero) x += *cp;
+                            *cp = x;
+                        }
+                    }
+                }
+            }
+        }
+        if(W > 1) __syncthreads();   // red is reused by the next tile
+    }
+}
+
+template<int FM, int FN, int W>
+static hipError_t launch_stacked_t(const stack_desc *st, const stack_pair *sp,
+    const stack_arow *ar, const stack_crow *cr, const gemm_tile *tiles, uint32_t ntiles,
+    hipStream_t s) {
+    hipLaunchKernelGGL((gemm_stacked_kernel<FM, FN, W>), dim3(ntiles), dim3(64 * W), 0, s,
+        st, sp, ar, cr, tiles, ntiles);
+    return hipGetLastError();
+}
+
+hipError_t launch_gemm_stacked(int cfg, const stack_desc *st, const stack_pair *sp,
+    const stack_arow *ar, const stack_crow *cr, const gemm_tile *tiles, uint32_t ntiles,
+    hipStream_t s) {
+    if(ntiles == 0) return hipSuccess;
+    switch(cfg) {
+    case STACK_16x64_W4: return launch_stacked_t<1, 4, 4>(st, sp, ar, cr, tiles, ntiles, s);
+    case STACK_32x32_W4: return launch_stacked_t<2, 2, 4>(st, sp, ar, cr, tiles, ntiles, s);
+    case STACK_32x64_W4: return launch_stacked_t<2, 4, 4>(st, sp, ar, cr, tiles, ntiles, s);
+    case STACK_64x32_W4: return launch_stacked_t<4, 2, 4>(st, sp, ar, cr, tiles, ntiles, s);
+    default: return hipErrorInvalidValue;
+    }
+}
+
 hipError_t launch_gemm(int cfg, const gemm_cblock *cb,
     const gemm_pair *pairs, const gemm_tile *tiles, uint32_t ntiles,
     uint32_t nwg, hipStream_t s) {
     if(ntiles == 0) return hipSuccess;
     if(nwg == 0 || nwg > ntiles) nwg = ntiles;
+    cfg = unstacked(cfg);
     switch(cfg) {
     case TILE_32x64: return launch_gemm_t<32, 64>(cb, pairs, tiles, ntiles, nwg, s);
     case TILE_64x32: return launch_gemm_t<64, 32>(cb, pairs, tiles, ntiles, nwg, s);

@@ -19,6 +19,7 @@
 #include "lt_runtime.h"
 
 #include <map>
+#include <tuple>
 
 #include <chrono>
 #include <cmath>
@@ -180,6 +181,19 @@ struct contract_plan {
 
     // device copies
     dev_lists lists, tiles;
+    // stacks (C blocks sharing their B operands) for the STACK_* kernels
+    std::vector<dev::stack_desc> stacks;
+    std::vector<dev::stack_pair> spairs;
+    std::vector<dev::stack_arow> arows;
+    std::vector<dev::stack_crow> crows;
+    std::vector<double> stack_ksum;
+    std::vector<dev::gemm_tile> stiles;
+    dev_lists stk, stk_tiles;
+    dev::stack_desc *d_stacks;
+    dev::stack_pair *d_spairs;
+    dev::stack_arow *d_arows;
+    dev::stack_crow *d_crows;
+    dev::gemm_tile *d_stiles;
     dev::perm_desc *d_prep, *d_wback;
     uint32_t *d_prep_cd, *d_prep_co, *d_wb_cd, *d_wb_co;
     uint32_t n_prep_chunks, n_wb_chunks;
@@ -211,6 +225,7 @@ struct contract_plan {
         d_prep_co(0), d_wb_cd(0), d_wb_co(0), n_prep_chunks(0), n_wb_chunks(0),
         d_cb(0), d_pairs(0), d_tiles_sorted(0), d_tiles_block(0), scratch(0),
         scratch_elems(0), ctx(0), events(false), graph(0), gexec(0), graph_launches(0),
+        d_stacks(0), d_spairs(0), d_arows(0), d_crows(0), d_stiles(0),
         oz_slices(0), d_oz_split(0), d_oz_items(0), n_oz_items(0), d_oz_pairs(0),
         d_oz_tiles(0), n_oz_tiles(0) { }
     ~contract_plan() {
@@ -220,6 +235,15 @@ struct contract_plan {
         if(events) for(int i = 0; i < 4; i++) hipEventDestroy(ev[i]);
     }
 };
+
+//! The grouped GEMM launch of a plan in configuration k.
+hipError_t launch_grouped(const contract_plan &pl, int k, hipStream_t s) {
+    if(dev::is_stacked(k))
+        return dev::launch_gemm_stacked(k, pl.d_stacks, pl.d_spairs, pl.d_arows, pl.d_crows,
+            pl.d_stiles, (uint32_t)pl.stiles.size(), s);
+    uint32_t nt = (uint32_t)pl.tiles_sorted.size();
+    return dev::launch_gemm(k, pl.d_cb, pl.d_pairs, pl.d_tiles_sorted, nt, nt, s);
+}
 
 // ---------------------------------------------------------------------------
 
@@ -558,6 +582,81 @@ void btod_contract2::build_plan(btensor &c, bool add, double dscale) {
 
     pl.cb_ksum = cb_ksum;
 
+    // Pairs of every C block in a fixed order of their B operands, so that C
+    // blocks reading the same B operands list them identically; such blocks
+    // form a stack (rows concatenated, each B chunk read once for all).
+    typedef std::tuple<uintptr_t, uint32_t, uint32_t, uint32_t> bkey_t;
+    auto bkey = [](const dev::gemm_pair &g) {
+        return bkey_t((uintptr_t)g.b, g.flags, g.ldb, g.k);
+    };
+    for(size_t b = 0; b < pl.cblocks.size(); b++) {
+        std::stable_sort(pl.pairs.begin() + pl.cblocks[b].pair_begin,
+            pl.pairs.begin() + pl.cblocks[b].pair_end,
+            [&](const dev::gemm_pair &x, const dev::gemm_pair &y) { return bkey(x) < bkey(y); });
+    }
+    {
+        std::map<std::pair<uint32_t, std::vector<bkey_t> >, std::vector<size_t> > groups;
+        std::vector<std::pair<uint32_t, std::vector<bkey_t> > > order;
+        for(size_t b = 0; b < pl.cblocks.size(); b++) {
+            const dev::gemm_cblock &cb = pl.cblocks[b];
+            std::pair<uint32_t, std::vector<bkey_t> > key;
+            key.first = cb.n;
+            for(uint32_t p = cb.pair_begin; p < cb.pair_end; p++) key.second.push_back(bkey(pl.pairs[p]));
+            std::vector<size_t> &g = groups[key];
+            if(g.empty()) order.push_back(key);
+            g.push_back(b);
+        }
+        for(size_t gi = 0; gi < order.size(); gi++) {
+            const std::vector<size_t> &bl = groups[order[gi]];
+            const dev::gemm_cblock &c0 = pl.cblocks[bl[0]];
+            dev::stack_desc sd;
+            std::memset(&sd, 0, sizeof(sd));
+            sd.n = c0.n;
+            sd.npairs = c0.pair_end - c0.pair_begin;
+            sd.pair_begin = (uint32_t)pl.spairs.size();
+            sd.arow_begin = (uint32_t)pl.arows.size();
+            sd.crow_begin = (uint32_t)pl.crows.size();
+            size_t rows = 0;
+            for(size_t q = 0; q < bl.size(); q++) rows += pl.cblocks[bl[q]].m;
+            lt_require(rows < (size_t(1) << 20) && pl.arows.size() + rows * sd.npairs < (size_t(1) << 31),
+                LT_ERR_PARAM, "stack too large");
+            sd.rows = (uint32_t)rows;
+            double ks = 0.0;
+            for(uint32_t j = 0; j < sd.npairs; j++) {
+                const dev::gemm_pair &g = pl.pairs[c0.pair_begin + j];
+                dev::stack_pair sp;
+                std::memset(&sp, 0, sizeof(sp));
+                sp.b = g.b; sp.k = g.k; sp.ldb = g.ldb; sp.flags = g.flags;
+                pl.spairs.push_back(sp);
+                ks += double(g.k);
+                for(size_t q = 0; q < bl.size(); q++) {
+                    const dev::gemm_cblock &cq = pl.cblocks[bl[q]];
+                    const dev::gemm_pair &gq = pl.pairs[cq.pair_begin + j];
+                    for(uint32_t r = 0; r < cq.m; r++) {
+                        dev::stack_arow ar;
+                        std::memset(&ar, 0, sizeof(ar));
+                        ar.p = (gq.flags & dev::kPairAKMajor) ? gq.a + r : gq.a + size_t(r) * gq.lda;
+                        ar.d = gq.d;
+                        ar.lda = gq.lda;
+                        pl.arows.push_back(ar);
+                    }
+                }
+            }
+            for(size_t q = 0; q < bl.size(); q++) {
+                const dev::gemm_cblock &cq = pl.cblocks[bl[q]];
+                for(uint32_t r = 0; r < cq.m; r++) {
+                    dev::stack_crow cr;
+                    std::memset(&cr, 0, sizeof(cr));
+                    cr.c = cq.c + size_t(r) * cq.ldc;
+                    cr.zero = (uint32_t)cq.zero;
+                    pl.crows.push_back(cr);
+                }
+            }
+            pl.stacks.push_back(sd);
+            pl.stack_ksum.push_back(ks);
+        }
+    }
+
     // Tile lists of one kernel configuration: tiles grouped by C block (one
     // launch per block) and the same tiles longest-first (grouped launch).
     // Grouped launch orders (the autotuner measures both): 0, tiles longest
@@ -730,6 +829,10 @@ void btod_contract2::build_plan(btensor &c, bool add, double dscale) {
     size_t o_wco = sz; sz = align256(sz + wco.size() * 4);
     size_t o_cb = sz; sz = align256(sz + pl.cblocks.size() * sizeof(dev::gemm_cblock));
     size_t o_pr = sz; sz = align256(sz + pl.pairs.size() * sizeof(dev::gemm_pair));
+    size_t o_st = sz; sz = align256(sz + pl.stacks.size() * sizeof(dev::stack_desc));
+    size_t o_sp = sz; sz = align256(sz + pl.spairs.size() * sizeof(dev::stack_pair));
+    size_t o_ar = sz; sz = align256(sz + pl.arows.size() * sizeof(dev::stack_arow));
+    size_t o_cr = sz; sz = align256(sz + pl.crows.size() * sizeof(dev::stack_crow));
     std::vector<char> host(sz, 0);
     auto put = [&](size_t off, const void *src, size_t n) { if(n) std::memcpy(&host[off], src, n); };
     put(o_prep, pl.prep.data(), pl.prep.size() * sizeof(dev::perm_desc));
@@ -740,6 +843,10 @@ void btod_contract2::build_plan(btensor &c, bool add, double dscale) {
     put(o_wco, wco.data(), wco.size() * 4);
     put(o_cb, pl.cblocks.data(), pl.cblocks.size() * sizeof(dev::gemm_cblock));
     put(o_pr, pl.pairs.data(), pl.pairs.size() * sizeof(dev::gemm_pair));
+    put(o_st, pl.stacks.data(), pl.stacks.size() * sizeof(dev::stack_desc));
+    put(o_sp, pl.spairs.data(), pl.spairs.size() * sizeof(dev::stack_pair));
+    put(o_ar, pl.arows.data(), pl.arows.size() * sizeof(dev::stack_arow));
+    put(o_cr, pl.crows.data(), pl.crows.size() * sizeof(dev::stack_crow));
     pl.lists.ctx = &ctx;
     pl.lists.mem = ctx.alloc(sz);
     pl.lists.bytes = sz;
@@ -753,6 +860,10 @@ void btod_contract2::build_plan(btensor &c, bool add, double dscale) {
     pl.d_wb_co = (uint32_t*)(base + o_wco);
     pl.d_cb = (dev::gemm_cblock*)(base + o_cb);
     pl.d_pairs = (dev::gemm_pair*)(base + o_pr);
+    pl.d_stacks = (dev::stack_desc*)(base + o_st);
+    pl.d_spairs = (dev::stack_pair*)(base + o_sp);
+    pl.d_arows = (dev::stack_arow*)(base + o_ar);
+    pl.d_crows = (dev::stack_crow*)(base + o_cr);
 
     // tile lists live in their own allocation (replaced by the autotuner)
     auto upload_tiles = [&](int k, int order) {
@@ -773,6 +884,33 @@ void btod_contract2::build_plan(btensor &c, bool add, double dscale) {
         pl.d_tiles_block = (dev::gemm_tile*)((char*)pl.tiles.mem + align256(nt));
         pl.tile_cfg = k;
         pl.tile_order = order;
+        pl.stiles.clear();
+        if(dev::is_stacked(k)) {
+            // stack tiles, longest (largest k sum) first
+            uint32_t bm = 64, bn = 64;
+            dev::tile_shape(k, bm, bn);
+            std::vector<std::pair<double, dev::gemm_tile> > st;
+            for(size_t si = 0; si < pl.stacks.size(); si++) {
+                uint32_t tm = (pl.stacks[si].rows + bm - 1) / bm, tn = (pl.stacks[si].n + bn - 1) / bn;
+                lt_require(tm < 65536 && tn < 65536, LT_ERR_PARAM, "stack too large");
+                for(uint32_t i = 0; i < tm; i++) for(uint32_t j = 0; j < tn; j++) {
+                    dev::gemm_tile t;
+                    t.cblock = (uint32_t)si; t.tm = (uint16_t)i; t.tn = (uint16_t)j;
+                    st.push_back(std::make_pair(pl.stack_ksum[si], t));
+                }
+            }
+            std::stable_sort(st.begin(), st.end(),
+                [](const std::pair<double, dev::gemm_tile> &x, const std::pair<double, dev::gemm_tile> &y) {
+                    return x.first > y.first; });
+            for(size_t i = 0; i < st.size(); i++) pl.stiles.push_back(st[i].second);
+            size_t nb = pl.stiles.size() * sizeof(dev::gemm_tile);
+            if(pl.stk_tiles.mem) ctx.free(pl.stk_tiles.mem);
+            pl.stk_tiles.ctx = &ctx;
+            pl.stk_tiles.mem = ctx.alloc(nb ? nb : 256);
+            pl.stk_tiles.bytes = nb ? nb : 256;
+            if(nb) ctx.upload(pl.stk_tiles.mem, pl.stiles.data(), nb);
+            pl.d_stiles = (dev::gemm_tile*)pl.stk_tiles.mem;
+        }
     };
     int order = (cfg == m_tuned_cfg && m_tuned_order >= 0) ? m_tuned_order : 0;
     upload_tiles(cfg, order);
@@ -794,6 +932,7 @@ void btod_contract2::build_plan(btensor &c, bool add, double dscale) {
         int bestk = cfg, besto = 0;
         for(size_t ci = 0; ci < 2 * candidates.size(); ci++) {
             int k = candidates[ci / 2], o = int(ci % 2);
+            if(o == 1 && dev::is_stacked(k)) continue;     // one order for stacks
             upload_tiles(k, o);
             uint32_t nt = (uint32_t)pl.tiles_sorted.size();
             // one timed launch; a short one (first-launch overheads are
@@ -801,8 +940,7 @@ void btod_contract2::build_plan(btensor &c, bool add, double dscale) {
             float ms = 0.f;
             for(int rep = 0; rep < 2; rep++) {
                 hip_check(hipEventRecord(pl.ev[0], s), "hipEventRecord");
-                hip_check(dev::launch_gemm(k, pl.d_cb, pl.d_pairs,
-                    pl.d_tiles_sorted, nt, nt, s), "autotune");
+                hip_check(launch_grouped(pl, k, s), "autotune");
                 hip_check(hipEventRecord(pl.ev[1], s), "hipEventRecord");
                 hip_check(hipStreamSynchronize(s), "hipStreamSynchronize");
                 hip_check(hipEventElapsedTime(&ms, pl.ev[0], pl.ev[1]), "hipEventElapsedTime");
@@ -974,9 +1112,7 @@ void btod_contract2::enqueue(int mode, hipStream_t s, bool timed, size_t &launch
             launches++;
         }
     } else {
-        uint32_t nt = (uint32_t)pl.tiles_sorted.size();
-        hip_check(dev::launch_gemm(pl.tile_cfg, pl.d_cb, pl.d_pairs,
-            pl.d_tiles_sorted, nt, nt, s), "gemm (grouped)");
+        hip_check(launch_grouped(pl, pl.tile_cfg, s), "gemm (grouped)");
         launches++;
     }
     if(timed) hip_check(hipEventRecord(pl.ev[2], s), "hipEventRecord");
