@@ -380,49 +380,52 @@ __device__ __forceinline__ void seek_chunk(const gemm_pair *__restrict__ pairs,
     }
 }
 
+//! Element of a uniform base pointer at a 32-bit byte offset (one
+//! address register per load, base in scalar registers).
+__device__ __forceinline__ double ldg(const double *base, uint32_t off_bytes) {
+    return *(gdouble *)((const __attribute__((address_space(1))) char *)base + off_bytes);
+}
+
+// Loads of one chunk.  k beyond the pair's extent reads element K - 1 (in
+// bounds); mma_chunk multiplies those A values by zero, so the loaded values
+// are not consumed here and the loads of the next chunk stay in flight
+// while the current chunk is on the matrix cores.  Blocks are < 2^31
+// elements, element offsets in bytes fit in 32 bits.
 template<int FM, int FN, bool TA, bool TB>
 __device__ __forceinline__ void load_chunk(double (&af)[FM][4], double (&bf)[FN][4],
     const gemm_pair &pr, int c, const int (&arow)[FM], const int (&bcol)[FN], int lg) {
-    const int K = int(pr.k);
-    const int kk = 16 * c + 4 * lg;
-    gdouble *pa = (gdouble *)pr.a;
-    gdouble *pb = (gdouble *)pr.b;
-    int kc[4];
-    bool kv[4];
+    const uint32_t K = pr.k;
+    const uint32_t kk = 16 * c + 4 * lg;
+    uint32_t kc[4];
 #pragma unroll
-    for(int j = 0; j < 4; j++) {
-        kv[j] = kk + j < K;
-        kc[j] = kv[j] ? kk + j : K - 1;
-    }
+    for(int j = 0; j < 4; j++) kc[j] = min(kk + j, K - 1);
 #pragma unroll
     for(int i = 0; i < FM; i++) {
+        const uint32_t r = uint32_t(arow[i]);
 #pragma unroll
-        for(int j = 0; j < 4; j++) {
-            const size_t off = TA ? size_t(kc[j]) * pr.lda + arow[i]
-                                  : size_t(arow[i]) * pr.lda + kc[j];
-            const double x = pa[off];
-            af[i][j] = kv[j] ? x : 0.0;
-        }
+        for(int j = 0; j < 4; j++)
+            af[i][j] = ldg(pr.a, 8u * (TA ? kc[j] * pr.lda + r : r * pr.lda + kc[j]));
     }
 #pragma unroll
     for(int f = 0; f < FN; f++) {
+        const uint32_t n = uint32_t(bcol[f]);
 #pragma unroll
-        for(int j = 0; j < 4; j++) {
-            const size_t off = TB ? size_t(bcol[f]) * pr.ldb + kc[j]
-                                  : size_t(kc[j]) * pr.ldb + bcol[f];
-            const double x = pb[off];
-            bf[f][j] = kv[j] ? x : 0.0;
-        }
+        for(int j = 0; j < 4; j++)
+            bf[f][j] = ldg(pr.b, 8u * (TB ? n * pr.ldb + kc[j] : kc[j] * pr.ldb + n));
     }
 }
 
+//! A scaled by the pair's coefficient, by zero for k beyond its extent.
 template<int FM, int FN>
 __device__ __forceinline__ void mma_chunk(double4_t (&acc)[FM][FN],
-    double (&af)[FM][4], const double (&bf)[FN][4], double d) {
+    double (&af)[FM][4], const double (&bf)[FN][4], double d, uint32_t K, int c, int lg) {
+    const uint32_t kk = 16 * c + 4 * lg;
 #pragma unroll
-    for(int i = 0; i < FM; i++)
+    for(int q = 0; q < 4; q++) {
+        const double dq = kk + q < K ? d : 0.0;
 #pragma unroll
-        for(int q = 0; q < 4; q++) af[i][q] *= d;
+        for(int i = 0; i < FM; i++) af[i][q] *= dq;
+    }
 #pragma unroll
     for(int q = 0; q < 4; q++)
 #pragma unroll
@@ -494,7 +497,7 @@ gemm_direct_kernel(const gemm_cblock *__restrict__ cbs,
                 if(!hy) { py = px; cy = cx; }       // dummy in-bounds load
                 const gemm_pair pry = pairs[py];
                 load_chunk_any<FM, FN>(ay, by, pry, cy, arow, bcol, lg);
-                mma_chunk<FM, FN>(acc, ax, bx, prx.d);
+                mma_chunk<FM, FN>(acc, ax, bx, prx.d, prx.k, cx, lg);
                 if(!hy) break;
                 uint32_t pz = py;
                 int cz = cy + W;
@@ -503,7 +506,7 @@ gemm_direct_kernel(const gemm_cblock *__restrict__ cbs,
                 if(!hz) { pz = py; cz = cy; }
                 prx = pairs[pz];
                 load_chunk_any<FM, FN>(ax, bx, prx, cz, arow, bcol, lg);
-                mma_chunk<FM, FN>(acc, ay, by, pry.d);
+                mma_chunk<FM, FN>(acc, ay, by, pry.d, pry.k, cy, lg);
                 if(!hz) break;
                 px = pz;
                 cx = cz;
@@ -587,35 +590,23 @@ struct st_rowsrc {
 template<int FM, int FN, bool TA, bool TB>
 __device__ __forceinline__ void load_chunk_st(double (&af)[FM][4], double (&bf)[FN][4],
     const st_rowsrc<FM> &ar, const stack_pair &sp, int c, const int (&bcol)[FN], int lg) {
-    const int K = int(sp.k);
-    const int kk = 16 * c + 4 * lg;
-    gdouble *pb = (gdouble *)sp.b;
-    int kc[4];
-    bool kv[4];
+    const uint32_t K = sp.k;
+    const uint32_t kk = 16 * c + 4 * lg;
+    uint32_t kc[4];
 #pragma unroll
-    for(int j = 0; j < 4; j++) {
-        kv[j] = kk + j < K;
-        kc[j] = kv[j] ? kk + j : K - 1;
-    }
+    for(int j = 0; j < 4; j++) kc[j] = min(kk + j, K - 1);
 #pragma unroll
     for(int i = 0; i < FM; i++) {
         gdouble *pa = (gdouble *)ar.p[i];
 #pragma unroll
-        for(int j = 0; j < 4; j++) {
-            const size_t off = TA ? size_t(kc[j]) * ar.lda[i] : size_t(kc[j]);
-            const double x = pa[off];
-            af[i][j] = kv[j] ? x : 0.0;
-        }
+        for(int j = 0; j < 4; j++) af[i][j] = pa[TA ? kc[j] * ar.lda[i] : kc[j]];
     }
 #pragma unroll
     for(int f = 0; f < FN; f++) {
+        const uint32_t n = uint32_t(bcol[f]);
 #pragma unroll
-        for(int j = 0; j < 4; j++) {
-            const size_t off = TB ? size_t(bcol[f]) * sp.ldb + kc[j]
-                                  : size_t(kc[j]) * sp.ldb + bcol[f];
-            const double x = pb[off];
-            bf[f][j] = kv[j] ? x : 0.0;
-        }
+        for(int j = 0; j < 4; j++)
+            bf[f][j] = ldg(sp.b, 8u * (TB ? n * sp.ldb + kc[j] : kc[j] * sp.ldb + n));
     }
 }
 
@@ -644,11 +635,15 @@ __device__ __forceinline__ void st_rows(st_rowsrc<FM> &r, const stack_arow *__re
 
 template<int FM, int FN>
 __device__ __forceinline__ void mma_chunk_st(double4_t (&acc)[FM][FN],
-    double (&af)[FM][4], const double (&bf)[FN][4], const st_rowsrc<FM> &r) {
+    double (&af)[FM][4], const double (&bf)[FN][4], const st_rowsrc<FM> &r,
+    uint32_t K, int c, int lg) {
+    const uint32_t kk = 16 * c + 4 * lg;
 #pragma unroll
-    for(int i = 0; i < FM; i++)
+    for(int q = 0; q < 4; q++) {
+        const bool kv = kk + q < K;
 #pragma unroll
-        for(int q = 0; q < 4; q++) af[i][q] *= r.d[i];
+        for(int i = 0; i < FM; i++) af[i][q] *= kv ? r.d[i] : 0.0;
+    }
 #pragma unroll
     for(int q = 0; q < 4; q++)
 #pragma unroll
@@ -720,7 +715,7 @@ gemm_stacked_kernel(const stack_desc *__restrict__ stacks,
                 if(py == px) ry = rx;
                 else st_rows<FM>(ry, arow, st.arow_begin + (py - pbeg) * st.rows, srow);
                 load_chunk_st_any<FM, FN>(ay, by, ry, sy, cy, bcol, lg);
-                mma_chunk_st<FM, FN>(acc, ax, bx, rx);
+                mma_chunk_st<FM, FN>(acc, ax, bx, rx, sx.k, cx, lg);
                 if(!hy) break;
                 uint32_t pz = py;
                 int cz = cy + W;
@@ -731,7 +726,7 @@ gemm_stacked_kernel(const stack_desc *__restrict__ stacks,
                 if(pz == py) rx = ry;
                 else st_rows<FM>(rx, arow, st.arow_begin + (pz - pbeg) * st.rows, srow);
                 load_chunk_st_any<FM, FN>(ax, bx, rx, sx, cz, bcol, lg);
-                mma_chunk_st<FM, FN>(acc, ay, by, ry);
+                mma_chunk_st<FM, FN>(acc, ay, by, ry, sy.k, cy, lg);
                 if(!hz) break;
                 px = pz;
                 cx = cz;
