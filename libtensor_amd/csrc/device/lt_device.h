@@ -102,6 +102,9 @@ struct stack_desc {
 };
 struct stack_pair {
     const double *b;
+    //! B read through offset tables (not null): B(k, n) = b[koff[k] + noff[n]]
+    //! (a permuted operand read in place from its stored block)
+    const int32_t *koff, *noff;
     uint32_t k, ldb, flags, pad;      //!< flags: kPairAKMajor / kPairBNMajor
 };
 //! A operand of one row for one pair: row r of A at p[k] (or p[k * lda]

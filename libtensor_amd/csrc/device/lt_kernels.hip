@@ -580,21 +580,27 @@ static hipError_t launch_direct_t(const gemm_cblock *cb,
 // are dealt to the W waves as in the direct kernel, so a C element gets the
 // same sums in the same order as there.
 
-template<int FM>
+template<int FM, int FN>
 struct st_rowsrc {
     const double *p[FM];
     double d[FM];
     uint32_t lda[FM];
+    int32_t no[FN];           //!< B column offsets (offset-table pairs)
 };
 
-template<int FM, int FN, bool TA, bool TB>
+template<int FM, int FN, bool TA, bool TB, bool TBL>
 __device__ __forceinline__ void load_chunk_st(double (&af)[FM][4], double (&bf)[FN][4],
-    const st_rowsrc<FM> &ar, const stack_pair &sp, int c, const int (&bcol)[FN], int lg) {
+    const st_rowsrc<FM, FN> &ar, const stack_pair &sp, int c, const int (&bcol)[FN], int lg) {
     const uint32_t K = sp.k;
     const uint32_t kk = 16 * c + 4 * lg;
     uint32_t kc[4];
 #pragma unroll
     for(int j = 0; j < 4; j++) kc[j] = min(kk + j, K - 1);
+    int32_t ko[4];
+    if(TBL) {
+#pragma unroll
+        for(int j = 0; j < 4; j++) ko[j] = sp.koff[kc[j]];
+    }
 #pragma unroll
     for(int i = 0; i < FM; i++) {
         gdouble *pa = (gdouble *)ar.p[i];
@@ -605,25 +611,34 @@ __device__ __forceinline__ void load_chunk_st(double (&af)[FM][4], double (&bf)[
     for(int f = 0; f < FN; f++) {
         const uint32_t n = uint32_t(bcol[f]);
 #pragma unroll
-        for(int j = 0; j < 4; j++)
-            bf[f][j] = ldg(sp.b, 8u * (TB ? n * sp.ldb + kc[j] : kc[j] * sp.ldb + n));
+        for(int j = 0; j < 4; j++) {
+            const uint32_t e = TBL ? uint32_t(ko[j] + ar.no[f])
+                                   : (TB ? n * sp.ldb + kc[j] : kc[j] * sp.ldb + n);
+            bf[f][j] = ldg(sp.b, 8u * e);
+        }
     }
 }
 
 template<int FM, int FN>
 __device__ __forceinline__ void load_chunk_st_any(double (&af)[FM][4], double (&bf)[FN][4],
-    const st_rowsrc<FM> &ar, const stack_pair &sp, int c, const int (&bcol)[FN], int lg) {
+    const st_rowsrc<FM, FN> &ar, const stack_pair &sp, int c, const int (&bcol)[FN], int lg) {
+    const bool ta = (sp.flags & kPairAKMajor) != 0;
+    if(sp.koff) {
+        if(ta) load_chunk_st<FM, FN, true, false, true>(af, bf, ar, sp, c, bcol, lg);
+        else load_chunk_st<FM, FN, false, false, true>(af, bf, ar, sp, c, bcol, lg);
+        return;
+    }
     switch(sp.flags & (kPairAKMajor | kPairBNMajor)) {
-    case 0: load_chunk_st<FM, FN, false, false>(af, bf, ar, sp, c, bcol, lg); break;
-    case kPairAKMajor: load_chunk_st<FM, FN, true, false>(af, bf, ar, sp, c, bcol, lg); break;
-    case kPairBNMajor: load_chunk_st<FM, FN, false, true>(af, bf, ar, sp, c, bcol, lg); break;
-    default: load_chunk_st<FM, FN, true, true>(af, bf, ar, sp, c, bcol, lg); break;
+    case 0: load_chunk_st<FM, FN, false, false, false>(af, bf, ar, sp, c, bcol, lg); break;
+    case kPairAKMajor: load_chunk_st<FM, FN, true, false, false>(af, bf, ar, sp, c, bcol, lg); break;
+    case kPairBNMajor: load_chunk_st<FM, FN, false, true, false>(af, bf, ar, sp, c, bcol, lg); break;
+    default: load_chunk_st<FM, FN, true, true, false>(af, bf, ar, sp, c, bcol, lg); break;
     }
 }
 
-template<int FM>
-__device__ __forceinline__ void st_rows(st_rowsrc<FM> &r, const stack_arow *__restrict__ arow,
-    uint32_t base, const int (&srow)[FM]) {
+template<int FM, int FN>
+__device__ __forceinline__ void st_rows(st_rowsrc<FM, FN> &r, const stack_arow *__restrict__ arow,
+    uint32_t base, const int (&srow)[FM], const stack_pair &sp, const int (&bcol)[FN]) {
 #pragma unroll
     for(int i = 0; i < FM; i++) {
         const stack_arow e = arow[base + srow[i]];
@@ -631,11 +646,13 @@ __device__ __forceinline__ void st_rows(st_rowsrc<FM> &r, const stack_arow *__re
         r.d[i] = e.d;
         r.lda[i] = e.lda;
     }
+#pragma unroll
+    for(int f = 0; f < FN; f++) r.no[f] = sp.noff ? sp.noff[bcol[f]] : 0;
 }
 
 template<int FM, int FN>
 __device__ __forceinline__ void mma_chunk_st(double4_t (&acc)[FM][FN],
-    double (&af)[FM][4], const double (&bf)[FN][4], const st_rowsrc<FM> &r,
+    double (&af)[FM][4], const double (&bf)[FN][4], const st_rowsrc<FM, FN> &r,
     uint32_t K, int c, int lg) {
     const uint32_t kk = 16 * c + 4 * lg;
 #pragma unroll
@@ -701,9 +718,9 @@ gemm_stacked_kernel(const stack_desc *__restrict__ stacks,
         seek_chunk_st(spairs, pend, px, cx);
         if(px < pend) {
             double ax[FM][4], bx[FN][4], ay[FM][4], by[FN][4];
-            st_rowsrc<FM> rx, ry;
+            st_rowsrc<FM, FN> rx, ry;
             stack_pair sx = spairs[px];
-            st_rows<FM>(rx, arow, st.arow_begin + (px - pbeg) * st.rows, srow);
+            st_rows<FM, FN>(rx, arow, st.arow_begin + (px - pbeg) * st.rows, srow, sx, bcol);
             load_chunk_st_any<FM, FN>(ax, bx, rx, sx, cx, bcol, lg);
             while(true) {
                 uint32_t py = px;
@@ -713,7 +730,7 @@ gemm_stacked_kernel(const stack_desc *__restrict__ stacks,
                 if(!hy) { py = px; cy = cx; }       // dummy in-bounds load
                 const stack_pair sy = spairs[py];
                 if(py == px) ry = rx;
-                else st_rows<FM>(ry, arow, st.arow_begin + (py - pbeg) * st.rows, srow);
+                else st_rows<FM, FN>(ry, arow, st.arow_begin + (py - pbeg) * st.rows, srow, sy, bcol);
                 load_chunk_st_any<FM, FN>(ay, by, ry, sy, cy, bcol, lg);
                 mma_chunk_st<FM, FN>(acc, ax, bx, rx, sx.k, cx, lg);
                 if(!hy) break;
@@ -724,7 +741,7 @@ gemm_stacked_kernel(const stack_desc *__restrict__ stacks,
                 if(!hz) { pz = py; cz = cy; }
                 sx = spairs[pz];
                 if(pz == py) rx = ry;
-                else st_rows<FM>(rx, arow, st.arow_begin + (pz - pbeg) * st.rows, srow);
+                else st_rows<FM, FN>(rx, arow, st.arow_begin + (pz - pbeg) * st.rows, srow, sx, bcol);
                 load_chunk_st_any<FM, FN>(ax, bx, rx, sx, cz, bcol, lg);
                 mma_chunk_st<FM, FN>(acc, ay, by, ry, sy.k, cy, lg);
                 if(!hz) break;

@@ -188,6 +188,13 @@ struct contract_plan {
     std::vector<dev::stack_crow> crows;
     std::vector<double> stack_ksum;
     std::vector<dev::gemm_tile> stiles;
+    // offset tables of operands read in place by the stacked kernels
+    std::vector<int32_t> otab;
+    std::vector<std::pair<int64_t, int64_t> > spair_tab;  // (koff, noff) indexes or -1
+    std::vector<dev::perm_desc> prep_st;      // prepare list of the stacked launches
+    uint32_t n_prep_st_chunks;
+    dev::perm_desc *d_prep_st;
+    uint32_t *d_pst_cd, *d_pst_co;
     dev_lists stk, stk_tiles;
     dev::stack_desc *d_stacks;
     dev::stack_pair *d_spairs;
@@ -226,6 +233,7 @@ struct contract_plan {
         d_cb(0), d_pairs(0), d_tiles_sorted(0), d_tiles_block(0), scratch(0),
         scratch_elems(0), ctx(0), events(false), graph(0), gexec(0), graph_launches(0),
         d_stacks(0), d_spairs(0), d_arows(0), d_crows(0), d_stiles(0),
+        n_prep_st_chunks(0), d_prep_st(0), d_pst_cd(0), d_pst_co(0),
         oz_slices(0), d_oz_split(0), d_oz_items(0), n_oz_items(0), d_oz_pairs(0),
         d_oz_tiles(0), n_oz_tiles(0) { }
     ~contract_plan() {
@@ -420,6 +428,46 @@ void btod_contract2::build_plan(btensor &c, bool add, double dscale) {
         return false;
     };
 
+    // The operand in the kernels' B slot (B, or A when C is computed
+    // transposed) can also be read in place through offset tables by the
+    // stacked kernels: element (k, n) of the aligned block is
+    // Y[koff[k] + noff[n]] of its stored block Y.  The offset of an element
+    // is linear in each index, so it splits into a row part and a column
+    // part of the aligned matrix.
+    const bool tc = gf0.tc;
+    struct tabref { const double *y; double alpha; int64_t koff, noff; };
+    auto make_tables = [&](btensor &t, const dims_t &bid, const src_t &s, const dims_t &dz,
+        size_t nfirst, bool first_is_k, tabref &r) {
+        dims_t dy = t.bis().block_dims(get_index(s.y, bid));
+        size_t n = dy.n;
+        std::vector<int64_t> ys(n), w(n);
+        int64_t acc = 1;
+        for(size_t d = n; d-- > 0;) { ys[d] = acc; acc *= (int64_t)dy[d]; }
+        for(size_t i = 0; i < n; i++) w[i] = ys[s.tr.perm[i]];
+        auto table = [&](size_t i0, size_t i1) {
+            int64_t at = (int64_t)pl.otab.size();
+            size_t cnt = 1;
+            for(size_t i = i0; i < i1; i++) cnt *= dz[i];
+            std::vector<size_t> idx(n, 0);
+            for(size_t e = 0; e < cnt; e++) {
+                int64_t off = 0;
+                for(size_t i = i0; i < i1; i++) off += (int64_t)idx[i] * w[i];
+                pl.otab.push_back((int32_t)off);
+                for(size_t i = i1; i-- > i0;) {
+                    if(++idx[i] < dz[i]) break;
+                    idx[i] = 0;
+                }
+            }
+            return at;
+        };
+        int64_t rt = table(0, nfirst), ct = table(nfirst, n);
+        r.y = t.block_ptr(s.y);
+        r.alpha = s.tr.st.c;
+        r.koff = first_is_k ? rt : ct;
+        r.noff = first_is_k ? ct : rt;
+    };
+    std::unordered_map<size_t, tabref> taba, tabb;
+
     size_t scratch = 0;
     std::unordered_map<size_t, size_t> offa, offb;    // aligned block -> scratch offset
     std::unordered_map<size_t, opref> refa, refb;
@@ -434,6 +482,7 @@ void btod_contract2::build_plan(btensor &c, bool add, double dscale) {
             scratch += (dz.size() + 31) & ~size_t(31);
             prepa.push_back(std::make_pair(usea[i], s));
             r.p = 0; r.scale = 1.0; r.flip = false; r.ldf = 0;
+            if(tc) make_tables(m_a, bida, s, dz, gf0.ta ? contr.K : contr.N, gf0.ta, taba[usea[i]]);
         }
         refa[usea[i]] = r;
     }
@@ -447,6 +496,7 @@ void btod_contract2::build_plan(btensor &c, bool add, double dscale) {
             scratch += (dz.size() + 31) & ~size_t(31);
             prepb.push_back(std::make_pair(useb[i], s));
             r.p = 0; r.scale = 1.0; r.flip = false; r.ldf = 0;
+            if(!tc) make_tables(m_b, bidb, s, dz, gf0.tb ? contr.M : contr.K, !gf0.tb, tabb[useb[i]]);
         }
         refb[useb[i]] = r;
     }
@@ -492,14 +542,17 @@ void btod_contract2::build_plan(btensor &c, bool add, double dscale) {
         pl.prep.push_back(make_perm_desc(m_b.block_ptr(s.y), dy, s.tr,
             pl.scratch + offb[prepb[i].first], false));
     }
+    // the stacked launches read the kernel-B operands through their tables
+    if(!tc) pl.prep_st.assign(pl.prep.begin(), pl.prep.begin() + prepa.size());
+    else pl.prep_st.assign(pl.prep.begin() + prepa.size(), pl.prep.end());
 
     // 2. GEMM work lists
-    const bool tc = gf0.tc;
     // with C stored as (j, i) the product is computed as C^T = B^T A^T
     pl.ta = tc ? !gf0.tb : gf0.ta;
     pl.tb = tc ? !gf0.ta : gf0.tb;
 
     double flops = 0.0;
+    std::vector<tabref> pair_tab;
     std::vector<std::pair<size_t, size_t> > mn(batchc.size());
     for(size_t t = 0; t < batchc.size(); t++) {
         index_t ic = get_index(batchc[t], bidct);
@@ -570,6 +623,13 @@ void btod_contract2::build_plan(btensor &c, bool add, double dscale) {
             gp.d = d * ra.scale * rb.scale;
             if(gp.d == 0.0) continue;
             pl.pairs.push_back(gp);
+            {
+                // kernel-B operand through tables (stacked launches)
+                const std::unordered_map<size_t, tabref> &tm = tc ? taba : tabb;
+                std::unordered_map<size_t, tabref>::const_iterator ti = tm.find(tc ? p->aia : p->aib);
+                if(ti != tm.end()) pair_tab.push_back(ti->second);
+                else { tabref z; z.y = 0; z.alpha = 1.0; z.koff = z.noff = -1; pair_tab.push_back(z); }
+            }
             flops += 2.0 * double(g.m) * double(g.n) * double(g.k);
             ksum += double(g.k);
         }
@@ -590,9 +650,16 @@ void btod_contract2::build_plan(btensor &c, bool add, double dscale) {
         return bkey_t((uintptr_t)g.b, g.flags, g.ldb, g.k);
     };
     for(size_t b = 0; b < pl.cblocks.size(); b++) {
-        std::stable_sort(pl.pairs.begin() + pl.cblocks[b].pair_begin,
-            pl.pairs.begin() + pl.cblocks[b].pair_end,
-            [&](const dev::gemm_pair &x, const dev::gemm_pair &y) { return bkey(x) < bkey(y); });
+        uint32_t p0 = pl.cblocks[b].pair_begin, p1 = pl.cblocks[b].pair_end;
+        std::vector<uint32_t> ord(p1 - p0);
+        std::iota(ord.begin(), ord.end(), p0);
+        std::stable_sort(ord.begin(), ord.end(), [&](uint32_t x, uint32_t y) {
+            return bkey(pl.pairs[x]) < bkey(pl.pairs[y]); });
+        std::vector<dev::gemm_pair> gs(ord.size());
+        std::vector<tabref> ts(ord.size());
+        for(size_t i = 0; i < ord.size(); i++) { gs[i] = pl.pairs[ord[i]]; ts[i] = pair_tab[ord[i]]; }
+        std::copy(gs.begin(), gs.end(), pl.pairs.begin() + p0);
+        std::copy(ts.begin(), ts.end(), pair_tab.begin() + p0);
     }
     {
         std::map<std::pair<uint32_t, std::vector<bkey_t> >, std::vector<size_t> > groups;
@@ -624,10 +691,17 @@ void btod_contract2::build_plan(btensor &c, bool add, double dscale) {
             double ks = 0.0;
             for(uint32_t j = 0; j < sd.npairs; j++) {
                 const dev::gemm_pair &g = pl.pairs[c0.pair_begin + j];
+                const tabref &tb = pair_tab[c0.pair_begin + j];
                 dev::stack_pair sp;
                 std::memset(&sp, 0, sizeof(sp));
                 sp.b = g.b; sp.k = g.k; sp.ldb = g.ldb; sp.flags = g.flags;
+                double bscale = 1.0;
+                if(tb.koff >= 0) {
+                    sp.b = tb.y;
+                    bscale = tb.alpha;    // the copy would have carried it
+                }
                 pl.spairs.push_back(sp);
+                pl.spair_tab.push_back(std::make_pair(tb.koff, tb.noff));
                 ks += double(g.k);
                 for(size_t q = 0; q < bl.size(); q++) {
                     const dev::gemm_cblock &cq = pl.cblocks[bl[q]];
@@ -636,7 +710,7 @@ void btod_contract2::build_plan(btensor &c, bool add, double dscale) {
                         dev::stack_arow ar;
                         std::memset(&ar, 0, sizeof(ar));
                         ar.p = (gq.flags & dev::kPairAKMajor) ? gq.a + r : gq.a + size_t(r) * gq.lda;
-                        ar.d = gq.d;
+                        ar.d = gq.d * bscale;
                         ar.lda = gq.lda;
                         pl.arows.push_back(ar);
                     }
@@ -815,11 +889,13 @@ void btod_contract2::build_plan(btensor &c, bool add, double dscale) {
     }
 
     // upload all lists in one allocation
-    std::vector<uint32_t> pcd, pco, wcd, wco;
+    std::vector<uint32_t> pcd, pco, wcd, wco, scd, sco;
     make_chunks(pl.prep, pcd, pco);
     make_chunks(pl.wback, wcd, wco);
+    make_chunks(pl.prep_st, scd, sco);
     pl.n_prep_chunks = (uint32_t)pcd.size();
     pl.n_wb_chunks = (uint32_t)wcd.size();
+    pl.n_prep_st_chunks = (uint32_t)scd.size();
     size_t sz = 0;
     size_t o_prep = sz; sz = align256(sz + pl.prep.size() * sizeof(dev::perm_desc));
     size_t o_wb = sz; sz = align256(sz + pl.wback.size() * sizeof(dev::perm_desc));
@@ -833,6 +909,10 @@ void btod_contract2::build_plan(btensor &c, bool add, double dscale) {
     size_t o_sp = sz; sz = align256(sz + pl.spairs.size() * sizeof(dev::stack_pair));
     size_t o_ar = sz; sz = align256(sz + pl.arows.size() * sizeof(dev::stack_arow));
     size_t o_cr = sz; sz = align256(sz + pl.crows.size() * sizeof(dev::stack_crow));
+    size_t o_ot = sz; sz = align256(sz + pl.otab.size() * sizeof(int32_t));
+    size_t o_ps = sz; sz = align256(sz + pl.prep_st.size() * sizeof(dev::perm_desc));
+    size_t o_scd = sz; sz = align256(sz + scd.size() * 4);
+    size_t o_sco = sz; sz = align256(sz + sco.size() * 4);
     std::vector<char> host(sz, 0);
     auto put = [&](size_t off, const void *src, size_t n) { if(n) std::memcpy(&host[off], src, n); };
     put(o_prep, pl.prep.data(), pl.prep.size() * sizeof(dev::perm_desc));
@@ -844,12 +924,25 @@ void btod_contract2::build_plan(btensor &c, bool add, double dscale) {
     put(o_cb, pl.cblocks.data(), pl.cblocks.size() * sizeof(dev::gemm_cblock));
     put(o_pr, pl.pairs.data(), pl.pairs.size() * sizeof(dev::gemm_pair));
     put(o_st, pl.stacks.data(), pl.stacks.size() * sizeof(dev::stack_desc));
-    put(o_sp, pl.spairs.data(), pl.spairs.size() * sizeof(dev::stack_pair));
+    put(o_ot, pl.otab.data(), pl.otab.size() * sizeof(int32_t));
+    put(o_ps, pl.prep_st.data(), pl.prep_st.size() * sizeof(dev::perm_desc));
+    put(o_scd, scd.data(), scd.size() * 4);
+    put(o_sco, sco.data(), sco.size() * 4);
     put(o_ar, pl.arows.data(), pl.arows.size() * sizeof(dev::stack_arow));
     put(o_cr, pl.crows.data(), pl.crows.size() * sizeof(dev::stack_crow));
     pl.lists.ctx = &ctx;
     pl.lists.mem = ctx.alloc(sz);
     pl.lists.bytes = sz;
+    {
+        // stack pairs with their offset tables at their device addresses
+        const int32_t *ot = (const int32_t*)((char*)pl.lists.mem + o_ot);
+        for(size_t j = 0; j < pl.spairs.size(); j++) {
+            if(pl.spair_tab[j].first < 0) continue;
+            pl.spairs[j].koff = ot + pl.spair_tab[j].first;
+            pl.spairs[j].noff = ot + pl.spair_tab[j].second;
+        }
+        put(o_sp, pl.spairs.data(), pl.spairs.size() * sizeof(dev::stack_pair));
+    }
     ctx.upload(pl.lists.mem, host.data(), sz);
     char *base = (char*)pl.lists.mem;
     pl.d_prep = (dev::perm_desc*)(base + o_prep);
@@ -864,6 +957,9 @@ void btod_contract2::build_plan(btensor &c, bool add, double dscale) {
     pl.d_spairs = (dev::stack_pair*)(base + o_sp);
     pl.d_arows = (dev::stack_arow*)(base + o_ar);
     pl.d_crows = (dev::stack_crow*)(base + o_cr);
+    pl.d_prep_st = (dev::perm_desc*)(base + o_ps);
+    pl.d_pst_cd = (uint32_t*)(base + o_scd);
+    pl.d_pst_co = (uint32_t*)(base + o_sco);
 
     // tile lists live in their own allocation (replaced by the autotuner)
     auto upload_tiles = [&](int k, int order) {
@@ -920,7 +1016,7 @@ void btod_contract2::build_plan(btensor &c, bool add, double dscale) {
     pl.events = true;
 
     // Autotuner: time every candidate configuration on this plan's own work
-    // list (grouped launch), keep the fastest.  Results of these
+    // list (prepare permute and grouped launch), keep the fastest.  Results of these
     // launches are overwritten by the perform that follows (never run for
     // the accumulating path).
     if(candidates.size() > 1) {
@@ -928,6 +1024,15 @@ void btod_contract2::build_plan(btensor &c, bool add, double dscale) {
         if(pl.n_prep_chunks)
             hip_check(dev::launch_permute(pl.d_prep, pl.d_prep_cd, pl.d_prep_co,
                 pl.n_prep_chunks, s), "permute (prepare)");
+        // each candidate is timed with the prepare permute it needs (the
+        // stacked kernels read B operands in place and prepare less)
+        auto prep_for = [&](int k) -> hipError_t {
+            if(dev::is_stacked(k))
+                return pl.n_prep_st_chunks ? dev::launch_permute(pl.d_prep_st, pl.d_pst_cd,
+                    pl.d_pst_co, pl.n_prep_st_chunks, s) : hipSuccess;
+            return pl.n_prep_chunks ? dev::launch_permute(pl.d_prep, pl.d_prep_cd,
+                pl.d_prep_co, pl.n_prep_chunks, s) : hipSuccess;
+        };
         float best = 1e30f;
         int bestk = cfg, besto = 0;
         for(size_t ci = 0; ci < 2 * candidates.size(); ci++) {
@@ -940,6 +1045,7 @@ void btod_contract2::build_plan(btensor &c, bool add, double dscale) {
             float ms = 0.f;
             for(int rep = 0; rep < 2; rep++) {
                 hip_check(hipEventRecord(pl.ev[0], s), "hipEventRecord");
+                hip_check(prep_for(k), "autotune (prepare)");
                 hip_check(launch_grouped(pl, k, s), "autotune");
                 hip_check(hipEventRecord(pl.ev[1], s), "hipEventRecord");
                 hip_check(hipStreamSynchronize(s), "hipStreamSynchronize");
@@ -1091,7 +1197,13 @@ void btod_contract2::build_ozaki(int slices) {
 
 void btod_contract2::enqueue(int mode, hipStream_t s, bool timed, size_t &launches) {
     contract_plan &pl = *m_plan;
-    if(pl.n_prep_chunks) {
+    if(mode == MODE_GROUPED && dev::is_stacked(pl.tile_cfg)) {
+        if(pl.n_prep_st_chunks) {
+            hip_check(dev::launch_permute(pl.d_prep_st, pl.d_pst_cd, pl.d_pst_co,
+                pl.n_prep_st_chunks, s), "permute (prepare)");
+            launches++;
+        }
+    } else if(pl.n_prep_chunks) {
         hip_check(dev::launch_permute(pl.d_prep, pl.d_prep_cd, pl.d_prep_co,
             pl.n_prep_chunks, s), "permute (prepare)");
         launches++;
