@@ -54,7 +54,7 @@ typedef struct lt_stats {
     int32_t tile_cfg;
     int32_t ozaki_slices;
     double ozaki_max_rel_err;
-    double tune_ms[16];      /* autotuner: grouped GEMM time per kernel
+    double tune_ms[32];      /* autotuner: grouped GEMM time per kernel
                                configuration (0 = not measured) */
     int32_t graph;           /* 1: the last perform ran as a HIP graph
                                 (t_gemm_ms is then 0: not separately timed) */
@@ -195,7 +195,7 @@ int lt_btod_contract2_clst(lt_btod_contract2 *op, size_t aidx_c, double *out,
 int lt_btod_contract2_compute_block_host(lt_btod_contract2 *op, size_t aidx_c,
     double *host);
 int lt_btod_contract2_stats(lt_btod_contract2 *op, lt_stats *out);
-/* Kernel configuration of the contraction GEMM: 0..11 fixed, -1 chosen by a
+/* Kernel configuration of the contraction GEMM: 0..18 fixed, -1 chosen by a
  * padding estimate, -2 (default) measured on the plan's own work list when
  * it is built (autotuner). */
 int lt_btod_contract2_set_tile(lt_btod_contract2 *op, int cfg);

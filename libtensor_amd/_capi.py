@@ -15,7 +15,7 @@ c_int_p = ctypes.POINTER(ctypes.c_int)
 vp = ctypes.c_void_p
 
 # GEMM kernel configurations (lt_btod_contract2_set_tile: 0 .. GEMM_CONFIGS - 1)
-GEMM_CONFIGS = 16
+GEMM_CONFIGS = 19
 LT_MODE_PER_BLOCK = 0
 LT_MODE_GROUPED = 1
 LT_MODE_OZAKI = 2
@@ -37,7 +37,7 @@ class LtStats(ctypes.Structure):
         ("tile_cfg", ctypes.c_int32),
         ("ozaki_slices", ctypes.c_int32),
         ("ozaki_max_rel_err", ctypes.c_double),
-        ("tune_ms", ctypes.c_double * 16),
+        ("tune_ms", ctypes.c_double * 32),
         ("graph", ctypes.c_int32),
         ("pad", ctypes.c_int32),
     ]

@@ -83,12 +83,22 @@ enum { TILE_64x64 = 0, TILE_32x64 = 1, TILE_64x32 = 2, TILE_32x32 = 3,
        SPLITK_64x32_W4 = 7, SPLITK_16x64_W2 = 8, SPLITK_32x32_W2 = 9,
        SPLITK_16x32_W4 = 10, SPLITK_32x16_W4 = 11,
        STACK_16x64_W4 = 12, STACK_32x32_W4 = 13, STACK_32x64_W4 = 14,
-       STACK_64x32_W4 = 15,
-       TILE_COUNT = 16 };
+       STACK_64x32_W4 = 15, STACK_48x64_W4 = 16, STACK_64x48_W4 = 17,
+       STACK_48x32_W4 = 18,
+       TILE_COUNT = 19 };
 void tile_shape(int cfg, uint32_t &bm, uint32_t &bn);
-inline bool is_stacked(int cfg) { return cfg >= STACK_16x64_W4 && cfg <= STACK_64x32_W4; }
-//! The per-C-block configuration with the same tile and summation order.
-inline int unstacked(int cfg) { return is_stacked(cfg) ? cfg - STACK_16x64_W4 + SPLITK_16x64_W4 : cfg; }
+inline bool is_stacked(int cfg) { return cfg >= STACK_16x64_W4 && cfg <= STACK_48x32_W4; }
+//! The per-C-block configuration with the same summation order (same
+//! split-K width; the tile shape only partitions the output).
+inline int unstacked(int cfg) {
+    switch(cfg) {
+    case STACK_16x64_W4: return SPLITK_16x64_W4;
+    case STACK_32x32_W4: return SPLITK_32x32_W4;
+    case STACK_32x64_W4: case STACK_48x64_W4: return SPLITK_32x64_W4;
+    case STACK_64x32_W4: case STACK_64x48_W4: case STACK_48x32_W4: return SPLITK_64x32_W4;
+    default: return cfg;
+    }
+}
 
 //! A stack: C blocks whose contraction lists read the same B operands in the
 //! same order (same layouts and extents); their rows are concatenated

@@ -40,6 +40,9 @@ void tile_shape(int cfg, uint32_t &bm, uint32_t &bn) {
     case SPLITK_32x32_W4: case SPLITK_32x32_W2: case STACK_32x32_W4: bm = 32; bn = 32; break;
     case SPLITK_32x64_W4: case STACK_32x64_W4: bm = 32; bn = 64; break;
     case SPLITK_64x32_W4: case STACK_64x32_W4: bm = 64; bn = 32; break;
+    case STACK_48x64_W4: bm = 48; bn = 64; break;
+    case STACK_64x48_W4: bm = 64; bn = 48; break;
+    case STACK_48x32_W4: bm = 48; bn = 32; break;
     case SPLITK_16x32_W4: bm = 16; bn = 32; break;
     case SPLITK_32x16_W4: bm = 32; bn = 16; break;
     default: bm = 64; bn = 64; break;
@@ -819,6 +822,9 @@ hipError_t launch_gemm_stacked(int cfg, const stack_desc *st, const stack_pair *
     case STACK_32x32_W4: return launch_stacked_t<2, 2, 4>(st, sp, ar, cr, tiles, ntiles, s);
     case STACK_32x64_W4: return launch_stacked_t<2, 4, 4>(st, sp, ar, cr, tiles, ntiles, s);
     case STACK_64x32_W4: return launch_stacked_t<4, 2, 4>(st, sp, ar, cr, tiles, ntiles, s);
+    case STACK_48x64_W4: return launch_stacked_t<3, 4, 4>(st, sp, ar, cr, tiles, ntiles, s);
+    case STACK_64x48_W4: return launch_stacked_t<4, 3, 4>(st, sp, ar, cr, tiles, ntiles, s);
+    case STACK_48x32_W4: return launch_stacked_t<3, 2, 4>(st, sp, ar, cr, tiles, ntiles, s);
     default: return hipErrorInvalidValue;
     }
 }

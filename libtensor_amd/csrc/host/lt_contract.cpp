@@ -777,8 +777,10 @@ void btod_contract2::build_plan(btensor &c, bool add, double dscale) {
     // launch per block) and the grouped-launch order above.
     auto make_tiles = [&](int cfg, int order, std::vector<dev::gemm_tile> &by_block,
         std::vector<uint32_t> &begin, std::vector<dev::gemm_tile> &sorted) {
+        // C-block tiles of the kernel that runs them (stacked configurations
+        // run C-block lists with their unstacked twin)
         uint32_t bm = 64, bn = 64;
-        dev::tile_shape(cfg, bm, bn);
+        dev::tile_shape(dev::unstacked(cfg), bm, bn);
         by_block.clear();
         begin.clear();
         std::vector<double> gcost(ngroups, 0.0);
