@@ -276,7 +276,7 @@ def main():
     if os.path.exists(tpath):
         with open(tpath) as f:
             tr = json.load(f)
-        if tr.get("gemm_kernel_cfg") == int(st["tile_cfg"]):
+        if int(st["tile_cfg"]) in tr.get("gemm_kernel_cfgs", []):
             traffic = tr["traffic_bytes_per_launch"]
     line = {
         "metric": METRIC, "value": value, "unit": "TFLOP/s", "n_gpus": 1,
