@@ -43,6 +43,8 @@ void tile_shape(int cfg, uint32_t &bm, uint32_t &bn) {
     case STACK_48x64_W4: bm = 48; bn = 64; break;
     case STACK_64x48_W4: bm = 64; bn = 48; break;
     case STACK_48x32_W4: bm = 48; bn = 32; break;
+    case STACK_64x48_W4_SB: case STACK_64x48_W2: bm = 64; bn = 48; break;
+    case STACK_48x64_W2: bm = 48; bn = 64; break;
     case SPLITK_16x32_W4: bm = 16; bn = 32; break;
     case SPLITK_32x16_W4: bm = 32; bn = 16; break;
     default: bm = 64; bn = 64; break;
@@ -684,7 +686,7 @@ __device__ __forceinline__ void seek_chunk_st(const stack_pair *__restrict__ sp,
     }
 }
 
-template<int FM, int FN, int W>
+template<int FM, int FN, int W, bool DB>
 __global__ void __launch_bounds__(64 * W)
 gemm_stacked_kernel(const stack_desc *__restrict__ stacks,
     const stack_pair *__restrict__ spairs, const stack_arow *__restrict__ arow,
@@ -719,7 +721,24 @@ gemm_stacked_kernel(const stack_desc *__restrict__ stacks,
         uint32_t px = pbeg;
         int cx = wave;
         seek_chunk_st(spairs, pend, px, cx);
-        if(px < pend) {
+        if(!DB) {
+            // single buffer: fewer registers, latency hidden by other waves
+            double ax[FM][4], bx[FN][4];
+            st_rowsrc<FM, FN> rx;
+            uint32_t pr = 0xffffffffu;
+            stack_pair sx;
+            while(px < pend) {
+                if(px != pr) {
+                    sx = spairs[px];
+                    st_rows<FM, FN>(rx, arow, st.arow_begin + (px - pbeg) * st.rows, srow, sx, bcol);
+                    pr = px;
+                }
+                load_chunk_st_any<FM, FN>(ax, bx, rx, sx, cx, bcol, lg);
+                mma_chunk_st<FM, FN>(acc, ax, bx, rx, sx.k, cx, lg);
+                cx += W;
+                seek_chunk_st(spairs, pend, px, cx);
+            }
+        } else if(px < pend) {
             double ax[FM][4], bx[FN][4], ay[FM][4], by[FN][4];
             st_rowsrc<FM, FN> rx, ry;
             stack_pair sx = spairs[px];
@@ -804,11 +823,11 @@ gemm_stacked_kernel(const stack_desc *__restrict__ stacks,
     }
 }
 
-template<int FM, int FN, int W>
+template<int FM, int FN, int W, bool DB = true>
 static hipError_t launch_stacked_t(const stack_desc *st, const stack_pair *sp,
     const stack_arow *ar, const stack_crow *cr, const gemm_tile *tiles, uint32_t ntiles,
     hipStream_t s) {
-    hipLaunchKernelGGL((gemm_stacked_kernel<FM, FN, W>), dim3(ntiles), dim3(64 * W), 0, s,
+    hipLaunchKernelGGL((gemm_stacked_kernel<FM, FN, W, DB>), dim3(ntiles), dim3(64 * W), 0, s,
         st, sp, ar, cr, tiles, ntiles);
     return hipGetLastError();
 }
@@ -825,6 +844,9 @@ hipError_t launch_gemm_stacked(int cfg, const stack_desc *st, const stack_pair *
     case STACK_48x64_W4: return launch_stacked_t<3, 4, 4>(st, sp, ar, cr, tiles, ntiles, s);
     case STACK_64x48_W4: return launch_stacked_t<4, 3, 4>(st, sp, ar, cr, tiles, ntiles, s);
     case STACK_48x32_W4: return launch_stacked_t<3, 2, 4>(st, sp, ar, cr, tiles, ntiles, s);
+    case STACK_64x48_W4_SB: return launch_stacked_t<4, 3, 4, false>(st, sp, ar, cr, tiles, ntiles, s);
+    case STACK_64x48_W2: return launch_stacked_t<4, 3, 2>(st, sp, ar, cr, tiles, ntiles, s);
+    case STACK_48x64_W2: return launch_stacked_t<3, 4, 2>(st, sp, ar, cr, tiles, ntiles, s);
     default: return hipErrorInvalidValue;
     }
 }
