@@ -729,6 +729,10 @@ void btod_contract2::build_plan(btensor &c, bool add, double dscale) {
             pl.stacks.push_back(sd);
             pl.stack_ksum.push_back(ks);
         }
+        dev::stack_desc z;
+        std::memset(&z, 0, sizeof(z));
+        pl.stacks.push_back(z);      // empty stack of the padding tiles
+        pl.stack_ksum.push_back(0.0);
     }
 
     // Tile lists of one kernel configuration: tiles grouped by C block (one
@@ -1000,7 +1004,39 @@ void btod_contract2::build_plan(btensor &c, bool add, double dscale) {
             std::stable_sort(st.begin(), st.end(),
                 [](const std::pair<double, dev::gemm_tile> &x, const std::pair<double, dev::gemm_tile> &y) {
                     return x.first > y.first; });
-            for(size_t i = 0; i < st.size(); i++) pl.stiles.push_back(st[i].second);
+            if(order == 0) {
+                for(size_t i = 0; i < st.size(); i++) pl.stiles.push_back(st[i].second);
+            } else {
+                // XCD-aware: stacks to XCDs longest-processing-time first, each
+                // XCD's tiles longest first at positions x, x + nx, ...; the
+                // padding tiles name the empty stack (no rows)
+                const size_t ns = pl.stacks.size() - 1;
+                std::vector<double> scost(ns, 0.0);
+                for(size_t i = 0; i < st.size(); i++) scost[st[i].second.cblock] += st[i].first;
+                std::vector<size_t> so(ns);
+                std::iota(so.begin(), so.end(), size_t(0));
+                std::stable_sort(so.begin(), so.end(), [&](size_t x, size_t y) {
+                    return scost[x] > scost[y]; });
+                std::vector<double> load(nx, 0.0);
+                std::vector<int> sx(ns, 0);
+                for(size_t i = 0; i < ns; i++) {
+                    int x = int(std::min_element(load.begin(), load.end()) - load.begin());
+                    sx[so[i]] = x;
+                    load[x] += scost[so[i]];
+                }
+                std::vector<std::vector<dev::gemm_tile> > q(nx);
+                size_t qmax = 0;
+                for(size_t i = 0; i < st.size(); i++) {
+                    std::vector<dev::gemm_tile> &qq = q[sx[st[i].second.cblock]];
+                    qq.push_back(st[i].second);
+                    qmax = std::max(qmax, qq.size());
+                }
+                dev::gemm_tile empty;
+                empty.cblock = (uint32_t)ns; empty.tm = 0; empty.tn = 0;
+                pl.stiles.assign(qmax * nx, empty);
+                for(int x = 0; x < nx; x++)
+                    for(size_t l = 0; l < q[x].size(); l++) pl.stiles[l * nx + x] = q[x][l];
+            }
             size_t nb = pl.stiles.size() * sizeof(dev::gemm_tile);
             if(pl.stk_tiles.mem) ctx.free(pl.stk_tiles.mem);
             pl.stk_tiles.ctx = &ctx;
@@ -1039,7 +1075,6 @@ void btod_contract2::build_plan(btensor &c, bool add, double dscale) {
         int bestk = cfg, besto = 0;
         for(size_t ci = 0; ci < 2 * candidates.size(); ci++) {
             int k = candidates[ci / 2], o = int(ci % 2);
-            if(o == 1 && dev::is_stacked(k)) continue;     // one order for stacks
             upload_tiles(k, o);
             uint32_t nt = (uint32_t)pl.tiles_sorted.size();
             // one timed launch; a short one (first-launch overheads are
