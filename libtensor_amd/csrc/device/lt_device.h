@@ -64,6 +64,10 @@ struct gemm_pair {
     double d;
 };
 static const uint32_t kPairAKMajor = 1, kPairBNMajor = 2;
+//! Stacked kernels: k-contiguous rows of A (kPairVecA) or of B
+//! (kPairVecB, with kPairBNMajor) start 16-byte aligned and the extent is
+//! even, so a lane loads two k at once.
+static const uint32_t kPairVecA = 4, kPairVecB = 8;
 
 //! Work item of a grouped GEMM launch: one output tile of one C block.
 struct gemm_tile {

@@ -624,10 +624,66 @@ __device__ __forceinline__ void load_chunk_st(double (&af)[FM][4], double (&bf)[
     }
 }
 
+typedef double double2_t __attribute__((ext_vector_type(2)));
+typedef const double2_t __attribute__((address_space(1))) gdouble2;
+
+// Row-major A (k contiguous per row, 16-byte aligned rows, even extent):
+// each lane loads its four k as two 16-byte loads.  With an even extent the
+// pairs (kk, kk + 1) and (kk + 2, kk + 3) (kk even) are each either inside
+// or outside the extent; an outside pair reads the last pair (in bounds)
+// and is zeroed through the A scale.  B is read k-contiguous the same way
+// (VB, n-major) or through its offset tables (TBL).
+template<int FM, int FN, bool VB, bool TBL>
+__device__ __forceinline__ void load_chunk_st_vec(double (&af)[FM][4], double (&bf)[FN][4],
+    const st_rowsrc<FM, FN> &ar, const stack_pair &sp, int c, const int (&bcol)[FN], int lg) {
+    const uint32_t K = sp.k;
+    const uint32_t kk = 16 * c + 4 * lg;
+    const uint32_t k0 = min(kk, K - 2), k2 = min(kk + 2, K - 2);
+#pragma unroll
+    for(int i = 0; i < FM; i++) {
+        gdouble *pa = (gdouble *)ar.p[i];
+        const double2_t v0 = *(gdouble2 *)(pa + k0);
+        const double2_t v1 = *(gdouble2 *)(pa + k2);
+        af[i][0] = v0.x; af[i][1] = v0.y; af[i][2] = v1.x; af[i][3] = v1.y;
+    }
+    if(VB) {
+#pragma unroll
+        for(int f = 0; f < FN; f++) {
+            gdouble *pb = (gdouble *)sp.b + size_t(uint32_t(bcol[f]) * sp.ldb);
+            const double2_t v0 = *(gdouble2 *)(pb + k0);
+            const double2_t v1 = *(gdouble2 *)(pb + k2);
+            bf[f][0] = v0.x; bf[f][1] = v0.y; bf[f][2] = v1.x; bf[f][3] = v1.y;
+        }
+    } else {
+        uint32_t kc[4];
+#pragma unroll
+        for(int j = 0; j < 4; j++) kc[j] = min(kk + j, K - 1);
+        int32_t ko[4];
+#pragma unroll
+        for(int j = 0; j < 4; j++) ko[j] = TBL ? sp.koff[kc[j]] : 0;
+#pragma unroll
+        for(int f = 0; f < FN; f++) {
+            const uint32_t n = uint32_t(bcol[f]);
+#pragma unroll
+            for(int j = 0; j < 4; j++) {
+                const uint32_t e = TBL ? uint32_t(ko[j] + ar.no[f])
+                                       : ((sp.flags & kPairBNMajor) ? n * sp.ldb + kc[j] : kc[j] * sp.ldb + n);
+                bf[f][j] = ldg(sp.b, 8u * e);
+            }
+        }
+    }
+}
+
 template<int FM, int FN>
 __device__ __forceinline__ void load_chunk_st_any(double (&af)[FM][4], double (&bf)[FN][4],
     const st_rowsrc<FM, FN> &ar, const stack_pair &sp, int c, const int (&bcol)[FN], int lg) {
     const bool ta = (sp.flags & kPairAKMajor) != 0;
+    if(sp.flags & kPairVecA) {
+        if(sp.flags & kPairVecB) load_chunk_st_vec<FM, FN, true, false>(af, bf, ar, sp, c, bcol, lg);
+        else if(sp.koff) load_chunk_st_vec<FM, FN, false, true>(af, bf, ar, sp, c, bcol, lg);
+        else load_chunk_st_vec<FM, FN, false, false>(af, bf, ar, sp, c, bcol, lg);
+        return;
+    }
     if(sp.koff) {
         if(ta) load_chunk_st<FM, FN, true, false, true>(af, bf, ar, sp, c, bcol, lg);
         else load_chunk_st<FM, FN, false, false, true>(af, bf, ar, sp, c, bcol, lg);

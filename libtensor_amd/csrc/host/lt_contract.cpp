@@ -700,6 +700,17 @@ void btod_contract2::build_plan(btensor &c, bool add, double dscale) {
                     sp.b = tb.y;
                     bscale = tb.alpha;    // the copy would have carried it
                 }
+                // two-k loads: k-contiguous rows, even extent, 16-byte aligned rows
+                if(!(g.flags & dev::kPairAKMajor) && (g.k % 2) == 0) {
+                    bool ok = true;
+                    for(size_t q = 0; q < bl.size() && ok; q++) {
+                        const dev::gemm_pair &gq = pl.pairs[pl.cblocks[bl[q]].pair_begin + j];
+                        ok = (gq.lda % 2) == 0 && ((uintptr_t)gq.a % 16) == 0;
+                    }
+                    if(ok) sp.flags |= dev::kPairVecA;
+                    if(ok && tb.koff < 0 && (g.flags & dev::kPairBNMajor) && (g.ldb % 2) == 0 &&
+                        ((uintptr_t)g.b % 16) == 0) sp.flags |= dev::kPairVecB;
+                }
                 pl.spairs.push_back(sp);
                 pl.spair_tab.push_back(std::make_pair(tb.koff, tb.noff));
                 ks += double(g.k);
