@@ -68,6 +68,9 @@ static const uint32_t kPairAKMajor = 1, kPairBNMajor = 2;
 //! (kPairVecB, with kPairBNMajor) start 16-byte aligned and the extent is
 //! even, so a lane loads two k at once.
 static const uint32_t kPairVecA = 4, kPairVecB = 8;
+//! Offset-table B whose k offsets come in adjacent even pairs and whose n
+//! offsets are even (two k per load through the tables).
+static const uint32_t kPairVecBT = 16;
 
 //! Work item of a grouped GEMM launch: one output tile of one C block.
 struct gemm_tile {

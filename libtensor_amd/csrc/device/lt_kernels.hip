@@ -633,7 +633,7 @@ typedef const double2_t __attribute__((address_space(1))) gdouble2;
 // or outside the extent; an outside pair reads the last pair (in bounds)
 // and is zeroed through the A scale.  B is read k-contiguous the same way
 // (VB, n-major) or through its offset tables (TBL).
-template<int FM, int FN, bool VB, bool TBL>
+template<int FM, int FN, bool VB, bool TBL, bool TBV = false>
 __device__ __forceinline__ void load_chunk_st_vec(double (&af)[FM][4], double (&bf)[FN][4],
     const st_rowsrc<FM, FN> &ar, const stack_pair &sp, int c, const int (&bcol)[FN], int lg) {
     const uint32_t K = sp.k;
@@ -646,7 +646,17 @@ __device__ __forceinline__ void load_chunk_st_vec(double (&af)[FM][4], double (&
         const double2_t v1 = *(gdouble2 *)(pa + k2);
         af[i][0] = v0.x; af[i][1] = v0.y; af[i][2] = v1.x; af[i][3] = v1.y;
     }
-    if(VB) {
+    if(TBV) {
+        // table pairs: koff[k0 + 1] = koff[k0] + 1, all offsets even
+        const int32_t o0 = sp.koff[k0], o2 = sp.koff[k2];
+#pragma unroll
+        for(int f = 0; f < FN; f++) {
+            gdouble *pb = (gdouble *)sp.b;
+            const double2_t v0 = *(gdouble2 *)(pb + uint32_t(o0 + ar.no[f]));
+            const double2_t v1 = *(gdouble2 *)(pb + uint32_t(o2 + ar.no[f]));
+            bf[f][0] = v0.x; bf[f][1] = v0.y; bf[f][2] = v1.x; bf[f][3] = v1.y;
+        }
+    } else if(VB) {
 #pragma unroll
         for(int f = 0; f < FN; f++) {
             gdouble *pb = (gdouble *)sp.b + size_t(uint32_t(bcol[f]) * sp.ldb);
@@ -680,6 +690,7 @@ __device__ __forceinline__ void load_chunk_st_any(double (&af)[FM][4], double (&
     const bool ta = (sp.flags & kPairAKMajor) != 0;
     if(sp.flags & kPairVecA) {
         if(sp.flags & kPairVecB) load_chunk_st_vec<FM, FN, true, false>(af, bf, ar, sp, c, bcol, lg);
+        else if(sp.flags & kPairVecBT) load_chunk_st_vec<FM, FN, false, true, true>(af, bf, ar, sp, c, bcol, lg);
         else if(sp.koff) load_chunk_st_vec<FM, FN, false, true>(af, bf, ar, sp, c, bcol, lg);
         else load_chunk_st_vec<FM, FN, false, false>(af, bf, ar, sp, c, bcol, lg);
         return;

@@ -710,6 +710,15 @@ void btod_contract2::build_plan(btensor &c, bool add, double dscale) {
                     if(ok) sp.flags |= dev::kPairVecA;
                     if(ok && tb.koff < 0 && (g.flags & dev::kPairBNMajor) && (g.ldb % 2) == 0 &&
                         ((uintptr_t)g.b % 16) == 0) sp.flags |= dev::kPairVecB;
+                    if(ok && tb.koff >= 0 && ((uintptr_t)tb.y % 16) == 0) {
+                        bool pairs_ok = true;
+                        for(uint32_t kk = 0; kk + 1 < g.k && pairs_ok; kk += 2)
+                            pairs_ok = pl.otab[tb.koff + kk] % 2 == 0 &&
+                                pl.otab[tb.koff + kk + 1] == pl.otab[tb.koff + kk] + 1;
+                        for(uint32_t n = 0; n < sd.n && pairs_ok; n++)
+                            pairs_ok = pl.otab[tb.noff + n] % 2 == 0;
+                        if(pairs_ok) sp.flags |= dev::kPairVecBT;
+                    }
                 }
                 pl.spairs.push_back(sp);
                 pl.spair_tab.push_back(std::make_pair(tb.koff, tb.noff));
