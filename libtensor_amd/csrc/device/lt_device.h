@@ -82,7 +82,8 @@ struct gemm_tile {
 //! operands; SPLITK_*_Wn: one workgroup of n waves per tile, the tile's
 //! k chunks dealt to the waves, operands loaded straight into MFMA
 //! fragments, partial tiles summed in LDS in a fixed order.
-//! STACK_*: the split-K kernels over stacks (_SB: single-buffered) (C blocks sharing their B
+//! STACK_*: the split-K kernels over stacks (_SB: single-buffered, _On:
+//! compiled for n waves per SIMD) (C blocks sharing their B
 //! operands, rows concatenated); used by grouped launches, per-block
 //! launches run the SPLITK configuration of the same shape.
 enum { TILE_64x64 = 0, TILE_32x64 = 1, TILE_64x32 = 2, TILE_32x32 = 3,
@@ -92,10 +93,11 @@ enum { TILE_64x64 = 0, TILE_32x64 = 1, TILE_64x32 = 2, TILE_32x32 = 3,
        STACK_16x64_W4 = 12, STACK_32x32_W4 = 13, STACK_32x64_W4 = 14,
        STACK_64x32_W4 = 15, STACK_48x64_W4 = 16, STACK_64x48_W4 = 17,
        STACK_48x32_W4 = 18, STACK_64x48_W4_SB = 19, STACK_64x48_W2 = 20,
-       STACK_48x64_W2 = 21,
-       TILE_COUNT = 22 };
+       STACK_48x64_W2 = 21, STACK_48x32_W4_O2 = 22, STACK_32x32_W4_O3 = 23,
+       STACK_16x64_W4_O3 = 24,
+       TILE_COUNT = 25 };
 void tile_shape(int cfg, uint32_t &bm, uint32_t &bn);
-inline bool is_stacked(int cfg) { return cfg >= STACK_16x64_W4 && cfg <= STACK_48x64_W2; }
+inline bool is_stacked(int cfg) { return cfg >= STACK_16x64_W4 && cfg <= STACK_16x64_W4_O3; }
 //! The per-C-block configuration with the same summation order (same
 //! split-K width; the tile shape only partitions the output).
 inline int unstacked(int cfg) {
@@ -104,6 +106,7 @@ inline int unstacked(int cfg) {
     case STACK_32x32_W4: return SPLITK_32x32_W4;
     case STACK_32x64_W4: case STACK_48x64_W4: return SPLITK_32x64_W4;
     case STACK_64x32_W4: case STACK_64x48_W4: case STACK_48x32_W4: case STACK_64x48_W4_SB:
+    case STACK_48x32_W4_O2: case STACK_32x32_W4_O3: case STACK_16x64_W4_O3:
         return SPLITK_64x32_W4;
     case STACK_64x48_W2: case STACK_48x64_W2: return SPLITK_32x32_W2;
     default: return cfg;

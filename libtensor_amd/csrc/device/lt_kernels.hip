@@ -45,6 +45,9 @@ void tile_shape(int cfg, uint32_t &bm, uint32_t &bn) {
     case STACK_48x32_W4: bm = 48; bn = 32; break;
     case STACK_64x48_W4_SB: case STACK_64x48_W2: bm = 64; bn = 48; break;
     case STACK_48x64_W2: bm = 48; bn = 64; break;
+    case STACK_48x32_W4_O2: bm = 48; bn = 32; break;
+    case STACK_32x32_W4_O3: bm = 32; bn = 32; break;
+    case STACK_16x64_W4_O3: bm = 16; bn = 64; break;
     case SPLITK_16x32_W4: bm = 16; bn = 32; break;
     case SPLITK_32x16_W4: bm = 32; bn = 16; break;
     default: bm = 64; bn = 64; break;
@@ -754,14 +757,40 @@ __device__ __forceinline__ void seek_chunk_st(const stack_pair *__restrict__ sp,
 }
 
 template<int FM, int FN, int W, bool DB>
+__device__ __forceinline__ void stacked_body(const stack_desc *__restrict__ stacks,
+    const stack_pair *__restrict__ spairs, const stack_arow *__restrict__ arow,
+    const stack_crow *__restrict__ crow, const gemm_tile *__restrict__ tiles,
+    uint32_t ntiles, double *red);
+
+template<int FM, int FN, int W, bool DB>
 __global__ void __launch_bounds__(64 * W)
 gemm_stacked_kernel(const stack_desc *__restrict__ stacks,
     const stack_pair *__restrict__ spairs, const stack_arow *__restrict__ arow,
     const stack_crow *__restrict__ crow, const gemm_tile *__restrict__ tiles,
     uint32_t ntiles) {
+    __shared__ double red[(W > 1 ? W - 1 : 1) * FM * FN * 4 * 64];
+    stacked_body<FM, FN, W, DB>(stacks, spairs, arow, crow, tiles, ntiles, red);
+}
+
+//! The same kernel compiled for OCC waves per SIMD (register budget
+//! 512 / OCC per lane, accumulators in architectural registers).
+template<int FM, int FN, int W, int OCC>
+__global__ void __launch_bounds__(64 * W) __attribute__((amdgpu_waves_per_eu(OCC)))
+gemm_stacked_occ_kernel(const stack_desc *__restrict__ stacks,
+    const stack_pair *__restrict__ spairs, const stack_arow *__restrict__ arow,
+    const stack_crow *__restrict__ crow, const gemm_tile *__restrict__ tiles,
+    uint32_t ntiles) {
+    __shared__ double red[(W > 1 ? W - 1 : 1) * FM * FN * 4 * 64];
+    stacked_body<FM, FN, W, true>(stacks, spairs, arow, crow, tiles, ntiles, red);
+}
+
+template<int FM, int FN, int W, bool DB>
+__device__ __forceinline__ void stacked_body(const stack_desc *__restrict__ stacks,
+    const stack_pair *__restrict__ spairs, const stack_arow *__restrict__ arow,
+    const stack_crow *__restrict__ crow, const gemm_tile *__restrict__ tiles,
+    uint32_t ntiles, double *red) {
 
     constexpr int TM = 16 * FM, TN = 16 * FN;
-    __shared__ double red[(W > 1 ? W - 1 : 1) * FM * FN * 4 * 64];
     const int lane = threadIdx.x & 63;
     const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
     const int lr = lane & 15, lg = lane >> 4;
@@ -890,6 +919,15 @@ gemm_stacked_kernel(const stack_desc *__restrict__ stacks,
     }
 }
 
+template<int FM, int FN, int W, int OCC>
+static hipError_t launch_stacked_occ_t(const stack_desc *st, const stack_pair *sp,
+    const stack_arow *ar, const stack_crow *cr, const gemm_tile *tiles, uint32_t ntiles,
+    hipStream_t s) {
+    hipLaunchKernelGGL((gemm_stacked_occ_kernel<FM, FN, W, OCC>), dim3(ntiles), dim3(64 * W), 0, s,
+        st, sp, ar, cr, tiles, ntiles);
+    return hipGetLastError();
+}
+
 template<int FM, int FN, int W, bool DB = true>
 static hipError_t launch_stacked_t(const stack_desc *st, const stack_pair *sp,
     const stack_arow *ar, const stack_crow *cr, const gemm_tile *tiles, uint32_t ntiles,
@@ -914,6 +952,9 @@ hipError_t launch_gemm_stacked(int cfg, const stack_desc *st, const stack_pair *
     case STACK_64x48_W4_SB: return launch_stacked_t<4, 3, 4, false>(st, sp, ar, cr, tiles, ntiles, s);
     case STACK_64x48_W2: return launch_stacked_t<4, 3, 2>(st, sp, ar, cr, tiles, ntiles, s);
     case STACK_48x64_W2: return launch_stacked_t<3, 4, 2>(st, sp, ar, cr, tiles, ntiles, s);
+    case STACK_48x32_W4_O2: return launch_stacked_occ_t<3, 2, 4, 2>(st, sp, ar, cr, tiles, ntiles, s);
+    case STACK_32x32_W4_O3: return launch_stacked_occ_t<2, 2, 4, 3>(st, sp, ar, cr, tiles, ntiles, s);
+    case STACK_16x64_W4_O3: return launch_stacked_occ_t<1, 4, 4, 3>(st, sp, ar, cr, tiles, ntiles, s);
     default: return hipErrorInvalidValue;
     }
 }
