@@ -92,6 +92,15 @@ int lt_dev_download(lt_ctx *ctx, void *host, const void *src, size_t bytes);
 int lt_tod_copy(lt_ctx *ctx, size_t n, const size_t *dims_a, const double *a,
     const size_t *perm, double c, int zero, double *b);
 
+/* tod_add<N> (libtensor/dense_tensor/tod_add.h:54-100: the constructor
+ * takes the first operand, add_op() the others, perform(zero, b)):
+ * b = (zero ? 0 : b) + sum_i c[i] P_i(a[i]), operands added in order (one
+ * accumulating permute per operand).  All a[i] have dimensions dims (before
+ * their permutation); perms holds nops permutations of n entries each (NULL:
+ * identities). */
+int lt_tod_add(lt_ctx *ctx, size_t n, const size_t *dims, size_t nops,
+    const double *const *a, const size_t *perms, const double *c, int zero, double *b);
+
 /* tod_set<N>(v).perform(zero, a) (libtensor/dense_tensor/tod_set.h:37). */
 int lt_tod_set(lt_ctx *ctx, size_t nelem, double v, double *a);
 

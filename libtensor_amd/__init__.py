@@ -16,7 +16,7 @@ from ._capi import (GEMM_CONFIGS, LT_MODE_GROUPED, LT_MODE_OZAKI, LT_MODE_PER_BL
 
 __all__ = ["Context", "product_table", "block_index_space", "block_tensor",
            "se_perm", "se_part", "se_label", "contraction2", "btod_contract2",
-           "device_array", "tod_copy", "tod_set", "tod_contract2", "tod_dotprod", "btod_dotprod", "server",
+           "device_array", "tod_copy", "tod_add", "tod_set", "tod_contract2", "tod_dotprod", "btod_dotprod", "server",
            "LtError", "GEMM_CONFIGS", "LT_MODE_PER_BLOCK", "LT_MODE_GROUPED", "LT_MODE_OZAKI"]
 
 
@@ -426,6 +426,22 @@ def tod_copy(a, b, perm=None, c=1.0, zero=True):
     """b = c P(a) (zero) or b += c P(a) (libtensor/dense_tensor/tod_copy.h)."""
     p = None if perm is None else sizes(perm)
     call("lt_tod_copy", a.ctx._h, len(a.shape), sizes(a.shape), a.ptr, p, float(c),
+         1 if zero else 0, b.ptr)
+
+
+def tod_add(ops, b, zero=True):
+    """b = (zero ? 0 : b) + sum c_i P_i(a_i) for ops = [(a_i, perm_i, c_i)], all a_i of
+    the same dimensions (libtensor/dense_tensor/tod_add.h)."""
+    a0 = ops[0][0]
+    n = len(a0.shape)
+    ptrs = (ctypes.c_void_p * len(ops))(*[a.ptr for a, _, _ in ops])
+    perms = []
+    for a, p, _ in ops:
+        if tuple(a.shape) != tuple(a0.shape):
+            raise LtError(-2, "tod_add: operands of different shapes")
+        perms += list(range(n)) if p is None else list(p)
+    coeffs = (ctypes.c_double * len(ops))(*[float(c) for _, _, c in ops])
+    call("lt_tod_add", a0.ctx._h, n, sizes(a0.shape), len(ops), ptrs, sizes(perms), coeffs,
          1 if zero else 0, b.ptr)
 
 

@@ -70,6 +70,8 @@ _SIGS = {
     "lt_tod_copy": [vp, ctypes.c_size_t, c_size_p, vp, c_size_p, ctypes.c_double,
                     ctypes.c_int, vp],
     "lt_tod_set": [vp, ctypes.c_size_t, ctypes.c_double, vp],
+    "lt_tod_add": [vp, ctypes.c_size_t, c_size_p, ctypes.c_size_t, ctypes.POINTER(vp),
+                   c_size_p, c_double_p, ctypes.c_int, vp],
     "lt_tod_contract2": [vp, ctypes.c_size_t, ctypes.c_size_t, ctypes.c_size_t,
                          c_size_p, c_size_p, c_size_p, c_size_p, vp, c_size_p, vp,
                          ctypes.c_double, ctypes.c_int, vp],

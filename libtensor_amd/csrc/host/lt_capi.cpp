@@ -208,6 +208,32 @@ int lt_tod_copy(lt_ctx *ctx, size_t n, const size_t *dims_a, const double *a,
     });
 }
 
+int lt_tod_add(lt_ctx *ctx, size_t n, const size_t *dims, size_t nops,
+    const double *const *a, const size_t *perms, const double *c, int zero, double *b) {
+    return guard([&]() {
+        need(ctx, "ctx"); need(dims, "dims"); need(b, "b");
+        ctx->c->require_device("tod_add");
+        lt_require(n >= 1 && n <= LT_MAX_ORDER, LT_ERR_PARAM, "order");
+        lt_require(nops >= 1, LT_ERR_PARAM, "tod_add needs an operand");
+        need(a, "a"); need(c, "c");
+        dims_t da = dims_t::from(n, dims);
+        // every permuted operand has the dimensions of the first one
+        dims_t d0;
+        for(size_t i = 0; i < nops; i++) {
+            need(a[i], "a[i]");
+            perm_t p = perm_or_identity(n, perms ? perms + i * n : 0);
+            dims_t di = da;
+            di.permute(p);
+            if(i == 0) d0 = di;
+            lt_require(di == d0, LT_ERR_DIMS, "tod_add: operands of different dimensions");
+        }
+        for(size_t i = 0; i < nops; i++) {
+            tensor_tr tr(perm_or_identity(n, perms ? perms + i * n : 0), scalar_tr(c[i]));
+            dense_permute(*ctx->c, a[i], da, tr, b, !(zero && i == 0));
+        }
+    });
+}
+
 int lt_tod_set(lt_ctx *ctx, size_t nelem, double v, double *a) {
     return guard([&]() {
         need(ctx, "ctx"); need(a, "a");

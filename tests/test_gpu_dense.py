@@ -115,3 +115,19 @@ def test_set(gpu_ctx):
     d = lt.device_array(gpu_ctx, (1000,))
     lt.tod_set(d, 3.5)
     assert np.all(d.download() == 3.5)
+
+
+def test_tod_add(gpu_ctx):
+    """tod_add: b = (zero ? 0 : b) + sum c_i P_i(a_i), operands in order."""
+    rng = np.random.default_rng(8)
+    shape = (4, 6, 4, 6)
+    arrs = [rng.standard_normal(shape) for _ in range(3)]
+    perms = [None, (2, 1, 0, 3), (0, 3, 2, 1)]
+    coeffs = [1.0, -0.5, 2.0]
+    das = [lt.device_array(gpu_ctx, shape, x) for x in arrs]
+    ref = sum(O.copy(x, p, c) for x, p, c in zip(arrs, perms, coeffs))
+    db = lt.device_array(gpu_ctx, ref.shape)
+    lt.tod_add(list(zip(das, perms, coeffs)), db, zero=True)
+    assert np.allclose(db.download(), ref, rtol=0, atol=1e-13)
+    lt.tod_add(list(zip(das, perms, coeffs)), db, zero=False)
+    assert np.allclose(db.download(), 2 * ref, rtol=0, atol=1e-13)
