@@ -204,7 +204,7 @@ int lt_btod_contract2_clst(lt_btod_contract2 *op, size_t aidx_c, double *out,
 int lt_btod_contract2_compute_block_host(lt_btod_contract2 *op, size_t aidx_c,
     double *host);
 int lt_btod_contract2_stats(lt_btod_contract2 *op, lt_stats *out);
-/* Kernel configuration of the contraction GEMM: 0..24 fixed, -1 chosen by a
+/* Kernel configuration of the contraction GEMM: 0..26 fixed, -1 chosen by a
  * padding estimate, -2 (default) measured on the plan's own work list when
  * it is built (autotuner). */
 int lt_btod_contract2_set_tile(lt_btod_contract2 *op, int cfg);

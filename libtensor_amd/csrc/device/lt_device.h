@@ -94,10 +94,12 @@ enum { TILE_64x64 = 0, TILE_32x64 = 1, TILE_64x32 = 2, TILE_32x32 = 3,
        STACK_64x32_W4 = 15, STACK_48x64_W4 = 16, STACK_64x48_W4 = 17,
        STACK_48x32_W4 = 18, STACK_64x48_W4_SB = 19, STACK_64x48_W2 = 20,
        STACK_48x64_W2 = 21, STACK_48x32_W4_O2 = 22, STACK_32x32_W4_O3 = 23,
-       STACK_16x64_W4_O3 = 24,
-       TILE_COUNT = 25 };
+       STACK_16x64_W4_O3 = 24, STACK_VAR_64x32_W4 = 25, STACK_VAR_64x48_W4 = 26,
+       TILE_COUNT = 27 };
 void tile_shape(int cfg, uint32_t &bm, uint32_t &bn);
-inline bool is_stacked(int cfg) { return cfg >= STACK_16x64_W4 && cfg <= STACK_16x64_W4_O3; }
+inline bool is_stacked(int cfg) { return cfg >= STACK_16x64_W4 && cfg <= STACK_VAR_64x48_W4; }
+//! Stacked configurations over tiles of varying shape (var_tile lists).
+inline bool is_var(int cfg) { return cfg == STACK_VAR_64x32_W4 || cfg == STACK_VAR_64x48_W4; }
 //! The per-C-block configuration with the same summation order (same
 //! split-K width; the tile shape only partitions the output).
 inline int unstacked(int cfg) {
@@ -107,6 +109,7 @@ inline int unstacked(int cfg) {
     case STACK_32x64_W4: case STACK_48x64_W4: return SPLITK_32x64_W4;
     case STACK_64x32_W4: case STACK_64x48_W4: case STACK_48x32_W4: case STACK_64x48_W4_SB:
     case STACK_48x32_W4_O2: case STACK_32x32_W4_O3: case STACK_16x64_W4_O3:
+    case STACK_VAR_64x32_W4: case STACK_VAR_64x48_W4:
         return SPLITK_64x32_W4;
     case STACK_64x48_W2: case STACK_48x64_W2: return SPLITK_32x32_W2;
     default: return cfg;
@@ -141,6 +144,16 @@ struct stack_crow {
     double *c;                        //!< start of the row in its C block
     uint32_t zero, pad;
 };
+//! Tile of a varying-shape stacked launch: rows 16 r16 .. + 16 fm,
+//! columns 16 c16 .. + 16 fn of stack `stack`.
+struct var_tile {
+    uint32_t stack;
+    uint16_t r16, c16;
+    uint16_t fm, fn;
+};
+hipError_t launch_gemm_stacked_var(int cfg, const stack_desc *st, const stack_pair *sp,
+    const stack_arow *ar, const stack_crow *cr, const var_tile *tiles, uint32_t ntiles,
+    hipStream_t s);
 hipError_t launch_gemm_stacked(int cfg, const stack_desc *st, const stack_pair *sp,
     const stack_arow *ar, const stack_crow *cr, const gemm_tile *tiles, uint32_t ntiles,
     hipStream_t s);

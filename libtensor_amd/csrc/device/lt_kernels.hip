@@ -784,139 +784,197 @@ gemm_stacked_occ_kernel(const stack_desc *__restrict__ stacks,
     stacked_body<FM, FN, W, true>(stacks, spairs, arow, crow, tiles, ntiles, red);
 }
 
+//! One (16 FM) x (16 FN) tile of a stack at rows m0, columns n0.
 template<int FM, int FN, int W, bool DB>
-__device__ __forceinline__ void stacked_body(const stack_desc *__restrict__ stacks,
+__device__ __forceinline__ void stacked_tile(const stack_desc &st,
     const stack_pair *__restrict__ spairs, const stack_arow *__restrict__ arow,
-    const stack_crow *__restrict__ crow, const gemm_tile *__restrict__ tiles,
-    uint32_t ntiles, double *red) {
+    const stack_crow *__restrict__ crow, int m0, int n0, double *red) {
 
-    constexpr int TM = 16 * FM, TN = 16 * FN;
     const int lane = threadIdx.x & 63;
     const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
     const int lr = lane & 15, lg = lane >> 4;
+    const int mrem = int(st.rows) - m0, nrem = int(st.n) - n0;
 
-    for(uint32_t t = blockIdx.x; t < ntiles; t += gridDim.x) {
-        const gemm_tile tile = tiles[t];
-        const stack_desc st = stacks[tile.cblock];
-        const int m0 = int(tile.tm) * TM, n0 = int(tile.tn) * TN;
-        const int mrem = int(st.rows) - m0, nrem = int(st.n) - n0;
+    double4_t acc[FM][FN];
+#pragma unroll
+    for(int i = 0; i < FM; i++)
+#pragma unroll
+        for(int j = 0; j < FN; j++) acc[i][j] = double4_t{0.0, 0.0, 0.0, 0.0};
 
-        double4_t acc[FM][FN];
+    int srow[FM], bcol[FN];
 #pragma unroll
-        for(int i = 0; i < FM; i++)
+    for(int i = 0; i < FM; i++) srow[i] = (16 * i + lr) < mrem ? m0 + 16 * i + lr : 0;
 #pragma unroll
-            for(int j = 0; j < FN; j++) acc[i][j] = double4_t{0.0, 0.0, 0.0, 0.0};
+    for(int j = 0; j < FN; j++) bcol[j] = (16 * j + lr) < nrem ? n0 + 16 * j + lr : 0;
 
-        int srow[FM], bcol[FN];
-#pragma unroll
-        for(int i = 0; i < FM; i++) srow[i] = (16 * i + lr) < mrem ? m0 + 16 * i + lr : 0;
-#pragma unroll
-        for(int j = 0; j < FN; j++) bcol[j] = (16 * j + lr) < nrem ? n0 + 16 * j + lr : 0;
-
-        const uint32_t pbeg = st.pair_begin, pend = st.pair_begin + st.npairs;
-        uint32_t px = pbeg;
-        int cx = wave;
-        seek_chunk_st(spairs, pend, px, cx);
-        if(!DB) {
-            // single buffer: fewer registers, latency hidden by other waves
-            double ax[FM][4], bx[FN][4];
-            st_rowsrc<FM, FN> rx;
-            uint32_t pr = 0xffffffffu;
-            stack_pair sx;
-            while(px < pend) {
-                if(px != pr) {
-                    sx = spairs[px];
-                    st_rows<FM, FN>(rx, arow, st.arow_begin + (px - pbeg) * st.rows, srow, sx, bcol);
-                    pr = px;
-                }
-                load_chunk_st_any<FM, FN>(ax, bx, rx, sx, cx, bcol, lg);
-                mma_chunk_st<FM, FN>(acc, ax, bx, rx, sx.k, cx, lg);
-                cx += W;
-                seek_chunk_st(spairs, pend, px, cx);
+    const uint32_t pbeg = st.pair_begin, pend = st.pair_begin + st.npairs;
+    uint32_t px = pbeg;
+    int cx = wave;
+    seek_chunk_st(spairs, pend, px, cx);
+    if(!DB) {
+        // single buffer: fewer registers, latency hidden by other waves
+        double ax[FM][4], bx[FN][4];
+        st_rowsrc<FM, FN> rx;
+        uint32_t pr = 0xffffffffu;
+        stack_pair sx;
+        while(px < pend) {
+            if(px != pr) {
+                sx = spairs[px];
+                st_rows<FM, FN>(rx, arow, st.arow_begin + (px - pbeg) * st.rows, srow, sx, bcol);
+                pr = px;
             }
-        } else if(px < pend) {
-            double ax[FM][4], bx[FN][4], ay[FM][4], by[FN][4];
-            st_rowsrc<FM, FN> rx, ry;
-            stack_pair sx = spairs[px];
-            st_rows<FM, FN>(rx, arow, st.arow_begin + (px - pbeg) * st.rows, srow, sx, bcol);
             load_chunk_st_any<FM, FN>(ax, bx, rx, sx, cx, bcol, lg);
-            while(true) {
-                uint32_t py = px;
-                int cy = cx + W;
-                seek_chunk_st(spairs, pend, py, cy);
-                const bool hy = py < pend;
-                if(!hy) { py = px; cy = cx; }       // dummy in-bounds load
-                const stack_pair sy = spairs[py];
-                if(py == px) ry = rx;
-                else st_rows<FM, FN>(ry, arow, st.arow_begin + (py - pbeg) * st.rows, srow, sy, bcol);
-                load_chunk_st_any<FM, FN>(ay, by, ry, sy, cy, bcol, lg);
-                mma_chunk_st<FM, FN>(acc, ax, bx, rx, sx.k, cx, lg);
-                if(!hy) break;
-                uint32_t pz = py;
-                int cz = cy + W;
-                seek_chunk_st(spairs, pend, pz, cz);
-                const bool hz = pz < pend;
-                if(!hz) { pz = py; cz = cy; }
-                sx = spairs[pz];
-                if(pz == py) rx = ry;
-                else st_rows<FM, FN>(rx, arow, st.arow_begin + (pz - pbeg) * st.rows, srow, sx, bcol);
-                load_chunk_st_any<FM, FN>(ax, bx, rx, sx, cz, bcol, lg);
-                mma_chunk_st<FM, FN>(acc, ay, by, ry, sy.k, cy, lg);
-                if(!hz) break;
-                px = pz;
-                cx = cz;
-            }
+            mma_chunk_st<FM, FN>(acc, ax, bx, rx, sx.k, cx, lg);
+            cx += W;
+            seek_chunk_st(spairs, pend, px, cx);
         }
+    } else if(px < pend) {
+        double ax[FM][4], bx[FN][4], ay[FM][4], by[FN][4];
+        st_rowsrc<FM, FN> rx, ry;
+        stack_pair sx = spairs[px];
+        st_rows<FM, FN>(rx, arow, st.arow_begin + (px - pbeg) * st.rows, srow, sx, bcol);
+        load_chunk_st_any<FM, FN>(ax, bx, rx, sx, cx, bcol, lg);
+        while(true) {
+            uint32_t py = px;
+            int cy = cx + W;
+            seek_chunk_st(spairs, pend, py, cy);
+            const bool hy = py < pend;
+            if(!hy) { py = px; cy = cx; }       // dummy in-bounds load
+            const stack_pair sy = spairs[py];
+            if(py == px) ry = rx;
+            else st_rows<FM, FN>(ry, arow, st.arow_begin + (py - pbeg) * st.rows, srow, sy, bcol);
+            load_chunk_st_any<FM, FN>(ay, by, ry, sy, cy, bcol, lg);
+            mma_chunk_st<FM, FN>(acc, ax, bx, rx, sx.k, cx, lg);
+            if(!hy) break;
+            uint32_t pz = py;
+            int cz = cy + W;
+            seek_chunk_st(spairs, pend, pz, cz);
+            const bool hz = pz < pend;
+            if(!hz) { pz = py; cz = cy; }
+            sx = spairs[pz];
+            if(pz == py) rx = ry;
+            else st_rows<FM, FN>(rx, arow, st.arow_begin + (pz - pbeg) * st.rows, srow, sx, bcol);
+            load_chunk_st_any<FM, FN>(ax, bx, rx, sx, cz, bcol, lg);
+            mma_chunk_st<FM, FN>(acc, ay, by, ry, sy.k, cy, lg);
+            if(!hz) break;
+            px = pz;
+            cx = cz;
+        }
+    }
 
-        if(W > 1) {
-            if(wave > 0) {
-                double *r = red + (wave - 1) * (FM * FN * 4 * 64);
+    if(W > 1) {
+        if(wave > 0) {
+            double *r = red + (wave - 1) * (FM * FN * 4 * 64);
+#pragma unroll
+            for(int i = 0; i < FM; i++)
+#pragma unroll
+                for(int j = 0; j < FN; j++)
+#pragma unroll
+                    for(int v = 0; v < 4; v++)
+                        r[((i * FN + j) * 4 + v) * 64 + lane] = acc[i][j][v];
+        }
+        __syncthreads();
+        if(wave == 0) {
+            for(int w = 1; w < W; w++) {
+                const double *r = red + (w - 1) * (FM * FN * 4 * 64);
 #pragma unroll
                 for(int i = 0; i < FM; i++)
 #pragma unroll
                     for(int j = 0; j < FN; j++)
 #pragma unroll
                         for(int v = 0; v < 4; v++)
-                            r[((i * FN + j) * 4 + v) * 64 + lane] = acc[i][j][v];
-            }
-            __syncthreads();
-            if(wave == 0) {
-                for(int w = 1; w < W; w++) {
-                    const double *r = red + (w - 1) * (FM * FN * 4 * 64);
-#pragma unroll
-                    for(int i = 0; i < FM; i++)
-#pragma unroll
-                        for(int j = 0; j < FN; j++)
-#pragma unroll
-                            for(int v = 0; v < 4; v++)
-                                acc[i][j][v] += r[((i * FN + j) * 4 + v) * 64 + lane];
-                }
+                            acc[i][j][v] += r[((i * FN + j) * 4 + v) * 64 + lane];
             }
         }
+    }
 
-        if(wave == 0) {
+    if(wave == 0) {
 #pragma unroll
-            for(int i = 0; i < FM; i++) {
+        for(int i = 0; i < FM; i++) {
 #pragma unroll
-                for(int v = 0; v < 4; v++) {
-                    const int row = 16 * i + lg + 4 * v;
-                    if(row >= mrem) continue;
-                    const stack_crow cr = crow[st.crow_begin + m0 + row];
+            for(int v = 0; v < 4; v++) {
+                const int row = 16 * i + lg + 4 * v;
+                if(row >= mrem) continue;
+                const stack_crow cr = crow[st.crow_begin + m0 + row];
 #pragma unroll
-                    for(int j = 0; j < FN; j++) {
-                        const int col = 16 * j + lr;
-                        if(col < nrem) {
-                            double *cp = cr.c + (n0 + col);
-                            double x = acc[i][j][v];
-                            if(!cr.zero) x += *cp;
-                            *cp = x;
-                        }
+                for(int j = 0; j < FN; j++) {
+                    const int col = 16 * j + lr;
+                    if(col < nrem) {
+                        double *cp = cr.c + (n0 + col);
+                        double x = acc[i][j][v];
+                        if(!cr.zero) x += *cp;
+                        *cp = x;
                     }
                 }
             }
         }
-        if(W > 1) __syncthreads();   // red is reused by the next tile
     }
+    if(W > 1) __syncthreads();   // red is reused by the next tile
+}
+
+template<int FM, int FN, int W, bool DB>
+__device__ __forceinline__ void stacked_body(const stack_desc *__restrict__ stacks,
+    const stack_pair *__restrict__ spairs, const stack_arow *__restrict__ arow,
+    const stack_crow *__restrict__ crow, const gemm_tile *__restrict__ tiles,
+    uint32_t ntiles, double *red) {
+    for(uint32_t t = blockIdx.x; t < ntiles; t += gridDim.x) {
+        const gemm_tile tile = tiles[t];
+        const stack_desc st = stacks[tile.cblock];
+        stacked_tile<FM, FN, W, DB>(st, spairs, arow, crow, int(tile.tm) * 16 * FM,
+            int(tile.tn) * 16 * FN, red);
+    }
+}
+
+//! Stacked kernel over tiles of varying height and width (16 fm x 16 fn,
+//! fm <= 4, fn <= FNX): every stack is covered by full tiles and one
+//! narrower tile per remainder, so rows and columns are padded to 16 only.
+template<int W, int FNX>
+__global__ void __launch_bounds__(64 * W)
+gemm_stacked_var_kernel(const stack_desc *__restrict__ stacks,
+    const stack_pair *__restrict__ spairs, const stack_arow *__restrict__ arow,
+    const stack_crow *__restrict__ crow, const var_tile *__restrict__ tiles,
+    uint32_t ntiles) {
+    __shared__ double red[(W > 1 ? W - 1 : 1) * 4 * FNX * 4 * 64];
+    for(uint32_t t = blockIdx.x; t < ntiles; t += gridDim.x) {
+        const var_tile vt = tiles[t];
+        const stack_desc st = stacks[vt.stack];
+        const int m0 = 16 * int(vt.r16), n0 = 16 * int(vt.c16);
+        switch(vt.fm * 4 + vt.fn) {
+        case 1 * 4 + 1: stacked_tile<1, 1, W, true>(st, spairs, arow, crow, m0, n0, red); break;
+        case 2 * 4 + 1: stacked_tile<2, 1, W, true>(st, spairs, arow, crow, m0, n0, red); break;
+        case 3 * 4 + 1: stacked_tile<3, 1, W, true>(st, spairs, arow, crow, m0, n0, red); break;
+        case 4 * 4 + 1: stacked_tile<4, 1, W, true>(st, spairs, arow, crow, m0, n0, red); break;
+        case 1 * 4 + 2: stacked_tile<1, 2, W, true>(st, spairs, arow, crow, m0, n0, red); break;
+        case 2 * 4 + 2: stacked_tile<2, 2, W, true>(st, spairs, arow, crow, m0, n0, red); break;
+        case 3 * 4 + 2: stacked_tile<3, 2, W, true>(st, spairs, arow, crow, m0, n0, red); break;
+        case 4 * 4 + 2: stacked_tile<4, 2, W, true>(st, spairs, arow, crow, m0, n0, red); break;
+        default:
+            if(FNX >= 3) {
+                switch(vt.fm) {
+                case 1: stacked_tile<1, (FNX >= 3 ? 3 : 1), W, true>(st, spairs, arow, crow, m0, n0, red); break;
+                case 2: stacked_tile<2, (FNX >= 3 ? 3 : 1), W, true>(st, spairs, arow, crow, m0, n0, red); break;
+                case 3: stacked_tile<3, (FNX >= 3 ? 3 : 1), W, true>(st, spairs, arow, crow, m0, n0, red); break;
+                default: stacked_tile<4, (FNX >= 3 ? 3 : 1), W, true>(st, spairs, arow, crow, m0, n0, red); break;
+                }
+            }
+            break;
+        }
+    }
+}
+
+hipError_t launch_gemm_stacked_var(int cfg, const stack_desc *st, const stack_pair *sp,
+    const stack_arow *ar, const stack_crow *cr, const var_tile *tiles, uint32_t ntiles,
+    hipStream_t s) {
+    if(ntiles == 0) return hipSuccess;
+    if(cfg == STACK_VAR_64x32_W4)
+        hipLaunchKernelGGL((gemm_stacked_var_kernel<4, 2>), dim3(ntiles), dim3(256), 0, s,
+            st, sp, ar, cr, tiles, ntiles);
+    else if(cfg == STACK_VAR_64x48_W4)
+        hipLaunchKernelGGL((gemm_stacked_var_kernel<4, 3>), dim3(ntiles), dim3(256), 0, s,
+            st, sp, ar, cr, tiles, ntiles);
+    else
+        return hipErrorInvalidValue;
+    return hipGetLastError();
 }
 
 template<int FM, int FN, int W, int OCC>

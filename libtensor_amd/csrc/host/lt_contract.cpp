@@ -188,6 +188,7 @@ struct contract_plan {
     std::vector<dev::stack_crow> crows;
     std::vector<double> stack_ksum;
     std::vector<dev::gemm_tile> stiles;
+    std::vector<dev::var_tile> vtiles;
     // offset tables of operands read in place by the stacked kernels
     std::vector<int32_t> otab;
     std::vector<std::pair<int64_t, int64_t> > spair_tab;  // (koff, noff) indexes or -1
@@ -201,6 +202,7 @@ struct contract_plan {
     dev::stack_arow *d_arows;
     dev::stack_crow *d_crows;
     dev::gemm_tile *d_stiles;
+    dev::var_tile *d_vtiles;
     dev::perm_desc *d_prep, *d_wback;
     uint32_t *d_prep_cd, *d_prep_co, *d_wb_cd, *d_wb_co;
     uint32_t n_prep_chunks, n_wb_chunks;
@@ -232,7 +234,7 @@ struct contract_plan {
         d_prep_co(0), d_wb_cd(0), d_wb_co(0), n_prep_chunks(0), n_wb_chunks(0),
         d_cb(0), d_pairs(0), d_tiles_sorted(0), d_tiles_block(0), scratch(0),
         scratch_elems(0), ctx(0), events(false), graph(0), gexec(0), graph_launches(0),
-        d_stacks(0), d_spairs(0), d_arows(0), d_crows(0), d_stiles(0),
+        d_stacks(0), d_spairs(0), d_arows(0), d_crows(0), d_stiles(0), d_vtiles(0),
         n_prep_st_chunks(0), d_prep_st(0), d_pst_cd(0), d_pst_co(0),
         oz_slices(0), d_oz_split(0), d_oz_items(0), n_oz_items(0), d_oz_pairs(0),
         d_oz_tiles(0), n_oz_tiles(0) { }
@@ -246,6 +248,9 @@ struct contract_plan {
 
 //! The grouped GEMM launch of a plan in configuration k.
 hipError_t launch_grouped(const contract_plan &pl, int k, hipStream_t s) {
+    if(dev::is_var(k))
+        return dev::launch_gemm_stacked_var(k, pl.d_stacks, pl.d_spairs, pl.d_arows, pl.d_crows,
+            pl.d_vtiles, (uint32_t)pl.vtiles.size(), s);
     if(dev::is_stacked(k))
         return dev::launch_gemm_stacked(k, pl.d_stacks, pl.d_spairs, pl.d_arows, pl.d_crows,
             pl.d_stiles, (uint32_t)pl.stiles.size(), s);
@@ -1007,7 +1012,39 @@ void btod_contract2::build_plan(btensor &c, bool add, double dscale) {
         pl.tile_cfg = k;
         pl.tile_order = order;
         pl.stiles.clear();
-        if(dev::is_stacked(k)) {
+        pl.vtiles.clear();
+        if(dev::is_var(k)) {
+            // full 64-row tiles and one tile per row / column remainder,
+            // largest (k sum x area) first
+            const uint32_t fnx = (k == dev::STACK_VAR_64x48_W4) ? 3 : 2;
+            std::vector<std::pair<double, dev::var_tile> > vt;
+            for(size_t si = 0; si + 1 < pl.stacks.size(); si++) {
+                const dev::stack_desc &sd = pl.stacks[si];
+                for(uint32_t r0 = 0; r0 < sd.rows; r0 += 64) {
+                    uint32_t fm = std::min<uint32_t>(4, (sd.rows - r0 + 15) / 16);
+                    for(uint32_t c0 = 0; c0 < sd.n; c0 += 16 * fnx) {
+                        uint32_t fn = std::min<uint32_t>(fnx, (sd.n - c0 + 15) / 16);
+                        lt_require(r0 / 16 < 65536 && c0 / 16 < 65536, LT_ERR_PARAM, "stack too large");
+                        dev::var_tile t;
+                        t.stack = (uint32_t)si;
+                        t.r16 = (uint16_t)(r0 / 16); t.c16 = (uint16_t)(c0 / 16);
+                        t.fm = (uint16_t)fm; t.fn = (uint16_t)fn;
+                        vt.push_back(std::make_pair(pl.stack_ksum[si] * fm * fn, t));
+                    }
+                }
+            }
+            std::stable_sort(vt.begin(), vt.end(),
+                [](const std::pair<double, dev::var_tile> &x, const std::pair<double, dev::var_tile> &y) {
+                    return x.first > y.first; });
+            for(size_t i = 0; i < vt.size(); i++) pl.vtiles.push_back(vt[i].second);
+            size_t nb = pl.vtiles.size() * sizeof(dev::var_tile);
+            if(pl.stk_tiles.mem) ctx.free(pl.stk_tiles.mem);
+            pl.stk_tiles.ctx = &ctx;
+            pl.stk_tiles.mem = ctx.alloc(nb ? nb : 256);
+            pl.stk_tiles.bytes = nb ? nb : 256;
+            if(nb) ctx.upload(pl.stk_tiles.mem, pl.vtiles.data(), nb);
+            pl.d_vtiles = (dev::var_tile*)pl.stk_tiles.mem;
+        } else if(dev::is_stacked(k)) {
             // stack tiles, longest (largest k sum) first
             uint32_t bm = 64, bn = 64;
             dev::tile_shape(k, bm, bn);
