@@ -269,15 +269,16 @@ def main():
         serve = serving_mix(ctx, lt, W, args.serve_clients, args.serve_rounds, args.serve_rate)
 
     achieved = flops / (gemm_avg * 1e-3) / 1e12
-    # HBM bytes per GEMM launch from the committed rocprofv3 counter runs
-    # (valid for the kernel configuration they were taken with)
+    # HBM bytes per GEMM launch from the committed rocprofv3 counter runs,
+    # looked up for the kernel configuration and tile order chosen here
     traffic = None
     tpath = os.path.join(HERE, "profiles", "pp_ladder_o40_v200_traffic.json")
     if os.path.exists(tpath):
         with open(tpath) as f:
             tr = json.load(f)
-        if int(st["tile_cfg"]) in tr.get("gemm_kernel_cfgs", []):
-            traffic = tr["traffic_bytes_per_launch"]
+        for e in tr.get("launches", []):
+            if e["cfg"] == int(st["tile_cfg"]) and e["order"] == int(st["tile_order"]):
+                traffic = e["traffic_bytes"]
     line = {
         "metric": METRIC, "value": value, "unit": "TFLOP/s", "n_gpus": 1,
         "steps": args.steps, "warmup": args.warmup, "ms_per_step": ms_per_step,
@@ -295,7 +296,8 @@ def main():
         "per_block_ms_per_step": per_block_ms,
         "gemm_ms": gemm_avg,
         "plan_ms": st["t_plan_ms"], "schedule_ms": st["t_schedule_ms"],
-        "gemm_kernel_cfg": int(st["tile_cfg"]), "autotune_ms": st["tune_ms"],
+        "gemm_kernel_cfg": int(st["tile_cfg"]), "gemm_tile_order": int(st["tile_order"]),
+        "autotune_ms": st["tune_ms"],
         "p50_ms": serve["p50_ms"] if serve else None,
         "p99_ms": serve["p99_ms"] if serve else None,
         "serving": serve,

@@ -58,7 +58,8 @@ typedef struct lt_stats {
                                configuration (0 = not measured) */
     int32_t graph;           /* 1: the last perform ran as a HIP graph
                                 (t_gemm_ms is then 0: not separately timed) */
-    int32_t pad;
+    int32_t tile_order;      /* grouped launch tile order: 0 longest C block
+                                first, 1 tiles dealt to XCD groups */
 } lt_stats;
 
 /* Message of the calling thread's last failed call. */

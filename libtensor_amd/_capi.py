@@ -39,7 +39,7 @@ class LtStats(ctypes.Structure):
         ("ozaki_max_rel_err", ctypes.c_double),
         ("tune_ms", ctypes.c_double * 32),
         ("graph", ctypes.c_int32),
-        ("pad", ctypes.c_int32),
+        ("tile_order", ctypes.c_int32),
     ]
 
 

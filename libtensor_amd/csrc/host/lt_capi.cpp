@@ -853,7 +853,7 @@ int lt_btod_contract2_stats(lt_btod_contract2 *op, lt_stats *out) {
         out->ozaki_max_rel_err = s.ozaki_max_rel_err;
         for(int i = 0; i < 32; i++) out->tune_ms[i] = i < dev::TILE_COUNT ? s.tune_ms[i] : 0.0;
         out->graph = s.graph;
-        out->pad = 0;
+        out->tile_order = s.tile_order;
     });
 }
 

@@ -1163,6 +1163,7 @@ void btod_contract2::build_plan(btensor &c, bool add, double dscale) {
     m_stats.n_unfold = pl.prep.size();
     m_stats.flops = flops;
     m_stats.tile_cfg = cfg;
+    m_stats.tile_order = pl.tile_order;
     double bytes = 0.0;
     for(size_t i = 0; i < pl.prep.size(); i++) bytes += 16.0 * pl.prep[i].total;
     for(size_t i = 0; i < pl.wback.size(); i++)

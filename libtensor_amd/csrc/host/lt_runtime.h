@@ -161,6 +161,7 @@ struct exec_stats {
     int ozaki_slices;
     double tune_ms[dev::TILE_COUNT]; //!< autotuner timings (0 = not measured)
     int graph;              //!< 1 if the last perform ran as a HIP graph
+    int tile_order;         //!< grouped tile order (0 longest first, 1 XCD groups)
 };
 
 struct contract_plan;
