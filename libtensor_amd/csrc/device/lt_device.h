@@ -71,6 +71,10 @@ static const uint32_t kPairVecA = 4, kPairVecB = 8;
 //! Offset-table B whose k offsets come in adjacent even pairs and whose n
 //! offsets are even (two k per load through the tables).
 static const uint32_t kPairVecBT = 16;
+//! Offset-table B whose tables are linear in at most two index groups:
+//! koff[k] = (k / kd) ks0 + (k % kd) ks1 and noff[n] likewise, computed in
+//! the kernel instead of read.
+static const uint32_t kPairOffArith = 32;
 
 //! Work item of a grouped GEMM launch: one output tile of one C block.
 struct gemm_tile {
@@ -82,17 +86,18 @@ struct gemm_tile {
 //! operands; SPLITK_*_Wn: one workgroup of n waves per tile, the tile's
 //! k chunks dealt to the waves, operands loaded straight into MFMA
 //! fragments, partial tiles summed in LDS in a fixed order.
-//! STACK_*: the split-K kernels over stacks (_SB: single-buffered, _On:
-//! compiled for n waves per SIMD) (C blocks sharing their B
-//! operands, rows concatenated); used by grouped launches, per-block
-//! launches run the SPLITK configuration of the same shape.
+//! STACK_*: the split-K kernels over stacks (C blocks sharing their B
+//! operands, rows concatenated; _On: compiled for n waves per SIMD; _P1:
+//! the two-stage pipeline that loads a pair's row table when it starts
+//! the pair's first chunk); used by grouped launches, per-block launches
+//! run the SPLITK configuration of the same shape.
 enum { TILE_64x64 = 0, TILE_32x64 = 1, TILE_64x32 = 2, TILE_32x32 = 3,
        SPLITK_16x64_W4 = 4, SPLITK_32x32_W4 = 5, SPLITK_32x64_W4 = 6,
        SPLITK_64x32_W4 = 7, SPLITK_16x64_W2 = 8, SPLITK_32x32_W2 = 9,
        SPLITK_16x32_W4 = 10, SPLITK_32x16_W4 = 11,
        STACK_16x64_W4 = 12, STACK_32x32_W4 = 13, STACK_32x64_W4 = 14,
        STACK_64x32_W4 = 15, STACK_48x64_W4 = 16, STACK_64x48_W4 = 17,
-       STACK_48x32_W4 = 18, STACK_64x48_W4_SB = 19, STACK_64x48_W2 = 20,
+       STACK_48x32_W4 = 18, STACK_64x32_W4_P1 = 19, STACK_64x48_W2 = 20,
        STACK_48x64_W2 = 21, STACK_48x32_W4_O2 = 22, STACK_32x32_W4_O3 = 23,
        STACK_16x64_W4_O3 = 24, STACK_VAR_64x32_W4 = 25, STACK_VAR_64x48_W4 = 26,
        TILE_COUNT = 27 };
@@ -107,7 +112,7 @@ inline int unstacked(int cfg) {
     case STACK_16x64_W4: return SPLITK_16x64_W4;
     case STACK_32x32_W4: return SPLITK_32x32_W4;
     case STACK_32x64_W4: case STACK_48x64_W4: return SPLITK_32x64_W4;
-    case STACK_64x32_W4: case STACK_64x48_W4: case STACK_48x32_W4: case STACK_64x48_W4_SB:
+    case STACK_64x32_W4: case STACK_64x48_W4: case STACK_48x32_W4: case STACK_64x32_W4_P1:
     case STACK_48x32_W4_O2: case STACK_32x32_W4_O3: case STACK_16x64_W4_O3:
     case STACK_VAR_64x32_W4: case STACK_VAR_64x48_W4:
         return SPLITK_64x32_W4;
@@ -131,7 +136,11 @@ struct stack_pair {
     //! B read through offset tables (not null): B(k, n) = b[koff[k] + noff[n]]
     //! (a permuted operand read in place from its stored block)
     const int32_t *koff, *noff;
-    uint32_t k, ldb, flags, pad;      //!< flags: kPairAKMajor / kPairBNMajor
+    uint32_t k, ldb, flags, pad;      //!< flags: kPairAKMajor / kPairBNMajor ...
+    //! kPairOffArith: the tables as koff[k] = (k / kd) ks0 + (k % kd) ks1,
+    //! noff[n] = (n / nd) ns0 + (n % nd) ns1
+    fastdiv kd, nd;
+    int32_t ks0, ks1, ns0, ns1;
 };
 //! A operand of one row for one pair: row r of A at p[k] (or p[k * lda]
 //! with kPairAKMajor), coefficient d.

@@ -39,11 +39,11 @@ void tile_shape(int cfg, uint32_t &bm, uint32_t &bn) {
     case SPLITK_16x64_W4: case SPLITK_16x64_W2: case STACK_16x64_W4: bm = 16; bn = 64; break;
     case SPLITK_32x32_W4: case SPLITK_32x32_W2: case STACK_32x32_W4: bm = 32; bn = 32; break;
     case SPLITK_32x64_W4: case STACK_32x64_W4: bm = 32; bn = 64; break;
-    case SPLITK_64x32_W4: case STACK_64x32_W4: bm = 64; bn = 32; break;
+    case SPLITK_64x32_W4: case STACK_64x32_W4: case STACK_64x32_W4_P1: bm = 64; bn = 32; break;
     case STACK_48x64_W4: bm = 48; bn = 64; break;
     case STACK_64x48_W4: bm = 64; bn = 48; break;
     case STACK_48x32_W4: bm = 48; bn = 32; break;
-    case STACK_64x48_W4_SB: case STACK_64x48_W2: bm = 64; bn = 48; break;
+    case STACK_64x48_W2: bm = 64; bn = 48; break;
     case STACK_48x64_W2: bm = 48; bn = 64; break;
     case STACK_48x32_W4_O2: bm = 48; bn = 32; break;
     case STACK_32x32_W4_O3: bm = 32; bn = 32; break;
@@ -596,119 +596,141 @@ struct st_rowsrc {
     int32_t no[FN];           //!< B column offsets (offset-table pairs)
 };
 
-template<int FM, int FN, bool TA, bool TB, bool TBL>
-__device__ __forceinline__ void load_chunk_st(double (&af)[FM][4], double (&bf)[FN][4],
-    const st_rowsrc<FM, FN> &ar, const stack_pair &sp, int c, const int (&bcol)[FN], int lg) {
-    const uint32_t K = sp.k;
-    const uint32_t kk = 16 * c + 4 * lg;
-    uint32_t kc[4];
-#pragma unroll
-    for(int j = 0; j < 4; j++) kc[j] = min(kk + j, K - 1);
-    int32_t ko[4];
-    if(TBL) {
-#pragma unroll
-        for(int j = 0; j < 4; j++) ko[j] = sp.koff[kc[j]];
-    }
-#pragma unroll
-    for(int i = 0; i < FM; i++) {
-        gdouble *pa = (gdouble *)ar.p[i];
-#pragma unroll
-        for(int j = 0; j < 4; j++) af[i][j] = pa[TA ? kc[j] * ar.lda[i] : kc[j]];
-    }
-#pragma unroll
-    for(int f = 0; f < FN; f++) {
-        const uint32_t n = uint32_t(bcol[f]);
-#pragma unroll
-        for(int j = 0; j < 4; j++) {
-            const uint32_t e = TBL ? uint32_t(ko[j] + ar.no[f])
-                                   : (TB ? n * sp.ldb + kc[j] : kc[j] * sp.ldb + n);
-            bf[f][j] = ldg(sp.b, 8u * e);
-        }
-    }
+//! Offset of index k in a table linear in two index groups:
+//! (k / f.d) s0 + (k % f.d) s1.
+__device__ __forceinline__ int32_t off_arith(uint32_t k, const fastdiv &f, int32_t s0, int32_t s1) {
+    const uint32_t q = fdiv(k, f);
+    return int32_t(q) * s0 + int32_t(k - q * f.d) * s1;
+}
+
+//! B k offsets of one chunk: none (KT 0), read from the table (KT 1) or
+//! computed (KT 2).
+template<int KT>
+__device__ __forceinline__ int32_t koff_at(const stack_pair &sp, uint32_t k) {
+    if(KT == 1) return sp.koff[k];
+    if(KT == 2) return off_arith(k, sp.kd, sp.ks0, sp.ks1);
+    return 0;
 }
 
 typedef double double2_t __attribute__((ext_vector_type(2)));
 typedef const double2_t __attribute__((address_space(1))) gdouble2;
 
-// Row-major A (k contiguous per row, 16-byte aligned rows, even extent):
-// each lane loads its four k as two 16-byte loads.  With an even extent the
-// pairs (kk, kk + 1) and (kk + 2, kk + 3) (kk even) are each either inside
-// or outside the extent; an outside pair reads the last pair (in bounds)
-// and is zeroed through the A scale.  B is read k-contiguous the same way
-// (VB, n-major) or through its offset tables (TBL).
-template<int FM, int FN, bool VB, bool TBL, bool TBV = false>
-__device__ __forceinline__ void load_chunk_st_vec(double (&af)[FM][4], double (&bf)[FN][4],
-    const st_rowsrc<FM, FN> &ar, const stack_pair &sp, int c, const int (&bcol)[FN], int lg) {
+// Operand loads of one chunk (16 k) of a stacked tile.  Lane (lr, lg) holds
+// k = 16 c + 4 lg + q (q = 0..3) of its fragment rows (A) and columns (B).
+// A k past the extent reads an in-bounds element and is zeroed through the
+// A scale.  A and B are loaded independently, each in the form its layout
+// allows:
+//   A: k-contiguous rows, 16-byte aligned, even extent (kPairVecA): two
+//      16-byte loads per row; k-major (kPairAKMajor): one element per k,
+//      consecutive rows at consecutive addresses; else one element per k.
+//   B: k-contiguous columns, aligned, even extent (kPairVecB); offset tables
+//      with adjacent even k pairs (kPairVecBT): two 16-byte loads per
+//      column; offset tables (computed or read), n-major or k-major: one
+//      element per k.
+// With an even extent the pairs (kk, kk + 1) and (kk + 2, kk + 3) (kk even)
+// are each inside or outside the extent; an outside pair reads the last
+// pair.
+template<int FM, int FN>
+__device__ __forceinline__ void load_a_st(double (&af)[FM][4], const st_rowsrc<FM, FN> &ar,
+    const stack_pair &sp, int c, int lg) {
     const uint32_t K = sp.k;
     const uint32_t kk = 16 * c + 4 * lg;
-    const uint32_t k0 = min(kk, K - 2), k2 = min(kk + 2, K - 2);
+    if(sp.flags & kPairVecA) {
+        const uint32_t k0 = min(kk, K - 2), k2 = min(kk + 2, K - 2);
 #pragma unroll
-    for(int i = 0; i < FM; i++) {
-        gdouble *pa = (gdouble *)ar.p[i];
-        const double2_t v0 = *(gdouble2 *)(pa + k0);
-        const double2_t v1 = *(gdouble2 *)(pa + k2);
-        af[i][0] = v0.x; af[i][1] = v0.y; af[i][2] = v1.x; af[i][3] = v1.y;
-    }
-    if(TBV) {
-        // table pairs: koff[k0 + 1] = koff[k0] + 1, all offsets even
-        const int32_t o0 = sp.koff[k0], o2 = sp.koff[k2];
-#pragma unroll
-        for(int f = 0; f < FN; f++) {
-            gdouble *pb = (gdouble *)sp.b;
-            const double2_t v0 = *(gdouble2 *)(pb + uint32_t(o0 + ar.no[f]));
-            const double2_t v1 = *(gdouble2 *)(pb + uint32_t(o2 + ar.no[f]));
-            bf[f][0] = v0.x; bf[f][1] = v0.y; bf[f][2] = v1.x; bf[f][3] = v1.y;
+        for(int i = 0; i < FM; i++) {
+            gdouble *pa = (gdouble *)ar.p[i];
+            const double2_t v0 = *(gdouble2 *)(pa + k0);
+            const double2_t v1 = *(gdouble2 *)(pa + k2);
+            af[i][0] = v0.x; af[i][1] = v0.y; af[i][2] = v1.x; af[i][3] = v1.y;
         }
-    } else if(VB) {
+        return;
+    }
+    uint32_t kc[4];
 #pragma unroll
-        for(int f = 0; f < FN; f++) {
-            gdouble *pb = (gdouble *)sp.b + size_t(uint32_t(bcol[f]) * sp.ldb);
-            const double2_t v0 = *(gdouble2 *)(pb + k0);
-            const double2_t v1 = *(gdouble2 *)(pb + k2);
-            bf[f][0] = v0.x; bf[f][1] = v0.y; bf[f][2] = v1.x; bf[f][3] = v1.y;
+    for(int j = 0; j < 4; j++) kc[j] = min(kk + j, K - 1);
+    if(sp.flags & kPairAKMajor) {
+#pragma unroll
+        for(int i = 0; i < FM; i++) {
+            gdouble *pa = (gdouble *)ar.p[i];
+#pragma unroll
+            for(int j = 0; j < 4; j++) af[i][j] = pa[kc[j] * ar.lda[i]];
         }
     } else {
-        uint32_t kc[4];
 #pragma unroll
-        for(int j = 0; j < 4; j++) kc[j] = min(kk + j, K - 1);
-        int32_t ko[4];
+        for(int i = 0; i < FM; i++) {
+            gdouble *pa = (gdouble *)ar.p[i];
 #pragma unroll
-        for(int j = 0; j < 4; j++) ko[j] = TBL ? sp.koff[kc[j]] : 0;
+            for(int j = 0; j < 4; j++) af[i][j] = pa[kc[j]];
+        }
+    }
+}
+
+template<int FN, int KT>
+__device__ __forceinline__ void load_b_vt(double (&bf)[FN][4], const int32_t (&no)[FN],
+    const stack_pair &sp, uint32_t k0, uint32_t k2) {
+    // table pairs: koff[k0 + 1] = koff[k0] + 1, all offsets even
+    const int32_t o0 = koff_at<KT>(sp, k0), o2 = koff_at<KT>(sp, k2);
+    gdouble *pb = (gdouble *)sp.b;
 #pragma unroll
-        for(int f = 0; f < FN; f++) {
-            const uint32_t n = uint32_t(bcol[f]);
+    for(int f = 0; f < FN; f++) {
+        const double2_t v0 = *(gdouble2 *)(pb + uint32_t(o0 + no[f]));
+        const double2_t v1 = *(gdouble2 *)(pb + uint32_t(o2 + no[f]));
+        bf[f][0] = v0.x; bf[f][1] = v0.y; bf[f][2] = v1.x; bf[f][3] = v1.y;
+    }
+}
+
+template<int FN, int KT>
+__device__ __forceinline__ void load_b_el(double (&bf)[FN][4], const int32_t (&no)[FN],
+    const stack_pair &sp, const uint32_t (&kc)[4], const int (&bcol)[FN]) {
+    int32_t ko[4];
 #pragma unroll
-            for(int j = 0; j < 4; j++) {
-                const uint32_t e = TBL ? uint32_t(ko[j] + ar.no[f])
-                                       : ((sp.flags & kPairBNMajor) ? n * sp.ldb + kc[j] : kc[j] * sp.ldb + n);
-                bf[f][j] = ldg(sp.b, 8u * e);
-            }
+    for(int j = 0; j < 4; j++) ko[j] = koff_at<KT>(sp, kc[j]);
+#pragma unroll
+    for(int f = 0; f < FN; f++) {
+        const uint32_t n = uint32_t(bcol[f]);
+#pragma unroll
+        for(int j = 0; j < 4; j++) {
+            const uint32_t e = KT ? uint32_t(ko[j] + no[f])
+                                  : ((sp.flags & kPairBNMajor) ? n * sp.ldb + kc[j] : kc[j] * sp.ldb + n);
+            bf[f][j] = ldg(sp.b, 8u * e);
         }
     }
 }
 
 template<int FM, int FN>
+__device__ __forceinline__ void load_b_st(double (&bf)[FN][4], const st_rowsrc<FM, FN> &ar,
+    const stack_pair &sp, int c, const int (&bcol)[FN], int lg) {
+    const uint32_t K = sp.k;
+    const uint32_t kk = 16 * c + 4 * lg;
+    const bool arith = (sp.flags & kPairOffArith) != 0;
+    if(sp.flags & (kPairVecB | kPairVecBT)) {
+        const uint32_t k0 = min(kk, K - 2), k2 = min(kk + 2, K - 2);
+        if(sp.flags & kPairVecB) {
+#pragma unroll
+            for(int f = 0; f < FN; f++) {
+                gdouble *pb = (gdouble *)sp.b + size_t(uint32_t(bcol[f]) * sp.ldb);
+                const double2_t v0 = *(gdouble2 *)(pb + k0);
+                const double2_t v1 = *(gdouble2 *)(pb + k2);
+                bf[f][0] = v0.x; bf[f][1] = v0.y; bf[f][2] = v1.x; bf[f][3] = v1.y;
+            }
+        } else if(arith) load_b_vt<FN, 2>(bf, ar.no, sp, k0, k2);
+        else load_b_vt<FN, 1>(bf, ar.no, sp, k0, k2);
+        return;
+    }
+    uint32_t kc[4];
+#pragma unroll
+    for(int j = 0; j < 4; j++) kc[j] = min(kk + j, K - 1);
+    if(arith) load_b_el<FN, 2>(bf, ar.no, sp, kc, bcol);
+    else if(sp.koff) load_b_el<FN, 1>(bf, ar.no, sp, kc, bcol);
+    else load_b_el<FN, 0>(bf, ar.no, sp, kc, bcol);
+}
+
+template<int FM, int FN>
 __device__ __forceinline__ void load_chunk_st_any(double (&af)[FM][4], double (&bf)[FN][4],
     const st_rowsrc<FM, FN> &ar, const stack_pair &sp, int c, const int (&bcol)[FN], int lg) {
-    const bool ta = (sp.flags & kPairAKMajor) != 0;
-    if(sp.flags & kPairVecA) {
-        if(sp.flags & kPairVecB) load_chunk_st_vec<FM, FN, true, false>(af, bf, ar, sp, c, bcol, lg);
-        else if(sp.flags & kPairVecBT) load_chunk_st_vec<FM, FN, false, true, true>(af, bf, ar, sp, c, bcol, lg);
-        else if(sp.koff) load_chunk_st_vec<FM, FN, false, true>(af, bf, ar, sp, c, bcol, lg);
-        else load_chunk_st_vec<FM, FN, false, false>(af, bf, ar, sp, c, bcol, lg);
-        return;
-    }
-    if(sp.koff) {
-        if(ta) load_chunk_st<FM, FN, true, false, true>(af, bf, ar, sp, c, bcol, lg);
-        else load_chunk_st<FM, FN, false, false, true>(af, bf, ar, sp, c, bcol, lg);
-        return;
-    }
-    switch(sp.flags & (kPairAKMajor | kPairBNMajor)) {
-    case 0: load_chunk_st<FM, FN, false, false, false>(af, bf, ar, sp, c, bcol, lg); break;
-    case kPairAKMajor: load_chunk_st<FM, FN, true, false, false>(af, bf, ar, sp, c, bcol, lg); break;
-    case kPairBNMajor: load_chunk_st<FM, FN, false, true, false>(af, bf, ar, sp, c, bcol, lg); break;
-    default: load_chunk_st<FM, FN, true, true, false>(af, bf, ar, sp, c, bcol, lg); break;
-    }
+    load_a_st<FM, FN>(af, ar, sp, c, lg);
+    load_b_st<FM, FN>(bf, ar, sp, c, bcol, lg);
 }
 
 template<int FM, int FN>
@@ -721,20 +743,25 @@ __device__ __forceinline__ void st_rows(st_rowsrc<FM, FN> &r, const stack_arow *
         r.d[i] = e.d;
         r.lda[i] = e.lda;
     }
+    if(sp.flags & kPairOffArith) {
 #pragma unroll
-    for(int f = 0; f < FN; f++) r.no[f] = sp.noff ? sp.noff[bcol[f]] : 0;
+        for(int f = 0; f < FN; f++) r.no[f] = off_arith(uint32_t(bcol[f]), sp.nd, sp.ns0, sp.ns1);
+    } else {
+#pragma unroll
+        for(int f = 0; f < FN; f++) r.no[f] = sp.noff ? sp.noff[bcol[f]] : 0;
+    }
 }
 
 template<int FM, int FN>
 __device__ __forceinline__ void mma_chunk_st(double4_t (&acc)[FM][FN],
-    double (&af)[FM][4], const double (&bf)[FN][4], const st_rowsrc<FM, FN> &r,
+    double (&af)[FM][4], const double (&bf)[FN][4], const double (&d)[FM],
     uint32_t K, int c, int lg) {
     const uint32_t kk = 16 * c + 4 * lg;
 #pragma unroll
     for(int q = 0; q < 4; q++) {
         const bool kv = kk + q < K;
 #pragma unroll
-        for(int i = 0; i < FM; i++) af[i][q] *= kv ? r.d[i] : 0.0;
+        for(int i = 0; i < FM; i++) af[i][q] *= kv ? d[i] : 0.0;
     }
 #pragma unroll
     for(int q = 0; q < 4; q++)
@@ -756,20 +783,203 @@ __device__ __forceinline__ void seek_chunk_st(const stack_pair *__restrict__ sp,
     }
 }
 
-template<int FM, int FN, int W, bool DB>
+//! Operand forms of a stacked pair with a branch-free load path: A in
+//! 16-byte row loads and B k-contiguous (VA_VB), row-major (VA_ROW), through
+//! computed offsets element by element (VA_AR) or in 16-byte pairs
+//! (VA_VTAR); everything else takes the general path (ANY).
+enum { ST_FORM_ANY = 0, ST_FORM_VA_VB = 1, ST_FORM_VA_ROW = 2, ST_FORM_VA_AR = 3,
+       ST_FORM_VA_VTAR = 4 };
+
+__device__ __forceinline__ int st_form(const stack_pair &sp) {
+    const uint32_t f = sp.flags;
+    if(!(f & kPairVecA)) return ST_FORM_ANY;
+    if(f & kPairVecB) return ST_FORM_VA_VB;
+    if(f & kPairVecBT) return (f & kPairOffArith) ? ST_FORM_VA_VTAR : ST_FORM_ANY;
+    if(f & kPairOffArith) return ST_FORM_VA_AR;
+    if(!sp.koff && !(f & kPairBNMajor)) return ST_FORM_VA_ROW;
+    return ST_FORM_ANY;
+}
+
+//! Row sources of one pair without the B column offsets (computed or read
+//! when the operands are loaded, so the row table loads do not wait for the
+//! pair's descriptor).
+template<int FM, int FN>
+__device__ __forceinline__ void st_rows_a(st_rowsrc<FM, FN> &r, const stack_arow *__restrict__ arow,
+    uint32_t base, const int (&srow)[FM]) {
+#pragma unroll
+    for(int i = 0; i < FM; i++) {
+        const stack_arow e = arow[base + srow[i]];
+        r.p[i] = e.p;
+        r.d[i] = e.d;
+        r.lda[i] = e.lda;
+    }
+}
+
+template<int FM, int FN>
+__device__ __forceinline__ void st_cols(st_rowsrc<FM, FN> &r, const stack_pair &sp,
+    const int (&bcol)[FN]) {
+    if(sp.flags & kPairOffArith) {
+#pragma unroll
+        for(int f = 0; f < FN; f++) r.no[f] = off_arith(uint32_t(bcol[f]), sp.nd, sp.ns0, sp.ns1);
+    } else {
+#pragma unroll
+        for(int f = 0; f < FN; f++) r.no[f] = sp.noff ? sp.noff[bcol[f]] : 0;
+    }
+}
+
+//! Operands of one chunk in the pair's form (FORM != ANY: no branches).
+template<int FM, int FN, int FORM>
+__device__ __forceinline__ void load_form(double (&af)[FM][4], double (&bf)[FN][4],
+    st_rowsrc<FM, FN> &ar, const stack_pair &sp, int c, const int (&bcol)[FN], int lg) {
+    if(FORM == ST_FORM_ANY) {
+        st_cols<FM, FN>(ar, sp, bcol);
+        load_chunk_st_any<FM, FN>(af, bf, ar, sp, c, bcol, lg);
+        return;
+    }
+    const uint32_t K = sp.k;
+    const uint32_t kk = 16 * c + 4 * lg;
+    const uint32_t k0 = min(kk, K - 2), k2 = min(kk + 2, K - 2);
+#pragma unroll
+    for(int i = 0; i < FM; i++) {
+        gdouble *pa = (gdouble *)ar.p[i];
+        const double2_t v0 = *(gdouble2 *)(pa + k0);
+        const double2_t v1 = *(gdouble2 *)(pa + k2);
+        af[i][0] = v0.x; af[i][1] = v0.y; af[i][2] = v1.x; af[i][3] = v1.y;
+    }
+    if(FORM == ST_FORM_VA_VB) {
+#pragma unroll
+        for(int f = 0; f < FN; f++) {
+            gdouble *pb = (gdouble *)sp.b + size_t(uint32_t(bcol[f]) * sp.ldb);
+            const double2_t v0 = *(gdouble2 *)(pb + k0);
+            const double2_t v1 = *(gdouble2 *)(pb + k2);
+            bf[f][0] = v0.x; bf[f][1] = v0.y; bf[f][2] = v1.x; bf[f][3] = v1.y;
+        }
+        return;
+    }
+    int32_t no[FN];
+#pragma unroll
+    for(int f = 0; f < FN; f++)
+        no[f] = FORM == ST_FORM_VA_ROW ? bcol[f] : off_arith(uint32_t(bcol[f]), sp.nd, sp.ns0, sp.ns1);
+    if(FORM == ST_FORM_VA_VTAR) {
+        load_b_vt<FN, 2>(bf, no, sp, k0, k2);
+        return;
+    }
+    uint32_t kc[4];
+#pragma unroll
+    for(int j = 0; j < 4; j++) kc[j] = min(kk + j, K - 1);
+    int32_t ko[4];
+#pragma unroll
+    for(int j = 0; j < 4; j++)
+        ko[j] = FORM == ST_FORM_VA_ROW ? int32_t(kc[j] * sp.ldb) : off_arith(kc[j], sp.kd, sp.ks0, sp.ks1);
+#pragma unroll
+    for(int f = 0; f < FN; f++)
+#pragma unroll
+        for(int j = 0; j < 4; j++) bf[f][j] = ldg(sp.b, 8u * uint32_t(ko[j] + no[f]));
+}
+
+//! Cursor of a wave over its chunks of a stack's pair list (pair p, chunk
+//! c of that pair) with the pair's descriptor and the lane's row sources.
+//! After the last chunk it stays on it with valid = false, so a load issued
+//! from it stays in bounds.
+template<int FM, int FN>
+struct st_cursor {
+    uint32_t p;
+    int c;
+    bool valid;
+    stack_pair sp;
+    st_rowsrc<FM, FN> r;
+};
+
+//! Moves the cursor W chunks on.  A new pair's descriptor and row sources
+//! are loaded here, one chunk before the chunk's operands.
+template<int FM, int FN, int W>
+__device__ __forceinline__ void st_advance(st_cursor<FM, FN> &u,
+    const stack_pair *__restrict__ spairs, const stack_arow *__restrict__ arow,
+    const stack_desc &st, const int (&srow)[FM], const int (&bcol)[FN]) {
+    if(!u.valid) return;
+    int c = u.c + W;
+    const int nch = (int(u.sp.k) + 15) >> 4;
+    if(c < nch) { u.c = c; return; }
+    uint32_t p = u.p + 1;
+    const uint32_t pend = st.pair_begin + st.npairs;
+    c -= nch;
+    seek_chunk_st(spairs, pend, p, c);
+    if(p >= pend) { u.valid = false; return; }
+    u.sp = spairs[p];
+    st_rows<FM, FN>(u.r, arow, st.arow_begin + (p - st.pair_begin) * st.rows, srow, u.sp, bcol);
+    u.p = p;
+    u.c = c;
+}
+
+//! As st_advance, loading only the new pair's row sources (st_rows_a).
+template<int FM, int FN, int W>
+__device__ __forceinline__ void st_advance_a(st_cursor<FM, FN> &u,
+    const stack_pair *__restrict__ spairs, const stack_arow *__restrict__ arow,
+    const stack_desc &st, const int (&srow)[FM]) {
+    if(!u.valid) return;
+    int c = u.c + W;
+    const int nch = (int(u.sp.k) + 15) >> 4;
+    if(c < nch) { u.c = c; return; }
+    uint32_t p = u.p + 1;
+    const uint32_t pend = st.pair_begin + st.npairs;
+    c -= nch;
+    seek_chunk_st(spairs, pend, p, c);
+    if(p >= pend) { u.valid = false; return; }
+    u.sp = spairs[p];
+    st_rows_a<FM, FN>(u.r, arow, st.arow_begin + (p - st.pair_begin) * st.rows, srow);
+    u.p = p;
+    u.c = c;
+}
+
+//! Loads chunk u (in its pair's form) and multiplies the chunk held in
+//! (ac, bc): one basic block per form, so the scheduler can interleave the
+//! loads with the products.
+template<int FM, int FN>
+__device__ __forceinline__ void st_step(double4_t (&acc)[FM][FN], double (&an)[FM][4],
+    double (&bn)[FN][4], st_cursor<FM, FN> &u, double (&ac)[FM][4], const double (&bc)[FN][4],
+    const double (&dc)[FM], uint32_t kc, int cc, const int (&bcol)[FN], int lg) {
+    switch(st_form(u.sp)) {
+    case ST_FORM_VA_VB:
+        load_form<FM, FN, ST_FORM_VA_VB>(an, bn, u.r, u.sp, u.c, bcol, lg);
+        mma_chunk_st<FM, FN>(acc, ac, bc, dc, kc, cc, lg);
+        break;
+    case ST_FORM_VA_ROW:
+        load_form<FM, FN, ST_FORM_VA_ROW>(an, bn, u.r, u.sp, u.c, bcol, lg);
+        mma_chunk_st<FM, FN>(acc, ac, bc, dc, kc, cc, lg);
+        break;
+    case ST_FORM_VA_AR:
+        load_form<FM, FN, ST_FORM_VA_AR>(an, bn, u.r, u.sp, u.c, bcol, lg);
+        mma_chunk_st<FM, FN>(acc, ac, bc, dc, kc, cc, lg);
+        break;
+    case ST_FORM_VA_VTAR:
+        load_form<FM, FN, ST_FORM_VA_VTAR>(an, bn, u.r, u.sp, u.c, bcol, lg);
+        mma_chunk_st<FM, FN>(acc, ac, bc, dc, kc, cc, lg);
+        break;
+    default:
+        load_form<FM, FN, ST_FORM_ANY>(an, bn, u.r, u.sp, u.c, bcol, lg);
+        mma_chunk_st<FM, FN>(acc, ac, bc, dc, kc, cc, lg);
+        break;
+    }
+}
+
+template<int FM, int FN, int W, int PIPE>
 __device__ __forceinline__ void stacked_body(const stack_desc *__restrict__ stacks,
     const stack_pair *__restrict__ spairs, const stack_arow *__restrict__ arow,
     const stack_crow *__restrict__ crow, const gemm_tile *__restrict__ tiles,
     uint32_t ntiles, double *red);
 
-template<int FM, int FN, int W, bool DB>
+//! PIPE 0: one chunk in registers at a time; 1: the next chunk's operands
+//! load during the current chunk's products; 2: also the descriptor and row
+//! sources of the chunk after that; 3: as 2, with one branch-free block of
+//! loads and products per operand form.
+template<int FM, int FN, int W, int PIPE>
 __global__ void __launch_bounds__(64 * W)
 gemm_stacked_kernel(const stack_desc *__restrict__ stacks,
     const stack_pair *__restrict__ spairs, const stack_arow *__restrict__ arow,
     const stack_crow *__restrict__ crow, const gemm_tile *__restrict__ tiles,
     uint32_t ntiles) {
     __shared__ double red[(W > 1 ? W - 1 : 1) * FM * FN * 4 * 64];
-    stacked_body<FM, FN, W, DB>(stacks, spairs, arow, crow, tiles, ntiles, red);
+    stacked_body<FM, FN, W, PIPE>(stacks, spairs, arow, crow, tiles, ntiles, red);
 }
 
 //! The same kernel compiled for OCC waves per SIMD (register budget
@@ -781,11 +991,11 @@ gemm_stacked_occ_kernel(const stack_desc *__restrict__ stacks,
     const stack_crow *__restrict__ crow, const gemm_tile *__restrict__ tiles,
     uint32_t ntiles) {
     __shared__ double red[(W > 1 ? W - 1 : 1) * FM * FN * 4 * 64];
-    stacked_body<FM, FN, W, true>(stacks, spairs, arow, crow, tiles, ntiles, red);
+    stacked_body<FM, FN, W, 2>(stacks, spairs, arow, crow, tiles, ntiles, red);
 }
 
 //! One (16 FM) x (16 FN) tile of a stack at rows m0, columns n0.
-template<int FM, int FN, int W, bool DB>
+template<int FM, int FN, int W, int PIPE>
 __device__ __forceinline__ void stacked_tile(const stack_desc &st,
     const stack_pair *__restrict__ spairs, const stack_arow *__restrict__ arow,
     const stack_crow *__restrict__ crow, int m0, int n0, double *red) {
@@ -811,7 +1021,7 @@ __device__ __forceinline__ void stacked_tile(const stack_desc &st,
     uint32_t px = pbeg;
     int cx = wave;
     seek_chunk_st(spairs, pend, px, cx);
-    if(!DB) {
+    if(PIPE == 0) {
         // single buffer: fewer registers, latency hidden by other waves
         double ax[FM][4], bx[FN][4];
         st_rowsrc<FM, FN> rx;
@@ -824,11 +1034,11 @@ __device__ __forceinline__ void stacked_tile(const stack_desc &st,
                 pr = px;
             }
             load_chunk_st_any<FM, FN>(ax, bx, rx, sx, cx, bcol, lg);
-            mma_chunk_st<FM, FN>(acc, ax, bx, rx, sx.k, cx, lg);
+            mma_chunk_st<FM, FN>(acc, ax, bx, rx.d, sx.k, cx, lg);
             cx += W;
             seek_chunk_st(spairs, pend, px, cx);
         }
-    } else if(px < pend) {
+    } else if(PIPE == 1 && px < pend) {
         double ax[FM][4], bx[FN][4], ay[FM][4], by[FN][4];
         st_rowsrc<FM, FN> rx, ry;
         stack_pair sx = spairs[px];
@@ -844,7 +1054,7 @@ __device__ __forceinline__ void stacked_tile(const stack_desc &st,
             if(py == px) ry = rx;
             else st_rows<FM, FN>(ry, arow, st.arow_begin + (py - pbeg) * st.rows, srow, sy, bcol);
             load_chunk_st_any<FM, FN>(ay, by, ry, sy, cy, bcol, lg);
-            mma_chunk_st<FM, FN>(acc, ax, bx, rx, sx.k, cx, lg);
+            mma_chunk_st<FM, FN>(acc, ax, bx, rx.d, sx.k, cx, lg);
             if(!hy) break;
             uint32_t pz = py;
             int cz = cy + W;
@@ -855,14 +1065,92 @@ __device__ __forceinline__ void stacked_tile(const stack_desc &st,
             if(pz == py) rx = ry;
             else st_rows<FM, FN>(rx, arow, st.arow_begin + (pz - pbeg) * st.rows, srow, sx, bcol);
             load_chunk_st_any<FM, FN>(ax, bx, rx, sx, cz, bcol, lg);
-            mma_chunk_st<FM, FN>(acc, ay, by, ry, sy.k, cy, lg);
+            mma_chunk_st<FM, FN>(acc, ay, by, ry.d, sy.k, cy, lg);
             if(!hz) break;
             px = pz;
             cx = cz;
         }
+    } else if(PIPE == 2 && px < pend) {
+        // three stages: operands of chunk i + 1 and the descriptor and row
+        // sources of chunk i + 2 are in flight during the products of chunk i
+        st_cursor<FM, FN> u;
+        u.p = px;
+        u.c = cx;
+        u.valid = true;
+        u.sp = spairs[px];
+        st_rows<FM, FN>(u.r, arow, st.arow_begin + (px - pbeg) * st.rows, srow, u.sp, bcol);
+        double ax[FM][4], bx[FN][4], ay[FM][4], by[FN][4];
+        double dx[FM], dy[FM];
+        uint32_t kx, ky;
+        int ccx, ccy;
+        load_chunk_st_any<FM, FN>(ax, bx, u.r, u.sp, u.c, bcol, lg);
+#pragma unroll
+        for(int i = 0; i < FM; i++) dx[i] = u.r.d[i];
+        kx = u.sp.k;
+        ccx = u.c;
+        st_advance<FM, FN, W>(u, spairs, arow, st, srow, bcol);
+        while(true) {
+            const bool vy = u.valid;            // else a dummy in-bounds load
+            load_chunk_st_any<FM, FN>(ay, by, u.r, u.sp, u.c, bcol, lg);
+#pragma unroll
+            for(int i = 0; i < FM; i++) dy[i] = u.r.d[i];
+            ky = u.sp.k;
+            ccy = u.c;
+            st_advance<FM, FN, W>(u, spairs, arow, st, srow, bcol);
+            mma_chunk_st<FM, FN>(acc, ax, bx, dx, kx, ccx, lg);
+            if(!vy) break;
+            const bool vx = u.valid;
+            load_chunk_st_any<FM, FN>(ax, bx, u.r, u.sp, u.c, bcol, lg);
+#pragma unroll
+            for(int i = 0; i < FM; i++) dx[i] = u.r.d[i];
+            kx = u.sp.k;
+            ccx = u.c;
+            st_advance<FM, FN, W>(u, spairs, arow, st, srow, bcol);
+            mma_chunk_st<FM, FN>(acc, ay, by, dy, ky, ccy, lg);
+            if(!vx) break;
+        }
+    } else if(PIPE == 3 && px < pend) {
+        // u1: the chunk loaded during the current products; u2: the chunk
+        // after it, whose descriptor and row sources load one step earlier
+        st_cursor<FM, FN> u1, u2;
+        u1.p = px;
+        u1.c = cx;
+        u1.valid = true;
+        u1.sp = spairs[px];
+        st_rows_a<FM, FN>(u1.r, arow, st.arow_begin + (px - pbeg) * st.rows, srow);
+        double ax[FM][4], bx[FN][4], ay[FM][4], by[FN][4];
+        double dx[FM], dy[FM];
+        load_form<FM, FN, ST_FORM_ANY>(ax, bx, u1.r, u1.sp, u1.c, bcol, lg);
+#pragma unroll
+        for(int i = 0; i < FM; i++) dx[i] = u1.r.d[i];
+        uint32_t kx = u1.sp.k, ky;
+        int ccx = u1.c, ccy;
+        u2 = u1;
+        st_advance_a<FM, FN, W>(u2, spairs, arow, st, srow);
+        while(true) {
+            u1 = u2;
+            st_advance_a<FM, FN, W>(u2, spairs, arow, st, srow);
+            const bool vy = u1.valid;           // else a dummy in-bounds load
+            st_step<FM, FN>(acc, ay, by, u1, ax, bx, dx, kx, ccx, bcol, lg);
+#pragma unroll
+            for(int i = 0; i < FM; i++) dy[i] = u1.r.d[i];
+            ky = u1.sp.k;
+            ccy = u1.c;
+            if(!vy) break;
+            u1 = u2;
+            st_advance_a<FM, FN, W>(u2, spairs, arow, st, srow);
+            const bool vx = u1.valid;
+            st_step<FM, FN>(acc, ax, bx, u1, ay, by, dy, ky, ccy, bcol, lg);
+#pragma unroll
+            for(int i = 0; i < FM; i++) dx[i] = u1.r.d[i];
+            kx = u1.sp.k;
+            ccx = u1.c;
+            if(!vx) break;
+        }
     }
 
     if(W > 1) {
+
         if(wave > 0) {
             double *r = red + (wave - 1) * (FM * FN * 4 * 64);
 #pragma unroll
@@ -912,7 +1200,7 @@ __device__ __forceinline__ void stacked_tile(const stack_desc &st,
     if(W > 1) __syncthreads();   // red is reused by the next tile
 }
 
-template<int FM, int FN, int W, bool DB>
+template<int FM, int FN, int W, int PIPE>
 __device__ __forceinline__ void stacked_body(const stack_desc *__restrict__ stacks,
     const stack_pair *__restrict__ spairs, const stack_arow *__restrict__ arow,
     const stack_crow *__restrict__ crow, const gemm_tile *__restrict__ tiles,
@@ -920,7 +1208,7 @@ __device__ __forceinline__ void stacked_body(const stack_desc *__restrict__ stac
     for(uint32_t t = blockIdx.x; t < ntiles; t += gridDim.x) {
         const gemm_tile tile = tiles[t];
         const stack_desc st = stacks[tile.cblock];
-        stacked_tile<FM, FN, W, DB>(st, spairs, arow, crow, int(tile.tm) * 16 * FM,
+        stacked_tile<FM, FN, W, PIPE>(st, spairs, arow, crow, int(tile.tm) * 16 * FM,
             int(tile.tn) * 16 * FN, red);
     }
 }
@@ -940,21 +1228,21 @@ gemm_stacked_var_kernel(const stack_desc *__restrict__ stacks,
         const stack_desc st = stacks[vt.stack];
         const int m0 = 16 * int(vt.r16), n0 = 16 * int(vt.c16);
         switch(vt.fm * 4 + vt.fn) {
-        case 1 * 4 + 1: stacked_tile<1, 1, W, true>(st, spairs, arow, crow, m0, n0, red); break;
-        case 2 * 4 + 1: stacked_tile<2, 1, W, true>(st, spairs, arow, crow, m0, n0, red); break;
-        case 3 * 4 + 1: stacked_tile<3, 1, W, true>(st, spairs, arow, crow, m0, n0, red); break;
-        case 4 * 4 + 1: stacked_tile<4, 1, W, true>(st, spairs, arow, crow, m0, n0, red); break;
-        case 1 * 4 + 2: stacked_tile<1, 2, W, true>(st, spairs, arow, crow, m0, n0, red); break;
-        case 2 * 4 + 2: stacked_tile<2, 2, W, true>(st, spairs, arow, crow, m0, n0, red); break;
-        case 3 * 4 + 2: stacked_tile<3, 2, W, true>(st, spairs, arow, crow, m0, n0, red); break;
-        case 4 * 4 + 2: stacked_tile<4, 2, W, true>(st, spairs, arow, crow, m0, n0, red); break;
+        case 1 * 4 + 1: stacked_tile<1, 1, W, 2>(st, spairs, arow, crow, m0, n0, red); break;
+        case 2 * 4 + 1: stacked_tile<2, 1, W, 2>(st, spairs, arow, crow, m0, n0, red); break;
+        case 3 * 4 + 1: stacked_tile<3, 1, W, 2>(st, spairs, arow, crow, m0, n0, red); break;
+        case 4 * 4 + 1: stacked_tile<4, 1, W, 2>(st, spairs, arow, crow, m0, n0, red); break;
+        case 1 * 4 + 2: stacked_tile<1, 2, W, 2>(st, spairs, arow, crow, m0, n0, red); break;
+        case 2 * 4 + 2: stacked_tile<2, 2, W, 2>(st, spairs, arow, crow, m0, n0, red); break;
+        case 3 * 4 + 2: stacked_tile<3, 2, W, 2>(st, spairs, arow, crow, m0, n0, red); break;
+        case 4 * 4 + 2: stacked_tile<4, 2, W, 2>(st, spairs, arow, crow, m0, n0, red); break;
         default:
             if(FNX >= 3) {
                 switch(vt.fm) {
-                case 1: stacked_tile<1, (FNX >= 3 ? 3 : 1), W, true>(st, spairs, arow, crow, m0, n0, red); break;
-                case 2: stacked_tile<2, (FNX >= 3 ? 3 : 1), W, true>(st, spairs, arow, crow, m0, n0, red); break;
-                case 3: stacked_tile<3, (FNX >= 3 ? 3 : 1), W, true>(st, spairs, arow, crow, m0, n0, red); break;
-                default: stacked_tile<4, (FNX >= 3 ? 3 : 1), W, true>(st, spairs, arow, crow, m0, n0, red); break;
+                case 1: stacked_tile<1, (FNX >= 3 ? 3 : 1), W, 2>(st, spairs, arow, crow, m0, n0, red); break;
+                case 2: stacked_tile<2, (FNX >= 3 ? 3 : 1), W, 2>(st, spairs, arow, crow, m0, n0, red); break;
+                case 3: stacked_tile<3, (FNX >= 3 ? 3 : 1), W, 2>(st, spairs, arow, crow, m0, n0, red); break;
+                default: stacked_tile<4, (FNX >= 3 ? 3 : 1), W, 2>(st, spairs, arow, crow, m0, n0, red); break;
                 }
             }
             break;
@@ -986,11 +1274,11 @@ static hipError_t launch_stacked_occ_t(const stack_desc *st, const stack_pair *s
     return hipGetLastError();
 }
 
-template<int FM, int FN, int W, bool DB = true>
+template<int FM, int FN, int W, int PIPE = 2>
 static hipError_t launch_stacked_t(const stack_desc *st, const stack_pair *sp,
     const stack_arow *ar, const stack_crow *cr, const gemm_tile *tiles, uint32_t ntiles,
     hipStream_t s) {
-    hipLaunchKernelGGL((gemm_stacked_kernel<FM, FN, W, DB>), dim3(ntiles), dim3(64 * W), 0, s,
+    hipLaunchKernelGGL((gemm_stacked_kernel<FM, FN, W, PIPE>), dim3(ntiles), dim3(64 * W), 0, s,
         st, sp, ar, cr, tiles, ntiles);
     return hipGetLastError();
 }
@@ -1003,11 +1291,11 @@ hipError_t launch_gemm_stacked(int cfg, const stack_desc *st, const stack_pair *
     case STACK_16x64_W4: return launch_stacked_t<1, 4, 4>(st, sp, ar, cr, tiles, ntiles, s);
     case STACK_32x32_W4: return launch_stacked_t<2, 2, 4>(st, sp, ar, cr, tiles, ntiles, s);
     case STACK_32x64_W4: return launch_stacked_t<2, 4, 4>(st, sp, ar, cr, tiles, ntiles, s);
-    case STACK_64x32_W4: return launch_stacked_t<4, 2, 4>(st, sp, ar, cr, tiles, ntiles, s);
+    case STACK_64x32_W4: return launch_stacked_t<4, 2, 4, 3>(st, sp, ar, cr, tiles, ntiles, s);
     case STACK_48x64_W4: return launch_stacked_t<3, 4, 4>(st, sp, ar, cr, tiles, ntiles, s);
-    case STACK_64x48_W4: return launch_stacked_t<4, 3, 4>(st, sp, ar, cr, tiles, ntiles, s);
+    case STACK_64x48_W4: return launch_stacked_t<4, 3, 4, 3>(st, sp, ar, cr, tiles, ntiles, s);
     case STACK_48x32_W4: return launch_stacked_t<3, 2, 4>(st, sp, ar, cr, tiles, ntiles, s);
-    case STACK_64x48_W4_SB: return launch_stacked_t<4, 3, 4, false>(st, sp, ar, cr, tiles, ntiles, s);
+    case STACK_64x32_W4_P1: return launch_stacked_t<4, 2, 4, 1>(st, sp, ar, cr, tiles, ntiles, s);
     case STACK_64x48_W2: return launch_stacked_t<4, 3, 2>(st, sp, ar, cr, tiles, ntiles, s);
     case STACK_48x64_W2: return launch_stacked_t<3, 4, 2>(st, sp, ar, cr, tiles, ntiles, s);
     case STACK_48x32_W4_O2: return launch_stacked_occ_t<3, 2, 4, 2>(st, sp, ar, cr, tiles, ntiles, s);

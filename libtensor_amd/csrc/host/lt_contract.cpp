@@ -438,9 +438,15 @@ void btod_contract2::build_plan(btensor &c, bool add, double dscale) {
     // stacked kernels: element (k, n) of the aligned block is
     // Y[koff[k] + noff[n]] of its stored block Y.  The offset of an element
     // is linear in each index, so it splits into a row part and a column
-    // part of the aligned matrix.
+    // part of the aligned matrix.  Where each part is linear in at most two
+    // groups of indices, the kernels compute the offsets instead of reading
+    // them: off(e) = (e / d) s0 + (e % d) s1.
     const bool tc = gf0.tc;
-    struct tabref { const double *y; double alpha; int64_t koff, noff; };
+    struct linform { uint32_t d; int32_t s0, s1; };
+    struct tabref {
+        const double *y; double alpha; int64_t koff, noff;
+        bool arith; linform kf, nf;
+    };
     auto make_tables = [&](btensor &t, const dims_t &bid, const src_t &s, const dims_t &dz,
         size_t nfirst, bool first_is_k, tabref &r) {
         dims_t dy = t.bis().block_dims(get_index(s.y, bid));
@@ -465,11 +471,40 @@ void btod_contract2::build_plan(btensor &c, bool add, double dscale) {
             }
             return at;
         };
+        // groups of consecutive indices with one stride (extent-1 indices
+        // dropped); at most two groups give the linear form
+        auto linear = [&](size_t i0, size_t i1, int64_t at, linform &f) {
+            std::vector<std::pair<int64_t, int64_t> > g;    // (extent, stride)
+            int64_t cnt = 1;
+            for(size_t i = i0; i < i1; i++) {
+                cnt *= (int64_t)dz[i];
+                if(dz[i] == 1) continue;
+                if(!g.empty() && g.back().second == w[i] * (int64_t)dz[i]) {
+                    g.back().first *= (int64_t)dz[i];
+                    g.back().second = w[i];
+                } else g.push_back(std::make_pair((int64_t)dz[i], w[i]));
+            }
+            int64_t d = 2, s0 = 0, s1 = 0;
+            if(g.size() == 1) { d = g[0].first; s1 = g[0].second; }
+            else if(g.size() == 2) { d = g[1].first; s0 = g[0].second; s1 = g[1].second; }
+            else if(g.size() > 2) return false;
+            if(cnt >= (int64_t(1) << 30) || d >= (int64_t(1) << 30)) return false;
+            f.d = (uint32_t)d; f.s0 = (int32_t)s0; f.s1 = (int32_t)s1;
+            for(int64_t e = 0; e < cnt; e++)
+                lt_require((e / d) * s0 + (e % d) * s1 == pl.otab[at + e], LT_ERR_INTERNAL,
+                    "offset table is not linear in its index groups");
+            return true;
+        };
         int64_t rt = table(0, nfirst), ct = table(nfirst, n);
+        linform rf, cf;
+        bool rl = linear(0, nfirst, rt, rf), cl = linear(nfirst, n, ct, cf);
         r.y = t.block_ptr(s.y);
         r.alpha = s.tr.st.c;
         r.koff = first_is_k ? rt : ct;
         r.noff = first_is_k ? ct : rt;
+        r.arith = rl && cl;
+        r.kf = first_is_k ? rf : cf;
+        r.nf = first_is_k ? cf : rf;
     };
     std::unordered_map<size_t, tabref> taba, tabb;
 
@@ -633,7 +668,12 @@ void btod_contract2::build_plan(btensor &c, bool add, double dscale) {
                 const std::unordered_map<size_t, tabref> &tm = tc ? taba : tabb;
                 std::unordered_map<size_t, tabref>::const_iterator ti = tm.find(tc ? p->aia : p->aib);
                 if(ti != tm.end()) pair_tab.push_back(ti->second);
-                else { tabref z; z.y = 0; z.alpha = 1.0; z.koff = z.noff = -1; pair_tab.push_back(z); }
+                else {
+                    tabref z;
+                    std::memset(&z, 0, sizeof(z));
+                    z.y = 0; z.alpha = 1.0; z.koff = z.noff = -1; z.arith = false;
+                    pair_tab.push_back(z);
+                }
             }
             flops += 2.0 * double(g.m) * double(g.n) * double(g.k);
             ksum += double(g.k);
@@ -704,18 +744,26 @@ void btod_contract2::build_plan(btensor &c, bool add, double dscale) {
                 if(tb.koff >= 0) {
                     sp.b = tb.y;
                     bscale = tb.alpha;    // the copy would have carried it
+                    if(tb.arith) {
+                        sp.flags |= dev::kPairOffArith;
+                        sp.kd = dev::make_fastdiv(tb.kf.d);
+                        sp.ks0 = tb.kf.s0; sp.ks1 = tb.kf.s1;
+                        sp.nd = dev::make_fastdiv(tb.nf.d);
+                        sp.ns0 = tb.nf.s0; sp.ns1 = tb.nf.s1;
+                    }
                 }
-                // two-k loads: k-contiguous rows, even extent, 16-byte aligned rows
-                if(!(g.flags & dev::kPairAKMajor) && (g.k % 2) == 0) {
-                    bool ok = true;
+                // two-k loads: k-contiguous rows (A) or columns (B), even
+                // extent, 16-byte aligned rows; A and B independently
+                if((g.k % 2) == 0) {
+                    bool ok = !(g.flags & dev::kPairAKMajor);
                     for(size_t q = 0; q < bl.size() && ok; q++) {
                         const dev::gemm_pair &gq = pl.pairs[pl.cblocks[bl[q]].pair_begin + j];
                         ok = (gq.lda % 2) == 0 && ((uintptr_t)gq.a % 16) == 0;
                     }
                     if(ok) sp.flags |= dev::kPairVecA;
-                    if(ok && tb.koff < 0 && (g.flags & dev::kPairBNMajor) && (g.ldb % 2) == 0 &&
+                    if(tb.koff < 0 && (g.flags & dev::kPairBNMajor) && (g.ldb % 2) == 0 &&
                         ((uintptr_t)g.b % 16) == 0) sp.flags |= dev::kPairVecB;
-                    if(ok && tb.koff >= 0 && ((uintptr_t)tb.y % 16) == 0) {
+                    if(tb.koff >= 0 && ((uintptr_t)tb.y % 16) == 0) {
                         bool pairs_ok = true;
                         for(uint32_t kk = 0; kk + 1 < g.k && pairs_ok; kk += 2)
                             pairs_ok = pl.otab[tb.koff + kk] % 2 == 0 &&
