@@ -15,7 +15,7 @@ c_int_p = ctypes.POINTER(ctypes.c_int)
 vp = ctypes.c_void_p
 
 # GEMM kernel configurations (lt_btod_contract2_set_tile: 0 .. GEMM_CONFIGS - 1)
-GEMM_CONFIGS = 27
+GEMM_CONFIGS = 28
 LT_MODE_PER_BLOCK = 0
 LT_MODE_GROUPED = 1
 LT_MODE_OZAKI = 2

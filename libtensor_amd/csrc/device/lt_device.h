@@ -87,7 +87,8 @@ struct gemm_tile {
 //! k chunks dealt to the waves, operands loaded straight into MFMA
 //! fragments, partial tiles summed in LDS in a fixed order.
 //! STACK_*: the split-K kernels over stacks (C blocks sharing their B
-//! operands, rows concatenated; _On: compiled for n waves per SIMD; _P1:
+//! operands, rows concatenated; _On: compiled for n waves per SIMD; LDS: A
+//! chunks staged in LDS for two column halves, two k groups; _P1:
 //! the two-stage pipeline that loads a pair's row table when it starts
 //! the pair's first chunk); used by grouped launches, per-block launches
 //! run the SPLITK configuration of the same shape.
@@ -100,9 +101,9 @@ enum { TILE_64x64 = 0, TILE_32x64 = 1, TILE_64x32 = 2, TILE_32x32 = 3,
        STACK_48x32_W4 = 18, STACK_64x32_W4_P1 = 19, STACK_64x48_W2 = 20,
        STACK_48x64_W2 = 21, STACK_48x32_W4_O2 = 22, STACK_32x32_W4_O3 = 23,
        STACK_16x64_W4_O3 = 24, STACK_VAR_64x32_W4 = 25, STACK_VAR_64x48_W4 = 26,
-       TILE_COUNT = 27 };
+       STACK_LDS_64x64 = 27, TILE_COUNT = 28 };
 void tile_shape(int cfg, uint32_t &bm, uint32_t &bn);
-inline bool is_stacked(int cfg) { return cfg >= STACK_16x64_W4 && cfg <= STACK_VAR_64x48_W4; }
+inline bool is_stacked(int cfg) { return cfg >= STACK_16x64_W4 && cfg <= STACK_LDS_64x64; }
 //! Stacked configurations over tiles of varying shape (var_tile lists).
 inline bool is_var(int cfg) { return cfg == STACK_VAR_64x32_W4 || cfg == STACK_VAR_64x48_W4; }
 //! The per-C-block configuration with the same summation order (same
@@ -116,7 +117,7 @@ inline int unstacked(int cfg) {
     case STACK_48x32_W4_O2: case STACK_32x32_W4_O3: case STACK_16x64_W4_O3:
     case STACK_VAR_64x32_W4: case STACK_VAR_64x48_W4:
         return SPLITK_64x32_W4;
-    case STACK_64x48_W2: case STACK_48x64_W2: return SPLITK_32x32_W2;
+    case STACK_64x48_W2: case STACK_48x64_W2: case STACK_LDS_64x64: return SPLITK_32x32_W2;
     default: return cfg;
     }
 }
@@ -129,7 +130,9 @@ inline int unstacked(int cfg) {
 //! crow[crow_begin + r].
 struct stack_desc {
     uint32_t rows, n, npairs, pair_begin;
-    uint32_t arow_begin, crow_begin, pad0, pad1;
+    uint32_t arow_begin, crow_begin;
+    uint32_t nchunk;                  //!< sum over pairs of ceil(k / 16)
+    uint32_t pad1;
 };
 struct stack_pair {
     const double *b;

@@ -40,6 +40,7 @@ void tile_shape(int cfg, uint32_t &bm, uint32_t &bn) {
     case SPLITK_32x32_W4: case SPLITK_32x32_W2: case STACK_32x32_W4: bm = 32; bn = 32; break;
     case SPLITK_32x64_W4: case STACK_32x64_W4: bm = 32; bn = 64; break;
     case SPLITK_64x32_W4: case STACK_64x32_W4: case STACK_64x32_W4_P1: bm = 64; bn = 32; break;
+    case STACK_LDS_64x64: bm = 64; bn = 64; break;
     case STACK_48x64_W4: bm = 48; bn = 64; break;
     case STACK_64x48_W4: bm = 64; bn = 48; break;
     case STACK_48x32_W4: bm = 48; bn = 32; break;
@@ -1213,6 +1214,208 @@ __device__ __forceinline__ void stacked_body(const stack_desc *__restrict__ stac
     }
 }
 
+// LDS-staged stacked kernel.  A workgroup of four waves computes a
+// (16 FM) x (32 FN) tile: waves g = 0, 1 (k group) x h = 0, 1 (column half).
+// The chunks of the stack's pair list are dealt to the two k groups in turn
+// (as to the two waves of the two-wave split-K kernels); the two waves of a
+// group share the group's A chunk, staged in LDS once per chunk (each lane
+// loads 16 bytes of one row, scaled by the row coefficient and zeroed past
+// the extent), and load their own B columns into registers.  Group 1's
+// partial tile is added to group 0's in LDS at the end, so every C element
+// gets the same sums in the same order as in the two-wave split-K kernels.
+static const int kLdsRowStride = 18;    // doubles per staged A row (144 bytes)
+
+//! Staging of one A chunk: lane (row srw + 16 j, segment seg) holds k =
+//! 16 c + 2 seg and 2 seg + 1 of its rows.
+template<int FM, int FN>
+__device__ __forceinline__ void lds_stage_load(double2_t (&sa)[FM], const st_rowsrc<FM, FN> &r,
+    const stack_pair &sp, int c, int seg) {
+    const uint32_t K = sp.k;
+    const uint32_t k = 16 * c + 2 * seg;
+    if(sp.flags & kPairVecA) {
+        const uint32_t k0 = min(k, K - 2);
+#pragma unroll
+        for(int j = 0; j < FM; j++) sa[j] = *(gdouble2 *)((gdouble *)r.p[j] + k0);
+        return;
+    }
+    const uint32_t k0 = min(k, K - 1), k1 = min(k + 1, K - 1);
+    if(sp.flags & kPairAKMajor) {
+#pragma unroll
+        for(int j = 0; j < FM; j++) {
+            gdouble *pa = (gdouble *)r.p[j];
+            sa[j].x = pa[k0 * r.lda[j]];
+            sa[j].y = pa[k1 * r.lda[j]];
+        }
+    } else {
+#pragma unroll
+        for(int j = 0; j < FM; j++) {
+            gdouble *pa = (gdouble *)r.p[j];
+            sa[j].x = pa[k0];
+            sa[j].y = pa[k1];
+        }
+    }
+}
+
+template<int FM, int FN>
+__device__ __forceinline__ void lds_stage_store(double *buf, const double2_t (&sa)[FM],
+    const st_rowsrc<FM, FN> &r, uint32_t K, int c, int srw, int seg) {
+    const uint32_t k = 16 * c + 2 * seg;
+    const bool v0 = k < K, v1 = k + 1 < K;
+#pragma unroll
+    for(int j = 0; j < FM; j++) {
+        double2_t w;
+        w.x = sa[j].x * (v0 ? r.d[j] : 0.0);
+        w.y = sa[j].y * (v1 ? r.d[j] : 0.0);
+        *(double2_t *)(buf + (srw + 16 * j) * kLdsRowStride + 2 * seg) = w;
+    }
+}
+
+template<int FM, int FN>
+__device__ __forceinline__ void lds_tile(const stack_desc &st,
+    const stack_pair *__restrict__ spairs, const stack_arow *__restrict__ arow,
+    const stack_crow *__restrict__ crow, int m0, int n0, double *lds) {
+    constexpr int ABUF = 16 * FM * kLdsRowStride;
+    const int lane = threadIdx.x & 63;
+    const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    const int g = wave & 1, h = wave >> 1;
+    const int lr = lane & 15, lg = lane >> 4;
+    const int lig = lane + 64 * h;
+    const int seg = lig & 7, srw = lig >> 3;
+    const int nw = n0 + 16 * FN * h;
+    const int mrem = int(st.rows) - m0, nrem = int(st.n) - nw;
+    double *abuf = lds + g * 2 * ABUF;
+
+    double4_t acc[FM][FN];
+#pragma unroll
+    for(int i = 0; i < FM; i++)
+#pragma unroll
+        for(int j = 0; j < FN; j++) acc[i][j] = double4_t{0.0, 0.0, 0.0, 0.0};
+
+    int srow[FM], bcol[FN];
+#pragma unroll
+    for(int j = 0; j < FM; j++) srow[j] = (srw + 16 * j) < mrem ? m0 + srw + 16 * j : 0;
+#pragma unroll
+    for(int f = 0; f < FN; f++) bcol[f] = (16 * f + lr) < nrem ? nw + 16 * f + lr : 0;
+
+    const int nchunk = int(st.nchunk);
+    if(nchunk > 0) {
+        const uint32_t pbeg = st.pair_begin, pend = st.pair_begin + st.npairs;
+        // the group's first chunk; a group without chunks keeps an in-bounds
+        // cursor on chunk 0 and only joins the barriers
+        st_cursor<FM, FN> u, u2;
+        u.p = pbeg;
+        u.c = g;
+        seek_chunk_st(spairs, pend, u.p, u.c);
+        u.valid = u.p < pend;
+        if(!u.valid) { u.p = pbeg; u.c = 0; }
+        u.sp = spairs[u.p];
+        st_rows_a<FM, FN>(u.r, arow, st.arow_begin + (u.p - pbeg) * st.rows, srow);
+        st_cols<FM, FN>(u.r, u.sp, bcol);
+        double2_t sa[FM];
+        double bx[FN][4], by[FN][4], af[FM][4];
+        lds_stage_load<FM, FN>(sa, u.r, u.sp, u.c, seg);
+        load_b_st<FM, FN>(bx, u.r, u.sp, u.c, bcol, lg);
+        lds_stage_store<FM, FN>(abuf, sa, u.r, u.sp.k, u.c, srw, seg);
+        u2 = u;
+        st_advance_a<FM, FN, 2>(u2, spairs, arow, st, srow);
+        __syncthreads();
+        const int niter = (nchunk + 1) >> 1;
+        for(int it = 0; it < niter; it++) {
+            const bool cur = u.valid;
+            // next chunk: its A staged through registers, its B straight in
+            st_cols<FM, FN>(u2.r, u2.sp, bcol);
+            lds_stage_load<FM, FN>(sa, u2.r, u2.sp, u2.c, seg);
+            load_b_st<FM, FN>(by, u2.r, u2.sp, u2.c, bcol, lg);
+            const double *rb = abuf + (it & 1) * ABUF;
+#pragma unroll
+            for(int i = 0; i < FM; i++) {
+                const double *rp = rb + (16 * i + lr) * kLdsRowStride + 4 * lg;
+                const double2_t v0 = *(const double2_t *)rp;
+                const double2_t v1 = *(const double2_t *)(rp + 2);
+                af[i][0] = v0.x; af[i][1] = v0.y; af[i][2] = v1.x; af[i][3] = v1.y;
+            }
+            if(cur) {
+#pragma unroll
+                for(int q = 0; q < 4; q++)
+#pragma unroll
+                    for(int i = 0; i < FM; i++)
+#pragma unroll
+                        for(int j = 0; j < FN; j++)
+                            acc[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(
+                                af[i][q], bx[j][q], acc[i][j], 0, 0, 0);
+            }
+            lds_stage_store<FM, FN>(abuf + ((it + 1) & 1) * ABUF, sa, u2.r, u2.sp.k, u2.c, srw, seg);
+            u = u2;
+            st_advance_a<FM, FN, 2>(u2, spairs, arow, st, srow);
+#pragma unroll
+            for(int f = 0; f < FN; f++)
+#pragma unroll
+                for(int q = 0; q < 4; q++) bx[f][q] = by[f][q];
+            __syncthreads();
+        }
+    }
+
+    // group 1's partial tile onto group 0's, then the C rows
+    double *red = lds;
+    if(g == 1) {
+        double *r = red + h * (FM * FN * 4 * 64);
+#pragma unroll
+        for(int i = 0; i < FM; i++)
+#pragma unroll
+            for(int j = 0; j < FN; j++)
+#pragma unroll
+                for(int v = 0; v < 4; v++)
+                    r[((i * FN + j) * 4 + v) * 64 + lane] = acc[i][j][v];
+    }
+    __syncthreads();
+    if(g == 0) {
+        const double *r = red + h * (FM * FN * 4 * 64);
+#pragma unroll
+        for(int i = 0; i < FM; i++)
+#pragma unroll
+            for(int j = 0; j < FN; j++)
+#pragma unroll
+                for(int v = 0; v < 4; v++)
+                    acc[i][j][v] += r[((i * FN + j) * 4 + v) * 64 + lane];
+#pragma unroll
+        for(int i = 0; i < FM; i++) {
+#pragma unroll
+            for(int v = 0; v < 4; v++) {
+                const int row = 16 * i + lg + 4 * v;
+                if(row >= mrem) continue;
+                const stack_crow cr = crow[st.crow_begin + m0 + row];
+#pragma unroll
+                for(int j = 0; j < FN; j++) {
+                    const int col = 16 * j + lr;
+                    if(col < nrem) {
+                        double *cp = cr.c + (nw + col);
+                        double x = acc[i][j][v];
+                        if(!cr.zero) x += *cp;
+                        *cp = x;
+                    }
+                }
+            }
+        }
+    }
+    __syncthreads();   // LDS is reused by the next tile
+}
+
+template<int FM, int FN>
+__global__ void __launch_bounds__(256)
+gemm_stacked_lds_kernel(const stack_desc *__restrict__ stacks,
+    const stack_pair *__restrict__ spairs, const stack_arow *__restrict__ arow,
+    const stack_crow *__restrict__ crow, const gemm_tile *__restrict__ tiles,
+    uint32_t ntiles) {
+    constexpr int NA = 4 * 16 * FM * kLdsRowStride, NR = 2 * FM * FN * 4 * 64;
+    __shared__ double lds[NA > NR ? NA : NR];
+    for(uint32_t t = blockIdx.x; t < ntiles; t += gridDim.x) {
+        const gemm_tile tile = tiles[t];
+        const stack_desc st = stacks[tile.cblock];
+        lds_tile<FM, FN>(st, spairs, arow, crow, int(tile.tm) * 16 * FM,
+            int(tile.tn) * 32 * FN, lds);
+    }
+}
+
 //! Stacked kernel over tiles of varying height and width (16 fm x 16 fn,
 //! fm <= 4, fn <= FNX): every stack is covered by full tiles and one
 //! narrower tile per remainder, so rows and columns are padded to 16 only.
@@ -1292,6 +1495,10 @@ hipError_t launch_gemm_stacked(int cfg, const stack_desc *st, const stack_pair *
     case STACK_32x32_W4: return launch_stacked_t<2, 2, 4>(st, sp, ar, cr, tiles, ntiles, s);
     case STACK_32x64_W4: return launch_stacked_t<2, 4, 4>(st, sp, ar, cr, tiles, ntiles, s);
     case STACK_64x32_W4: return launch_stacked_t<4, 2, 4, 3>(st, sp, ar, cr, tiles, ntiles, s);
+    case STACK_LDS_64x64:
+        hipLaunchKernelGGL((gemm_stacked_lds_kernel<4, 2>), dim3(ntiles), dim3(256), 0, s,
+            st, sp, ar, cr, tiles, ntiles);
+        return hipGetLastError();
     case STACK_48x64_W4: return launch_stacked_t<3, 4, 4>(st, sp, ar, cr, tiles, ntiles, s);
     case STACK_64x48_W4: return launch_stacked_t<4, 3, 4, 3>(st, sp, ar, cr, tiles, ntiles, s);
     case STACK_48x32_W4: return launch_stacked_t<3, 2, 4>(st, sp, ar, cr, tiles, ntiles, s);

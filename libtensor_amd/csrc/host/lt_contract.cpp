@@ -776,6 +776,7 @@ void btod_contract2::build_plan(btensor &c, bool add, double dscale) {
                 pl.spairs.push_back(sp);
                 pl.spair_tab.push_back(std::make_pair(tb.koff, tb.noff));
                 ks += double(g.k);
+                sd.nchunk += (g.k + 15) / 16;
                 for(size_t q = 0; q < bl.size(); q++) {
                     const dev::gemm_cblock &cq = pl.cblocks[bl[q]];
                     const dev::gemm_pair &gq = pl.pairs[cq.pair_begin + j];
