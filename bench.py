@@ -76,7 +76,7 @@ def serving_mix(ctx, lt, W, n_clients, rounds, rate, seed=1005):
         clients.append((p["name"], a, b, op, c, digest, op.stats()["flops"]))
     log("serving: %d clients ready in %.1fs" % (len(clients), time.time() - t0))
     srv = lt.server(ctx, max_batch=64)
-    lat, bad = [], []
+    lat, qlat, bad = [], [], []
     lock = threading.Lock()
 
     def client(i):
@@ -87,6 +87,7 @@ def serving_mix(ctx, lt, W, n_clients, rounds, rate, seed=1005):
             st = srv.wait(srv.submit(op, c))
             with lock:
                 lat.append(st["t_total_ms"])
+                qlat.append(st["t_queue_ms"])
 
     t1 = time.perf_counter()
     th = [threading.Thread(target=client, args=(i,)) for i in range(len(clients))]
@@ -106,6 +107,10 @@ def serving_mix(ctx, lt, W, n_clients, rounds, rate, seed=1005):
            "max_batch": int(st["max_batch"]), "arrival_rate_per_s": rate,
            "wall_s": wall, "tflops": flops / wall / 1e12,
            "p50_ms": float(np.percentile(lat, 50)), "p99_ms": float(np.percentile(lat, 99)),
+           "queue_p50_ms": float(np.percentile(qlat, 50)),
+           "queue_p99_ms": float(np.percentile(qlat, 99)),
+           "request_gflop_min": min(x[6] for x in clients) / 1e9,
+           "request_gflop_max": max(x[6] for x in clients) / 1e9,
            "results_same_as_alone": not bad, "differing": bad}
     log("serving: %s" % json.dumps(out))
     return out
@@ -150,6 +155,19 @@ def ozaki_sweep(ctx, lt, W, prob, op, ref, flops, slices=(3, 4, 5, 6, 7), steps=
     return {"sweep": out, "slices_for_1e-12": min(need) if need else None}
 
 
+# name -> (problem builder, input seed, description)
+WORKLOADS = {
+    "pp-ladder": (lambda W: W.pp_ladder_o40_v200(), 1002,
+                  "pp-ladder R[ijab] = sum_cd T[ijcd] V[abcd], o=40 v=200, alpha/beta se_part, "
+                  "C2v se_label, (ij)(ab)(cd) antisymmetric, uneven blocks 4-24"),
+    "ring-o10-v40": (lambda W: W.ring(10, 40), 1001,
+                     "C[ijab] = sum_kc A[ikac] B[kjcb], o=10 v=40, one block, no symmetry"),
+    "ring-o60-v300": (lambda W: W.ring_batched_o60_v300(), 1003,
+                      "C[ijab] = sum_kc A[ikac] B[kjcb], o=60 v=300, uneven blocks U{8..96} "
+                      "(seed 1003), no symmetry"),
+}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -169,7 +187,14 @@ def main():
     ap.add_argument("--serve-rounds", type=int, default=3)
     ap.add_argument("--serve-rate", type=float, default=50.0,
                     help="aggregate Poisson arrival rate of the serving mix (1/s)")
+    ap.add_argument("--workload", default="pp-ladder", choices=sorted(WORKLOADS),
+                    help="pp-ladder (default, the headline line); ring-o10-v40 (one "
+                         "block, permuted operands) and ring-o60-v300 (uneven blocks): "
+                         "side measurements without the host baseline, the split-precision "
+                         "sweep or the serving mix")
     args = ap.parse_args()
+    if args.workload != "pp-ladder":
+        args.no_cpu = args.no_serve = args.no_ozaki = True
     if args.gpus != 1:
         print("bench.py runs on one GPU only: this build measures the single-GPU "
               "path (see DESIGN.md for how the contraction would shard)", file=sys.stderr)
@@ -188,9 +213,10 @@ def main():
         print(json.dumps({"copy_probe_bytes_read": n * 8, "copy_probe_bytes_written": n * 8,
                           "launches": 3}))
         return
-    prob = W.pp_ladder_o40_v200()
+    make_prob, seed, wl_desc = WORKLOADS[args.workload]
+    prob = make_prob(W)
     t0 = time.time()
-    a, b, op = W.build_random(ctx, prob, 1002)
+    a, b, op = W.build_random(ctx, prob, seed)
     c = W.result_tensor(ctx, prob)
     log("inputs ready in %.1fs: %d + %d canonical blocks" % (
         time.time() - t0, len(a.req_nonzero_blocks()), len(b.req_nonzero_blocks())))
@@ -273,7 +299,7 @@ def main():
     # looked up for the kernel configuration and tile order chosen here
     traffic = None
     tpath = os.path.join(HERE, "profiles", "pp_ladder_o40_v200_traffic.json")
-    if os.path.exists(tpath):
+    if args.workload == "pp-ladder" and os.path.exists(tpath):
         with open(tpath) as f:
             tr = json.load(f)
         for e in tr.get("launches", []):
@@ -284,9 +310,7 @@ def main():
         "steps": args.steps, "warmup": args.warmup, "ms_per_step": ms_per_step,
         "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
         "dtype": "f64", "data": "synthetic",
-        "config": {"workload": "pp-ladder R[ijab] = sum_cd T[ijcd] V[abcd], o=40 v=200, "
-                               "alpha/beta se_part, C2v se_label, (ij)(ab)(cd) antisymmetric, "
-                               "uneven blocks 4-24", "mode": args.mode,
+        "config": {"workload": wl_desc, "mode": args.mode,
                    "c_blocks": int(st["n_cblocks"]), "block_pairs": int(st["n_pairs"]),
                    "flops_per_step": flops},
         "roofline": {"bound": "mfma", "achieved": achieved, "peak": FP64_MFMA_PEAK,

@@ -11,13 +11,13 @@ import ctypes
 import numpy as np
 
 from . import _capi
-from ._capi import (GEMM_CONFIGS, LT_MODE_GROUPED, LT_MODE_OZAKI, LT_MODE_PER_BLOCK, LtError,
-                    LtRequestStats, LtStats, call, dptr, ints, sizes)
+from ._capi import (GEMM_CONFIGS, GEMM_STACKED_FIRST, LT_MODE_GROUPED, LT_MODE_OZAKI,
+                    LT_MODE_PER_BLOCK, LtError, LtRequestStats, LtStats, call, dptr, ints, sizes)
 
 __all__ = ["Context", "product_table", "block_index_space", "block_tensor",
            "se_perm", "se_part", "se_label", "contraction2", "btod_contract2",
            "device_array", "tod_copy", "tod_add", "tod_set", "tod_contract2", "tod_dotprod", "btod_dotprod", "server",
-           "LtError", "GEMM_CONFIGS", "LT_MODE_PER_BLOCK", "LT_MODE_GROUPED", "LT_MODE_OZAKI"]
+           "LtError", "GEMM_CONFIGS", "GEMM_STACKED_FIRST", "LT_MODE_PER_BLOCK", "LT_MODE_GROUPED", "LT_MODE_OZAKI"]
 
 
 def _abs_index(idx, dims):

@@ -16,6 +16,8 @@ vp = ctypes.c_void_p
 
 # GEMM kernel configurations (lt_btod_contract2_set_tile: 0 .. GEMM_CONFIGS - 1)
 GEMM_CONFIGS = 28
+# configurations from here on are the stacked kernels (measured by the autotuner)
+GEMM_STACKED_FIRST = 12
 LT_MODE_PER_BLOCK = 0
 LT_MODE_GROUPED = 1
 LT_MODE_OZAKI = 2

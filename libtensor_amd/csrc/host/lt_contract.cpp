@@ -949,7 +949,11 @@ void btod_contract2::build_plan(btensor &c, bool add, double dscale) {
             est.push_back(std::make_pair(waste + ntile * 1024.0 * kmean, k));
         }
         std::sort(est.begin(), est.end());
-        for(size_t i = 0; i < est.size(); i++) candidates.push_back(est[i].second);
+        // the autotuner measures the stacked configurations only: on every
+        // work list measured they beat the per-C-block kernels in grouped
+        // launches, and each candidate costs two timed launches
+        for(size_t i = 0; i < est.size(); i++)
+            if(cfg != -2 || dev::is_stacked(est[i].second)) candidates.push_back(est[i].second);
         if(cfg == -1 || add) candidates.resize(1);
         // the configuration measured for these operands (same pairs, same
         // work list for any result tensor) is reused without measuring again
