@@ -204,8 +204,18 @@ int lt_btod_contract2_clst(lt_btod_contract2 *op, size_t aidx_c, double *out,
 /* compute_block(zero = true, ic, identity, blk) on the host (FP64 loops). */
 int lt_btod_contract2_compute_block_host(lt_btod_contract2 *op, size_t aidx_c,
     double *host);
+/* compute_block(zero, ic, tensor_transf(perm, coeff), blk)
+ * (libtensor/block_tensor/btod_contract2.h:124,
+ * gen_block_tensor/impl/gen_bto_contract2_impl.h:312-345): block aidx_c of
+ * the result (any block of C's block index space, canonical or not),
+ * computed on the device from the resident canonical blocks of A and B and
+ * transformed into blk (device array, dims of the block permuted by perm;
+ * zero: blk := ..., else blk += ...).  perm may be NULL (identity).  For
+ * direct block tensors (gen_bto_contract2::compute_block). */
+int lt_btod_contract2_compute_block(lt_btod_contract2 *op, int zero, size_t aidx_c,
+    const size_t *perm, double coeff, double *blk);
 int lt_btod_contract2_stats(lt_btod_contract2 *op, lt_stats *out);
-/* Kernel configuration of the contraction GEMM: 0..26 fixed, -1 chosen by a
+/* Kernel configuration of the contraction GEMM: 0..27 fixed, -1 chosen by a
  * padding estimate, -2 (default) measured on the plan's own work list when
  * it is built (autotuner). */
 int lt_btod_contract2_set_tile(lt_btod_contract2 *op, int cfg);

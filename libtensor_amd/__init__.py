@@ -360,6 +360,14 @@ class btod_contract2(object):
             out.append((aia, acia, pa, ca, aib, acib, pb, cb))
         return out
 
+    def compute_block(self, aidx_c, blk, zero=True, perm=None, coeff=1.0):
+        """blk (=, or += unless zero) coeff * P(block aidx_c of the result), computed
+        on the device (gen_bto_contract2::compute_block); blk is a device_array
+        with the block's dims permuted by perm."""
+        p = None if perm is None else sizes(perm)
+        call("lt_btod_contract2_compute_block", self._h, 1 if zero else 0, int(aidx_c), p,
+             float(coeff), blk.ptr)
+
     def compute_block_host(self, aidx_c, dims):
         arr = np.empty(dims, dtype=np.float64)
         call("lt_btod_contract2_compute_block_host", self._h, int(aidx_c), dptr(arr))

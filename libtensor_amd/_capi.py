@@ -121,6 +121,8 @@ _SIGS = {
                                      c_double_p, c_int_p],
     "lt_btod_contract2_clst": [vp, ctypes.c_size_t, c_double_p, ctypes.c_size_t, c_size_p],
     "lt_btod_contract2_compute_block_host": [vp, ctypes.c_size_t, c_double_p],
+    "lt_btod_contract2_compute_block": [vp, ctypes.c_int, ctypes.c_size_t, c_size_p,
+                                        ctypes.c_double, vp],
     "lt_btod_contract2_stats": [vp, ctypes.POINTER(LtStats)],
     "lt_btod_contract2_set_tile": [vp, ctypes.c_int],
     "lt_btod_contract2_set_graph": [vp, ctypes.c_int],

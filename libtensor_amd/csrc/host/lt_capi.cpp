@@ -84,13 +84,10 @@ contraction2 make_contraction(size_t N, size_t M, size_t K, const size_t *permc,
     return c;
 }
 
-// Dense helper: dst = alpha * P(src) with the batched permute kernel.
-void dense_permute(context &ctx, const double *src, const dims_t &dsrc,
-    const tensor_tr &tr, double *dst, bool accumulate);
-
 } // unnamed namespace
 
-namespace {
+namespace lta {
+
 void dense_permute(context &ctx, const double *src, const dims_t &dsrc,
     const tensor_tr &tr, double *dst, bool accumulate) {
     dev::perm_desc d = make_perm_desc_public(src, dsrc, tr, dst, accumulate);
@@ -115,6 +112,104 @@ void dense_permute(context &ctx, const double *src, const dims_t &dsrc,
         throw;
     }
     ctx.free(mem);
+}
+
+void dense_contract2(context &cx, const contraction2 &contr, const dims_t &da, const double *a,
+    const dims_t &db, const double *b, double d, bool zero, double *c) {
+    dims_t dc = contract2_dims(contr, da, db);
+    if(d == 0.0) {
+        if(zero) {
+            cx.activate();
+            hip_check(dev::launch_fill(c, dc.size(), 0.0, cx.stream()), "fill");
+            cx.sync();
+        }
+        return;
+    }
+    // align as tod_contract2::perform (tod_contract2_impl.h:124-135),
+    // permute A / B into scratch if needed, one GEMM, permute C back
+    contraction2_align al(contr);
+    contraction2 ca(contr);
+    ca.permute_a(al.perma);
+    ca.permute_b(al.permb);
+    ca.permute_c(al.permc);
+    dims_t da2(da), db2(db), dc2(dc);
+    da2.permute(al.perma);
+    db2.permute(al.permb);
+    dc2.permute(al.permc);
+    std::vector<void*> tmp;
+    try {
+        const double *pa = a, *pb = b;
+        if(!al.perma.is_identity()) {
+            double *t = (double*)cx.alloc(da.size() * 8);
+            tmp.push_back(t);
+            dense_permute(cx, a, da, tensor_tr(al.perma, scalar_tr()), t, false);
+            pa = t;
+        }
+        if(!al.permb.is_identity()) {
+            double *t = (double*)cx.alloc(db.size() * 8);
+            tmp.push_back(t);
+            dense_permute(cx, b, db, tensor_tr(al.permb, scalar_tr()), t, false);
+            pb = t;
+        }
+        double *pc = c;
+        bool cperm = !al.permc.is_identity();
+        if(cperm) {
+            pc = (double*)cx.alloc(dc.size() * 8);
+            tmp.push_back(pc);
+        }
+        gemm_form g = aligned_gemm_form(ca, da2, db2);
+        dev::gemm_cblock cb;
+        dev::gemm_pair gp;
+        std::memset(&cb, 0, sizeof(cb));
+        std::memset(&gp, 0, sizeof(gp));
+        size_t lda = g.ta ? g.m : g.k, ldb = g.tb ? g.k : g.n;
+        bool ta = g.ta, tb = g.tb;
+        if(!g.tc) {
+            cb.m = (uint32_t)g.m; cb.n = (uint32_t)g.n;
+            gp.a = pa; gp.b = pb; gp.lda = (uint32_t)lda; gp.ldb = (uint32_t)ldb;
+        } else {
+            cb.m = (uint32_t)g.n; cb.n = (uint32_t)g.m;
+            gp.a = pb; gp.b = pa; gp.lda = (uint32_t)ldb; gp.ldb = (uint32_t)lda;
+            ta = !g.tb; tb = !g.ta;
+        }
+        cb.ldc = cb.n;
+        cb.c = pc;
+        cb.zero = (cperm || zero) ? 1 : 0;
+        cb.pair_begin = 0;
+        cb.pair_end = 1;
+        gp.k = (uint32_t)g.k;
+        gp.d = d;
+        gp.flags = (ta ? dev::kPairAKMajor : 0) | (tb ? dev::kPairBNMajor : 0);
+        uint32_t bm = 64, bn = 64;
+        dev::tile_shape(dev::TILE_64x64, bm, bn);
+        std::vector<dev::gemm_tile> tiles;
+        for(uint32_t i = 0; i < (cb.m + bm - 1) / bm; i++)
+            for(uint32_t j = 0; j < (cb.n + bn - 1) / bn; j++) {
+                dev::gemm_tile t; t.cblock = 0; t.tm = (uint16_t)i; t.tn = (uint16_t)j;
+                tiles.push_back(t);
+            }
+        size_t bytes = 512 + 512 + tiles.size() * sizeof(dev::gemm_tile);
+        char *mem = (char*)cx.alloc(bytes);
+        tmp.push_back(mem);
+        std::vector<char> host(bytes, 0);
+        std::memcpy(&host[0], &cb, sizeof(cb));
+        std::memcpy(&host[512], &gp, sizeof(gp));
+        std::memcpy(&host[1024], tiles.data(), tiles.size() * sizeof(dev::gemm_tile));
+        cx.upload(mem, host.data(), bytes);
+        cx.activate();
+        hip_check(dev::launch_gemm(dev::TILE_64x64, (dev::gemm_cblock*)mem,
+            (dev::gemm_pair*)(mem + 512), (dev::gemm_tile*)(mem + 1024),
+            (uint32_t)tiles.size(), (uint32_t)tiles.size(), cx.stream()), "gemm");
+        cx.sync();
+        if(cperm) {
+            dense_permute(cx, pc, dc2, tensor_tr(al.permc.inverse(), scalar_tr()),
+                c, zero == 0);
+        }
+    } catch(...) {
+        for(size_t i = 0; i < tmp.size(); i++) cx.free(tmp[i]);
+        throw;
+    }
+    for(size_t i = 0; i < tmp.size(); i++) cx.free(tmp[i]);
 }
 }
 
@@ -255,101 +350,8 @@ int lt_tod_contract2(lt_ctx *ctx, size_t N, size_t M, size_t K,
         context &cx = *ctx->c;
         cx.require_device("tod_contract2");
         contraction2 contr = make_contraction(N, M, K, permc, ka, kb);
-        dims_t da = dims_t::from(N + K, dims_a), db = dims_t::from(M + K, dims_b);
-        dims_t dc = contract2_dims(contr, da, db);
-        if(d == 0.0) {
-            if(zero) {
-                cx.activate();
-                hip_check(dev::launch_fill(c, dc.size(), 0.0, cx.stream()), "fill");
-                cx.sync();
-            }
-            return;
-        }
-        // align as tod_contract2::perform (tod_contract2_impl.h:124-135),
-        // permute A / B into scratch if needed, one GEMM, permute C back
-        contraction2_align al(contr);
-        contraction2 ca(contr);
-        ca.permute_a(al.perma);
-        ca.permute_b(al.permb);
-        ca.permute_c(al.permc);
-        dims_t da2(da), db2(db), dc2(dc);
-        da2.permute(al.perma);
-        db2.permute(al.permb);
-        dc2.permute(al.permc);
-        std::vector<void*> tmp;
-        try {
-            const double *pa = a, *pb = b;
-            if(!al.perma.is_identity()) {
-                double *t = (double*)cx.alloc(da.size() * 8);
-                tmp.push_back(t);
-                dense_permute(cx, a, da, tensor_tr(al.perma, scalar_tr()), t, false);
-                pa = t;
-            }
-            if(!al.permb.is_identity()) {
-                double *t = (double*)cx.alloc(db.size() * 8);
-                tmp.push_back(t);
-                dense_permute(cx, b, db, tensor_tr(al.permb, scalar_tr()), t, false);
-                pb = t;
-            }
-            double *pc = c;
-            bool cperm = !al.permc.is_identity();
-            if(cperm) {
-                pc = (double*)cx.alloc(dc.size() * 8);
-                tmp.push_back(pc);
-            }
-            gemm_form g = aligned_gemm_form(ca, da2, db2);
-            dev::gemm_cblock cb;
-            dev::gemm_pair gp;
-            std::memset(&cb, 0, sizeof(cb));
-            std::memset(&gp, 0, sizeof(gp));
-            size_t lda = g.ta ? g.m : g.k, ldb = g.tb ? g.k : g.n;
-            bool ta = g.ta, tb = g.tb;
-            if(!g.tc) {
-                cb.m = (uint32_t)g.m; cb.n = (uint32_t)g.n;
-                gp.a = pa; gp.b = pb; gp.lda = (uint32_t)lda; gp.ldb = (uint32_t)ldb;
-            } else {
-                cb.m = (uint32_t)g.n; cb.n = (uint32_t)g.m;
-                gp.a = pb; gp.b = pa; gp.lda = (uint32_t)ldb; gp.ldb = (uint32_t)lda;
-                ta = !g.tb; tb = !g.ta;
-            }
-            cb.ldc = cb.n;
-            cb.c = pc;
-            cb.zero = (cperm || zero) ? 1 : 0;
-            cb.pair_begin = 0;
-            cb.pair_end = 1;
-            gp.k = (uint32_t)g.k;
-            gp.d = d;
-            gp.flags = (ta ? dev::kPairAKMajor : 0) | (tb ? dev::kPairBNMajor : 0);
-            uint32_t bm = 64, bn = 64;
-            dev::tile_shape(dev::TILE_64x64, bm, bn);
-            std::vector<dev::gemm_tile> tiles;
-            for(uint32_t i = 0; i < (cb.m + bm - 1) / bm; i++)
-                for(uint32_t j = 0; j < (cb.n + bn - 1) / bn; j++) {
-                    dev::gemm_tile t; t.cblock = 0; t.tm = (uint16_t)i; t.tn = (uint16_t)j;
-                    tiles.push_back(t);
-                }
-            size_t bytes = 512 + 512 + tiles.size() * sizeof(dev::gemm_tile);
-            char *mem = (char*)cx.alloc(bytes);
-            tmp.push_back(mem);
-            std::vector<char> host(bytes, 0);
-            std::memcpy(&host[0], &cb, sizeof(cb));
-            std::memcpy(&host[512], &gp, sizeof(gp));
-            std::memcpy(&host[1024], tiles.data(), tiles.size() * sizeof(dev::gemm_tile));
-            cx.upload(mem, host.data(), bytes);
-            cx.activate();
-            hip_check(dev::launch_gemm(dev::TILE_64x64, (dev::gemm_cblock*)mem,
-                (dev::gemm_pair*)(mem + 512), (dev::gemm_tile*)(mem + 1024),
-                (uint32_t)tiles.size(), (uint32_t)tiles.size(), cx.stream()), "gemm");
-            cx.sync();
-            if(cperm) {
-                dense_permute(cx, pc, dc2, tensor_tr(al.permc.inverse(), scalar_tr()),
-                    c, zero == 0);
-            }
-        } catch(...) {
-            for(size_t i = 0; i < tmp.size(); i++) cx.free(tmp[i]);
-            throw;
-        }
-        for(size_t i = 0; i < tmp.size(); i++) cx.free(tmp[i]);
+        dense_contract2(cx, contr, dims_t::from(N + K, dims_a), a, dims_t::from(M + K, dims_b),
+            b, d, zero != 0, c);
     });
 }
 
@@ -830,6 +832,17 @@ int lt_btod_contract2_compute_block_host(lt_btod_contract2 *op, size_t aidx_c, d
         need(op, "op"); need(host, "host");
         const symmetry &s = op->op->symc();
         op->op->compute_block_host(get_index(aidx_c, s.bis.block_index_dims()), host);
+    });
+}
+
+int lt_btod_contract2_compute_block(lt_btod_contract2 *op, int zero, size_t aidx_c,
+    const size_t *perm, double coeff, double *blk) {
+    return guard([&]() {
+        need(op, "op"); need(blk, "blk");
+        const symmetry &s = op->op->symc();
+        const size_t n = s.bis.order();
+        tensor_tr tr(perm_or_identity(n, perm), scalar_tr(coeff));
+        op->op->compute_block(zero != 0, get_index(aidx_c, s.bis.block_index_dims()), tr, blk);
     });
 }
 

@@ -1639,6 +1639,54 @@ void btod_contract2::perform_add(btensor &c, double d, int mode) {
 
 // ---------------------------------------------------------------------------
 
+void btod_contract2::compute_block(bool zero, const index_t &ic, const tensor_tr &trc,
+    double *blk) {
+    // gen_bto_contract2::compute_block (gen_bto_contract2_impl.h:312-345):
+    // the full contraction list of the block in the original index space,
+    // then gen_bto_contract2_block::compute_block (impl :100-205): every
+    // pair contracted from the canonical blocks with the connection permuted
+    // by their transformations and the scalars folded into d, pairs with
+    // d == 0 skipped (tod_contract2_impl.h:129), the sum transformed by trc
+    // into the output block.
+    context &ctx = m_a.ctx();
+    ctx.require_device("btod_contract2::compute_block");
+    dims_t bida = m_a.bidims(), bidb = m_b.bidims();
+    block_list bla(bida, m_a.nonzero_blocks()), blb(bidb, m_b.nonzero_blocks());
+    block_list blax(bida), blbx(bidb);
+    unfold_block_list(m_a.sym(), bla, blax);
+    unfold_block_list(m_b.sym(), blb, blbx);
+    contr_list cl;
+    build_clst_full(m_contr, m_a.sym(), m_b.sym(), blax, blbx,
+        m_symc.bis.block_index_dims(), ic, false, cl);
+    dims_t dc = m_symc.bis.block_dims(ic);
+    lt_require(trc.perm.n == dc.n, LT_ERR_PARAM, "block transformation order");
+    double *sum = (double*)ctx.alloc(std::max<size_t>(dc.size(), 1) * sizeof(double));
+    try {
+        bool first = true;
+        for(contr_list::const_iterator p = cl.begin(); p != cl.end(); ++p) {
+            double d = p->tra.st.c * m_ka * p->trb.st.c * m_kb * m_kc;
+            if(d == 0.0) continue;
+            dims_t da = m_a.bis().block_dims(get_index(p->acia, bida));
+            dims_t db = m_b.bis().block_dims(get_index(p->acib, bidb));
+            contraction2 cc(m_contr);
+            cc.permute_a(p->tra.perm.inverse());
+            cc.permute_b(p->trb.perm.inverse());
+            dense_contract2(ctx, cc, da, m_a.block_ptr(p->acia), db, m_b.block_ptr(p->acib),
+                d, first, sum);
+            first = false;
+        }
+        if(first) {
+            ctx.activate();
+            hip_check(dev::launch_fill(sum, dc.size(), 0.0, ctx.stream()), "fill");
+        }
+        dense_permute(ctx, sum, dc, trc, blk, !zero);
+    } catch(...) {
+        ctx.free(sum);
+        throw;
+    }
+    ctx.free(sum);
+}
+
 void btod_contract2::compute_block_host(const index_t &ic, double *out) {
     // Direct path of libtensor (gen_bto_contract2::compute_block): full
     // contraction list in the original space, canonical blocks with their

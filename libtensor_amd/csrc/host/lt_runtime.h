@@ -164,6 +164,14 @@ struct exec_stats {
     int tile_order;         //!< grouped tile order (0 longest first, 1 XCD groups)
 };
 
+//! Dense helpers on a context's stream (both synchronous): dst = or +=
+//! tr(src) (tod_copy), and C = or += d contract(A, B) of device arrays
+//! (tod_contract2: align, permute into scratch, one GEMM, permute back).
+void dense_permute(context &ctx, const double *src, const dims_t &dsrc,
+    const tensor_tr &tr, double *dst, bool accumulate);
+void dense_contract2(context &cx, const contraction2 &contr, const dims_t &da, const double *a,
+    const dims_t &db, const double *b, double d, bool zero, double *c);
+
 struct contract_plan;
 class btod_contract2;
 
@@ -206,6 +214,11 @@ public:
     //! One block computed directly on the host from canonical blocks
     //! (gen_bto_contract2::compute_block, reference-order numerics on CPU).
     void compute_block_host(const index_t &ic, double *out);
+    //! One block computed directly on the device from the resident canonical
+    //! blocks (gen_bto_contract2::compute_block, direct block tensors):
+    //! blk (=, or += unless zero) trc(C_ic), blk a device array in the layout
+    //! of the permuted block.
+    void compute_block(bool zero, const index_t &ic, const tensor_tr &trc, double *blk);
     //! Contraction list of a C block in the original index space.
     void clst(const index_t &ic, contr_list &out);
     //! Runs several performs (grouped mode) as one batch on their common

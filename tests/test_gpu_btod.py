@@ -226,3 +226,37 @@ def test_btod_dotprod(gpu_ctx, name):
     A = a.to_dense()
     got = lt.btod_dotprod(a, a)
     assert abs(got - float(np.sum(A * A))) <= 1e-12 * float(np.sum(A * A))
+
+
+@pytest.mark.parametrize("name", G.names())
+def test_compute_block_device(gpu_ctx, name):
+    """compute_block on the device (direct tensors): scheduled blocks and
+    arbitrary blocks of the result space (canonical or not, allowed or not)
+    against the golden dense result, identity and permuted + scaled +
+    accumulated, and against the host restatement."""
+    _say("compute_block", name)
+    prob, dense, meta, result = G.load(name)
+    a, b, op = P.build(gpu_ctx, prob, dense)
+    c = P.make_result_tensor(gpu_ctx, prob)
+    nblk = int(np.prod(c.bidims))
+    rng = np.random.default_rng(7)
+    picks = list(meta["schedule"][:4]) + [int(x) for x in rng.integers(0, nblk, size=6)]
+    scale = float(np.max(np.abs(result))) or 1.0
+    for bi in picks:
+        idx = c.index(bi)
+        st, dims = c.block_start(idx), c.block_dims(idx)
+        ref = result[tuple(slice(s0, s0 + n) for s0, n in zip(st, dims))]
+        out = lt.device_array(gpu_ctx, tuple(dims))
+        op.compute_block(bi, out)
+        got = out.download()
+        assert np.max(np.abs(got - ref)) <= TOL * scale, bi
+        host = op.compute_block_host(bi, tuple(dims))
+        assert np.max(np.abs(got - host)) <= TOL * scale, bi
+        perm = tuple(int(x) for x in rng.permutation(len(dims)))
+        pd = tuple(dims[p] for p in perm)
+        out2 = lt.device_array(gpu_ctx, pd, np.ones(pd))
+        op.compute_block(bi, out2, zero=False, perm=perm, coeff=-0.5)
+        exp = 1.0 - 0.5 * np.transpose(ref, perm)
+        assert np.max(np.abs(out2.download() - exp)) <= TOL * max(scale, 1.0), (bi, perm)
+        out.free()
+        out2.free()
