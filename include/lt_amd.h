@@ -236,6 +236,15 @@ int lt_btod_contract2_set_ozaki(lt_btod_contract2 *op, int slices);
 int lt_btod_dotprod(lt_btensor *a, const size_t *perma, double ca, lt_btensor *b,
     const size_t *permb, double cb, double *result);
 
+/* ---- btod_copy (libtensor/block_tensor/btod_copy.h:20-80:
+ *      btod_copy(bta, perm, c).perform(btb) and .perform(btb, d);
+ *      gen_block_tensor/impl/gen_bto_copy_impl.h:150-163, :302-330) ----
+ * add == 0: B := c P(A); B (block index space of A permuted by perm) takes
+ * the permuted symmetry of A.  add != 0: B += d c P(A); B must already have
+ * that symmetry.  perm may be NULL (identity). */
+int lt_btod_copy(lt_btensor *a, const size_t *perm, double c, lt_btensor *b, int add,
+    double d);
+
 /* ---- request loop: many calculations sharing one GPU.  No counterpart in
  *      the reference (its operations run synchronously in the caller's
  *      thread, libutil/thread_pool/thread_pool.h:22).  Block tensors stay

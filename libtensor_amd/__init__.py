@@ -16,7 +16,7 @@ from ._capi import (GEMM_CONFIGS, GEMM_STACKED_FIRST, LT_MODE_GROUPED, LT_MODE_O
 
 __all__ = ["Context", "product_table", "block_index_space", "block_tensor",
            "se_perm", "se_part", "se_label", "contraction2", "btod_contract2",
-           "device_array", "tod_copy", "tod_add", "tod_set", "tod_contract2", "tod_dotprod", "btod_dotprod", "server",
+           "device_array", "tod_copy", "tod_add", "tod_set", "tod_contract2", "tod_dotprod", "btod_dotprod", "btod_copy", "server",
            "LtError", "GEMM_CONFIGS", "GEMM_STACKED_FIRST", "LT_MODE_PER_BLOCK", "LT_MODE_GROUPED", "LT_MODE_OZAKI"]
 
 
@@ -483,6 +483,13 @@ def btod_dotprod(bta, btb, perma=None, permb=None, ca=1.0, cb=1.0):
     call("lt_btod_dotprod", bta._h, None if perma is None else sizes(perma), float(ca),
          btb._h, None if permb is None else sizes(permb), float(cb), ctypes.byref(r))
     return r.value
+
+
+def btod_copy(bta, btb, perm=None, c=1.0, add=False, d=1.0):
+    """btb = c P(bta) (btb takes the permuted symmetry), or btb += d c P(bta)
+    (libtensor/block_tensor/btod_copy.h: perform(btb), perform(btb, d))."""
+    call("lt_btod_copy", bta._h, None if perm is None else sizes(perm), float(c), btb._h,
+         1 if add else 0, float(d))
 
 
 class server(object):

@@ -907,6 +907,16 @@ int lt_btod_dotprod(lt_btensor *a, const size_t *perma, double ca, lt_btensor *b
     });
 }
 
+int lt_btod_copy(lt_btensor *a, const size_t *perm, double c, lt_btensor *b, int add,
+    double d) {
+    return guard([&]() {
+        need(a, "a"); need(b, "b");
+        size_t n = a->t->bis().order();
+        lt_require(b->t->bis().order() == n, LT_ERR_DIMS, "btod_copy: orders differ");
+        btod_copy(*a->t, tensor_tr(perm_or_identity(n, perm), scalar_tr(c)), *b->t, add != 0, d);
+    });
+}
+
 // ---- request loop ----
 
 int lt_server_create(lt_ctx *ctx, size_t max_batch, size_t hbm_reserve, lt_server **out) {

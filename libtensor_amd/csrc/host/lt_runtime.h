@@ -146,6 +146,14 @@ dev::perm_desc make_perm_desc_public(const double *src, const dims_t &dsrc,
 //! btod_dotprod(A, tra, B, trb).calculate() on resident block tensors.
 double btod_dotprod(btensor &a, const tensor_tr &tra, btensor &b, const tensor_tr &trb);
 
+//! btod_copy(a, tra).perform(b) (add false: B := tra(A), B takes the permuted
+//! symmetry of A) or perform(b, d) (add: B += d tra(A), B with that symmetry).
+void btod_copy(btensor &a, const tensor_tr &tra, btensor &b, bool add, double d);
+
+//! Runs a list of permute descriptors as one launch of the batched permute
+//! kernel on the context's stream and waits for it.
+void run_permutes(context &ctx, const std::vector<dev::perm_desc> &descs);
+
 enum exec_mode { MODE_PER_BLOCK = 0, MODE_GROUPED = 1, MODE_OZAKI = 2 };
 
 struct exec_stats {
