@@ -116,6 +116,44 @@ def serving_mix(ctx, lt, W, n_clients, rounds, rate, seed=1005):
     return out
 
 
+def block_ops(ctx, lt, prob, a, b, op, c, steps=10):
+    """btod_copy of the two-electron tensor V (every canonical block, one
+    batched permute launch; HBM-bound: 16 B per element) and compute_block of
+    the largest scheduled result block (direct tensors), timed on the host
+    clock around synchronous calls."""
+    perm = (1, 0, 2, 3)
+    td = prob["B"]
+    bis = lt.block_index_space([td["dims"][p] for p in perm])
+    for mask, pos in td.get("splits", []):
+        bis.split([mask[p] for p in perm], pos)
+    vt = lt.block_tensor(ctx, bis)
+    elems = sum(int(np.prod(b.block_dims(b.index(x)))) for x in b.req_nonzero_blocks())
+    lt.btod_copy(b, vt, perm, c=-1.0)
+    t0 = time.perf_counter()
+    for i in range(steps):
+        lt.btod_copy(b, vt, perm, c=-1.0)
+    ms = (time.perf_counter() - t0) * 1e3 / steps
+    gbs = 16.0 * elems / (ms * 1e-3) / 1e9
+    sched = op.get_schedule()
+    big = max(sched, key=lambda x: int(np.prod(c.block_dims(c.index(x)))))
+    dims = tuple(c.block_dims(c.index(big)))
+    out = lt.device_array(ctx, dims)
+    op.compute_block(big, out)
+    t0 = time.perf_counter()
+    for i in range(steps):
+        op.compute_block(big, out)
+    cb_ms = (time.perf_counter() - t0) * 1e3 / steps
+    out.free()
+    res = {"btod_copy": {"elements": elems, "ms": ms, "GB_per_s": gbs,
+                         "hbm_frac": gbs / 8000.0, "perm": list(perm),
+                         "note": "host clock around synchronous calls (includes the "
+                                 "descriptor upload)"},
+           "compute_block": {"block": int(big), "dims": list(dims),
+                             "pairs": len(op.clst(big)), "ms": cb_ms}}
+    log("block ops: %s" % json.dumps(res))
+    return res
+
+
 def ozaki_sweep(ctx, lt, W, prob, op, ref, flops, slices=(3, 4, 5, 6, 7), steps=5):
     """Split-precision mode on the benchmark contraction: error against the
     FP64 reference per block and overall, and time per perform, for each
@@ -294,6 +332,11 @@ def main():
     if not args.no_serve:
         serve = serving_mix(ctx, lt, W, args.serve_clients, args.serve_rounds, args.serve_rate)
 
+    # adjacent block operations on the same resident tensors
+    blockops = None
+    if args.workload == "pp-ladder":
+        blockops = block_ops(ctx, lt, prob, a, b, op, c)
+
     achieved = flops / (gemm_avg * 1e-3) / 1e12
     # HBM bytes per GEMM launch from the committed rocprofv3 counter runs,
     # looked up for the kernel configuration and tile order chosen here
@@ -326,6 +369,7 @@ def main():
         "p99_ms": serve["p99_ms"] if serve else None,
         "serving": serve,
         "ozaki": oz,
+        "block_ops": blockops,
     }
     print(json.dumps(line))
 

@@ -245,6 +245,18 @@ int lt_btod_dotprod(lt_btensor *a, const size_t *perma, double ca, lt_btensor *b
 int lt_btod_copy(lt_btensor *a, const size_t *perm, double c, lt_btensor *b, int add,
     double d);
 
+/* ---- btod_add (libtensor/block_tensor/btod_add.h: btod_add(bta, c) plus
+ *      add_op(bta, perm, c), then perform(btb) / perform(btb, d);
+ *      gen_block_tensor/impl/gen_bto_add_impl.h) ----
+ * B := sum_i c[i] P_i(A_i) (add == 0) or B += d sum_i ... (add != 0).  perms
+ * holds nops permutations of the tensor order (NULL: identities).  The
+ * operands' permuted symmetries must give every block the same orbit
+ * (LT_ERR_SYMMETRY otherwise; the reference also adds operands of different
+ * symmetry, into their intersection); B takes that symmetry, or must have it
+ * for add != 0. */
+int lt_btod_add(size_t nops, lt_btensor *const *a, const size_t *perms, const double *c,
+    lt_btensor *b, int add, double d);
+
 /* ---- request loop: many calculations sharing one GPU.  No counterpart in
  *      the reference (its operations run synchronously in the caller's
  *      thread, libutil/thread_pool/thread_pool.h:22).  Block tensors stay

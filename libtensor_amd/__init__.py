@@ -16,7 +16,7 @@ from ._capi import (GEMM_CONFIGS, GEMM_STACKED_FIRST, LT_MODE_GROUPED, LT_MODE_O
 
 __all__ = ["Context", "product_table", "block_index_space", "block_tensor",
            "se_perm", "se_part", "se_label", "contraction2", "btod_contract2",
-           "device_array", "tod_copy", "tod_add", "tod_set", "tod_contract2", "tod_dotprod", "btod_dotprod", "btod_copy", "server",
+           "device_array", "tod_copy", "tod_add", "tod_set", "tod_contract2", "tod_dotprod", "btod_dotprod", "btod_copy", "btod_add", "server",
            "LtError", "GEMM_CONFIGS", "GEMM_STACKED_FIRST", "LT_MODE_PER_BLOCK", "LT_MODE_GROUPED", "LT_MODE_OZAKI"]
 
 
@@ -490,6 +490,22 @@ def btod_copy(bta, btb, perm=None, c=1.0, add=False, d=1.0):
     (libtensor/block_tensor/btod_copy.h: perform(btb), perform(btb, d))."""
     call("lt_btod_copy", bta._h, None if perm is None else sizes(perm), float(c), btb._h,
          1 if add else 0, float(d))
+
+
+def btod_add(ops, btb, add=False, d=1.0):
+    """btb = sum c_i P_i(bta_i) for ops = [(bta_i, perm_i, c_i)] (btb takes the
+    operands' common permuted symmetry), or btb += d sum ... (libtensor/
+    block_tensor/btod_add.h: add_op, perform(btb), perform(btb, d))."""
+    n = btb.n
+    hs = (ctypes.c_void_p * len(ops))(*[x[0]._h.value for x in ops])
+    perms = None
+    if any(p is not None for _, p, _ in ops):
+        flat = []
+        for _, p, _ in ops:
+            flat += list(range(n)) if p is None else [int(v) for v in p]
+        perms = sizes(flat)
+    cs = (ctypes.c_double * len(ops))(*[float(x[2]) for x in ops])
+    call("lt_btod_add", len(ops), hs, perms, cs, btb._h, 1 if add else 0, float(d))
 
 
 class server(object):

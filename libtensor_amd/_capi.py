@@ -127,6 +127,8 @@ _SIGS = {
     "lt_btod_contract2_set_tile": [vp, ctypes.c_int],
     "lt_btod_contract2_set_graph": [vp, ctypes.c_int],
     "lt_btod_contract2_set_ozaki": [vp, ctypes.c_int],
+    "lt_btod_add": [ctypes.c_size_t, vp, c_size_p, c_double_p, vp, ctypes.c_int,
+                    ctypes.c_double],
     "lt_btod_copy": [vp, c_size_p, ctypes.c_double, vp, ctypes.c_int, ctypes.c_double],
     "lt_btod_dotprod": [vp, c_size_p, ctypes.c_double, vp, c_size_p, ctypes.c_double,
                         c_double_p],

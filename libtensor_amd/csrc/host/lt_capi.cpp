@@ -917,6 +917,25 @@ int lt_btod_copy(lt_btensor *a, const size_t *perm, double c, lt_btensor *b, int
     });
 }
 
+int lt_btod_add(size_t nops, lt_btensor *const *a, const size_t *perms, const double *c,
+    lt_btensor *b, int add, double d) {
+    return guard([&]() {
+        need(a, "a"); need(c, "c"); need(b, "b");
+        lt_require(nops > 0, LT_ERR_PARAM, "btod_add: no operands");
+        const size_t n = b->t->bis().order();
+        std::vector<btensor*> ops;
+        std::vector<tensor_tr> trs;
+        for(size_t i = 0; i < nops; i++) {
+            need(a[i], "a[i]");
+            lt_require(a[i]->t->bis().order() == n, LT_ERR_DIMS, "btod_add: orders differ");
+            ops.push_back(a[i]->t);
+            trs.push_back(tensor_tr(perm_or_identity(n, perms ? perms + i * n : 0),
+                scalar_tr(c[i])));
+        }
+        btod_add(ops, trs, *b->t, add != 0, d);
+    });
+}
+
 // ---- request loop ----
 
 int lt_server_create(lt_ctx *ctx, size_t max_batch, size_t hbm_reserve, lt_server **out) {

@@ -150,6 +150,12 @@ double btod_dotprod(btensor &a, const tensor_tr &tra, btensor &b, const tensor_t
 //! symmetry of A) or perform(b, d) (add: B += d tra(A), B with that symmetry).
 void btod_copy(btensor &a, const tensor_tr &tra, btensor &b, bool add, double d);
 
+//! btod_add: B := sum_i trs[i](ops[i]) (add false; B takes the operands'
+//! permuted symmetry, which must give every block the same orbit for all
+//! operands) or B += d sum_i trs[i](ops[i]) (add; B with that symmetry).
+void btod_add(const std::vector<btensor*> &ops, const std::vector<tensor_tr> &trs,
+    btensor &b, bool add, double d);
+
 //! Runs a list of permute descriptors as one launch of the batched permute
 //! kernel on the context's stream and waits for it.
 void run_permutes(context &ctx, const std::vector<dev::perm_desc> &descs);
