@@ -78,6 +78,10 @@ int lt_ctx_destroy(lt_ctx *ctx);
 int lt_ctx_sync(lt_ctx *ctx);
 int lt_ctx_mem(lt_ctx *ctx, size_t *used, size_t *cap);
 int lt_ctx_set_threads(lt_ctx *ctx, size_t nthreads);
+/* Device time (HIP events) of the kernels of the last block operation
+ * (lt_btod_copy, lt_btod_add) on this context.  No counterpart in the
+ * reference (its timings<> report host wall time). */
+int lt_ctx_last_op_ms(lt_ctx *ctx, double *ms);
 
 /* Device memory for the dense-tensor entry points. */
 int lt_dev_alloc(lt_ctx *ctx, size_t bytes, void **dptr);

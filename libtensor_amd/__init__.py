@@ -68,6 +68,12 @@ class Context(object):
     def set_threads(self, n):
         call("lt_ctx_set_threads", self._h, int(n))
 
+    def last_op_ms(self):
+        """Device time of the kernels of the last btod_copy / btod_add."""
+        v = ctypes.c_double()
+        call("lt_ctx_last_op_ms", self._h, ctypes.byref(v))
+        return v.value
+
 
 class product_table(object):
     """Point-group product table registered under an id

@@ -65,6 +65,7 @@ _SIGS = {
     "lt_ctx_sync": [vp],
     "lt_ctx_mem": [vp, c_size_p, c_size_p],
     "lt_ctx_set_threads": [vp, ctypes.c_size_t],
+    "lt_ctx_last_op_ms": [vp, c_double_p],
     "lt_dev_alloc": [vp, ctypes.c_size_t, ctypes.POINTER(vp)],
     "lt_dev_free": [vp, vp],
     "lt_dev_upload": [vp, vp, vp, ctypes.c_size_t],

@@ -36,7 +36,7 @@ bool same_orbits(const symmetry &x, const symmetry &y, const dims_t &bid) {
 
 } // unnamed namespace
 
-void run_permutes(context &ctx, const std::vector<dev::perm_desc> &descs) {
+void run_permutes(context &ctx, const std::vector<dev::perm_desc> &descs, double *ms) {
     if(descs.empty()) return;
     std::vector<uint32_t> cd, co;
     make_chunks(descs, cd, co);
@@ -49,16 +49,30 @@ void run_permutes(context &ctx, const std::vector<dev::perm_desc> &descs) {
     std::memcpy(&host[o_cd], cd.data(), 4 * cd.size());
     std::memcpy(&host[o_co], co.data(), 4 * co.size());
     char *mem = (char*)ctx.alloc(bytes);
+    hipEvent_t ev[2] = {0, 0};
     try {
         ctx.upload(mem, host.data(), bytes);
         ctx.activate();
+        if(ms) {
+            hip_check(hipEventCreate(&ev[0]), "hipEventCreate");
+            hip_check(hipEventCreate(&ev[1]), "hipEventCreate");
+            hip_check(hipEventRecord(ev[0], ctx.stream()), "hipEventRecord");
+        }
         hip_check(dev::launch_permute((dev::perm_desc*)mem, (uint32_t*)(mem + o_cd),
             (uint32_t*)(mem + o_co), (uint32_t)cd.size(), ctx.stream()), "permute");
+        if(ms) hip_check(hipEventRecord(ev[1], ctx.stream()), "hipEventRecord");
         ctx.sync();
+        if(ms) {
+            float t = 0.f;
+            hip_check(hipEventElapsedTime(&t, ev[0], ev[1]), "hipEventElapsedTime");
+            *ms += t;
+        }
     } catch(...) {
+        for(int i = 0; i < 2; i++) if(ev[i]) (void)hipEventDestroy(ev[i]);
         ctx.free(mem);
         throw;
     }
+    for(int i = 0; i < 2; i++) if(ev[i]) (void)hipEventDestroy(ev[i]);
     ctx.free(mem);
 }
 
@@ -95,7 +109,9 @@ void btod_copy(btensor &a, const tensor_tr &tra, btensor &b, bool add, double d)
         descs.push_back(make_perm_desc_public(a.block_ptr(ia), da, tr, b.block_ptr(ob.acidx), acc));
         b.mark_nonzero(ob.acidx);
     }
-    run_permutes(ctx, descs);
+    double ms = 0.0;
+    run_permutes(ctx, descs, &ms);
+    ctx.set_last_op_ms(ms);
 }
 
 void btod_add(const std::vector<btensor*> &ops, const std::vector<tensor_tr> &trs,
@@ -123,6 +139,7 @@ void btod_add(const std::vector<btensor*> &ops, const std::vector<tensor_tr> &tr
         b.zero_all();
         b.set_symmetry(syms[0]);
     }
+    double ms = 0.0;
     std::set<size_t> written;
     if(add) {
         const std::vector<size_t> nzb = b.nonzero_blocks();
@@ -148,8 +165,9 @@ void btod_add(const std::vector<btensor*> &ops, const std::vector<tensor_tr> &tr
             written.insert(ob.acidx);
             b.mark_nonzero(ob.acidx);
         }
-        run_permutes(ctx, descs);
+        run_permutes(ctx, descs, &ms);
     }
+    ctx.set_last_op_ms(ms);
 }
 
 } // namespace lta

@@ -258,6 +258,10 @@ int lt_ctx_set_threads(lt_ctx *ctx, size_t n) {
     return guard([&]() { need(ctx, "ctx"); ctx->c->set_nthreads(n); });
 }
 
+int lt_ctx_last_op_ms(lt_ctx *ctx, double *ms) {
+    return guard([&]() { need(ctx, "ctx"); need(ms, "ms"); *ms = ctx->c->last_op_ms(); });
+}
+
 int lt_dev_alloc(lt_ctx *ctx, size_t bytes, void **dptr) {
     return guard([&]() {
         need(ctx, "ctx"); need(dptr, "dptr");

@@ -49,6 +49,11 @@ public:
     size_t nthreads() const { return m_nthreads; }
     void set_nthreads(size_t n) { m_nthreads = n ? n : 1; }
 
+    //! Device time (HIP events) of the last block operation's kernels
+    //! (btod_copy, btod_add) on this context.
+    double last_op_ms() const { return m_last_op_ms; }
+    void set_last_op_ms(double ms) { m_last_op_ms = ms; }
+
 private:
     int m_device;
     hipStream_t m_stream;
@@ -57,6 +62,7 @@ private:
     std::mutex m_mtx;
     std::map<void*, size_t> m_sizes;
     size_t m_nthreads;
+    double m_last_op_ms = 0.0;
 };
 
 //! Slab allocator for the blocks of one tensor (few hipMallocs, stable
@@ -157,8 +163,9 @@ void btod_add(const std::vector<btensor*> &ops, const std::vector<tensor_tr> &tr
     btensor &b, bool add, double d);
 
 //! Runs a list of permute descriptors as one launch of the batched permute
-//! kernel on the context's stream and waits for it.
-void run_permutes(context &ctx, const std::vector<dev::perm_desc> &descs);
+//! kernel on the context's stream and waits for it; adds the launch's device
+//! time (HIP events) to *ms when ms is not null.
+void run_permutes(context &ctx, const std::vector<dev::perm_desc> &descs, double *ms = 0);
 
 enum exec_mode { MODE_PER_BLOCK = 0, MODE_GROUPED = 1, MODE_OZAKI = 2 };
 
