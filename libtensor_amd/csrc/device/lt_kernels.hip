@@ -828,25 +828,29 @@ __device__ __forceinline__ void st_cols(st_rowsrc<FM, FN> &r, const stack_pair &
     }
 }
 
-//! Operands of one chunk in the pair's form (FORM != ANY: no branches).
+//! B operand forms without a condition on A (for kernels that stage A
+//! separately): the same forms as st_form.
+__device__ __forceinline__ int st_bform(const stack_pair &sp) {
+    const uint32_t f = sp.flags;
+    if(f & kPairVecB) return ST_FORM_VA_VB;
+    if(f & kPairVecBT) return (f & kPairOffArith) ? ST_FORM_VA_VTAR : ST_FORM_ANY;
+    if(f & kPairOffArith) return ST_FORM_VA_AR;
+    if(!sp.koff && !(f & kPairBNMajor)) return ST_FORM_VA_ROW;
+    return ST_FORM_ANY;
+}
+
+//! B operand of one chunk in the pair's B form (FORM != ANY: no branches).
 template<int FM, int FN, int FORM>
-__device__ __forceinline__ void load_form(double (&af)[FM][4], double (&bf)[FN][4],
-    st_rowsrc<FM, FN> &ar, const stack_pair &sp, int c, const int (&bcol)[FN], int lg) {
+__device__ __forceinline__ void load_b_form(double (&bf)[FN][4], st_rowsrc<FM, FN> &ar,
+    const stack_pair &sp, int c, const int (&bcol)[FN], int lg) {
     if(FORM == ST_FORM_ANY) {
         st_cols<FM, FN>(ar, sp, bcol);
-        load_chunk_st_any<FM, FN>(af, bf, ar, sp, c, bcol, lg);
+        load_b_st<FM, FN>(bf, ar, sp, c, bcol, lg);
         return;
     }
     const uint32_t K = sp.k;
     const uint32_t kk = 16 * c + 4 * lg;
     const uint32_t k0 = min(kk, K - 2), k2 = min(kk + 2, K - 2);
-#pragma unroll
-    for(int i = 0; i < FM; i++) {
-        gdouble *pa = (gdouble *)ar.p[i];
-        const double2_t v0 = *(gdouble2 *)(pa + k0);
-        const double2_t v1 = *(gdouble2 *)(pa + k2);
-        af[i][0] = v0.x; af[i][1] = v0.y; af[i][2] = v1.x; af[i][3] = v1.y;
-    }
     if(FORM == ST_FORM_VA_VB) {
 #pragma unroll
         for(int f = 0; f < FN; f++) {
@@ -876,6 +880,28 @@ __device__ __forceinline__ void load_form(double (&af)[FM][4], double (&bf)[FN][
     for(int f = 0; f < FN; f++)
 #pragma unroll
         for(int j = 0; j < 4; j++) bf[f][j] = ldg(sp.b, 8u * uint32_t(ko[j] + no[f]));
+}
+
+//! Operands of one chunk in the pair's form (FORM != ANY: no branches).
+template<int FM, int FN, int FORM>
+__device__ __forceinline__ void load_form(double (&af)[FM][4], double (&bf)[FN][4],
+    st_rowsrc<FM, FN> &ar, const stack_pair &sp, int c, const int (&bcol)[FN], int lg) {
+    if(FORM == ST_FORM_ANY) {
+        st_cols<FM, FN>(ar, sp, bcol);
+        load_chunk_st_any<FM, FN>(af, bf, ar, sp, c, bcol, lg);
+        return;
+    }
+    const uint32_t K = sp.k;
+    const uint32_t kk = 16 * c + 4 * lg;
+    const uint32_t k0 = min(kk, K - 2), k2 = min(kk + 2, K - 2);
+#pragma unroll
+    for(int i = 0; i < FM; i++) {
+        gdouble *pa = (gdouble *)ar.p[i];
+        const double2_t v0 = *(gdouble2 *)(pa + k0);
+        const double2_t v1 = *(gdouble2 *)(pa + k2);
+        af[i][0] = v0.x; af[i][1] = v0.y; af[i][2] = v1.x; af[i][3] = v1.y;
+    }
+    load_b_form<FM, FN, FORM>(bf, ar, sp, c, bcol, lg);
 }
 
 //! Cursor of a wave over its chunks of a stack's pair list (pair p, chunk
@@ -1270,6 +1296,54 @@ __device__ __forceinline__ void lds_stage_store(double *buf, const double2_t (&s
     }
 }
 
+//! One step of the LDS-staged tile: stage chunk un (A through registers into
+//! wbuf, zeros when un is past the group's last chunk; B into bn) while the
+//! chunk staged in rbuf (with B in bc) is multiplied.  One basic block per
+//! B form.
+template<int FM, int FN, int FORM>
+__device__ __forceinline__ void lds_step(double4_t (&acc)[FM][FN], double (&bn)[FN][4],
+    const double (&bc)[FN][4], st_cursor<FM, FN> &un, const double *rbuf, double *wbuf,
+    const int (&bcol)[FN], int lr, int lg, int srw, int seg) {
+    double2_t sa[FM];
+    lds_stage_load<FM, FN>(sa, un.r, un.sp, un.c, seg);
+    load_b_form<FM, FN, FORM>(bn, un.r, un.sp, un.c, bcol, lg);
+    double af[FM][4];
+#pragma unroll
+    for(int i = 0; i < FM; i++) {
+        const double *rp = rbuf + (16 * i + lr) * kLdsRowStride + 4 * lg;
+        const double2_t v0 = *(const double2_t *)rp;
+        const double2_t v1 = *(const double2_t *)(rp + 2);
+        af[i][0] = v0.x; af[i][1] = v0.y; af[i][2] = v1.x; af[i][3] = v1.y;
+    }
+#pragma unroll
+    for(int q = 0; q < 4; q++)
+#pragma unroll
+        for(int i = 0; i < FM; i++)
+#pragma unroll
+            for(int j = 0; j < FN; j++)
+                acc[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(af[i][q], bc[j][q], acc[i][j],
+                    0, 0, 0);
+    lds_stage_store<FM, FN>(wbuf, sa, un.r, un.valid ? un.sp.k : 0u, un.c, srw, seg);
+}
+
+template<int FM, int FN>
+__device__ __forceinline__ void lds_step_any(double4_t (&acc)[FM][FN], double (&bn)[FN][4],
+    const double (&bc)[FN][4], st_cursor<FM, FN> &un, const double *rbuf, double *wbuf,
+    const int (&bcol)[FN], int lr, int lg, int srw, int seg) {
+    switch(st_bform(un.sp)) {
+    case ST_FORM_VA_VB:
+        lds_step<FM, FN, ST_FORM_VA_VB>(acc, bn, bc, un, rbuf, wbuf, bcol, lr, lg, srw, seg); break;
+    case ST_FORM_VA_ROW:
+        lds_step<FM, FN, ST_FORM_VA_ROW>(acc, bn, bc, un, rbuf, wbuf, bcol, lr, lg, srw, seg); break;
+    case ST_FORM_VA_AR:
+        lds_step<FM, FN, ST_FORM_VA_AR>(acc, bn, bc, un, rbuf, wbuf, bcol, lr, lg, srw, seg); break;
+    case ST_FORM_VA_VTAR:
+        lds_step<FM, FN, ST_FORM_VA_VTAR>(acc, bn, bc, un, rbuf, wbuf, bcol, lr, lg, srw, seg); break;
+    default:
+        lds_step<FM, FN, ST_FORM_ANY>(acc, bn, bc, un, rbuf, wbuf, bcol, lr, lg, srw, seg); break;
+    }
+}
+
 template<int FM, int FN>
 __device__ __forceinline__ void lds_tile(const stack_desc &st,
     const stack_pair *__restrict__ spairs, const stack_arow *__restrict__ arow,
@@ -1301,7 +1375,7 @@ __device__ __forceinline__ void lds_tile(const stack_desc &st,
     if(nchunk > 0) {
         const uint32_t pbeg = st.pair_begin, pend = st.pair_begin + st.npairs;
         // the group's first chunk; a group without chunks keeps an in-bounds
-        // cursor on chunk 0 and only joins the barriers
+        // cursor on chunk 0, stages zeros and only joins the barriers
         st_cursor<FM, FN> u, u2;
         u.p = pbeg;
         u.c = g;
@@ -1311,46 +1385,28 @@ __device__ __forceinline__ void lds_tile(const stack_desc &st,
         u.sp = spairs[u.p];
         st_rows_a<FM, FN>(u.r, arow, st.arow_begin + (u.p - pbeg) * st.rows, srow);
         st_cols<FM, FN>(u.r, u.sp, bcol);
-        double2_t sa[FM];
-        double bx[FN][4], by[FN][4], af[FM][4];
-        lds_stage_load<FM, FN>(sa, u.r, u.sp, u.c, seg);
-        load_b_st<FM, FN>(bx, u.r, u.sp, u.c, bcol, lg);
-        lds_stage_store<FM, FN>(abuf, sa, u.r, u.sp.k, u.c, srw, seg);
+        double bx[FN][4], by[FN][4];
+        {
+            double2_t sa[FM];
+            lds_stage_load<FM, FN>(sa, u.r, u.sp, u.c, seg);
+            load_b_st<FM, FN>(bx, u.r, u.sp, u.c, bcol, lg);
+            lds_stage_store<FM, FN>(abuf, sa, u.r, u.valid ? u.sp.k : 0u, u.c, srw, seg);
+        }
         u2 = u;
         st_advance_a<FM, FN, 2>(u2, spairs, arow, st, srow);
         __syncthreads();
+        // iteration it multiplies chunk it (LDS buffer it & 1, B in bx or by)
+        // and stages chunk it + 1 in the other buffer
         const int niter = (nchunk + 1) >> 1;
-        for(int it = 0; it < niter; it++) {
-            const bool cur = u.valid;
-            // next chunk: its A staged through registers, its B straight in
-            st_cols<FM, FN>(u2.r, u2.sp, bcol);
-            lds_stage_load<FM, FN>(sa, u2.r, u2.sp, u2.c, seg);
-            load_b_st<FM, FN>(by, u2.r, u2.sp, u2.c, bcol, lg);
-            const double *rb = abuf + (it & 1) * ABUF;
-#pragma unroll
-            for(int i = 0; i < FM; i++) {
-                const double *rp = rb + (16 * i + lr) * kLdsRowStride + 4 * lg;
-                const double2_t v0 = *(const double2_t *)rp;
-                const double2_t v1 = *(const double2_t *)(rp + 2);
-                af[i][0] = v0.x; af[i][1] = v0.y; af[i][2] = v1.x; af[i][3] = v1.y;
-            }
-            if(cur) {
-#pragma unroll
-                for(int q = 0; q < 4; q++)
-#pragma unroll
-                    for(int i = 0; i < FM; i++)
-#pragma unroll
-                        for(int j = 0; j < FN; j++)
-                            acc[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(
-                                af[i][q], bx[j][q], acc[i][j], 0, 0, 0);
-            }
-            lds_stage_store<FM, FN>(abuf + ((it + 1) & 1) * ABUF, sa, u2.r, u2.sp.k, u2.c, srw, seg);
-            u = u2;
+        for(int it = 0; it < niter; it += 2) {
+            st_cursor<FM, FN> un = u2;
             st_advance_a<FM, FN, 2>(u2, spairs, arow, st, srow);
-#pragma unroll
-            for(int f = 0; f < FN; f++)
-#pragma unroll
-                for(int q = 0; q < 4; q++) bx[f][q] = by[f][q];
+            lds_step_any<FM, FN>(acc, by, bx, un, abuf, abuf + ABUF, bcol, lr, lg, srw, seg);
+            __syncthreads();
+            if(it + 1 >= niter) break;
+            un = u2;
+            st_advance_a<FM, FN, 2>(u2, spairs, arow, st, srow);
+            lds_step_any<FM, FN>(acc, bx, by, un, abuf + ABUF, abuf, bcol, lr, lg, srw, seg);
             __syncthreads();
         }
     }
