@@ -486,9 +486,9 @@ gemm_direct_kernel(const gemm_cblock *__restrict__ cbs,
 
         int arow[FM], bcol[FN];
 #pragma unroll
-        for(int i = 0; i < FM; i++) arow[i] = (16 * i + lr) < mrem ? m0 + 16 * i + lr : 0;
+        for(int i = 0; i < FM; i++) arow[i] = (16 * i + lr) < mrem ? m0 + 16 * i + lr : m0;
 #pragma unroll
-        for(int j = 0; j < FN; j++) bcol[j] = (16 * j + lr) < nrem ? n0 + 16 * j + lr : 0;
+        for(int j = 0; j < FN; j++) bcol[j] = (16 * j + lr) < nrem ? n0 + 16 * j + lr : n0;
 
         const uint32_t pend = cb.pair_end;
         uint32_t px = cb.pair_begin;
@@ -1038,11 +1038,13 @@ __device__ __forceinline__ void stacked_tile(const stack_desc &st,
 #pragma unroll
         for(int j = 0; j < FN; j++) acc[i][j] = double4_t{0.0, 0.0, 0.0, 0.0};
 
+    // lanes of rows or columns past the extent read the tile's first row or
+    // column: their loads fall on cache lines the other lanes request anyway
     int srow[FM], bcol[FN];
 #pragma unroll
-    for(int i = 0; i < FM; i++) srow[i] = (16 * i + lr) < mrem ? m0 + 16 * i + lr : 0;
+    for(int i = 0; i < FM; i++) srow[i] = (16 * i + lr) < mrem ? m0 + 16 * i + lr : m0;
 #pragma unroll
-    for(int j = 0; j < FN; j++) bcol[j] = (16 * j + lr) < nrem ? n0 + 16 * j + lr : 0;
+    for(int j = 0; j < FN; j++) bcol[j] = (16 * j + lr) < nrem ? n0 + 16 * j + lr : n0;
 
     const uint32_t pbeg = st.pair_begin, pend = st.pair_begin + st.npairs;
     uint32_t px = pbeg;
@@ -1367,9 +1369,9 @@ __device__ __forceinline__ void lds_tile(const stack_desc &st,
 
     int srow[FM], bcol[FN];
 #pragma unroll
-    for(int j = 0; j < FM; j++) srow[j] = (srw + 16 * j) < mrem ? m0 + srw + 16 * j : 0;
+    for(int j = 0; j < FM; j++) srow[j] = (srw + 16 * j) < mrem ? m0 + srw + 16 * j : m0;
 #pragma unroll
-    for(int f = 0; f < FN; f++) bcol[f] = (16 * f + lr) < nrem ? nw + 16 * f + lr : 0;
+    for(int f = 0; f < FN; f++) bcol[f] = (16 * f + lr) < nrem ? nw + 16 * f + lr : n0;
 
     const int nchunk = int(st.nchunk);
     if(nchunk > 0) {

@@ -1048,17 +1048,18 @@ void btod_contract2::build_plan(btensor &c, bool add, double dscale) {
     // tile lists live in their own allocation (replaced by the autotuner)
     auto upload_tiles = [&](int k, int order) {
         make_tiles(k, order, pl.tiles_by_block, pl.block_tile_begin, pl.tiles_sorted);
+        // the sorted list can be longer than the per-block list (padding
+        // tiles of the XCD-aware order)
         size_t nt = pl.tiles_sorted.size() * sizeof(dev::gemm_tile);
-        size_t tsz = 2 * align256(nt);
+        size_t nb = pl.tiles_by_block.size() * sizeof(dev::gemm_tile);
+        size_t tsz = align256(nt) + align256(nb) + 256;
         if(pl.tiles.mem) ctx.free(pl.tiles.mem);
         pl.tiles.ctx = &ctx;
         pl.tiles.mem = ctx.alloc(tsz);
         pl.tiles.bytes = tsz;
         std::vector<char> th(tsz, 0);
-        if(nt) {
-            std::memcpy(&th[0], pl.tiles_sorted.data(), nt);
-            std::memcpy(&th[align256(nt)], pl.tiles_by_block.data(), nt);
-        }
+        if(nt) std::memcpy(&th[0], pl.tiles_sorted.data(), nt);
+        if(nb) std::memcpy(&th[align256(nt)], pl.tiles_by_block.data(), nb);
         ctx.upload(pl.tiles.mem, th.data(), tsz);
         pl.d_tiles_sorted = (dev::gemm_tile*)pl.tiles.mem;
         pl.d_tiles_block = (dev::gemm_tile*)((char*)pl.tiles.mem + align256(nt));
