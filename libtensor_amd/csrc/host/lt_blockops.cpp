@@ -48,7 +48,8 @@ void run_permutes(context &ctx, const std::vector<dev::perm_desc> &descs, double
     std::memcpy(&host[0], descs.data(), descs.size() * sizeof(dev::perm_desc));
     std::memcpy(&host[o_cd], cd.data(), 4 * cd.size());
     std::memcpy(&host[o_co], co.data(), 4 * co.size());
-    char *mem = (char*)ctx.alloc(bytes);
+    std::lock_guard<std::mutex> lk(ctx.staging_lock());
+    char *mem = (char*)ctx.staging(bytes);
     hipEvent_t ev[2] = {0, 0};
     try {
         ctx.upload(mem, host.data(), bytes);
@@ -69,11 +70,9 @@ void run_permutes(context &ctx, const std::vector<dev::perm_desc> &descs, double
         }
     } catch(...) {
         for(int i = 0; i < 2; i++) if(ev[i]) (void)hipEventDestroy(ev[i]);
-        ctx.free(mem);
         throw;
     }
     for(int i = 0; i < 2; i++) if(ev[i]) (void)hipEventDestroy(ev[i]);
-    ctx.free(mem);
 }
 
 void btod_copy(btensor &a, const tensor_tr &tra, btensor &b, bool add, double d) {

@@ -64,6 +64,18 @@ void context::activate() const {
     hip_check(hipSetDevice(m_device), "hipSetDevice");
 }
 
+void *context::staging(size_t bytes) {
+    if(bytes > m_stage_bytes) {
+        if(m_stage) free(m_stage);
+        m_stage = 0;
+        m_stage_bytes = 0;
+        size_t n = std::max<size_t>(bytes, 65536);
+        m_stage = alloc(n);
+        m_stage_bytes = n;
+    }
+    return m_stage;
+}
+
 void *context::alloc(size_t bytes) {
     if(bytes == 0) bytes = 8;
     std::lock_guard<std::mutex> lk(m_mtx);

@@ -54,6 +54,12 @@ public:
     double last_op_ms() const { return m_last_op_ms; }
     void set_last_op_ms(double ms) { m_last_op_ms = ms; }
 
+    //! Device staging area for the descriptor lists of block operations
+    //! (grown on demand, released with the context); hold staging_lock()
+    //! from the request until the work that reads it has completed.
+    void *staging(size_t bytes);
+    std::mutex &staging_lock() { return m_stage_mtx; }
+
 private:
     int m_device;
     hipStream_t m_stream;
@@ -63,6 +69,9 @@ private:
     std::map<void*, size_t> m_sizes;
     size_t m_nthreads;
     double m_last_op_ms = 0.0;
+    void *m_stage = nullptr;
+    size_t m_stage_bytes = 0;
+    std::mutex m_stage_mtx;
 };
 
 //! Slab allocator for the blocks of one tensor (few hipMallocs, stable
