@@ -238,8 +238,8 @@ def main():
     if args.workload != "pp-ladder":
         args.no_cpu = args.no_serve = args.no_ozaki = True
     if args.gpus != 1:
-        print("bench.py runs on one GPU only: this build measures the single-GPU "
-              "path (see DESIGN.md for how the contraction would shard)", file=sys.stderr)
+        print("bench.py runs on one GPU only: it measures one GPU's share of the "
+              "contraction; multi-GPU sharding is not built", file=sys.stderr)
         sys.exit(2)
 
     import libtensor_amd as lt
