@@ -1,10 +1,14 @@
 // CDNA4 (gfx950) kernels of the block-contraction path:
 //   * batched N-d permute / scale / accumulate (tod_copy, tod_add, symmetry
-//     unfolding, C write-back),
-//   * grouped FP64 GEMM on v_mfma_f64_16x16x4f64 with LDS-staged operand
-//     tiles; every workgroup owns one C tile and runs the whole contraction
-//     list of that C block, so each C element is written once and the sum
-//     order is fixed (no atomics),
+//     unfolding, C write-back): gather chunks or LDS-tiled transposes,
+//   * FP64 GEMM on v_mfma_f64_16x16x4f64 over the contraction lists of the
+//     C blocks: LDS-staged workgroup tiles (gemm_kernel), split-K tiles with
+//     operands loaded straight into MFMA fragments per C block
+//     (gemm_direct_kernel) or over stacks of C blocks that share their B
+//     operands (gemm_stacked_kernel, its varying-shape and LDS-staged
+//     variants).  Every workgroup owns one C tile and runs the whole list of
+//     that tile, the waves' partial tiles are added in LDS in wave order: each
+//     C element is written once with a fixed summation order (no atomics),
 //   * permuted dot product with a fixed-order two-pass reduction,
 //   * fill.
 #include <hip/hip_runtime.h>
