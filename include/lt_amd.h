@@ -41,6 +41,7 @@ typedef struct lt_ctx lt_ctx;
 typedef struct lt_btensor lt_btensor;
 typedef struct lt_symel lt_symel;
 typedef struct lt_btod_contract2 lt_btod_contract2;
+typedef struct lt_btod_symmetrize2 lt_btod_symmetrize2;
 typedef struct lt_server lt_server;
 
 typedef struct lt_stats {
@@ -260,6 +261,35 @@ int lt_btod_copy(lt_btensor *a, const size_t *perm, double c, lt_btensor *b, int
  * for add != 0. */
 int lt_btod_add(size_t nops, lt_btensor *const *a, const size_t *perms, const double *c,
     lt_btensor *b, int add, double d);
+
+/* ---- btod_symmetrize2 (libtensor/block_tensor/btod_symmetrize2.h:20-110;
+ *      gen_block_tensor/impl/gen_bto_symmetrize2_impl.h) ----
+ * B = X + s P(X), X the result of a btod_contract2 (create) or of another
+ * btod_symmetrize2 (create_nested), P a pair permutation (P P = 1, P not the
+ * identity; btod_symmetrize2(op, i1, i2, symm) is the transposition of i1
+ * and i2), s = +1 (symm != 0) or -1.  The result symmetry is libtensor's
+ * so_symmetrize of the symmetry of X (sym_orbit queries it), the schedule
+ * that of gen_bto_symmetrize2::make_schedule.  The operand must outlive the
+ * operation.  perform: B := result (B takes the result symmetry);
+ * perform_add: B += d * result for B with that symmetry (the reference's
+ * perform(btb, d)); mode as for btod_contract2, used for X.  X is kept in a
+ * resident temporary owned by the operation.  compute_block: blk (=, or +=
+ * unless zero) P_blk c B(aidx_b) on the device (btod_symmetrize2::
+ * compute_block, direct tensors); perm may be NULL. */
+int lt_btod_symmetrize2_create(lt_btod_contract2 *op, const size_t *perm, int symm,
+    lt_btod_symmetrize2 **out);
+int lt_btod_symmetrize2_create_nested(lt_btod_symmetrize2 *op, const size_t *perm, int symm,
+    lt_btod_symmetrize2 **out);
+int lt_btod_symmetrize2_destroy(lt_btod_symmetrize2 *op);
+int lt_btod_symmetrize2_perform(lt_btod_symmetrize2 *op, lt_btensor *b, int mode);
+int lt_btod_symmetrize2_perform_add(lt_btod_symmetrize2 *op, lt_btensor *b, double d,
+    int mode);
+int lt_btod_symmetrize2_schedule(lt_btod_symmetrize2 *op, size_t *out, size_t cap,
+    size_t *count);
+int lt_btod_symmetrize2_sym_orbit(lt_btod_symmetrize2 *op, size_t aidx, size_t *acidx,
+    size_t *perm, double *coeff, int *allowed);
+int lt_btod_symmetrize2_compute_block(lt_btod_symmetrize2 *op, int zero, size_t aidx_b,
+    const size_t *perm, double coeff, double *blk);
 
 /* ---- request loop: many calculations sharing one GPU.  No counterpart in
  *      the reference (its operations run synchronously in the caller's

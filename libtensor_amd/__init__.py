@@ -16,7 +16,7 @@ from ._capi import (GEMM_CONFIGS, GEMM_STACKED_FIRST, LT_MODE_GROUPED, LT_MODE_O
 
 __all__ = ["Context", "product_table", "block_index_space", "block_tensor",
            "se_perm", "se_part", "se_label", "contraction2", "btod_contract2",
-           "device_array", "tod_copy", "tod_add", "tod_set", "tod_contract2", "tod_dotprod", "btod_dotprod", "btod_copy", "btod_add", "server",
+           "device_array", "tod_copy", "tod_add", "tod_set", "tod_contract2", "tod_dotprod", "btod_dotprod", "btod_copy", "btod_add", "btod_symmetrize2", "server",
            "LtError", "GEMM_CONFIGS", "GEMM_STACKED_FIRST", "LT_MODE_PER_BLOCK", "LT_MODE_GROUPED", "LT_MODE_OZAKI"]
 
 
@@ -394,6 +394,70 @@ class btod_contract2(object):
 
     def set_ozaki(self, slices):
         call("lt_btod_contract2_set_ozaki", self._h, int(slices))
+
+
+class btod_symmetrize2(object):
+    """B = X + s P(X) for the result X of a btod_contract2 or of another
+    btod_symmetrize2 (libtensor/block_tensor/btod_symmetrize2.h).  perm is a
+    pair permutation (P P = 1); btod_symmetrize2(op, i1, i2, symm) of the
+    reference is the transposition of i1 and i2 (use ``pair(n, i1, i2)``)."""
+
+    def __init__(self, op, perm, symm):
+        h = ctypes.c_void_p()
+        if isinstance(op, btod_symmetrize2):
+            call("lt_btod_symmetrize2_create_nested", op._h, sizes(perm), 1 if symm else 0,
+                 ctypes.byref(h))
+        else:
+            call("lt_btod_symmetrize2_create", op._h, sizes(perm), 1 if symm else 0,
+                 ctypes.byref(h))
+        self._h = h
+        self.op = op          # the operand must outlive this operation
+        self.n = len(perm)
+
+    @staticmethod
+    def pair(n, i1, i2):
+        p = list(range(n))
+        p[i1], p[i2] = p[i2], p[i1]
+        return p
+
+    def __del__(self):
+        try:
+            if self._h:
+                call("lt_btod_symmetrize2_destroy", self._h)
+                self._h = None
+        except Exception:
+            pass
+
+    def perform(self, btb, mode=LT_MODE_GROUPED):
+        call("lt_btod_symmetrize2_perform", self._h, btb._h, int(mode))
+
+    def perform_add(self, btb, d=1.0, mode=LT_MODE_GROUPED):
+        call("lt_btod_symmetrize2_perform_add", self._h, btb._h, float(d), int(mode))
+
+    def get_schedule(self):
+        cnt = ctypes.c_size_t()
+        call("lt_btod_symmetrize2_schedule", self._h, None, 0, ctypes.byref(cnt))
+        out = (ctypes.c_size_t * max(1, cnt.value))()
+        call("lt_btod_symmetrize2_schedule", self._h, out, cnt.value, ctypes.byref(cnt))
+        return [out[i] for i in range(cnt.value)]
+
+    def sym_orbit(self, aidx):
+        """(canonical index, orbit transformation perm, coefficient, allowed)
+        of a block under the result symmetry."""
+        ac = ctypes.c_size_t()
+        perm = sizes([0] * self.n)
+        c = ctypes.c_double()
+        al = ctypes.c_int()
+        call("lt_btod_symmetrize2_sym_orbit", self._h, int(aidx), ctypes.byref(ac), perm,
+             ctypes.byref(c), ctypes.byref(al))
+        return ac.value, tuple(perm[i] for i in range(self.n)), c.value, bool(al.value)
+
+    def compute_block(self, aidx_b, blk, zero=True, perm=None, coeff=1.0):
+        """blk (=, or += unless zero) coeff * P(block aidx_b of the result) on the
+        device; blk is a device_array with the block's dims permuted by perm."""
+        p = None if perm is None else sizes(perm)
+        call("lt_btod_symmetrize2_compute_block", self._h, 1 if zero else 0, int(aidx_b), p,
+             float(coeff), blk.ptr)
 
 
 # ---------------------------------------------------------------------------

@@ -128,6 +128,16 @@ _SIGS = {
     "lt_btod_contract2_set_tile": [vp, ctypes.c_int],
     "lt_btod_contract2_set_graph": [vp, ctypes.c_int],
     "lt_btod_contract2_set_ozaki": [vp, ctypes.c_int],
+    "lt_btod_symmetrize2_create": [vp, c_size_p, ctypes.c_int, ctypes.POINTER(vp)],
+    "lt_btod_symmetrize2_create_nested": [vp, c_size_p, ctypes.c_int, ctypes.POINTER(vp)],
+    "lt_btod_symmetrize2_destroy": [vp],
+    "lt_btod_symmetrize2_perform": [vp, vp, ctypes.c_int],
+    "lt_btod_symmetrize2_perform_add": [vp, vp, ctypes.c_double, ctypes.c_int],
+    "lt_btod_symmetrize2_schedule": [vp, c_size_p, ctypes.c_size_t, c_size_p],
+    "lt_btod_symmetrize2_sym_orbit": [vp, ctypes.c_size_t, c_size_p, c_size_p,
+                                      c_double_p, c_int_p],
+    "lt_btod_symmetrize2_compute_block": [vp, ctypes.c_int, ctypes.c_size_t, c_size_p,
+                                          ctypes.c_double, vp],
     "lt_btod_add": [ctypes.c_size_t, vp, c_size_p, c_double_p, vp, ctypes.c_int,
                     ctypes.c_double],
     "lt_btod_copy": [vp, c_size_p, ctypes.c_double, vp, ctypes.c_int, ctypes.c_double],

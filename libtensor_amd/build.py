@@ -21,7 +21,7 @@ ARCH = "gfx950"
 
 HOST_SRC = ["host/lt_core.cpp", "host/lt_sym.cpp", "host/lt_sched.cpp",
             "host/lt_runtime.cpp", "host/lt_contract.cpp", "host/lt_dotprod.cpp",
-            "host/lt_blockops.cpp",
+            "host/lt_blockops.cpp", "host/lt_symmetrize.cpp",
             "host/lt_server.cpp", "host/lt_capi.cpp"]
 DEV_SRC = ["device/lt_kernels.hip", "device/lt_ozaki.hip"]
 

@@ -22,9 +22,6 @@
 
 namespace lta {
 
-namespace {
-
-// Same canonical block, transformation and allowedness for every block.
 bool same_orbits(const symmetry &x, const symmetry &y, const dims_t &bid) {
     for(size_t i = 0; i < bid.size(); i++) {
         orbit ox(x, i, true), oy(y, i, true);
@@ -33,8 +30,6 @@ bool same_orbits(const symmetry &x, const symmetry &y, const dims_t &bid) {
     }
     return true;
 }
-
-} // unnamed namespace
 
 void run_permutes(context &ctx, const std::vector<dev::perm_desc> &descs, double *ms) {
     if(descs.empty()) return;

@@ -19,6 +19,11 @@ struct lt_btensor { btensor *t; lt_ctx *ctx; };
 struct lt_symel { std::shared_ptr<sym_element> e; };
 struct lt_btod_contract2 { btod_contract2 *op; lt_btensor *a, *b; lt_ctx *ctx; };
 struct lt_server { server *s; lt_ctx *ctx; };
+struct lt_btod_symmetrize2 {
+    btod_symmetrize2 *s;
+    additive_op *inner;    //!< owned adapter of a btod_contract2 operand, or null
+    lt_ctx *ctx;
+};
 
 static void ctx_retain(lt_ctx *x) { x->refs++; }
 static void ctx_release(lt_ctx *x) {
@@ -801,6 +806,107 @@ int lt_btod_contract2_symc_orbit(lt_btod_contract2 *op, size_t aidx, size_t *aci
         if(acidx) *acidx = o.acidx;
         fill_transf(o.get_transf(aidx), s.bis.order(), perm, coeff);
         if(allowed) *allowed = o.allowed ? 1 : 0;
+    });
+}
+
+namespace {
+
+void symm_create(lt_ctx *ctx, additive_op *op, additive_op *owned, const size_t *perm,
+    int symm, lt_btod_symmetrize2 **out) {
+    const size_t n = op->sym().order();
+    lt_btod_symmetrize2 *h = new lt_btod_symmetrize2;
+    h->inner = owned;
+    h->ctx = ctx;
+    try {
+        h->s = new btod_symmetrize2(*op, perm_or_identity(n, perm), symm != 0);
+    } catch(...) {
+        delete owned;
+        delete h;
+        throw;
+    }
+    ctx_retain(ctx);
+    *out = h;
+}
+
+} // unnamed namespace
+
+int lt_btod_symmetrize2_create(lt_btod_contract2 *op, const size_t *perm, int symm,
+    lt_btod_symmetrize2 **out) {
+    return guard([&]() {
+        need(op, "op"); need(perm, "perm"); need(out, "out");
+        contract2_op *inner = new contract2_op(*op->op, *op->a->t);
+        symm_create(op->ctx, inner, inner, perm, symm, out);
+    });
+}
+
+int lt_btod_symmetrize2_create_nested(lt_btod_symmetrize2 *op, const size_t *perm, int symm,
+    lt_btod_symmetrize2 **out) {
+    return guard([&]() {
+        need(op, "op"); need(perm, "perm"); need(out, "out");
+        symm_create(op->ctx, op->s, 0, perm, symm, out);
+    });
+}
+
+int lt_btod_symmetrize2_destroy(lt_btod_symmetrize2 *s) {
+    return guard([&]() {
+        if(!s) return;
+        lt_ctx *ctx = s->ctx;
+        delete s->s;
+        delete s->inner;
+        delete s;
+        ctx_release(ctx);
+    });
+}
+
+int lt_btod_symmetrize2_perform(lt_btod_symmetrize2 *s, lt_btensor *b, int mode) {
+    return guard([&]() {
+        need(s, "op"); need(b, "b");
+        lt_require(mode >= 0 && mode <= 2, LT_ERR_PARAM, "mode");
+        s->s->perform(*b->t, mode);
+    });
+}
+
+int lt_btod_symmetrize2_perform_add(lt_btod_symmetrize2 *s, lt_btensor *b, double d,
+    int mode) {
+    return guard([&]() {
+        need(s, "op"); need(b, "b");
+        lt_require(mode >= 0 && mode <= 2, LT_ERR_PARAM, "mode");
+        s->s->perform_add(*b->t, d, mode);
+    });
+}
+
+int lt_btod_symmetrize2_schedule(lt_btod_symmetrize2 *s, size_t *out, size_t cap,
+    size_t *count) {
+    return guard([&]() {
+        need(s, "op"); need(count, "count");
+        const std::vector<size_t> &v = s->s->schedule();
+        *count = v.size();
+        if(out) for(size_t i = 0; i < v.size() && i < cap; i++) out[i] = v[i];
+    });
+}
+
+int lt_btod_symmetrize2_sym_orbit(lt_btod_symmetrize2 *s, size_t aidx, size_t *acidx,
+    size_t *perm, double *coeff, int *allowed) {
+    return guard([&]() {
+        need(s, "op");
+        const symmetry &y = s->s->sym();
+        lt_require(aidx < y.bis.block_index_dims().size(), LT_ERR_PARAM, "block index");
+        orbit o(y, aidx, true);
+        if(acidx) *acidx = o.acidx;
+        fill_transf(o.get_transf(aidx), y.bis.order(), perm, coeff);
+        if(allowed) *allowed = o.allowed ? 1 : 0;
+    });
+}
+
+int lt_btod_symmetrize2_compute_block(lt_btod_symmetrize2 *s, int zero, size_t aidx_b,
+    const size_t *perm, double coeff, double *blk) {
+    return guard([&]() {
+        need(s, "op"); need(blk, "blk");
+        const symmetry &y = s->s->sym();
+        const size_t n = y.bis.order();
+        lt_require(aidx_b < y.bis.block_index_dims().size(), LT_ERR_PARAM, "block index");
+        tensor_tr tr(perm_or_identity(n, perm), scalar_tr(coeff));
+        s->s->compute_block(zero != 0, get_index(aidx_b, y.bis.block_index_dims()), tr, blk);
     });
 }
 

@@ -171,6 +171,9 @@ void btod_copy(btensor &a, const tensor_tr &tra, btensor &b, bool add, double d)
 void btod_add(const std::vector<btensor*> &ops, const std::vector<tensor_tr> &trs,
     btensor &b, bool add, double d);
 
+//! Same canonical block, transformation and allowedness for every block.
+bool same_orbits(const symmetry &x, const symmetry &y, const dims_t &bid);
+
 //! Runs a list of permute descriptors as one launch of the batched permute
 //! kernel on the context's stream and waits for it; adds the launch's device
 //! time (HIP events) to *ms when ms is not null.
@@ -293,6 +296,90 @@ private:
     void run(btensor &c, int mode);
     void enqueue(int mode, hipStream_t s, bool timed, size_t &launches);
     void build_ozaki(int slices);
+};
+
+//! Operation whose result can be written into a block tensor or computed one
+//! block at a time (libtensor's additive_gen_bto, the operand of
+//! btod_symmetrize2: gen_block_tensor/additive_gen_bto.h:26-68).
+class additive_op {
+public:
+    virtual ~additive_op() { }
+    virtual context &ctx() = 0;
+    virtual const symmetry &sym() const = 0;
+    virtual const std::vector<size_t> &schedule() const = 0;
+    //! T := result (T zeroed, given the result symmetry); returns the
+    //! device time of the kernels in ms.
+    virtual double perform_into(btensor &t, int mode) = 0;
+    //! blk (=, or += unless zero) tr(block idx of the result) on the device.
+    virtual void compute_block(bool zero, const index_t &idx, const tensor_tr &tr,
+        double *blk) = 0;
+    virtual double flops() const = 0;
+};
+
+//! btod_contract2 as an additive_op.
+class contract2_op : public additive_op {
+public:
+    explicit contract2_op(btod_contract2 &op, btensor &a) : m_op(op), m_a(a) { }
+    context &ctx() { return m_a.ctx(); }
+    const symmetry &sym() const { return m_op.symc(); }
+    const std::vector<size_t> &schedule() const { return m_op.schedule(); }
+    double perform_into(btensor &t, int mode) {
+        m_op.perform(t, mode);
+        return m_op.stats().t_gpu_ms;
+    }
+    void compute_block(bool zero, const index_t &idx, const tensor_tr &tr, double *blk) {
+        m_op.compute_block(zero, idx, tr, blk);
+    }
+    double flops() const { return m_op.flops(); }
+private:
+    btod_contract2 &m_op;
+    btensor &m_a;
+};
+
+//! btod_symmetrize2<N> (libtensor/block_tensor/btod_symmetrize2.h:20-110;
+//! gen_block_tensor/impl/gen_bto_symmetrize2_impl.h): B = X + s P(X) for the
+//! result X of another operation and a pair permutation P, s = +1
+//! (symmetric) or -1 (antisymmetric).  The result symmetry and the schedule
+//! follow the reference; X is computed into a resident temporary and every
+//! block of B is gathered from it by the batched permute kernel.
+class btod_symmetrize2 : public additive_op {
+public:
+    btod_symmetrize2(additive_op &op, const perm_t &perm, bool symm);
+    ~btod_symmetrize2();
+
+    context &ctx() { return m_op.ctx(); }
+    const symmetry &sym() const { return m_sym; }
+    const std::vector<size_t> &schedule() const { return m_sched; }
+    double flops() const { return m_op.flops(); }
+
+    //! B := result (B zeroed, given the result symmetry).
+    void perform(btensor &b, int mode);
+    //! B += d * result for B with the symmetry of the result.
+    void perform_add(btensor &b, double d, int mode);
+    double perform_into(btensor &t, int mode);
+    //! gen_bto_symmetrize2::compute_block (impl :66-128) on the device.
+    void compute_block(bool zero, const index_t &ib, const tensor_tr &trb, double *blk);
+    //! Device time of the last perform: the operand's kernels and the
+    //! gather launches.
+    double last_ms() const { return m_last_ms; }
+
+private:
+    struct rec {
+        size_t ai;        //!< canonical block of X (in the operand's schedule)
+        tensor_tr tr;     //!< block of B = tr(X_ai)
+    };
+    additive_op &m_op;
+    perm_t m_perm;
+    bool m_symm;
+    symmetry m_sym;
+    std::vector<size_t> m_sched;
+    std::vector<std::pair<size_t, std::vector<rec> > > m_recs;  //!< by block of B
+    std::unique_ptr<btensor> m_x;
+    double m_last_ms;
+
+    void make_schedule();
+    const std::vector<rec> *records(size_t aib) const;
+    double run(btensor &b, bool add, double d, int mode);
 };
 
 } // namespace lta

@@ -4,6 +4,7 @@
 
 #include <algorithm>
 #include <list>
+#include <set>
 
 namespace lta {
 
@@ -322,7 +323,8 @@ se_label::se_label(const dims_t &bid, const std::string &tid) :
 
 se_label::se_label(const se_label &o) :
     sym_element(o), table_id(o.table_id), bidims(o.bidims), labels(o.labels),
-    rule(o.rule), src(o.src), keep(o.keep), rdims(o.rdims), rlabels(o.rlabels) {
+    rule(o.rule), src(o.src), keep(o.keep), rdims(o.rdims), rlabels(o.rlabels),
+    alts(o.alts) {
 }
 
 void se_label::set_label(size_t dim, size_t blk, size_t label) {
@@ -378,9 +380,21 @@ bool se_label::eval_rule(const size_t *lab) const {
     return false;
 }
 
-bool se_label::eval_labels(const size_t *lab) const {
+bool se_label::eval_one(const size_t *lab) const {
     if(src) return eval_reduced(lab);
     return eval_rule(lab);
+}
+
+bool se_label::eval_labels(const size_t *lab) const {
+    if(eval_one(lab)) return true;
+    // Permuted dimensions carry the same labels, so permuting the index
+    // permutes the label sequence.
+    for(size_t k = 0; k < alts.size(); k++) {
+        size_t l2[kMaxOrder];
+        for(size_t i = 0; i < bidims.n; i++) l2[i] = lab[alts[k].p[i]];
+        if(eval_one(l2)) return true;
+    }
+    return false;
 }
 
 bool se_label::eval_reduced(const size_t *own) const {
@@ -440,6 +454,13 @@ void se_label::permute(const perm_t &p) {
         for(size_t ip = 0; ip < rule.size(); ip++)
             for(size_t it = 0; it < rule[ip].size(); it++)
                 p.apply(rule[ip][it].seq);
+    }
+    // new index y = p(x): alternative q becomes p^-1, then q, then p
+    const perm_t pinv(p.inverse());
+    for(size_t k = 0; k < alts.size(); k++) {
+        perm_t q(pinv);
+        q.permute(alts[k]).permute(p);
+        alts[k] = q;
     }
     std::lock_guard<std::mutex> lk(m_memo_mtx);
     m_memo.clear();
@@ -1056,6 +1077,8 @@ void dirprod_label(size_t na, size_t nb, const perm_t &perm, const bis_t &bis3,
             for(size_t e = 0; e < s->size(); e++) {
                 const se_label &l = dynamic_cast<const se_label&>(*(*s)[e]);
                 lt_require(!l.src, LT_ERR_INTERNAL, "dirprod of a reduced label");
+                lt_require(l.alts.empty(), LT_ERR_SYMMETRY,
+                    "contraction of an operand with a symmetrized label element");
                 if(l.table_id != tables[t]) continue;
                 for(size_t i = 0; i < ns; i++) e3.labels[map[i]] = l.labels[i];
                 label_rule nr;
@@ -1477,6 +1500,216 @@ symmetry contract2_symmetry(const contraction2 &c, const symmetry &syma,
         else if(t == "label") reduce_label(rp, bisc, set, symc);
     }
     return symc;
+}
+
+// ---------------------------------------------------------------------------
+// Symmetrization over a pair permutation (so_symmetrize with two index
+// groups; libtensor/symmetry/inst/so_symmetrize_impl.h and the handlers
+// so_symmetrize_se_{perm,part,label}_impl.h).  With two groups the
+// reference's permutation generator visits the identity and the exchange of
+// the groups, which is P itself; the code below uses that directly.
+
+namespace {
+
+uint64_t perm_code(const perm_t &p) {
+    uint64_t c = 0;
+    for(size_t i = 0; i < p.n; i++) c = (c << 4) | p.p[i];
+    return c;
+}
+
+// se_perm handler (so_symmetrize_se_perm_impl.h:30-160): the group elements
+// (breadth-first from the distinct generators) that stay in the group after
+// conjugation by P, plus (P, s), as a new permutation group.
+void symmetrize_perm(size_t n, const std::vector<std::shared_ptr<sym_element> > *set,
+    const perm_t &pp, const scalar_tr &trp, symmetry &out) {
+
+    typedef permutation_group::gen_t gen_t;
+    if(!set || set->empty()) {
+        out.insert(se_perm(pp, trp));
+        return;
+    }
+    std::vector<gen_t> l0, l2;
+    std::list<gen_t> l1;
+    std::set<uint64_t> done;
+    done.insert(perm_code(perm_t(n)));
+    for(size_t e = 0; e < set->size(); e++) {
+        const se_perm &el = dynamic_cast<const se_perm&>(*(*set)[e]);
+        uint64_t c = perm_code(el.tr.perm);
+        if(done.count(c)) continue;
+        done.insert(c);
+        l0.push_back(gen_t(el.tr.perm, el.tr.st));
+        l1.push_back(l0.back());
+    }
+    while(!l1.empty()) {
+        const gen_t cur = l1.front();
+        for(size_t g = 0; g < l0.size(); g++) {
+            perm_t p(cur.first);
+            p.permute(l0[g].first);
+            uint64_t c = perm_code(p);
+            if(done.count(c)) continue;
+            done.insert(c);
+            scalar_tr tr(cur.second);
+            tr.transform(l0[g].second);
+            l1.push_back(gen_t(p, tr));
+        }
+        l2.push_back(cur);
+        l1.pop_front();
+    }
+    const perm_t pinv(pp.inverse());
+    scalar_tr trinv(trp);
+    trinv.invert();
+    permutation_group gx(n);
+    for(size_t g = 0; g < l0.size(); g++) {
+        perm_t pi(pinv);
+        pi.permute(l0[g].first).permute(pp);
+        scalar_tr ti(trinv);
+        ti.transform(l0[g].second).transform(trp);
+        gx.add_orbit(ti, pi);
+    }
+    permutation_group g2(n);
+    for(size_t i = 0; i < l2.size(); i++)
+        if(gx.is_member(l2[i].second, l2[i].first)) g2.add_orbit(l2[i].second, l2[i].first);
+    g2.add_orbit(trp, pp);
+    std::vector<gen_t> gs;
+    g2.convert(gs);
+    for(size_t i = 0; i < gs.size(); i++) out.insert(se_perm(gs[i].first, gs[i].second));
+}
+
+// Consistent map i1 -> i2 in the partition for both the index pair and its
+// image under P (so_symmetrize_se_part_impl.h, map_exists).
+bool part_map_both(const se_part &sp, const index_t &i1, const index_t &i2,
+    const perm_t &pp) {
+    index_t j1[2] = { i1, i1 }, j2[2] = { i2, i2 };
+    pp.apply(j1[1]);
+    pp.apply(j2[1]);
+    scalar_tr tr;
+    size_t k = 0;
+    for(; k < 2; k++) {
+        if(sp.map_exists(j1[k], j2[k])) {
+            tr = sp.get_transf(j1[k], j2[k]);
+            break;
+        }
+        if(!sp.is_forbidden(j1[k]) || !sp.is_forbidden(j2[k])) return false;
+    }
+    for(k++; k < 2; k++) {
+        if(sp.map_exists(j1[k], j2[k])) {
+            if(tr != sp.get_transf(j1[k], j2[k])) return false;
+        } else if(!sp.is_forbidden(j1[k]) || !sp.is_forbidden(j2[k])) {
+            return false;
+        }
+    }
+    return true;
+}
+
+// se_part handler (so_symmetrize_se_part_impl.h:20-75).  gmap lists the
+// paired dimensions the way the reference orders them: gmap[2k], gmap[2k+1]
+// are the k-th pair.
+void symmetrize_part(const std::vector<std::shared_ptr<sym_element> > &set,
+    const perm_t &pp, const std::vector<size_t> &gmap, symmetry &out) {
+
+    if(set.empty()) return;
+    se_part cp = combine_part(set);
+    const dims_t pdims = cp.pdims;
+    const size_t npair = gmap.size() / 2;
+    // The reference compares these entries of its map (for several pairs
+    // they are not the paired dimensions); kept as it is.
+    for(size_t j = 0; j < npair; j++)
+        if(pdims[gmap[npair + j]] != pdims[gmap[j]]) return;
+    const size_t n = pdims.n;
+    se_part sp2(cp.bis, pdims);
+    index_t i1(n);
+    do {
+        index_t p1(i1);
+        pp.apply(p1);
+        if(cp.is_forbidden(i1) && cp.is_forbidden(p1)) {
+            sp2.mark_forbidden(i1);
+            sp2.mark_forbidden(p1);
+            continue;
+        }
+        if(cp.is_forbidden(i1)) continue;
+        index_t i2 = cp.get_direct_map(i1);
+        bool found = false;
+        while(!found && i1 < i2) {
+            if(part_map_both(cp, i1, i2, pp)) found = true;
+            else i2 = cp.get_direct_map(i2);
+        }
+        if(found) {
+            scalar_tr tr = cp.get_transf(i1, i2);
+            index_t p2(i2);
+            pp.apply(p2);
+            sp2.add_map(i1, i2, tr);
+            sp2.add_map(p1, p2, tr);
+        }
+    } while(inc_index(i1, pdims));
+    out.insert(sp2);
+}
+
+// se_label handler (so_symmetrize_se_label_impl.h:20-95): the rule of every
+// element united with its copy permuted by P.
+void symmetrize_label(const std::vector<std::shared_ptr<sym_element> > &set,
+    const perm_t &pp, symmetry &out) {
+
+    for(size_t e = 0; e < set.size(); e++) {
+        const se_label &l1 = dynamic_cast<const se_label&>(*set[e]);
+        for(size_t i = 0; i < pp.n; i++) {
+            lt_require(l1.bidims[i] == l1.bidims[pp[i]] && l1.labels[i] == l1.labels[pp[i]],
+                LT_ERR_SYMMETRY, "so_symmetrize: incompatible dimensions");
+        }
+        se_label l2(l1);
+        std::vector<perm_t> alts(l1.alts);
+        alts.push_back(pp);
+        for(size_t k = 0; k < l1.alts.size(); k++) {
+            perm_t q(pp);
+            q.permute(l1.alts[k]);
+            alts.push_back(q);
+        }
+        l2.alts.clear();
+        for(size_t k = 0; k < alts.size(); k++) {
+            if(alts[k].is_identity()) continue;
+            bool seen = false;
+            for(size_t j = 0; j < l2.alts.size(); j++) if(l2.alts[j] == alts[k]) seen = true;
+            if(!seen) l2.alts.push_back(alts[k]);
+        }
+        out.insert(l2);
+    }
+}
+
+} // unnamed namespace
+
+symmetry so_symmetrize_pair(const symmetry &sym, const perm_t &perm, double s) {
+    const size_t n = sym.order();
+    perm_t p2(perm);
+    p2.permute(perm);
+    lt_require(perm.n == n && !perm.is_identity() && p2.is_identity(), LT_ERR_PARAM,
+        "symmetrize2: the permutation must be a non-trivial pair permutation");
+    bis_t pb(sym.bis);
+    pb.permute(perm);
+    lt_require(pb == sym.bis, LT_ERR_SYMMETRY,
+        "symmetrize2: the permutation does not map the block index space onto itself");
+    // Index groups as gen_bto_symmetrize2::make_symmetry numbers them: the
+    // k-th pair is (i, perm[i]) for the k-th i with perm[i] > i.
+    std::vector<size_t> gmap;
+    for(size_t i = 0; i < n; i++)
+        if(perm[i] > i) { gmap.push_back(i); gmap.push_back(perm[i]); }
+    const scalar_tr tr(s);
+    symmetry r(sym.bis);
+    bool perm_done = false;
+    for(size_t k = 0; k < sym.subsets.size(); k++) {
+        const std::string &t = sym.subsets[k].first;
+        const std::vector<std::shared_ptr<sym_element> > &set = sym.subsets[k].second;
+        if(t == "perm") {
+            perm_done = true;
+            symmetrize_perm(n, &set, perm, tr, r);
+        } else if(t == "part") {
+            symmetrize_part(set, perm, gmap, r);
+        } else if(t == "label") {
+            symmetrize_label(set, perm, r);
+        } else {
+            throw lt_error(LT_ERR_SYMMETRY, "so_symmetrize: unknown element type " + t);
+        }
+    }
+    if(!perm_done) symmetrize_perm(n, 0, perm, tr, r);
+    return r;
 }
 
 } // namespace lta

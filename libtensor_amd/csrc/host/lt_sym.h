@@ -142,6 +142,12 @@ struct se_label : public sym_element {
     std::vector<std::vector<size_t> > rdims;  //!< src dims of each reduction step
     std::vector<std::vector<size_t> > rlabels; //!< candidate labels per step
 
+    //! Symmetrized form: the element also allows a block whose index,
+    //! permuted by one of these permutations (idx'[i] = idx[q[i]]), the rule
+    //! allows (the union of the rule and its permuted copies that
+    //! so_symmetrize builds).
+    std::vector<perm_t> alts;
+
     se_label(const dims_t &bid, const std::string &tid);
     const char *type() const { return "label"; }
     sym_element *clone() const { return new se_label(*this); }
@@ -156,6 +162,7 @@ struct se_label : public sym_element {
     void permute(const perm_t &p);
 
 private:
+    bool eval_one(const size_t *lab) const;
     bool eval_rule(const size_t *lab) const;
     bool eval_reduced(const size_t *ownlab) const;
     mutable std::mutex m_memo_mtx;
@@ -248,6 +255,12 @@ private:
 
 //! Symmetry of P(tensor) (libtensor/symmetry/so_permute.h).
 symmetry so_permute(const symmetry &sym, const perm_t &p);
+
+//! Symmetry of X + s P(X) for a pair permutation P (P P = 1, P != 1):
+//! libtensor/symmetry/so_symmetrize.h with two index groups, as
+//! gen_bto_symmetrize2::make_symmetry calls it
+//! (gen_block_tensor/impl/gen_bto_symmetrize2_impl.h:131-149).
+symmetry so_symmetrize_pair(const symmetry &sym, const perm_t &perm, double s);
 
 //! Symmetry of the result of a contraction
 //! (libtensor/gen_block_tensor/impl/gen_bto_contract2_sym_impl.h:42-86):

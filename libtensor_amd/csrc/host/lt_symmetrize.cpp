@@ -1,0 +1,187 @@
+// btod_symmetrize2 on HBM-resident block tensors.
+//
+// The result symmetry is so_symmetrize of the operand's symmetry
+// (gen_block_tensor/impl/gen_bto_symmetrize2_impl.h:131-149) and the schedule
+// and per-block source lists follow make_schedule (:152-199): for every
+// scheduled block X_ai0 of the operand and every block j of its orbit, the
+// block j and its image P(j), where canonical in the new symmetry, receive
+// tr_j(X_ai0) and (P, s) tr_j(X_ai0).  perform() computes X once into a
+// resident temporary and gathers every block of B from it with the batched
+// permute kernel, one launch per source rank (the k-th source of every block
+// of B in launch k, so no two descriptors of a launch share a destination).
+#include "lt_runtime.h"
+
+#include <list>
+#include <map>
+#include <set>
+
+namespace lta {
+
+btod_symmetrize2::btod_symmetrize2(additive_op &op, const perm_t &perm, bool symm) :
+    m_op(op), m_perm(perm), m_symm(symm), m_last_ms(0.0) {
+
+    m_sym = so_symmetrize_pair(op.sym(), perm, symm ? 1.0 : -1.0);
+    make_schedule();
+}
+
+btod_symmetrize2::~btod_symmetrize2() { }
+
+void btod_symmetrize2::make_schedule() {
+    const symmetry &s0 = m_op.sym();
+    const dims_t bid = m_sym.bis.block_index_dims();
+    const double sc = m_symm ? 1.0 : -1.0;
+    std::map<size_t, std::vector<rec> > recs;
+    std::set<size_t> insch;
+    const std::vector<size_t> &sch0 = m_op.schedule();
+    for(size_t i = 0; i < sch0.size(); i++) {
+        const size_t ai0 = sch0[i];
+        orbit o(s0, ai0, true);
+        for(size_t j = 0; j < o.orb.size(); j++) {
+            const size_t aj1 = o.orb[j].first;
+            index_t j2 = get_index(aj1, bid);
+            m_perm.apply(j2);
+            const size_t aj2 = abs_index(j2, bid);
+            short_orbit so1(m_sym, aj1, false), so2(m_sym, aj2, false);
+            if(so1.acidx == aj1) {
+                if(insch.insert(aj1).second) m_sched.push_back(aj1);
+                rec r;
+                r.ai = ai0;
+                r.tr = o.get_transf(aj1);
+                recs[aj1].push_back(r);
+            }
+            if(so2.acidx == aj2) {
+                if(insch.insert(aj2).second) m_sched.push_back(aj2);
+                rec r;
+                r.ai = ai0;
+                r.tr = o.get_transf(aj1);
+                r.tr.perm.permute(m_perm);
+                r.tr.st.scale(sc);
+                recs[aj2].push_back(r);
+            }
+        }
+    }
+    m_recs.assign(recs.begin(), recs.end());
+}
+
+const std::vector<btod_symmetrize2::rec> *btod_symmetrize2::records(size_t aib) const {
+    std::vector<std::pair<size_t, std::vector<rec> > >::const_iterator it =
+        std::lower_bound(m_recs.begin(), m_recs.end(), aib,
+            [](const std::pair<size_t, std::vector<rec> > &x, size_t v) { return x.first < v; });
+    if(it == m_recs.end() || it->first != aib) return 0;
+    return &it->second;
+}
+
+double btod_symmetrize2::run(btensor &b, bool add, double d, int mode) {
+    context &cx = ctx();
+    cx.require_device("btod_symmetrize2");
+    lt_require(&b.ctx() == &cx, LT_ERR_PARAM, "tensors of different contexts");
+    lt_require(b.bis() == m_sym.bis, LT_ERR_DIMS, "btod_symmetrize2: block index space of B");
+    const dims_t bid = b.bidims();
+    if(add) {
+        lt_require(same_orbits(m_sym, b.sym(), bid), LT_ERR_SYMMETRY,
+            "btod_symmetrize2: B += ... needs B with the symmetry of the result");
+    }
+    if(!m_x) m_x.reset(new btensor(cx, m_sym.bis));
+    btensor &x = *m_x;
+    double ms = m_op.perform_into(x, mode);
+    if(!add) {
+        b.zero_all();
+        b.set_symmetry(m_sym);
+    }
+    std::vector<std::vector<dev::perm_desc> > rounds;
+    for(size_t i = 0; i < m_recs.size(); i++) {
+        const size_t ib = m_recs[i].first;
+        const std::vector<rec> &rs = m_recs[i].second;
+        const bool had = add && b.is_nonzero(ib);
+        size_t r = 0;
+        for(size_t k = 0; k < rs.size(); k++) {
+            if(!x.is_nonzero(rs[k].ai)) continue;
+            tensor_tr tr(rs[k].tr);
+            if(add) tr.st.scale(d);
+            if(rounds.size() <= r) rounds.resize(r + 1);
+            rounds[r].push_back(make_perm_desc_public(x.block_ptr(rs[k].ai),
+                x.bis().block_dims(get_index(rs[k].ai, bid)), tr, b.block_ptr(ib),
+                r > 0 || had));
+            r++;
+        }
+        if(r) b.mark_nonzero(ib);
+    }
+    for(size_t r = 0; r < rounds.size(); r++) run_permutes(cx, rounds[r], &ms);
+    m_last_ms = ms;
+    cx.set_last_op_ms(ms);
+    return ms;
+}
+
+void btod_symmetrize2::perform(btensor &b, int mode) {
+    run(b, false, 1.0, mode);
+}
+
+void btod_symmetrize2::perform_add(btensor &b, double d, int mode) {
+    run(b, true, d, mode);
+}
+
+double btod_symmetrize2::perform_into(btensor &t, int mode) {
+    return run(t, false, 1.0, mode);
+}
+
+void btod_symmetrize2::compute_block(bool zero, const index_t &ib, const tensor_tr &trb,
+    double *blk) {
+
+    // gen_bto_symmetrize2::compute_block (impl :66-128): sources grouped by
+    // operand block; a block used once is computed straight into blk, one
+    // used several times once into a temporary that is then added with each
+    // record's transformation.
+    context &cx = ctx();
+    cx.require_device("btod_symmetrize2::compute_block");
+    const dims_t bid = m_sym.bis.block_index_dims();
+    lt_require(ib.n == bid.n && trb.perm.n == bid.n, LT_ERR_PARAM, "block index order");
+    const std::vector<rec> *rp = records(abs_index(ib, bid));
+    if(!rp) {
+        // No source: the block is zero (the reference leaves blk untouched).
+        if(zero) {
+            dims_t db = m_sym.bis.block_dims(ib);
+            cx.activate();
+            hip_check(dev::launch_fill(blk, db.size(), 0.0, cx.stream()), "fill");
+            cx.sync();
+        }
+        return;
+    }
+    std::list<rec> l(rp->begin(), rp->end());
+    bool zero1 = zero;
+    while(!l.empty()) {
+        const size_t ai = l.front().ai;
+        size_t n = 0;
+        for(std::list<rec>::const_iterator j = l.begin(); j != l.end(); ++j)
+            if(j->ai == ai) n++;
+        tensor_tr tri(l.front().tr);
+        tri.transform(trb);
+        const index_t ia = get_index(ai, bid);
+        if(n == 1) {
+            m_op.compute_block(zero1, ia, tri, blk);
+            zero1 = false;
+            l.pop_front();
+            continue;
+        }
+        dims_t dt = m_sym.bis.block_dims(ia);
+        dt.permute(tri.perm);
+        double *tmp = (double*)cx.alloc(std::max<size_t>(dt.size(), 1) * sizeof(double));
+        try {
+            m_op.compute_block(true, ia, tri, tmp);
+            const tensor_tr tinv(tri.inverse());
+            for(std::list<rec>::iterator j = l.begin(); j != l.end();) {
+                if(j->ai != ai) { ++j; continue; }
+                tensor_tr trj(tinv);
+                trj.transform(j->tr).transform(trb);
+                dense_permute(cx, tmp, dt, trj, blk, !zero1);
+                zero1 = false;
+                j = l.erase(j);
+            }
+        } catch(...) {
+            cx.free(tmp);
+            throw;
+        }
+        cx.free(tmp);
+    }
+}
+
+} // namespace lta

@@ -10,10 +10,15 @@
 //                     same inputs as gen_bto_contract2_nzorb), timings
 //   <out>.result.bin  the result unfolded to a dense row-major array
 //                     (tod_btconv)
+// With symmetrize2 lines in the spec the contraction is wrapped in
+// btod_symmetrize2 (innermost first); <out>.txt then holds the symmetry of the
+// symmetrized result with the orbit transformation of every block ("symt"),
+// its schedule, and <out>.result.bin its result.
 // Options: --threads T (libutil::thread_pool of T workers), --repeat R
 // (timed performs after one warm-up), --no-symc, --no-clst, --no-result.
 #include <libtensor/libtensor.h>
 #include <libtensor/block_tensor/btod_contract2.h>
+#include <libtensor/block_tensor/btod_symmetrize2.h>
 #include <libtensor/block_tensor/btod_traits.h>
 #include <libtensor/core/orbit.h>
 #include <libtensor/core/orbit_list.h>
@@ -33,6 +38,7 @@
 #include <cstdio>
 #include <cstring>
 #include <sys/time.h>
+#include <type_traits>
 #include <vector>
 
 #include "spec_format.h"
@@ -190,6 +196,66 @@ void print_tr(FILE *f, const tensor_transf<N, double> &tr) {
     fprintf(f, " %.17g", tr.get_scalar_tr().get_coeff());
 }
 
+template<size_t NC>
+void print_symt(FILE *f, const symmetry<NC, double> &sym) {
+    dimensions<NC> bid = sym.get_bis().get_block_index_dims();
+    fprintf(f, "symt %zu\n", bid.get_size());
+    for(size_t a = 0; a < bid.get_size(); a++) {
+        orbit<NC, double> o(sym, a);
+        fprintf(f, "%zu %zu %d", a, o.get_acindex(), o.is_allowed() ? 1 : 0);
+        print_tr<NC>(f, o.get_transf(a));
+        fprintf(f, "\n");
+    }
+}
+
+// btod_symmetrize2 chain over the contraction (libtensor instantiates it for
+// orders 2..8 only).
+template<size_t NC>
+int run_symm(const spec::problem &p, additive_gen_bto<NC, btod_traits::bti_traits> &op0,
+    FILE *f, const std::string &out, const options &opt, std::true_type) {
+    std::vector<btod_symmetrize2<NC>*> chain;
+    additive_gen_bto<NC, btod_traits::bti_traits> *cur = &op0;
+    double t0 = wall_ms();
+    for(size_t i = 0; i < p.symm.size(); i++) {
+        chain.push_back(new btod_symmetrize2<NC>(*cur, make_perm<NC>(p.symm[i].perm),
+            p.symm[i].symm != 0));
+        cur = chain.back();
+    }
+    fprintf(f, "symmetrize_ms %.6f\n", wall_ms() - t0);
+    print_symt<NC>(f, cur->get_symmetry());
+    const assignment_schedule<NC, double> &sch = cur->get_schedule();
+    std::vector<size_t> sched;
+    for(typename assignment_schedule<NC, double>::iterator i = sch.begin(); i != sch.end(); ++i)
+        sched.push_back(sch.get_abs_index(i));
+    std::sort(sched.begin(), sched.end());
+    fprintf(f, "schedule %zu\n", sched.size());
+    for(size_t i = 0; i < sched.size(); i++) fprintf(f, "%zu\n", sched[i]);
+    block_tensor<NC, double, allocator<double> > btc(cur->get_bis());
+    t0 = wall_ms();
+    chain.back()->perform(btc);
+    fprintf(f, "perform_ms %.6f\n", wall_ms() - t0);
+    if(opt.result) {
+        dimensions<NC> dc = cur->get_bis().get_dims();
+        dense_tensor<NC, double, allocator<double> > t(dc);
+        tod_btconv<NC>(btc).perform(t);
+        dense_tensor_rd_ctrl<NC, double> ctl(t);
+        const double *pd = ctl.req_const_dataptr();
+        FILE *g = fopen((out + ".result.bin").c_str(), "wb");
+        fwrite(pd, 8, dc.get_size(), g);
+        fclose(g);
+        ctl.ret_const_dataptr(pd);
+    }
+    for(size_t i = chain.size(); i > 0; i--) delete chain[i - 1];
+    return 0;
+}
+
+template<size_t NC>
+int run_symm(const spec::problem &, additive_gen_bto<NC, btod_traits::bti_traits> &,
+    FILE *, const std::string &, const options &, std::false_type) {
+    fprintf(stderr, "symmetrize2 needs a result of order 2 or more\n");
+    return 3;
+}
+
 template<size_t N, size_t M, size_t K>
 int run(const spec::problem &p, const std::string &out, const options &opt) {
     enum { NA = N + K, NB = M + K, NC = N + M };
@@ -216,6 +282,13 @@ int run(const spec::problem &p, const std::string &out, const options &opt) {
     btod_contract2<N, M, K> op(contr, bta, p.sa, btb, p.sb, p.sc);
     double t_sched = wall_ms() - t0;
     fprintf(f, "schedule_ms %.6f\n", t_sched);
+    if(!p.symm.empty()) {
+        int rc = run_symm<NC>(p, op, f, out, opt,
+            std::integral_constant<bool, (NC >= 2)>());
+        fclose(f);
+        delete btbp;
+        return rc;
+    }
 
     const symmetry<NC, double> &symc = op.get_symmetry();
     dimensions<NC> bidc = symc.get_bis().get_block_index_dims();

@@ -81,6 +81,8 @@ def write_spec(prob, path, dense=None, tensors=None):
         c["N"], c["M"], c["K"], " ".join(str(p) for p in permc),
         " ".join("%d %d" % (a, b) for a, b in c["pairs"]),
         c.get("ka", 1.0), c.get("kb", 1.0), c.get("kc", 1.0)))
+    for perm, symm in prob.get("symm", []):
+        lines.append("symmetrize2 %s %d" % (" ".join(str(x) for x in perm), 1 if symm else 0))
     if prob.get("self"):
         lines.append("self 1")
     with open(path, "w") as f:
@@ -88,7 +90,7 @@ def write_spec(prob, path, dense=None, tensors=None):
 
 
 def parse_output(prefix, result_shape=None):
-    out = {"symc": {}, "schedule": [], "clst": {}}
+    out = {"symc": {}, "symt": {}, "schedule": [], "clst": {}}
     with open(prefix + ".txt") as f:
         lines = f.read().split("\n")
     i = 0
@@ -102,6 +104,13 @@ def parse_output(prefix, result_shape=None):
             for k in range(n):
                 a, ac, al = lines[i + k].split()
                 out["symc"][int(a)] = (int(ac), int(al))
+            i += n
+        elif l[0] == "symt":
+            n = int(l[1])
+            for k in range(n):
+                r = lines[i + k].split()
+                out["symt"][int(r[0])] = (int(r[1]), int(r[2]), [int(x) for x in r[3:-1]],
+                                          float(r[-1]))
             i += n
         elif l[0] == "schedule":
             n = int(l[1])

@@ -21,6 +21,9 @@
 //                                        u64 nelem, nelem doubles
 //   contract <N> <M> <K> <permc x N+M> <ka kb x K> <ka> <kb> <kc>
 //   self 1                               B is the same tensor as A
+//   symmetrize2 <perm x N+M> <symm 0|1>  wrap the contraction in
+//                                        btod_symmetrize2 (repeatable: each
+//                                        line wraps the previous operation)
 #pragma once
 
 #include <cstdio>
@@ -82,6 +85,8 @@ struct problem {
     std::vector<size_t> permc, ka, kb;
     double sa, sb, sc;
     bool self;
+    struct symm2 { std::vector<size_t> perm; int symm; };
+    std::vector<symm2> symm;        // in file order (innermost first)
     problem() : N(0), M(0), K(0), sa(1), sb(1), sc(1), self(false) { }
 };
 
@@ -154,6 +159,9 @@ inline problem parse(const std::string &path) {
             p.permc = read_sz(in, p.N + p.M);
             for(size_t i = 0; i < p.K; i++) { size_t a, b; in >> a >> b; p.ka.push_back(a); p.kb.push_back(b); }
             in >> p.sa >> p.sb >> p.sc;
+        } else if(kw == "symmetrize2") {
+            problem::symm2 y; y.perm = read_sz(in, p.N + p.M); in >> y.symm;
+            p.symm.push_back(y);
         } else if(kw == "self") {
             int s; in >> s; p.self = s != 0;
         } else {

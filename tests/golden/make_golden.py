@@ -9,7 +9,10 @@ coalesced contraction list of every scheduled block as printed by libtensor;
 
 The inputs are generated with libtensor_amd's block_tensor.random on a
 host-only context (no GPU needed).  Run:  python tests/golden/make_golden.py
+(--symm-only: only the btod_symmetrize2 fixtures symm_*, from the committed
+inputs of their base cases).
 """
+import json
 import os
 import random
 import shutil
@@ -19,9 +22,11 @@ import tempfile
 HERE = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, os.path.dirname(HERE))
 sys.path.insert(0, os.path.dirname(os.path.dirname(HERE)))
+sys.path.insert(0, os.path.join(os.path.dirname(os.path.dirname(HERE)), "oracle"))
 
 import problems as P  # noqa: E402
 import libtensor_amd as lt  # noqa: E402
+import numpy as np  # noqa: E402
 
 
 def random_problem(rng, N, M, K, sym=False):
@@ -116,10 +121,61 @@ def cases():
     return out
 
 
+# btod_symmetrize2 over the contraction of a golden case (innermost first):
+# (name, base case, [(pair permutation, symmetric)], result stored).  The
+# inputs are those of the base case.  Cases libtensor rejects (a permutation
+# that does not map the block index space onto itself, or a sign that
+# contradicts the operand's symmetry) are stored with "error": true.
+SYMM_CASES = [
+    ("symm_ring_ijab", "ring_small", [([1, 0, 3, 2], True)]),
+    ("symm_ring_ab_ij", "ring_small", [([0, 1, 3, 2], False), ([1, 0, 2, 3], False)]),
+    ("symm_ladder_full_ijab", "ladder_full", [([1, 0, 3, 2], True)]),
+    ("symm_ladder_full_ab", "ladder_full", [([0, 1, 3, 2], False)]),
+    ("symm_ladder_full_ab_ij", "ladder_full", [([0, 1, 3, 2], False), ([1, 0, 2, 3], False)]),
+    ("symm_ladder_label_ijab", "ladder_label", [([1, 0, 3, 2], True)]),
+    ("symm_ladder_nosym_ij", "ladder_nosym", [([1, 0, 2, 3], False)]),
+    ("symm_rand_221", "rand_221", [([0, 1, 3, 2], True)]),
+    ("symm_rand_311", "rand_311", [([3, 1, 2, 0], False)]),
+    ("symm_randsym_211", "randsym_211", [([2, 1, 0], False)]),
+    ("symm_self_ladder", "self_ladder", [([2, 1, 0, 3], True)]),
+    ("symm_err_sign", "ladder_full", [([0, 1, 3, 2], True)]),
+    ("symm_err_bis", "rand_222", [([2, 1, 0, 3], False)]),
+]
+
+
+def write_symm(work):
+    """Fixtures symm_*.json (+ .npz with libtensor's result)."""
+    for name, base, symm in SYMM_CASES:
+        prob, dense, meta, result = P.load_case(os.path.join(HERE, base))
+        prob = dict(prob)
+        prob["symm"] = symm
+        out = {"base": base, "symm": symm}
+        try:
+            ref = P.run_ref(prob, dense, os.path.join(work, name))
+        except RuntimeError:
+            out["error"] = True
+            ref = None
+        if ref is not None:
+            out["schedule"] = ref["schedule"]
+            out["symt"] = {str(k): list(v) for k, v in ref["symt"].items()}
+            np.savez_compressed(os.path.join(HERE, name + ".npz"), result=ref["result"])
+        with open(os.path.join(HERE, name + ".json"), "w") as f:
+            json.dump(out, f)
+        print("%-26s %s" % (name, "error" if ref is None else
+                            "schedule %d" % len(ref["schedule"])))
+
+
 def main():
     if not P.ref_available():
         print("oracle/_ref/ref_driver is not built (make -C oracle ref)")
         return 1
+    if "--symm-only" in sys.argv:
+        work = tempfile.mkdtemp(prefix="lt_golden_")
+        try:
+            write_symm(work)
+        finally:
+            shutil.rmtree(work, ignore_errors=True)
+        return 0
     ctx = lt.Context(-1)
     work = tempfile.mkdtemp(prefix="lt_golden_")
     try:
@@ -128,6 +184,7 @@ def main():
             ref = P.run_ref(prob, dense, os.path.join(work, name))
             P.save_case(os.path.join(HERE, name), prob, dense, ref)
             print("%-22s schedule %4d" % (name, len(ref["schedule"])))
+        write_symm(work)
     finally:
         shutil.rmtree(work, ignore_errors=True)
     return 0
