@@ -118,9 +118,11 @@ def serving_mix(ctx, lt, W, n_clients, rounds, rate, seed=1005):
 
 def block_ops(ctx, lt, prob, a, b, op, c, steps=10):
     """btod_copy of the two-electron tensor V (every canonical block, one
-    batched permute launch; HBM-bound: 16 B per element) and compute_block of
-    the largest scheduled result block (direct tensors), timed on the host
-    clock around synchronous calls."""
+    batched permute launch; HBM-bound: 16 B per element), compute_block of
+    the largest scheduled result block (direct tensors), and btod_symmetrize2
+    of the contraction over (ij)(ab) (the contraction, then one gather launch
+    per source rank; gather HBM-bound), timed on the host clock around
+    synchronous calls and with HIP events around the kernels."""
     perm = (1, 0, 2, 3)
     td = prob["B"]
     bis = lt.block_index_space([td["dims"][p] for p in perm])
@@ -147,13 +149,34 @@ def block_ops(ctx, lt, prob, a, b, op, c, steps=10):
         op.compute_block(big, out)
     cb_ms = (time.perf_counter() - t0) * 1e3 / steps
     out.free()
+    # symm(i|j, a|b, contract(...)): B = R + P(R), P = (ij)(ab) (symmetric)
+    from libtensor_amd import workloads as W
+    sym = lt.btod_symmetrize2(op, (1, 0, 3, 2), True)
+    bs = W.result_tensor(ctx, prob)
+    sym.perform(bs)
+    sv = []
+    t0 = time.perf_counter()
+    for i in range(steps):
+        sym.perform(bs)
+        sv.append(sym.stats())
+    sy_ms = (time.perf_counter() - t0) * 1e3 / steps
+    g_ms = statistics.mean(x["gather_ms"] for x in sv)
+    g_gbs = sv[-1]["gather_bytes"] / (g_ms * 1e-3) / 1e9
+    symres = {"perm": [1, 0, 3, 2], "symmetric": True, "blocks": len(sym.get_schedule()),
+              "gpu_ms": statistics.mean(x["gpu_ms"] for x in sv), "gather_ms": g_ms,
+              "gather_bytes": sv[-1]["gather_bytes"], "gather_GB_per_s": g_gbs,
+              "gather_hbm_frac": g_gbs / 8000.0, "call_ms": sy_ms,
+              "note": "gpu_ms: contraction (grouped) + gather launches, HIP events; "
+                      "call_ms: host clock around the synchronous perform"}
+    del sym
     res = {"btod_copy": {"elements": elems, "kernel_ms": kms, "GB_per_s": gbs,
                          "hbm_frac": gbs / 8000.0, "call_ms": ms, "perm": list(perm),
                          "note": "kernel_ms: HIP events around the permute launch; call_ms: "
                                  "host clock around the synchronous call (orbits, "
                                  "descriptors, upload, launch)"},
            "compute_block": {"block": int(big), "dims": list(dims),
-                             "pairs": len(op.clst(big)), "ms": cb_ms}}
+                             "pairs": len(op.clst(big)), "ms": cb_ms},
+           "btod_symmetrize2": symres}
     log("block ops: %s" % json.dumps(res))
     return res
 

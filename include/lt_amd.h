@@ -80,8 +80,8 @@ int lt_ctx_sync(lt_ctx *ctx);
 int lt_ctx_mem(lt_ctx *ctx, size_t *used, size_t *cap);
 int lt_ctx_set_threads(lt_ctx *ctx, size_t nthreads);
 /* Device time (HIP events) of the kernels of the last block operation
- * (lt_btod_copy, lt_btod_add) on this context.  No counterpart in the
- * reference (its timings<> report host wall time). */
+ * (lt_btod_copy, lt_btod_add, lt_btod_symmetrize2_perform) on this context.
+ * No counterpart in the reference (its timings<> report host wall time). */
 int lt_ctx_last_op_ms(lt_ctx *ctx, double *ms);
 
 /* Device memory for the dense-tensor entry points. */
@@ -290,6 +290,13 @@ int lt_btod_symmetrize2_sym_orbit(lt_btod_symmetrize2 *op, size_t aidx, size_t *
     size_t *perm, double *coeff, int *allowed);
 int lt_btod_symmetrize2_compute_block(lt_btod_symmetrize2 *op, int zero, size_t aidx_b,
     const size_t *perm, double coeff, double *blk);
+/* Last perform: device time in ms (operand and gather kernels), gather
+ * kernels alone, bytes the gather moved (each source block read, each
+ * destination written, and also read where added to), and the operand's
+ * FLOPs.  Any pointer may be NULL.  No counterpart in the reference (its
+ * timings go to libutil's timings store). */
+int lt_btod_symmetrize2_stats(lt_btod_symmetrize2 *op, double *gpu_ms, double *gather_ms,
+    double *gather_bytes, double *flops);
 
 /* ---- request loop: many calculations sharing one GPU.  No counterpart in
  *      the reference (its operations run synchronously in the caller's

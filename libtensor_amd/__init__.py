@@ -452,6 +452,14 @@ class btod_symmetrize2(object):
              ctypes.byref(c), ctypes.byref(al))
         return ac.value, tuple(perm[i] for i in range(self.n)), c.value, bool(al.value)
 
+    def stats(self):
+        """Last perform: device ms (operand + gather), gather ms, gather bytes,
+        and the operand's FLOPs."""
+        v = [ctypes.c_double() for _ in range(4)]
+        call("lt_btod_symmetrize2_stats", self._h, *[ctypes.byref(x) for x in v])
+        return {"gpu_ms": v[0].value, "gather_ms": v[1].value, "gather_bytes": v[2].value,
+                "flops": v[3].value}
+
     def compute_block(self, aidx_b, blk, zero=True, perm=None, coeff=1.0):
         """blk (=, or += unless zero) coeff * P(block aidx_b of the result) on the
         device; blk is a device_array with the block's dims permuted by perm."""

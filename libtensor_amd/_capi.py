@@ -136,6 +136,7 @@ _SIGS = {
     "lt_btod_symmetrize2_schedule": [vp, c_size_p, ctypes.c_size_t, c_size_p],
     "lt_btod_symmetrize2_sym_orbit": [vp, ctypes.c_size_t, c_size_p, c_size_p,
                                       c_double_p, c_int_p],
+    "lt_btod_symmetrize2_stats": [vp, c_double_p, c_double_p, c_double_p, c_double_p],
     "lt_btod_symmetrize2_compute_block": [vp, ctypes.c_int, ctypes.c_size_t, c_size_p,
                                           ctypes.c_double, vp],
     "lt_btod_add": [ctypes.c_size_t, vp, c_size_p, c_double_p, vp, ctypes.c_int,

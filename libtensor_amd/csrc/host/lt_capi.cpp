@@ -898,6 +898,17 @@ int lt_btod_symmetrize2_sym_orbit(lt_btod_symmetrize2 *s, size_t aidx, size_t *a
     });
 }
 
+int lt_btod_symmetrize2_stats(lt_btod_symmetrize2 *s, double *gpu_ms, double *gather_ms,
+    double *gather_bytes, double *flops) {
+    return guard([&]() {
+        need(s, "op");
+        if(gpu_ms) *gpu_ms = s->s->last_ms();
+        if(gather_ms) *gather_ms = s->s->last_gather_ms();
+        if(gather_bytes) *gather_bytes = s->s->last_gather_bytes();
+        if(flops) *flops = s->s->flops();
+    });
+}
+
 int lt_btod_symmetrize2_compute_block(lt_btod_symmetrize2 *s, int zero, size_t aidx_b,
     const size_t *perm, double coeff, double *blk) {
     return guard([&]() {

@@ -50,7 +50,7 @@ public:
     void set_nthreads(size_t n) { m_nthreads = n ? n : 1; }
 
     //! Device time (HIP events) of the last block operation's kernels
-    //! (btod_copy, btod_add) on this context.
+    //! (btod_copy, btod_add, btod_symmetrize2) on this context.
     double last_op_ms() const { return m_last_op_ms; }
     void set_last_op_ms(double ms) { m_last_op_ms = ms; }
 
@@ -359,9 +359,12 @@ public:
     double perform_into(btensor &t, int mode);
     //! gen_bto_symmetrize2::compute_block (impl :66-128) on the device.
     void compute_block(bool zero, const index_t &ib, const tensor_tr &trb, double *blk);
-    //! Device time of the last perform: the operand's kernels and the
-    //! gather launches.
+    //! Device time of the last perform (the operand's kernels and the
+    //! gather launches), of the gather launches alone, and the bytes the
+    //! gather moved (source read, destination written, and read when added to).
     double last_ms() const { return m_last_ms; }
+    double last_gather_ms() const { return m_gather_ms; }
+    double last_gather_bytes() const { return m_gather_bytes; }
 
 private:
     struct rec {
@@ -375,7 +378,7 @@ private:
     std::vector<size_t> m_sched;
     std::vector<std::pair<size_t, std::vector<rec> > > m_recs;  //!< by block of B
     std::unique_ptr<btensor> m_x;
-    double m_last_ms;
+    double m_last_ms, m_gather_ms, m_gather_bytes;
 
     void make_schedule();
     const std::vector<rec> *records(size_t aib) const;

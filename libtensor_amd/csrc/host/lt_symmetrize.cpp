@@ -18,7 +18,8 @@
 namespace lta {
 
 btod_symmetrize2::btod_symmetrize2(additive_op &op, const perm_t &perm, bool symm) :
-    m_op(op), m_perm(perm), m_symm(symm), m_last_ms(0.0) {
+    m_op(op), m_perm(perm), m_symm(symm), m_last_ms(0.0), m_gather_ms(0.0),
+    m_gather_bytes(0.0) {
 
     m_sym = so_symmetrize_pair(op.sym(), perm, symm ? 1.0 : -1.0);
     make_schedule();
@@ -89,6 +90,7 @@ double btod_symmetrize2::run(btensor &b, bool add, double d, int mode) {
         b.set_symmetry(m_sym);
     }
     std::vector<std::vector<dev::perm_desc> > rounds;
+    double bytes = 0.0;
     for(size_t i = 0; i < m_recs.size(); i++) {
         const size_t ib = m_recs[i].first;
         const std::vector<rec> &rs = m_recs[i].second;
@@ -99,14 +101,20 @@ double btod_symmetrize2::run(btensor &b, bool add, double d, int mode) {
             tensor_tr tr(rs[k].tr);
             if(add) tr.st.scale(d);
             if(rounds.size() <= r) rounds.resize(r + 1);
-            rounds[r].push_back(make_perm_desc_public(x.block_ptr(rs[k].ai),
-                x.bis().block_dims(get_index(rs[k].ai, bid)), tr, b.block_ptr(ib),
-                r > 0 || had));
+            const dims_t dx = x.bis().block_dims(get_index(rs[k].ai, bid));
+            const bool acc = r > 0 || had;
+            rounds[r].push_back(make_perm_desc_public(x.block_ptr(rs[k].ai), dx, tr,
+                b.block_ptr(ib), acc));
+            bytes += 8.0 * dx.size() * (acc ? 3 : 2);
             r++;
         }
         if(r) b.mark_nonzero(ib);
     }
-    for(size_t r = 0; r < rounds.size(); r++) run_permutes(cx, rounds[r], &ms);
+    double gms = 0.0;
+    for(size_t r = 0; r < rounds.size(); r++) run_permutes(cx, rounds[r], &gms);
+    ms += gms;
+    m_gather_ms = gms;
+    m_gather_bytes = bytes;
     m_last_ms = ms;
     cx.set_last_op_ms(ms);
     return ms;

@@ -42,6 +42,8 @@ def test_perform(gpu_ctx, name, mode):
     assert np.max(np.abs(c.to_dense() - result)) <= TOL * scale
     nz = set(c.req_nonzero_blocks())
     assert nz <= set(meta["schedule"])
+    st = s.stats()
+    assert st["gather_bytes"] > 0 and st["gpu_ms"] >= st["gather_ms"] > 0
     s.perform_add(c, -0.5, mode)
     assert np.max(np.abs(c.to_dense() - 0.5 * result)) <= TOL * scale
     # a second perform reuses the operand's plan and the temporary
