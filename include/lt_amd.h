@@ -42,6 +42,7 @@ typedef struct lt_btensor lt_btensor;
 typedef struct lt_symel lt_symel;
 typedef struct lt_btod_contract2 lt_btod_contract2;
 typedef struct lt_btod_symmetrize2 lt_btod_symmetrize2;
+typedef struct lt_btod_contract3 lt_btod_contract3;
 typedef struct lt_server lt_server;
 
 typedef struct lt_stats {
@@ -262,10 +263,34 @@ int lt_btod_copy(lt_btensor *a, const size_t *perm, double c, lt_btensor *b, int
 int lt_btod_add(size_t nops, lt_btensor *const *a, const size_t *perms, const double *c,
     lt_btensor *b, int add, double d);
 
+/* ---- btod_contract3 (libtensor/block_tensor/btod_contract3.h:30-150;
+ *      gen_block_tensor/impl/gen_bto_contract3_impl.h) ----
+ * D = kd (A B) C: contr1 contracts A (order N1+K1) with B (order N2+K1+K2)
+ * into AB (order N1+N2+K2, index order permab, pairs ka1[i]-kb1[i]); contr2
+ * contracts AB with C (order N3+K2) into D (order N1+N2+N3, index order
+ * permd, pairs kab2[i]-kc2[i]), each as for lt_btod_contract2_create.  The
+ * result symmetry and schedule are the reference's (the second
+ * contraction's over AB's schedule).  AB is kept in a resident temporary
+ * owned by the operation.  perform: D := result; perform_add: D += k *
+ * result for D with the result symmetry; stats: device ms of the last
+ * perform (both contractions) and the FLOPs of both. */
+int lt_btod_contract3_create(size_t N1, size_t N2, size_t N3, size_t K1, size_t K2,
+    const size_t *permab, const size_t *ka1, const size_t *kb1, const size_t *permd,
+    const size_t *kab2, const size_t *kc2, lt_btensor *a, lt_btensor *b, lt_btensor *c,
+    double kd, lt_btod_contract3 **out);
+int lt_btod_contract3_destroy(lt_btod_contract3 *op);
+int lt_btod_contract3_perform(lt_btod_contract3 *op, lt_btensor *d, int mode);
+int lt_btod_contract3_perform_add(lt_btod_contract3 *op, lt_btensor *d, double k, int mode);
+int lt_btod_contract3_schedule(lt_btod_contract3 *op, size_t *out, size_t cap,
+    size_t *count);
+int lt_btod_contract3_sym_orbit(lt_btod_contract3 *op, size_t aidx, size_t *acidx,
+    size_t *perm, double *coeff, int *allowed);
+int lt_btod_contract3_stats(lt_btod_contract3 *op, double *gpu_ms, double *flops);
+
 /* ---- btod_symmetrize2 (libtensor/block_tensor/btod_symmetrize2.h:20-110;
  *      gen_block_tensor/impl/gen_bto_symmetrize2_impl.h) ----
- * B = X + s P(X), X the result of a btod_contract2 (create) or of another
- * btod_symmetrize2 (create_nested), P a pair permutation (P P = 1, P not the
+ * B = X + s P(X), X the result of a btod_contract2 (create), of another
+ * btod_symmetrize2 (create_nested) or of a btod_contract3 (create_c3), P a pair permutation (P P = 1, P not the
  * identity; btod_symmetrize2(op, i1, i2, symm) is the transposition of i1
  * and i2), s = +1 (symm != 0) or -1.  The result symmetry is libtensor's
  * so_symmetrize of the symmetry of X (sym_orbit queries it), the schedule
@@ -279,6 +304,10 @@ int lt_btod_add(size_t nops, lt_btensor *const *a, const size_t *perms, const do
 int lt_btod_symmetrize2_create(lt_btod_contract2 *op, const size_t *perm, int symm,
     lt_btod_symmetrize2 **out);
 int lt_btod_symmetrize2_create_nested(lt_btod_symmetrize2 *op, const size_t *perm, int symm,
+    lt_btod_symmetrize2 **out);
+/* Operand a btod_contract3 (libtensor's symmetrization of any additive
+ * operation; btod_symcontract3 is this with the symmetrized pair inside). */
+int lt_btod_symmetrize2_create_c3(lt_btod_contract3 *op, const size_t *perm, int symm,
     lt_btod_symmetrize2 **out);
 int lt_btod_symmetrize2_destroy(lt_btod_symmetrize2 *op);
 int lt_btod_symmetrize2_perform(lt_btod_symmetrize2 *op, lt_btensor *b, int mode);

@@ -16,7 +16,7 @@ from ._capi import (GEMM_CONFIGS, GEMM_STACKED_FIRST, LT_MODE_GROUPED, LT_MODE_O
 
 __all__ = ["Context", "product_table", "block_index_space", "block_tensor",
            "se_perm", "se_part", "se_label", "contraction2", "btod_contract2",
-           "device_array", "tod_copy", "tod_add", "tod_set", "tod_contract2", "tod_dotprod", "btod_dotprod", "btod_copy", "btod_add", "btod_symmetrize2", "server",
+           "device_array", "tod_copy", "tod_add", "tod_set", "tod_contract2", "tod_dotprod", "btod_dotprod", "btod_copy", "btod_add", "btod_symmetrize2", "btod_contract3", "server",
            "LtError", "GEMM_CONFIGS", "GEMM_STACKED_FIRST", "LT_MODE_PER_BLOCK", "LT_MODE_GROUPED", "LT_MODE_OZAKI"]
 
 
@@ -396,6 +396,66 @@ class btod_contract2(object):
         call("lt_btod_contract2_set_ozaki", self._h, int(slices))
 
 
+class btod_contract3(object):
+    """D = kd (A B) C (libtensor/block_tensor/btod_contract3.h): contr1 (a
+    contraction2 with N = N1, M = N2 + K2, K = K1) contracts A with B into the
+    intermediate AB, contr2 (N = N1 + N2, M = N3, K = K2) contracts AB with C."""
+
+    def __init__(self, contr1, contr2, bta, btb, btc, kd=1.0):
+        for ct in (contr1, contr2):
+            if len(ct.pairs) != ct.K:
+                raise LtError(-1, "contraction is incomplete")
+        n1, k1 = contr1.N, contr1.K
+        n3, k2 = contr2.M, contr2.K
+        n2 = contr1.M - k2
+        if n2 < 0 or contr2.N != n1 + n2:
+            raise LtError(-1, "contractions do not chain")
+        h = ctypes.c_void_p()
+        call("lt_btod_contract3_create", n1, n2, n3, k1, k2, sizes(contr1.permc),
+             sizes([p[0] for p in contr1.pairs]), sizes([p[1] for p in contr1.pairs]),
+             sizes(contr2.permc), sizes([p[0] for p in contr2.pairs]),
+             sizes([p[1] for p in contr2.pairs]), bta._h, btb._h, btc._h, float(kd),
+             ctypes.byref(h))
+        self._h = h
+        self.n = n1 + n2 + n3
+        self.a, self.b, self.c = bta, btb, btc
+
+    def __del__(self):
+        try:
+            if self._h:
+                call("lt_btod_contract3_destroy", self._h)
+                self._h = None
+        except Exception:
+            pass
+
+    def perform(self, btd, mode=LT_MODE_GROUPED):
+        call("lt_btod_contract3_perform", self._h, btd._h, int(mode))
+
+    def perform_add(self, btd, k=1.0, mode=LT_MODE_GROUPED):
+        call("lt_btod_contract3_perform_add", self._h, btd._h, float(k), int(mode))
+
+    def get_schedule(self):
+        cnt = ctypes.c_size_t()
+        call("lt_btod_contract3_schedule", self._h, None, 0, ctypes.byref(cnt))
+        out = (ctypes.c_size_t * max(1, cnt.value))()
+        call("lt_btod_contract3_schedule", self._h, out, cnt.value, ctypes.byref(cnt))
+        return [out[i] for i in range(cnt.value)]
+
+    def sym_orbit(self, aidx):
+        ac = ctypes.c_size_t()
+        perm = sizes([0] * self.n)
+        c = ctypes.c_double()
+        al = ctypes.c_int()
+        call("lt_btod_contract3_sym_orbit", self._h, int(aidx), ctypes.byref(ac), perm,
+             ctypes.byref(c), ctypes.byref(al))
+        return ac.value, tuple(perm[i] for i in range(self.n)), c.value, bool(al.value)
+
+    def stats(self):
+        v = [ctypes.c_double() for _ in range(2)]
+        call("lt_btod_contract3_stats", self._h, *[ctypes.byref(x) for x in v])
+        return {"gpu_ms": v[0].value, "flops": v[1].value}
+
+
 class btod_symmetrize2(object):
     """B = X + s P(X) for the result X of a btod_contract2 or of another
     btod_symmetrize2 (libtensor/block_tensor/btod_symmetrize2.h).  perm is a
@@ -406,6 +466,9 @@ class btod_symmetrize2(object):
         h = ctypes.c_void_p()
         if isinstance(op, btod_symmetrize2):
             call("lt_btod_symmetrize2_create_nested", op._h, sizes(perm), 1 if symm else 0,
+                 ctypes.byref(h))
+        elif isinstance(op, btod_contract3):
+            call("lt_btod_symmetrize2_create_c3", op._h, sizes(perm), 1 if symm else 0,
                  ctypes.byref(h))
         else:
             call("lt_btod_symmetrize2_create", op._h, sizes(perm), 1 if symm else 0,

@@ -128,6 +128,17 @@ _SIGS = {
     "lt_btod_contract2_set_tile": [vp, ctypes.c_int],
     "lt_btod_contract2_set_graph": [vp, ctypes.c_int],
     "lt_btod_contract2_set_ozaki": [vp, ctypes.c_int],
+    "lt_btod_contract3_create": [ctypes.c_size_t] * 5 + [c_size_p] * 6 + [vp, vp, vp,
+                                                                    ctypes.c_double,
+                                                                    ctypes.POINTER(vp)],
+    "lt_btod_contract3_destroy": [vp],
+    "lt_btod_contract3_perform": [vp, vp, ctypes.c_int],
+    "lt_btod_contract3_perform_add": [vp, vp, ctypes.c_double, ctypes.c_int],
+    "lt_btod_contract3_schedule": [vp, c_size_p, ctypes.c_size_t, c_size_p],
+    "lt_btod_contract3_sym_orbit": [vp, ctypes.c_size_t, c_size_p, c_size_p, c_double_p,
+                                    c_int_p],
+    "lt_btod_contract3_stats": [vp, c_double_p, c_double_p],
+    "lt_btod_symmetrize2_create_c3": [vp, c_size_p, ctypes.c_int, ctypes.POINTER(vp)],
     "lt_btod_symmetrize2_create": [vp, c_size_p, ctypes.c_int, ctypes.POINTER(vp)],
     "lt_btod_symmetrize2_create_nested": [vp, c_size_p, ctypes.c_int, ctypes.POINTER(vp)],
     "lt_btod_symmetrize2_destroy": [vp],

@@ -19,6 +19,7 @@ struct lt_btensor { btensor *t; lt_ctx *ctx; };
 struct lt_symel { std::shared_ptr<sym_element> e; };
 struct lt_btod_contract2 { btod_contract2 *op; lt_btensor *a, *b; lt_ctx *ctx; };
 struct lt_server { server *s; lt_ctx *ctx; };
+struct lt_btod_contract3 { btod_contract3 *op; lt_ctx *ctx; };
 struct lt_btod_symmetrize2 {
     btod_symmetrize2 *s;
     additive_op *inner;    //!< owned adapter of a btod_contract2 operand, or null
@@ -836,6 +837,97 @@ int lt_btod_symmetrize2_create(lt_btod_contract2 *op, const size_t *perm, int sy
         need(op, "op"); need(perm, "perm"); need(out, "out");
         contract2_op *inner = new contract2_op(*op->op, *op->a->t);
         symm_create(op->ctx, inner, inner, perm, symm, out);
+    });
+}
+
+int lt_btod_symmetrize2_create_c3(lt_btod_contract3 *op, const size_t *perm, int symm,
+    lt_btod_symmetrize2 **out) {
+    return guard([&]() {
+        need(op, "op"); need(perm, "perm"); need(out, "out");
+        symm_create(op->ctx, op->op, 0, perm, symm, out);
+    });
+}
+
+int lt_btod_contract3_create(size_t N1, size_t N2, size_t N3, size_t K1, size_t K2,
+    const size_t *permab, const size_t *ka1, const size_t *kb1, const size_t *permd,
+    const size_t *kab2, const size_t *kc2, lt_btensor *a, lt_btensor *b, lt_btensor *c,
+    double kd, lt_btod_contract3 **out) {
+    return guard([&]() {
+        need(a, "a"); need(b, "b"); need(c, "c"); need(out, "out");
+        if(K1) { need(ka1, "ka1"); need(kb1, "kb1"); }
+        if(K2) { need(kab2, "kab2"); need(kc2, "kc2"); }
+        lt_require(&a->t->ctx() == &b->t->ctx() && &a->t->ctx() == &c->t->ctx(), LT_ERR_PARAM,
+            "tensors of different contexts");
+        lt_require(a->t->bis().order() == N1 + K1 && b->t->bis().order() == N2 + K1 + K2 &&
+            c->t->bis().order() == N3 + K2, LT_ERR_DIMS, "btod_contract3: tensor orders");
+        contraction2 c1 = make_contraction(N1, N2 + K2, K1, permab, ka1, kb1);
+        contraction2 c2 = make_contraction(N1 + N2, N3, K2, permd, kab2, kc2);
+        lt_btod_contract3 *h = new lt_btod_contract3;
+        h->ctx = a->ctx;
+        try {
+            h->op = new btod_contract3(c1, c2, *a->t, *b->t, *c->t, kd);
+        } catch(...) {
+            delete h;
+            throw;
+        }
+        ctx_retain(h->ctx);
+        *out = h;
+    });
+}
+
+int lt_btod_contract3_destroy(lt_btod_contract3 *op) {
+    return guard([&]() {
+        if(!op) return;
+        lt_ctx *ctx = op->ctx;
+        delete op->op;
+        delete op;
+        ctx_release(ctx);
+    });
+}
+
+int lt_btod_contract3_perform(lt_btod_contract3 *op, lt_btensor *d, int mode) {
+    return guard([&]() {
+        need(op, "op"); need(d, "d");
+        lt_require(mode >= 0 && mode <= 2, LT_ERR_PARAM, "mode");
+        op->op->perform(*d->t, mode);
+    });
+}
+
+int lt_btod_contract3_perform_add(lt_btod_contract3 *op, lt_btensor *d, double k, int mode) {
+    return guard([&]() {
+        need(op, "op"); need(d, "d");
+        lt_require(mode >= 0 && mode <= 2, LT_ERR_PARAM, "mode");
+        op->op->perform_add(*d->t, k, mode);
+    });
+}
+
+int lt_btod_contract3_schedule(lt_btod_contract3 *op, size_t *out, size_t cap, size_t *count) {
+    return guard([&]() {
+        need(op, "op"); need(count, "count");
+        const std::vector<size_t> &v = op->op->schedule();
+        *count = v.size();
+        if(out) for(size_t i = 0; i < v.size() && i < cap; i++) out[i] = v[i];
+    });
+}
+
+int lt_btod_contract3_sym_orbit(lt_btod_contract3 *op, size_t aidx, size_t *acidx,
+    size_t *perm, double *coeff, int *allowed) {
+    return guard([&]() {
+        need(op, "op");
+        const symmetry &y = op->op->sym();
+        lt_require(aidx < y.bis.block_index_dims().size(), LT_ERR_PARAM, "block index");
+        orbit o(y, aidx, true);
+        if(acidx) *acidx = o.acidx;
+        fill_transf(o.get_transf(aidx), y.bis.order(), perm, coeff);
+        if(allowed) *allowed = o.allowed ? 1 : 0;
+    });
+}
+
+int lt_btod_contract3_stats(lt_btod_contract3 *op, double *gpu_ms, double *flops) {
+    return guard([&]() {
+        need(op, "op");
+        if(gpu_ms) *gpu_ms = op->op->last_ms();
+        if(flops) *flops = op->op->flops();
     });
 }
 

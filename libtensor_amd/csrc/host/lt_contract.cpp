@@ -18,6 +18,8 @@
 //      permuted into the canonical blocks of C.
 #include "lt_runtime.h"
 
+#include <atomic>
+
 #include <map>
 #include <tuple>
 
@@ -260,9 +262,16 @@ hipError_t launch_grouped(const contract_plan &pl, int k, hipStream_t s) {
 
 // ---------------------------------------------------------------------------
 
+namespace {
+uint64_t next_op_uid() {
+    static std::atomic<uint64_t> next(1);
+    return next++;
+}
+} // unnamed namespace
+
 btod_contract2::btod_contract2(const contraction2 &c, btensor &a, double ka,
     btensor &b, double kb, double kc) :
-    m_contr(c), m_a(a), m_b(b), m_ka(ka), m_kb(kb), m_kc(kc),
+    m_contr(c), m_a(a), m_b(b), m_ka(ka), m_kb(kb), m_kc(kc), m_uid(next_op_uid()),
     m_tile_cfg(-2), m_tuned_cfg(-1), m_tuned_order(-1), m_tuned_va(0), m_tuned_vb(0), m_use_graph(true), m_oz_slices(0), m_oz_int8(1) {
 
     lt_require(c.is_complete(), LT_ERR_PARAM, "contraction is incomplete");
@@ -1390,8 +1399,15 @@ void btod_contract2::run(btensor &c, int mode) {
     ctx.activate();
     hipStream_t s = ctx.stream();
     if(!pl.add) {
-        c.zero_all();
-        c.set_symmetry(m_symc);
+        // A result tensor this operation wrote last keeps its structure, and
+        // so its version (plans of operations that read it stay valid):
+        // every block of c_out is overwritten below.
+        std::vector<size_t> co(pl.c_out);
+        std::sort(co.begin(), co.end());
+        if(c.sym_tag() != m_uid || c.nonzero_blocks() != co) {
+            c.zero_all();
+            c.set_symmetry(m_symc, m_uid);
+        }
     }
     if(pl.empty) {
         m_stats.t_gpu_ms = m_stats.t_gemm_ms = 0.0;

@@ -184,6 +184,7 @@ btensor::btensor(context &ctx, const bis_t &bis) :
 
 symmetry &btensor::sym_rw() {
     touch();
+    m_sym_tag = 0;
     return m_sym;
 }
 
@@ -193,12 +194,14 @@ void btensor::split(const bool *mask, size_t pos) {
     for(size_t i = 0; i < m_sym.bis.order(); i++)
         if(mask[i]) m_sym.bis.split(i, pos);
     m_bidims = m_sym.bis.block_index_dims();
+    m_sym_tag = 0;
     touch();
 }
 
-void btensor::set_symmetry(const symmetry &s) {
+void btensor::set_symmetry(const symmetry &s, uint64_t tag) {
     lt_require(s.bis == m_sym.bis, LT_ERR_SYMMETRY, "symmetry of a different block index space");
     m_sym = s;
+    m_sym_tag = tag;
     touch();
 }
 

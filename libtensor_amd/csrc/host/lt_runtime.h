@@ -124,7 +124,11 @@ public:
     uint64_t uid() const { return m_uid; }
 
     void split(const bool *mask, size_t pos);
-    void set_symmetry(const symmetry &s);
+    //! Sets the symmetry; tag names the operation that set it (0: the caller).
+    void set_symmetry(const symmetry &s, uint64_t tag = 0);
+    //! The operation that set the symmetry last (0: the caller, or changed
+    //! since through sym_rw / split).
+    uint64_t sym_tag() const { return m_sym_tag; }
 
     //! Fills all canonical allowed blocks with random data symmetrized over
     //! each block's stabilizer (btod_random semantics).
@@ -138,6 +142,7 @@ private:
     std::map<size_t, double*> m_store;
     std::vector<size_t> m_nonzero;   // sorted
     uint64_t m_version, m_uid;
+    uint64_t m_sym_tag = 0;
     static std::atomic<uint64_t> s_version;
 };
 
@@ -280,6 +285,7 @@ private:
     contraction2 m_contr;
     btensor &m_a, &m_b;
     double m_ka, m_kb, m_kc;
+    uint64_t m_uid;         //!< tags the symmetry of the result tensors it writes
     symmetry m_symc;
     std::vector<size_t> m_sched;
     exec_stats m_stats;
@@ -334,6 +340,44 @@ public:
 private:
     btod_contract2 &m_op;
     btensor &m_a;
+};
+
+//! btod_contract3<N1, N2, N3, K1, K2> (libtensor/block_tensor/btod_contract3.h;
+//! gen_block_tensor/impl/gen_bto_contract3_impl.h): D = kd (A B) C, the
+//! intermediate AB = contr1(A, B) and D = contr2(AB, C).  The symmetry and
+//! schedule of AB and D are those of the two contractions
+//! (gen_bto_contract3::make_schedule, :501-526: the second contraction's
+//! non-zero orbits over AB's schedule).  AB is computed whole into a resident
+//! temporary instead of in batches of B.
+class btod_contract3 : public additive_op {
+public:
+    btod_contract3(const contraction2 &c1, const contraction2 &c2, btensor &a, btensor &b,
+        btensor &c, double kd);
+    ~btod_contract3();
+
+    context &ctx() { return m_a.ctx(); }
+    const symmetry &sym() const { return m_op2->symc(); }
+    const std::vector<size_t> &schedule() const { return m_op2->schedule(); }
+    double flops() const { return m_op1->flops() + m_op2->flops(); }
+
+    //! D := result (D zeroed, given the result symmetry).
+    void perform(btensor &d, int mode);
+    //! D += k * result for D with the symmetry of the result.
+    void perform_add(btensor &d, double k, int mode);
+    double perform_into(btensor &t, int mode);
+    //! One block of D on the device (AB computed first; the reference
+    //! leaves this unimplemented).
+    void compute_block(bool zero, const index_t &id, const tensor_tr &tr, double *blk);
+    double last_ms() const { return m_last_ms; }
+    btod_contract2 &first() { return *m_op1; }
+    btod_contract2 &second() { return *m_op2; }
+
+private:
+    btensor &m_a;
+    std::unique_ptr<btod_contract2> m_op1;
+    std::unique_ptr<btensor> m_ab;
+    std::unique_ptr<btod_contract2> m_op2;
+    double m_last_ms;
 };
 
 //! btod_symmetrize2<N> (libtensor/block_tensor/btod_symmetrize2.h:20-110;

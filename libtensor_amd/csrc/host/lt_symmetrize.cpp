@@ -1,4 +1,4 @@
-// btod_symmetrize2 on HBM-resident block tensors.
+// btod_symmetrize2 and btod_contract3 on HBM-resident block tensors.
 //
 // The result symmetry is so_symmetrize of the operand's symmetry
 // (gen_block_tensor/impl/gen_bto_symmetrize2_impl.h:131-149) and the schedule
@@ -16,6 +16,45 @@
 #include <set>
 
 namespace lta {
+
+btod_contract3::btod_contract3(const contraction2 &c1, const contraction2 &c2, btensor &a,
+    btensor &b, btensor &c, double kd) : m_a(a), m_last_ms(0.0) {
+
+    m_op1.reset(new btod_contract2(c1, a, 1.0, b, 1.0, 1.0));
+    // AB carries its symmetry and the first contraction's schedule as its
+    // non-zero blocks, so the second contraction's schedule is built over
+    // that list as the reference builds it.
+    m_ab.reset(new btensor(a.ctx(), m_op1->symc().bis));
+    m_ab->set_symmetry(m_op1->symc());
+    const std::vector<size_t> &sab = m_op1->schedule();
+    for(size_t i = 0; i < sab.size(); i++) m_ab->mark_nonzero(sab[i]);
+    m_op2.reset(new btod_contract2(c2, *m_ab, 1.0, c, 1.0, kd));
+}
+
+btod_contract3::~btod_contract3() { }
+
+void btod_contract3::perform(btensor &d, int mode) {
+    m_op1->perform(*m_ab, mode);
+    m_op2->perform(d, mode);
+    m_last_ms = m_op1->stats().t_gpu_ms + m_op2->stats().t_gpu_ms;
+}
+
+void btod_contract3::perform_add(btensor &d, double k, int mode) {
+    m_op1->perform(*m_ab, mode);
+    m_op2->perform_add(d, k, mode);
+    m_last_ms = m_op1->stats().t_gpu_ms + m_op2->stats().t_gpu_ms;
+}
+
+double btod_contract3::perform_into(btensor &t, int mode) {
+    perform(t, mode);
+    return m_last_ms;
+}
+
+void btod_contract3::compute_block(bool zero, const index_t &id, const tensor_tr &tr,
+    double *blk) {
+    m_op1->perform(*m_ab, MODE_GROUPED);
+    m_op2->compute_block(zero, id, tr, blk);
+}
 
 btod_symmetrize2::btod_symmetrize2(additive_op &op, const perm_t &perm, bool symm) :
     m_op(op), m_perm(perm), m_symm(symm), m_last_ms(0.0), m_gather_ms(0.0),

@@ -10,6 +10,8 @@
 //                     same inputs as gen_bto_contract2_nzorb), timings
 //   <out>.result.bin  the result unfolded to a dense row-major array
 //                     (tod_btconv)
+// With a contract3 line the problem is btod_contract3 of A, B and C (same
+// outputs as with symmetrize2).
 // With symmetrize2 lines in the spec the contraction is wrapped in
 // btod_symmetrize2 (innermost first); <out>.txt then holds the symmetry of the
 // symmetrized result with the orbit transformation of every block ("symt"),
@@ -19,6 +21,7 @@
 #include <libtensor/libtensor.h>
 #include <libtensor/block_tensor/btod_contract2.h>
 #include <libtensor/block_tensor/btod_symmetrize2.h>
+#include <libtensor/block_tensor/btod_contract3.h>
 #include <libtensor/block_tensor/btod_traits.h>
 #include <libtensor/core/orbit.h>
 #include <libtensor/core/orbit_list.h>
@@ -384,6 +387,54 @@ int run(const spec::problem &p, const std::string &out, const options &opt) {
     return 0;
 }
 
+// btod_contract3<N1, N2, N3, K1, K2> of A, B and C: the result symmetry with
+// every orbit transformation ("symt"), the schedule and the dense result.
+template<size_t N1, size_t N2, size_t N3, size_t K1, size_t K2>
+int run3(const spec::problem &p, const std::string &out, const options &opt) {
+    enum { NA = N1 + K1, NB = N2 + K1 + K2, NC = N3 + K2, NAB = N1 + N2 + K2,
+           ND = N1 + N2 + N3 };
+    block_tensor<NA, double, allocator<double> > bta(make_bis<NA>(p.a));
+    fill_tensor<NA>(p.a, bta);
+    block_tensor<NB, double, allocator<double> > btb(make_bis<NB>(p.b));
+    fill_tensor<NB>(p.b, btb);
+    block_tensor<NC, double, allocator<double> > btc(make_bis<NC>(p.c));
+    fill_tensor<NC>(p.c, btc);
+    contraction2<N1, N2 + K2, K1> contr1(make_perm<NAB>(p.permab));
+    for(size_t i = 0; i < K1; i++) contr1.contract(p.ka1[i], p.kb1[i]);
+    contraction2<N1 + N2, N3, K2> contr2(make_perm<ND>(p.permd));
+    for(size_t i = 0; i < K2; i++) contr2.contract(p.kab2[i], p.kc2[i]);
+
+    FILE *f = fopen((out + ".txt").c_str(), "w");
+    double t0 = wall_ms();
+    btod_contract3<N1, N2, N3, K1, K2> op(contr1, contr2, bta, btb, btc, p.kd);
+    fprintf(f, "schedule_ms %.6f\n", wall_ms() - t0);
+    print_symt<ND>(f, op.get_symmetry());
+    const assignment_schedule<ND, double> &sch = op.get_schedule();
+    std::vector<size_t> sched;
+    for(typename assignment_schedule<ND, double>::iterator i = sch.begin(); i != sch.end(); ++i)
+        sched.push_back(sch.get_abs_index(i));
+    std::sort(sched.begin(), sched.end());
+    fprintf(f, "schedule %zu\n", sched.size());
+    for(size_t i = 0; i < sched.size(); i++) fprintf(f, "%zu\n", sched[i]);
+    block_tensor<ND, double, allocator<double> > btd(op.get_bis());
+    t0 = wall_ms();
+    op.perform(btd);
+    fprintf(f, "perform_ms %.6f\n", wall_ms() - t0);
+    if(opt.result) {
+        dimensions<ND> dd = op.get_bis().get_dims();
+        dense_tensor<ND, double, allocator<double> > t(dd);
+        tod_btconv<ND>(btd).perform(t);
+        dense_tensor_rd_ctrl<ND, double> ctl(t);
+        const double *pd = ctl.req_const_dataptr();
+        FILE *g = fopen((out + ".result.bin").c_str(), "wb");
+        fwrite(pd, 8, dd.get_size(), g);
+        fclose(g);
+        ctl.ret_const_dataptr(pd);
+    }
+    fclose(f);
+    return 0;
+}
+
 typedef int (*runner)(const spec::problem &, const std::string &, const options &);
 
 struct entry { size_t n, m, k; runner r; };
@@ -397,6 +448,16 @@ const entry k_table[] = {
     E(1, 1, 3), E(2, 2, 3), E(1, 3, 3)
 };
 #undef E
+
+struct entry3 { size_t n1, n2, n3, k1, k2; runner r; };
+
+// The orders libtensor instantiates (block_tensor/impl/btod_contract3.C).
+#define E3(a, b, c, d, e) { a, b, c, d, e, &run3<a, b, c, d, e> }
+const entry3 k_table3[] = {
+    E3(1, 0, 1, 1, 1), E3(1, 1, 1, 1, 1), E3(1, 1, 2, 0, 0), E3(1, 1, 2, 1, 2),
+    E3(2, 0, 0, 0, 2), E3(2, 0, 1, 1, 2), E3(2, 0, 2, 1, 2)
+};
+#undef E3
 
 } // unnamed namespace
 
@@ -432,6 +493,15 @@ int main(int argc, char **argv) {
         }
         allocator<double>::init(4, 16, 16777216, 16777216);
         batching_policy_base::set_batch_size(size_t(1) << 40);
+        if(p.c3) {
+            for(size_t i = 0; i < sizeof(k_table3) / sizeof(k_table3[0]); i++) {
+                const entry3 &e = k_table3[i];
+                if(e.n1 == p.N1 && e.n2 == p.N2 && e.n3 == p.N3 && e.k1 == p.K1 &&
+                    e.k2 == p.K2) return e.r(p, argv[2], opt);
+            }
+            fprintf(stderr, "unsupported contract3 orders\n");
+            return 3;
+        }
         for(size_t i = 0; i < sizeof(k_table) / sizeof(k_table[0]); i++) {
             const entry &e = k_table[i];
             if(e.n == p.N && e.m == p.M && e.k == p.K) return e.r(p, argv[2], opt);

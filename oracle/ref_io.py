@@ -39,8 +39,10 @@ def write_spec(prob, path, dense=None, tensors=None):
         lines.append("table %s %d" % (tid, n))
         for l1, l2, lr in prods:
             lines.append("product %s %d %d %d" % (tid, l1, l2, lr))
-    for name in ("A", "B"):
+    for name in ("A", "B", "C"):
         if name == "B" and prob.get("self"):
+            continue
+        if name == "C" and "C" not in prob:
             continue
         td = prob[name]
         n = len(td["dims"])
@@ -75,12 +77,21 @@ def write_spec(prob, path, dense=None, tensors=None):
             f = os.path.join(d, "blocks_%s.bin" % name)
             write_blocks(tensors[name], f)
             lines.append("blocks %s %s" % (name, f))
-    c = prob["contr"]
-    permc = c.get("permc") or list(range(c["N"] + c["M"]))
-    lines.append("contract %d %d %d %s %s %.17g %.17g %.17g" % (
-        c["N"], c["M"], c["K"], " ".join(str(p) for p in permc),
-        " ".join("%d %d" % (a, b) for a, b in c["pairs"]),
-        c.get("ka", 1.0), c.get("kb", 1.0), c.get("kc", 1.0)))
+    if "contr3" in prob:
+        c = prob["contr3"]
+        lines.append("contract3 %d %d %d %d %d %s %s %s %s %.17g" % (
+            c["N1"], c["N2"], c["N3"], c["K1"], c["K2"],
+            " ".join(str(p) for p in c["permab"]),
+            " ".join("%d %d" % (a, b) for a, b in c["pairs1"]),
+            " ".join(str(p) for p in c["permd"]),
+            " ".join("%d %d" % (a, b) for a, b in c["pairs2"]), c.get("kd", 1.0)))
+    else:
+        c = prob["contr"]
+        permc = c.get("permc") or list(range(c["N"] + c["M"]))
+        lines.append("contract %d %d %d %s %s %.17g %.17g %.17g" % (
+            c["N"], c["M"], c["K"], " ".join(str(p) for p in permc),
+            " ".join("%d %d" % (a, b) for a, b in c["pairs"]),
+            c.get("ka", 1.0), c.get("kb", 1.0), c.get("kc", 1.0)))
     for perm, symm in prob.get("symm", []):
         lines.append("symmetrize2 %s %d" % (" ".join(str(x) for x in perm), 1 if symm else 0))
     if prob.get("self"):

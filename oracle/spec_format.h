@@ -21,6 +21,10 @@
 //                                        u64 nelem, nelem doubles
 //   contract <N> <M> <K> <permc x N+M> <ka kb x K> <ka> <kb> <kc>
 //   self 1                               B is the same tensor as A
+//   contract3 <N1> <N2> <N3> <K1> <K2> <permab x N1+N2+K2> <a b x K1>
+//             <permd x N1+N2+N3> <ab c x K2> <kd>
+//                                        btod_contract3 of A, B and C
+//                                        (tensor C declared like A and B)
 //   symmetrize2 <perm x N+M> <symm 0|1>  wrap the contraction in
 //                                        btod_symmetrize2 (repeatable: each
 //                                        line wraps the previous operation)
@@ -85,9 +89,15 @@ struct problem {
     std::vector<size_t> permc, ka, kb;
     double sa, sb, sc;
     bool self;
+    tensor c;                       // third operand of contract3
+    bool c3;                        // contract3 instead of contract
+    size_t N1, N2, N3, K1, K2;
+    std::vector<size_t> permab, permd, ka1, kb1, kab2, kc2;
+    double kd;
     struct symm2 { std::vector<size_t> perm; int symm; };
     std::vector<symm2> symm;        // in file order (innermost first)
-    problem() : N(0), M(0), K(0), sa(1), sb(1), sc(1), self(false) { }
+    problem() : N(0), M(0), K(0), sa(1), sb(1), sc(1), self(false), c3(false),
+        N1(0), N2(0), N3(0), K1(0), K2(0), kd(1) { }
 };
 
 inline std::vector<size_t> read_sz(std::istringstream &in, size_t n) {
@@ -123,7 +133,7 @@ inline problem parse(const std::string &path) {
         } else if(kw == "tensor" || kw == "split" || kw == "perm" || kw == "part" ||
             kw == "label" || kw == "dense" || kw == "blocks") {
             std::string name; in >> name;
-            tensor &t = (name == "A") ? p.a : p.b;
+            tensor &t = (name == "A") ? p.a : (name == "C") ? p.c : p.b;
             if(kw == "tensor") {
                 in >> t.n; t.dims = read_sz(in, t.n);
             } else if(kw == "split") {
@@ -159,6 +169,14 @@ inline problem parse(const std::string &path) {
             p.permc = read_sz(in, p.N + p.M);
             for(size_t i = 0; i < p.K; i++) { size_t a, b; in >> a >> b; p.ka.push_back(a); p.kb.push_back(b); }
             in >> p.sa >> p.sb >> p.sc;
+        } else if(kw == "contract3") {
+            p.c3 = true;
+            in >> p.N1 >> p.N2 >> p.N3 >> p.K1 >> p.K2;
+            p.permab = read_sz(in, p.N1 + p.N2 + p.K2);
+            for(size_t i = 0; i < p.K1; i++) { size_t a, b; in >> a >> b; p.ka1.push_back(a); p.kb1.push_back(b); }
+            p.permd = read_sz(in, p.N1 + p.N2 + p.N3);
+            for(size_t i = 0; i < p.K2; i++) { size_t a, b; in >> a >> b; p.kab2.push_back(a); p.kc2.push_back(b); }
+            in >> p.kd;
         } else if(kw == "symmetrize2") {
             problem::symm2 y; y.perm = read_sz(in, p.N + p.M); in >> y.symm;
             p.symm.push_back(y);

@@ -10,7 +10,7 @@ coalesced contraction list of every scheduled block as printed by libtensor;
 The inputs are generated with libtensor_amd's block_tensor.random on a
 host-only context (no GPU needed).  Run:  python tests/golden/make_golden.py
 (--symm-only: only the btod_symmetrize2 fixtures symm_*, from the committed
-inputs of their base cases).
+inputs of their base cases; --c3-only: only the btod_contract3 fixtures c3_*).
 """
 import json
 import os
@@ -143,6 +143,95 @@ SYMM_CASES = [
 ]
 
 
+def random_problem3(rng, N1, N2, N3, K1, K2, sym=False):
+    """btod_contract3 problem with random extents, splits and index orders.
+    sym: the two K2 indexes (and the two N3 indexes when N3 == 2) get equal
+    extents and splits, and B and C antisymmetry in them."""
+    letters = iter("abcdefghijklmnopqrstuvw")
+    o1 = [next(letters) for _ in range(N1)]
+    o2 = [next(letters) for _ in range(N2)]
+    o3 = [next(letters) for _ in range(N3)]
+    k1 = [next(letters) for _ in range(K1)]
+    k2 = [next(letters) for _ in range(K2)]
+    size, split = {}, {}
+    for i in o1 + o2 + o3 + k1 + k2:
+        size[i] = rng.choice([3, 4, 5, 6])
+        split[i] = rng.randint(1, size[i] - 1) if size[i] >= 4 else None
+    if sym:
+        for grp in (k2, o3):
+            if len(grp) == 2:
+                size[grp[1]], split[grp[1]] = size[grp[0]], split[grp[0]]
+    ia = o1 + k1
+    ib = o2 + k1 + k2
+    ic = o3 + k2
+    for x in (ia, ib, ic):
+        rng.shuffle(x)
+    rem1 = [i for i in ia if i not in k1] + [i for i in ib if i not in k1]
+    permab = list(range(len(rem1)))
+    rng.shuffle(permab)
+    iab = [rem1[permab[i]] for i in range(len(rem1))]
+    rem2 = [i for i in iab if i not in k2] + [i for i in ic if i not in k2]
+    permd = list(range(len(rem2)))
+    rng.shuffle(permd)
+    idd = [rem2[permd[i]] for i in range(len(rem2))]
+
+    def tensor(ix):
+        t = {"dims": [size[i] for i in ix], "splits": []}
+        for p, i in enumerate(ix):
+            if split[i] is not None:
+                t["splits"].append(([1 if q == p else 0 for q in range(len(ix))], split[i]))
+        return t
+    prob = {"A": tensor(ia), "B": tensor(ib), "C": tensor(ic), "D": tensor(idd),
+            "idx": {"A": ia, "B": ib, "C": ic, "D": idd},
+            "contr3": {"N1": N1, "N2": N2, "N3": N3, "K1": K1, "K2": K2, "permab": permab,
+                       "pairs1": [(ia.index(k), ib.index(k)) for k in k1],
+                       "permd": permd,
+                       "pairs2": [(iab.index(k), ic.index(k)) for k in k2], "kd": -1.5}}
+    if sym and K2 == 2:
+        for name, ix in (("B", ib), ("C", ic)):
+            p = list(range(len(ix)))
+            a, b = ix.index(k2[0]), ix.index(k2[1])
+            p[a], p[b] = b, a
+            prob[name]["elems"] = [("perm", p, -1.0)]
+        if N3 == 2:
+            p = list(range(len(ic)))
+            a, b = ic.index(o3[0]), ic.index(o3[1])
+            p[a], p[b] = b, a
+            prob["C"]["elems"].append(("perm", p, -1.0))
+    return prob
+
+
+def c3_cases():
+    rng = random.Random(20261203)
+    out = []
+    for (n1, n2, n3, k1, k2) in [(1, 0, 1, 1, 1), (1, 1, 1, 1, 1), (1, 1, 2, 0, 0),
+                                 (1, 1, 2, 1, 2), (2, 0, 0, 0, 2), (2, 0, 1, 1, 2),
+                                 (2, 0, 2, 1, 2)]:
+        out.append(("c3_%d%d%d%d%d" % (n1, n2, n3, k1, k2),
+                    random_problem3(rng, n1, n2, n3, k1, k2)))
+    for (n1, n2, n3, k1, k2) in [(1, 1, 2, 1, 2), (2, 0, 2, 1, 2)]:
+        out.append(("c3sym_%d%d%d%d%d" % (n1, n2, n3, k1, k2),
+                    random_problem3(rng, n1, n2, n3, k1, k2, sym=True)))
+    return out
+
+
+def write_c3(work):
+    """Fixtures c3_*.json (problem, result orbits, schedule) and .npz (inputs,
+    libtensor's result) for btod_contract3."""
+    ctx = lt.Context(-1)
+    for i, (name, prob) in enumerate(c3_cases()):
+        dense = P.make_data3(ctx, prob, 3000 + 11 * i)
+        shape = tuple(prob["D"]["dims"])
+        ref = P.ref_io.run_ref(prob, os.path.join(work, name), shape, dense=dense)
+        meta = {"prob": prob, "schedule": ref["schedule"],
+                "symt": {str(k): list(v) for k, v in ref["symt"].items()}}
+        with open(os.path.join(HERE, name + ".json"), "w") as f:
+            json.dump(meta, f)
+        np.savez_compressed(os.path.join(HERE, name + ".npz"), A=dense["A"], B=dense["B"],
+                            C=dense["C"], result=ref["result"])
+        print("%-22s schedule %4d" % (name, len(ref["schedule"])))
+
+
 def write_symm(work):
     """Fixtures symm_*.json (+ .npz with libtensor's result)."""
     for name, base, symm in SYMM_CASES:
@@ -169,10 +258,13 @@ def main():
     if not P.ref_available():
         print("oracle/_ref/ref_driver is not built (make -C oracle ref)")
         return 1
-    if "--symm-only" in sys.argv:
+    if "--symm-only" in sys.argv or "--c3-only" in sys.argv:
         work = tempfile.mkdtemp(prefix="lt_golden_")
         try:
-            write_symm(work)
+            if "--symm-only" in sys.argv:
+                write_symm(work)
+            if "--c3-only" in sys.argv:
+                write_c3(work)
         finally:
             shutil.rmtree(work, ignore_errors=True)
         return 0
@@ -185,6 +277,7 @@ def main():
             P.save_case(os.path.join(HERE, name), prob, dense, ref)
             print("%-22s schedule %4d" % (name, len(ref["schedule"])))
         write_symm(work)
+        write_c3(work)
     finally:
         shutil.rmtree(work, ignore_errors=True)
     return 0

@@ -11,9 +11,22 @@ GOLDEN = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
 
 
 def names():
-    """Contraction cases (the btod_symmetrize2 fixtures symm_* excluded)."""
+    """btod_contract2 cases (the btod_symmetrize2 fixtures symm_* and the
+    btod_contract3 fixtures c3* excluded)."""
     return sorted(os.path.basename(p)[:-5] for p in glob.glob(os.path.join(GOLDEN, "*.json"))
-                  if not os.path.basename(p).startswith("symm_"))
+                  if not os.path.basename(p).startswith(("symm_", "c3")))
+
+
+def c3_names():
+    return sorted(os.path.basename(p)[:-5] for p in glob.glob(os.path.join(GOLDEN, "c3*.json")))
+
+
+def load_c3(name):
+    """(problem, dense inputs, fixture, libtensor's result)."""
+    with open(os.path.join(GOLDEN, name + ".json")) as f:
+        meta = json.load(f)
+    z = np.load(os.path.join(GOLDEN, name + ".npz"), allow_pickle=False)
+    return meta["prob"], {k: z[k] for k in ("A", "B", "C")}, meta, z["result"]
 
 
 def symm_names(errors=False):

@@ -97,3 +97,50 @@ def load_case(path):
     dense = {k: z[k] for k in z.files if k in ("A", "B")}
     result = z["result"] if "result" in z.files else None
     return meta["prob"], dense, meta, result
+
+
+# ---------------------------------------------------------------------------
+# btod_contract3 problems: tensors A, B, C and D described by index ids
+# (prob["idx"][name] lists the id of every dimension; ids are letters, so the
+# dense result is an einsum).
+
+def make_data3(ctx, prob, seed):
+    W.register_tables(prob)
+    out = {}
+    for k, name in enumerate(("A", "B", "C")):
+        bt = W.build_tensor(ctx, prob[name])
+        W.fill_random(bt, prob[name], seed + 7 * k)
+        out[name] = bt.to_dense()
+    return out
+
+
+def build3(ctx, prob, dense):
+    """Block tensors A, B, C holding the dense data and the btod_contract3 op."""
+    W.register_tables(prob)
+    ts = []
+    for name in ("A", "B", "C"):
+        t = W.build_tensor(ctx, prob[name])
+        t.from_dense(dense[name])
+        ts.append(t)
+    c = prob["contr3"]
+    c1 = lt.contraction2(c["N1"], c["N2"] + c["K2"], c["K1"], c["permab"])
+    for ia, ib in c["pairs1"]:
+        c1.contract(ia, ib)
+    c2 = lt.contraction2(c["N1"] + c["N2"], c["N3"], c["K2"], c["permd"])
+    for ia, ib in c["pairs2"]:
+        c2.contract(ia, ib)
+    op = lt.btod_contract3(c1, c2, ts[0], ts[1], ts[2], c.get("kd", 1.0))
+    return ts, op
+
+
+def result_tensor3(ctx, prob):
+    return W.build_tensor(ctx, prob["D"])
+
+
+def einsum3(prob, dense):
+    """kd (A B) C of the dense inputs."""
+    ix = prob["idx"]
+    expr = "%s,%s,%s->%s" % ("".join(ix["A"]), "".join(ix["B"]), "".join(ix["C"]),
+                             "".join(ix["D"]))
+    return prob["contr3"].get("kd", 1.0) * np.einsum(expr, dense["A"], dense["B"], dense["C"],
+                                                     optimize=True)

@@ -1,0 +1,29 @@
+"""btod_contract3 host side (no GPU): the result symmetry (canonical block,
+orbit transformation and allowedness of every block) and the schedule of
+D = kd (A B) C against libtensor's (fixtures c3_* written by the real
+reference build for every order combination libtensor instantiates, two of
+them with antisymmetric operands), and the stored results against a numpy
+einsum of the dense inputs."""
+import numpy as np
+import pytest
+
+import golden_cases as G
+import problems as P
+
+
+@pytest.mark.parametrize("name", G.c3_names())
+def test_symmetry_and_schedule(host_ctx, name):
+    prob, dense, meta, result = G.load_c3(name)
+    ts, op = P.build3(host_ctx, prob, dense)
+    assert sorted(op.get_schedule()) == meta["schedule"]
+    for aidx, (ac, al, perm, c) in meta["symt"].items():
+        got = op.sym_orbit(int(aidx))
+        assert (got[0], got[3], list(got[1]), got[2]) == (ac, bool(al), perm, c), aidx
+
+
+@pytest.mark.parametrize("name", G.c3_names())
+def test_fixture_is_the_einsum(name):
+    prob, dense, meta, result = G.load_c3(name)
+    exp = P.einsum3(prob, dense)
+    scale = float(np.max(np.abs(exp))) or 1.0
+    assert np.max(np.abs(result - exp)) <= 1e-13 * scale
