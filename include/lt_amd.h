@@ -62,6 +62,9 @@ typedef struct lt_stats {
                                 (t_gemm_ms is then 0: not separately timed) */
     int32_t tile_order;      /* grouped launch tile order: 0 longest C block
                                 first, 1 tiles dealt to XCD groups */
+    uint64_t n_plans;        /* execution plans built so far (a plan is reused
+                                while the operands' structure, the result
+                                tensor and the mode of addition stay the same) */
 } lt_stats;
 
 /* Message of the calling thread's last failed call. */
@@ -286,6 +289,8 @@ int lt_btod_contract3_schedule(lt_btod_contract3 *op, size_t *out, size_t cap,
 int lt_btod_contract3_sym_orbit(lt_btod_contract3 *op, size_t aidx, size_t *acidx,
     size_t *perm, double *coeff, int *allowed);
 int lt_btod_contract3_stats(lt_btod_contract3 *op, double *gpu_ms, double *flops);
+/* Execution plans built so far by the two contractions (see lt_stats.n_plans). */
+int lt_btod_contract3_plans(lt_btod_contract3 *op, uint64_t *n);
 
 /* ---- btod_symmetrize2 (libtensor/block_tensor/btod_symmetrize2.h:20-110;
  *      gen_block_tensor/impl/gen_bto_symmetrize2_impl.h) ----

@@ -455,6 +455,12 @@ class btod_contract3(object):
         call("lt_btod_contract3_stats", self._h, *[ctypes.byref(x) for x in v])
         return {"gpu_ms": v[0].value, "flops": v[1].value}
 
+    def plans(self):
+        """Execution plans built so far by the two contractions."""
+        n = ctypes.c_uint64()
+        call("lt_btod_contract3_plans", self._h, ctypes.byref(n))
+        return n.value
+
 
 class btod_symmetrize2(object):
     """B = X + s P(X) for the result X of a btod_contract2 or of another

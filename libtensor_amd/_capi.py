@@ -42,6 +42,7 @@ class LtStats(ctypes.Structure):
         ("tune_ms", ctypes.c_double * 32),
         ("graph", ctypes.c_int32),
         ("tile_order", ctypes.c_int32),
+        ("n_plans", ctypes.c_uint64),
     ]
 
 
@@ -138,6 +139,7 @@ _SIGS = {
     "lt_btod_contract3_sym_orbit": [vp, ctypes.c_size_t, c_size_p, c_size_p, c_double_p,
                                     c_int_p],
     "lt_btod_contract3_stats": [vp, c_double_p, c_double_p],
+    "lt_btod_contract3_plans": [vp, ctypes.POINTER(ctypes.c_uint64)],
     "lt_btod_symmetrize2_create_c3": [vp, c_size_p, ctypes.c_int, ctypes.POINTER(vp)],
     "lt_btod_symmetrize2_create": [vp, c_size_p, ctypes.c_int, ctypes.POINTER(vp)],
     "lt_btod_symmetrize2_create_nested": [vp, c_size_p, ctypes.c_int, ctypes.POINTER(vp)],

@@ -923,6 +923,13 @@ int lt_btod_contract3_sym_orbit(lt_btod_contract3 *op, size_t aidx, size_t *acid
     });
 }
 
+int lt_btod_contract3_plans(lt_btod_contract3 *op, uint64_t *n) {
+    return guard([&]() {
+        need(op, "op"); need(n, "n");
+        *n = op->op->first().stats().n_plans + op->op->second().stats().n_plans;
+    });
+}
+
 int lt_btod_contract3_stats(lt_btod_contract3 *op, double *gpu_ms, double *flops) {
     return guard([&]() {
         need(op, "op");
@@ -1080,6 +1087,7 @@ int lt_btod_contract2_stats(lt_btod_contract2 *op, lt_stats *out) {
         for(int i = 0; i < 32; i++) out->tune_ms[i] = i < dev::TILE_COUNT ? s.tune_ms[i] : 0.0;
         out->graph = s.graph;
         out->tile_order = s.tile_order;
+        out->n_plans = s.n_plans;
     });
 }
 

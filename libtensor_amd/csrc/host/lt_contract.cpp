@@ -309,6 +309,7 @@ bool btod_contract2::plan_valid(btensor &c, bool add, double d, int) const {
 
 void btod_contract2::build_plan(btensor &c, bool add, double dscale) {
     double t0 = now_ms();
+    m_stats.n_plans++;
     context &ctx = m_a.ctx();
     m_plan.reset(new contract_plan());
     contract_plan &pl = *m_plan;

@@ -200,6 +200,7 @@ struct exec_stats {
     double tune_ms[dev::TILE_COUNT]; //!< autotuner timings (0 = not measured)
     int graph;              //!< 1 if the last perform ran as a HIP graph
     int tile_order;         //!< grouped tile order (0 longest first, 1 XCD groups)
+    size_t n_plans;         //!< execution plans built so far
 };
 
 //! Dense helpers on a context's stream (both synchronous): dst = or +=

@@ -201,6 +201,36 @@ def test_plan_not_reused_for_a_new_result_tensor(gpu_ctx):
         gc.collect()
 
 
+@pytest.mark.parametrize("mode", [lt.LT_MODE_GROUPED, lt.LT_MODE_PER_BLOCK])
+def test_repeated_perform_after_the_result_changed(gpu_ctx, mode):
+    """A perform into the tensor the operation wrote last keeps its structure;
+    once the tensor was changed in between (a block zeroed, another symmetry
+    set) the next perform rebuilds it, and the result is the same."""
+    prob, dense, meta, result = G.load("ladder_full")
+    a, b, op = P.build(gpu_ctx, prob, dense)
+    c = P.make_result_tensor(gpu_ctx, prob)
+    op.perform(c, mode)
+    nz = c.req_nonzero_blocks()
+    op.perform(c, mode)
+    assert c.req_nonzero_blocks() == nz
+    assert O.rel_err(c.to_dense(), result) < TOL
+    c.zero_block(c.index(nz[0]))
+    op.perform(c, mode)
+    assert c.req_nonzero_blocks() == nz
+    assert O.rel_err(c.to_dense(), result) < TOL
+    c.zero()
+    op.perform(c, mode)
+    assert c.req_nonzero_blocks() == nz
+    assert O.rel_err(c.to_dense(), result) < TOL
+    # another operation writes C and sets its symmetry
+    c2 = P.make_result_tensor(gpu_ctx, prob)
+    op.perform(c2, mode)
+    lt.btod_copy(c2, c, c=2.0)
+    assert O.rel_err(c.to_dense(), 2.0 * result) < TOL
+    op.perform(c, mode)
+    assert O.rel_err(c.to_dense(), result) < TOL
+
+
 @pytest.mark.parametrize("name", ["ladder_full", "ladder_spin", "rand_222", "randsym_222",
                                   "self_ladder", "ring_small"])
 def test_btod_dotprod(gpu_ctx, name):

@@ -39,6 +39,12 @@ def test_perform(gpu_ctx, name, mode):
     op.perform(d, mode)
     assert np.max(np.abs(d.to_dense() - result)) <= TOL * scale
     st = op.stats()
+    if meta["schedule"]:
+        # AB, written again by the first contraction, keeps its structure:
+        # the second contraction's plan for D is reused
+        n = op.plans()
+        op.perform(d, mode)
+        assert op.plans() == n
     assert st["flops"] > 0 or not meta["schedule"]
 
 
