@@ -42,6 +42,8 @@ typedef struct lt_btensor lt_btensor;
 typedef struct lt_symel lt_symel;
 typedef struct lt_btod_contract2 lt_btod_contract2;
 typedef struct lt_btod_symmetrize2 lt_btod_symmetrize2;
+/* btod_symmetrize3 handles are used with the lt_btod_symmetrize2_* calls. */
+typedef struct lt_btod_symmetrize2 lt_btod_symmetrize3;
 typedef struct lt_btod_contract3 lt_btod_contract3;
 typedef struct lt_server lt_server;
 
@@ -314,6 +316,20 @@ int lt_btod_symmetrize2_create_nested(lt_btod_symmetrize2 *op, const size_t *per
  * operation; btod_symcontract3 is this with the symmetrized pair inside). */
 int lt_btod_symmetrize2_create_c3(lt_btod_contract3 *op, const size_t *perm, int symm,
     lt_btod_symmetrize2 **out);
+/* btod_symmetrize3 (libtensor/block_tensor/btod_symmetrize3.h;
+ * gen_block_tensor/impl/gen_bto_symmetrize3_impl.h): B = sum over the six
+ * exchanges g of three index groups, generated by the pair permutations
+ * perm1 and perm2 (btod_symmetrize3(op, i1, i2, i3, symm) exchanges single
+ * indexes: perm1 = (i1 i2), perm2 = (i1 i3)), of s(g) g(X), s(g) = s for
+ * the three pair exchanges and 1 for the identity and the two cycles.
+ * Result symmetry and schedule as in the reference; the handle is used with
+ * the lt_btod_symmetrize2_* calls below (perform, perform_add, schedule,
+ * sym_orbit, compute_block, stats, destroy); create_nested takes another
+ * symmetrization as the operand. */
+int lt_btod_symmetrize3_create(lt_btod_contract2 *op, const size_t *perm1,
+    const size_t *perm2, int symm, lt_btod_symmetrize3 **out);
+int lt_btod_symmetrize3_create_nested(lt_btod_symmetrize2 *op, const size_t *perm1,
+    const size_t *perm2, int symm, lt_btod_symmetrize3 **out);
 int lt_btod_symmetrize2_destroy(lt_btod_symmetrize2 *op);
 int lt_btod_symmetrize2_perform(lt_btod_symmetrize2 *op, lt_btensor *b, int mode);
 int lt_btod_symmetrize2_perform_add(lt_btod_symmetrize2 *op, lt_btensor *b, double d,

@@ -16,7 +16,7 @@ from ._capi import (GEMM_CONFIGS, GEMM_STACKED_FIRST, LT_MODE_GROUPED, LT_MODE_O
 
 __all__ = ["Context", "product_table", "block_index_space", "block_tensor",
            "se_perm", "se_part", "se_label", "contraction2", "btod_contract2",
-           "device_array", "tod_copy", "tod_add", "tod_set", "tod_contract2", "tod_dotprod", "btod_dotprod", "btod_copy", "btod_add", "btod_symmetrize2", "btod_contract3", "server",
+           "device_array", "tod_copy", "tod_add", "tod_set", "tod_contract2", "tod_dotprod", "btod_dotprod", "btod_copy", "btod_add", "btod_symmetrize2", "btod_symmetrize3", "btod_contract3", "server",
            "LtError", "GEMM_CONFIGS", "GEMM_STACKED_FIRST", "LT_MODE_PER_BLOCK", "LT_MODE_GROUPED", "LT_MODE_OZAKI"]
 
 
@@ -535,6 +535,26 @@ class btod_symmetrize2(object):
         p = None if perm is None else sizes(perm)
         call("lt_btod_symmetrize2_compute_block", self._h, 1 if zero else 0, int(aidx_b), p,
              float(coeff), blk.ptr)
+
+
+class btod_symmetrize3(btod_symmetrize2):
+    """B = sum over the six exchanges of three index groups generated by the
+    pair permutations perm1 and perm2 of s(g) g(X) (libtensor/block_tensor/
+    btod_symmetrize3.h; btod_symmetrize3(op, i1, i2, i3, symm) of the
+    reference is perm1 = pair(n, i1, i2), perm2 = pair(n, i1, i3)).  X is the
+    result of a btod_contract2 or of another symmetrization."""
+
+    def __init__(self, op, perm1, perm2, symm):
+        h = ctypes.c_void_p()
+        if isinstance(op, btod_symmetrize2):
+            call("lt_btod_symmetrize3_create_nested", op._h, sizes(perm1), sizes(perm2),
+                 1 if symm else 0, ctypes.byref(h))
+        else:
+            call("lt_btod_symmetrize3_create", op._h, sizes(perm1), sizes(perm2),
+                 1 if symm else 0, ctypes.byref(h))
+        self._h = h
+        self.op = op
+        self.n = len(perm1)
 
 
 # ---------------------------------------------------------------------------

@@ -142,6 +142,9 @@ _SIGS = {
     "lt_btod_contract3_plans": [vp, ctypes.POINTER(ctypes.c_uint64)],
     "lt_btod_symmetrize2_create_c3": [vp, c_size_p, ctypes.c_int, ctypes.POINTER(vp)],
     "lt_btod_symmetrize2_create": [vp, c_size_p, ctypes.c_int, ctypes.POINTER(vp)],
+    "lt_btod_symmetrize3_create": [vp, c_size_p, c_size_p, ctypes.c_int, ctypes.POINTER(vp)],
+    "lt_btod_symmetrize3_create_nested": [vp, c_size_p, c_size_p, ctypes.c_int,
+                                          ctypes.POINTER(vp)],
     "lt_btod_symmetrize2_create_nested": [vp, c_size_p, ctypes.c_int, ctypes.POINTER(vp)],
     "lt_btod_symmetrize2_destroy": [vp],
     "lt_btod_symmetrize2_perform": [vp, vp, ctypes.c_int],

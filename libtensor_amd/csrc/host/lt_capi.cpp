@@ -21,7 +21,7 @@ struct lt_btod_contract2 { btod_contract2 *op; lt_btensor *a, *b; lt_ctx *ctx; }
 struct lt_server { server *s; lt_ctx *ctx; };
 struct lt_btod_contract3 { btod_contract3 *op; lt_ctx *ctx; };
 struct lt_btod_symmetrize2 {
-    btod_symmetrize2 *s;
+    symmetrizer *s;     //!< btod_symmetrize2 or btod_symmetrize3
     additive_op *inner;    //!< owned adapter of a btod_contract2 operand, or null
     lt_ctx *ctx;
 };
@@ -813,13 +813,17 @@ int lt_btod_contract2_symc_orbit(lt_btod_contract2 *op, size_t aidx, size_t *aci
 namespace {
 
 void symm_create(lt_ctx *ctx, additive_op *op, additive_op *owned, const size_t *perm,
-    int symm, lt_btod_symmetrize2 **out) {
+    const size_t *perm2, int symm, lt_btod_symmetrize2 **out) {
     const size_t n = op->sym().order();
     lt_btod_symmetrize2 *h = new lt_btod_symmetrize2;
     h->inner = owned;
     h->ctx = ctx;
     try {
-        h->s = new btod_symmetrize2(*op, perm_or_identity(n, perm), symm != 0);
+        if(perm2)
+            h->s = new btod_symmetrize3(*op, perm_or_identity(n, perm),
+                perm_or_identity(n, perm2), symm != 0);
+        else
+            h->s = new btod_symmetrize2(*op, perm_or_identity(n, perm), symm != 0);
     } catch(...) {
         delete owned;
         delete h;
@@ -836,7 +840,7 @@ int lt_btod_symmetrize2_create(lt_btod_contract2 *op, const size_t *perm, int sy
     return guard([&]() {
         need(op, "op"); need(perm, "perm"); need(out, "out");
         contract2_op *inner = new contract2_op(*op->op, *op->a->t);
-        symm_create(op->ctx, inner, inner, perm, symm, out);
+        symm_create(op->ctx, inner, inner, perm, 0, symm, out);
     });
 }
 
@@ -844,7 +848,7 @@ int lt_btod_symmetrize2_create_c3(lt_btod_contract3 *op, const size_t *perm, int
     lt_btod_symmetrize2 **out) {
     return guard([&]() {
         need(op, "op"); need(perm, "perm"); need(out, "out");
-        symm_create(op->ctx, op->op, 0, perm, symm, out);
+        symm_create(op->ctx, op->op, 0, perm, 0, symm, out);
     });
 }
 
@@ -942,7 +946,24 @@ int lt_btod_symmetrize2_create_nested(lt_btod_symmetrize2 *op, const size_t *per
     lt_btod_symmetrize2 **out) {
     return guard([&]() {
         need(op, "op"); need(perm, "perm"); need(out, "out");
-        symm_create(op->ctx, op->s, 0, perm, symm, out);
+        symm_create(op->ctx, op->s, 0, perm, 0, symm, out);
+    });
+}
+
+int lt_btod_symmetrize3_create(lt_btod_contract2 *op, const size_t *perm1,
+    const size_t *perm2, int symm, lt_btod_symmetrize3 **out) {
+    return guard([&]() {
+        need(op, "op"); need(perm1, "perm1"); need(perm2, "perm2"); need(out, "out");
+        contract2_op *inner = new contract2_op(*op->op, *op->a->t);
+        symm_create(op->ctx, inner, inner, perm1, perm2, symm, out);
+    });
+}
+
+int lt_btod_symmetrize3_create_nested(lt_btod_symmetrize2 *op, const size_t *perm1,
+    const size_t *perm2, int symm, lt_btod_symmetrize3 **out) {
+    return guard([&]() {
+        need(op, "op"); need(perm1, "perm1"); need(perm2, "perm2"); need(out, "out");
+        symm_create(op->ctx, op->s, 0, perm1, perm2, symm, out);
     });
 }
 

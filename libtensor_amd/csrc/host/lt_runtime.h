@@ -381,29 +381,25 @@ private:
     double m_last_ms;
 };
 
-//! btod_symmetrize2<N> (libtensor/block_tensor/btod_symmetrize2.h:20-110;
-//! gen_block_tensor/impl/gen_bto_symmetrize2_impl.h): B = X + s P(X) for the
-//! result X of another operation and a pair permutation P, s = +1
-//! (symmetric) or -1 (antisymmetric).  The result symmetry and the schedule
-//! follow the reference; X is computed into a resident temporary and every
-//! block of B is gathered from it by the batched permute kernel.
-class btod_symmetrize2 : public additive_op {
+//! Sum of transformed copies of another operation's result X
+//! (btod_symmetrize2/3): every canonical block of B is a list of sources
+//! (canonical block of X, transformation).  perform() computes X once into a
+//! resident temporary and gathers every block of B from it with the batched
+//! permute kernel, one launch per source rank (the k-th source of every
+//! block of B in launch k, so no two descriptors of a launch share a
+//! destination).
+class symmetrizer : public additive_op {
 public:
-    btod_symmetrize2(additive_op &op, const perm_t &perm, bool symm);
-    ~btod_symmetrize2();
-
     context &ctx() { return m_op.ctx(); }
     const symmetry &sym() const { return m_sym; }
     const std::vector<size_t> &schedule() const { return m_sched; }
     double flops() const { return m_op.flops(); }
 
     //! B := result (B zeroed, given the result symmetry).
-    void perform(btensor &b, int mode);
+    void perform(btensor &b, int mode) { run(b, false, 1.0, mode); }
     //! B += d * result for B with the symmetry of the result.
-    void perform_add(btensor &b, double d, int mode);
-    double perform_into(btensor &t, int mode);
-    //! gen_bto_symmetrize2::compute_block (impl :66-128) on the device.
-    void compute_block(bool zero, const index_t &ib, const tensor_tr &trb, double *blk);
+    void perform_add(btensor &b, double d, int mode) { run(b, true, d, mode); }
+    double perform_into(btensor &t, int mode) { return run(t, false, 1.0, mode); }
     //! Device time of the last perform (the operand's kernels and the
     //! gather launches), of the gather launches alone, and the bytes the
     //! gather moved (source read, destination written, and read when added to).
@@ -411,23 +407,58 @@ public:
     double last_gather_ms() const { return m_gather_ms; }
     double last_gather_bytes() const { return m_gather_bytes; }
 
-private:
+protected:
     struct rec {
         size_t ai;        //!< canonical block of X (in the operand's schedule)
-        tensor_tr tr;     //!< block of B = tr(X_ai)
+        tensor_tr tr;     //!< contribution tr(X_ai)
     };
+    explicit symmetrizer(additive_op &op) : m_op(op), m_last_ms(0.0), m_gather_ms(0.0),
+        m_gather_bytes(0.0) { }
+    const std::vector<rec> *records(size_t aib) const;
+
     additive_op &m_op;
-    perm_t m_perm;
-    bool m_symm;
     symmetry m_sym;
     std::vector<size_t> m_sched;
-    std::vector<std::pair<size_t, std::vector<rec> > > m_recs;  //!< by block of B
+    std::vector<std::pair<size_t, std::vector<rec> > > m_recs;  //!< by block of B, sorted
+
+private:
     std::unique_ptr<btensor> m_x;
     double m_last_ms, m_gather_ms, m_gather_bytes;
-
-    void make_schedule();
-    const std::vector<rec> *records(size_t aib) const;
     double run(btensor &b, bool add, double d, int mode);
+};
+
+//! btod_symmetrize2<N> (libtensor/block_tensor/btod_symmetrize2.h:20-110;
+//! gen_block_tensor/impl/gen_bto_symmetrize2_impl.h): B = X + s P(X) for the
+//! result X of another operation and a pair permutation P, s = +1
+//! (symmetric) or -1 (antisymmetric).  The result symmetry, the schedule and
+//! the sources of every block follow the reference.
+class btod_symmetrize2 : public symmetrizer {
+public:
+    btod_symmetrize2(additive_op &op, const perm_t &perm, bool symm);
+    //! gen_bto_symmetrize2::compute_block (impl :66-128) on the device.
+    void compute_block(bool zero, const index_t &ib, const tensor_tr &trb, double *blk);
+private:
+    perm_t m_perm;
+    bool m_symm;
+    void make_schedule();
+};
+
+//! btod_symmetrize3<N> (libtensor/block_tensor/btod_symmetrize3.h;
+//! gen_block_tensor/impl/gen_bto_symmetrize3_impl.h): B = sum over the six
+//! exchanges g of three index groups generated by the pair permutations p1
+//! and p2 of s(g) g(X), s(g) = s for the three pair exchanges and 1 for the
+//! identity and the two cycles.
+class btod_symmetrize3 : public symmetrizer {
+public:
+    btod_symmetrize3(additive_op &op, const perm_t &p1, const perm_t &p2, bool symm);
+    //! gen_bto_symmetrize3::compute_block (impl :66-128) on the device.
+    void compute_block(bool zero, const index_t &ib, const tensor_tr &trb, double *blk);
+private:
+    perm_t m_p1, m_p2;
+    bool m_symm;
+    std::vector<size_t> m_sched0;   //!< the operand's schedule, sorted
+    void make_schedule();
+    void block_records(size_t aib, std::vector<rec> &out) const;
 };
 
 } // namespace lta

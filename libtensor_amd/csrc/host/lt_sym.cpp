@@ -1503,13 +1503,53 @@ symmetry contract2_symmetry(const contraction2 &c, const symmetry &syma,
 }
 
 // ---------------------------------------------------------------------------
-// Symmetrization over a pair permutation (so_symmetrize with two index
-// groups; libtensor/symmetry/inst/so_symmetrize_impl.h and the handlers
-// so_symmetrize_se_{perm,part,label}_impl.h).  With two groups the
-// reference's permutation generator visits the identity and the exchange of
-// the groups, which is P itself; the code below uses that directly.
+// Symmetrization over index groups (so_symmetrize,
+// libtensor/symmetry/inst/so_symmetrize_impl.h and the handlers
+// so_symmetrize_se_{perm,part,label}_impl.h).  Group g (1..ngrp) holds the
+// dimensions with idxgrp == g; symidx orders the dimensions inside a group;
+// the symmetrization permutes whole groups.
 
 namespace {
+
+// Permutations of the positions that are not fixed, one exchange of
+// neighbouring positions per step: Trotter's algorithm 115 (Commun. ACM 5
+// (1962) 434) in the form libtensor enumerates it
+// (core/permutation_generator.h); the order of the sequence decides which
+// conjugations so_symmetrize composes.
+class perm_walk {
+public:
+    perm_walk(size_t n, const bool *fixed) : m_perm(n), m_m(0), m_done(false) {
+        for(size_t i = 0; i < kMaxOrder; i++) { m_up[i] = true; m_c[i] = 0; m_pos[i] = 0; }
+        for(size_t i = 0; i < n; i++) if(!fixed[i]) m_pos[m_m++] = i;
+        if(m_m < 2) m_done = true;
+    }
+    const perm_t &perm() const { return m_perm; }
+    bool step() {
+        if(m_done) return false;
+        size_t i = m_m - 1, carried = 0;
+        for(; i > 0; i--) {
+            if(m_up[i]) m_c[i]++;
+            else m_c[i]--;
+            if(m_c[i] == i + 1) { m_up[i] = false; continue; }
+            if(m_c[i] != 0) break;
+            m_up[i] = true;
+            carried++;
+        }
+        if(i == 0) m_done = true;
+        const size_t q = m_done ? carried : m_c[i] - 1 + carried;
+        m_perm.swap(m_pos[q], m_pos[q + 1]);
+        return !m_done;
+    }
+private:
+    perm_t m_perm;
+    size_t m_pos[kMaxOrder], m_c[kMaxOrder], m_m;
+    bool m_up[kMaxOrder], m_done;
+};
+
+struct sym_groups {
+    size_t n, ngrp, nidx;
+    seq_t idxgrp, symidx;
+};
 
 uint64_t perm_code(const perm_t &p) {
     uint64_t c = 0;
@@ -1517,14 +1557,33 @@ uint64_t perm_code(const perm_t &p) {
     return c;
 }
 
-// se_perm handler (so_symmetrize_se_perm_impl.h:30-160): the group elements
-// (breadth-first from the distinct generators) that stay in the group after
-// conjugation by P, plus (P, s), as a new permutation group.
-void symmetrize_perm(size_t n, const std::vector<std::shared_ptr<sym_element> > *set,
-    const perm_t &pp, const scalar_tr &trp, symmetry &out) {
+// Position of dimension (group g, index k), both counted from 1.
+size_t group_dim(const sym_groups &G, size_t g, size_t k) {
+    for(size_t j = 0; j < G.n; j++)
+        if(G.idxgrp[j] == g && G.symidx[j] == k) return j;
+    return G.n;
+}
+
+// se_perm handler (so_symmetrize_se_perm_impl.h): the group elements,
+// breadth-first from the distinct generators, that stay in the group under
+// every conjugation the walk over group permutations composes, plus the
+// cyclic (more than two groups) and pair permutations.
+void symmetrize_perm(const sym_groups &G,
+    const std::vector<std::shared_ptr<sym_element> > *set,
+    const scalar_tr &trp, const scalar_tr &trc, symmetry &out) {
 
     typedef permutation_group::gen_t gen_t;
+    const size_t n = G.n;
+    perm_t pp(n), cp(n);
+    for(size_t g = 1; g < G.ngrp; g++) {
+        for(size_t k = 1; k <= G.nidx; k++) {
+            const size_t i1 = group_dim(G, g, k), j = group_dim(G, g + 1, k);
+            if(g == 1) pp.swap(i1, j);
+            cp.swap(i1, j);
+        }
+    }
     if(!set || set->empty()) {
+        if(G.ngrp > 2) out.insert(se_perm(cp, trc));
         out.insert(se_perm(pp, trp));
         return;
     }
@@ -1555,114 +1614,181 @@ void symmetrize_perm(size_t n, const std::vector<std::shared_ptr<sym_element> > 
         l2.push_back(cur);
         l1.pop_front();
     }
-    const perm_t pinv(pp.inverse());
-    scalar_tr trinv(trp);
-    trinv.invert();
-    permutation_group gx(n);
-    for(size_t g = 0; g < l0.size(); g++) {
-        perm_t pi(pinv);
-        pi.permute(l0[g].first).permute(pp);
-        scalar_tr ti(trinv);
-        ti.transform(l0[g].second).transform(trp);
-        gx.add_orbit(ti, pi);
+    std::vector<char> keep(l2.size(), 1);
+    bool fixed[kMaxOrder];
+    for(size_t i = 0; i < n; i++) fixed[i] = G.idxgrp[i] == 0 || G.symidx[i] != 1;
+    perm_walk pw(n, fixed);
+    scalar_tr trx;
+    perm_t px(n), p1(n);
+    while(pw.step()) {
+        const perm_t &p2 = pw.perm();
+        if(G.ngrp == 2) {
+            trx.transform(trp);
+            px.permute(pp);
+        } else {
+            size_t i = 0;
+            for(; i < n && p2[i] == p1[i]; i++) { trx.transform(trc); px.permute(cp); }
+            trx.transform(trp);
+            px.permute(pp);
+            i++;
+            for(; i < n; i++) { trx.transform(trc); px.permute(cp); }
+        }
+        const perm_t pxinv(px.inverse());
+        scalar_tr trxinv(trx);
+        trxinv.invert();
+        p1 = p2;
+        permutation_group gx(n);
+        for(size_t g = 0; g < l0.size(); g++) {
+            perm_t pi(pxinv);
+            pi.permute(l0[g].first).permute(px);
+            scalar_tr ti(trxinv);
+            ti.transform(l0[g].second).transform(trx);
+            gx.add_orbit(ti, pi);
+        }
+        for(size_t i = 0; i < l2.size(); i++)
+            if(keep[i] && !gx.is_member(l2[i].second, l2[i].first)) keep[i] = 0;
     }
     permutation_group g2(n);
     for(size_t i = 0; i < l2.size(); i++)
-        if(gx.is_member(l2[i].second, l2[i].first)) g2.add_orbit(l2[i].second, l2[i].first);
+        if(keep[i]) g2.add_orbit(l2[i].second, l2[i].first);
+    if(G.ngrp > 2) g2.add_orbit(trc, cp);
     g2.add_orbit(trp, pp);
     std::vector<gen_t> gs;
     g2.convert(gs);
     for(size_t i = 0; i < gs.size(); i++) out.insert(se_perm(gs[i].first, gs[i].second));
 }
 
-// Consistent map i1 -> i2 in the partition for both the index pair and its
-// image under P (so_symmetrize_se_part_impl.h, map_exists).
-bool part_map_both(const se_part &sp, const index_t &i1, const index_t &i2,
-    const perm_t &pp) {
-    index_t j1[2] = { i1, i1 }, j2[2] = { i2, i2 };
-    pp.apply(j1[1]);
-    pp.apply(j2[1]);
+// The index i1 with its groups permuted by pn (so_symmetrize_se_part_impl.h:
+// map[(symidx - 1) ngrp + idxgrp - 1] is a dimension; pn permutes the
+// entries of every run of ngrp).
+index_t permute_groups(const index_t &i1, const perm_t &pn, const std::vector<size_t> &map,
+    size_t ngrp) {
+    index_t ix(i1);
+    for(size_t b = 0; b + ngrp <= map.size(); b += ngrp)
+        for(size_t j = 0; j < ngrp; j++) ix[map[b + j]] = i1[map[b + pn[j]]];
+    return ix;
+}
+
+std::vector<perm_t> group_perms(size_t ngrp, size_t n) {
+    bool fixed[kMaxOrder];
+    for(size_t i = 0; i < n; i++) fixed[i] = i >= ngrp;
+    std::vector<perm_t> out;
+    perm_walk pw(n, fixed);
+    do out.push_back(pw.perm()); while(pw.step());
+    return out;
+}
+
+// Consistent map i1 -> i2 of the partition for every group permutation of
+// the pair (so_symmetrize_se_part_impl.h, map_exists).
+bool part_map_all(const se_part &sp, const index_t &i1, const index_t &i2,
+    const std::vector<perm_t> &pns, const std::vector<size_t> &map, size_t ngrp) {
     scalar_tr tr;
     size_t k = 0;
-    for(; k < 2; k++) {
-        if(sp.map_exists(j1[k], j2[k])) {
-            tr = sp.get_transf(j1[k], j2[k]);
+    for(; k < pns.size(); k++) {
+        index_t j1 = permute_groups(i1, pns[k], map, ngrp);
+        index_t j2 = permute_groups(i2, pns[k], map, ngrp);
+        if(sp.map_exists(j1, j2)) {
+            tr = sp.get_transf(j1, j2);
             break;
         }
-        if(!sp.is_forbidden(j1[k]) || !sp.is_forbidden(j2[k])) return false;
+        if(!sp.is_forbidden(j1) || !sp.is_forbidden(j2)) return false;
     }
-    for(k++; k < 2; k++) {
-        if(sp.map_exists(j1[k], j2[k])) {
-            if(tr != sp.get_transf(j1[k], j2[k])) return false;
-        } else if(!sp.is_forbidden(j1[k]) || !sp.is_forbidden(j2[k])) {
+    for(k++; k < pns.size(); k++) {
+        index_t j1 = permute_groups(i1, pns[k], map, ngrp);
+        index_t j2 = permute_groups(i2, pns[k], map, ngrp);
+        if(sp.map_exists(j1, j2)) {
+            if(tr != sp.get_transf(j1, j2)) return false;
+        } else if(!sp.is_forbidden(j1) || !sp.is_forbidden(j2)) {
             return false;
         }
     }
     return true;
 }
 
-// se_part handler (so_symmetrize_se_part_impl.h:20-75).  gmap lists the
-// paired dimensions the way the reference orders them: gmap[2k], gmap[2k+1]
-// are the k-th pair.
-void symmetrize_part(const std::vector<std::shared_ptr<sym_element> > &set,
-    const perm_t &pp, const std::vector<size_t> &gmap, symmetry &out) {
+// se_part handler (so_symmetrize_se_part_impl.h:20-75).
+void symmetrize_part(const sym_groups &G,
+    const std::vector<std::shared_ptr<sym_element> > &set, symmetry &out) {
 
     if(set.empty()) return;
     se_part cp = combine_part(set);
     const dims_t pdims = cp.pdims;
-    const size_t npair = gmap.size() / 2;
-    // The reference compares these entries of its map (for several pairs
-    // they are not the paired dimensions); kept as it is.
-    for(size_t j = 0; j < npair; j++)
-        if(pdims[gmap[npair + j]] != pdims[gmap[j]]) return;
-    const size_t n = pdims.n;
+    std::vector<size_t> map(G.ngrp * G.nidx, G.n);
+    for(size_t i = 0; i < G.n; i++)
+        if(G.idxgrp[i]) map[(G.symidx[i] - 1) * G.ngrp + G.idxgrp[i] - 1] = i;
+    // The reference compares these entries of its map (for several indexes
+    // per group they are not the dimensions the symmetrization exchanges);
+    // kept as it is.
+    for(size_t g = 1; g < G.ngrp; g++)
+        for(size_t j = 0; j < G.nidx; j++)
+            if(pdims[map[g * G.nidx + j]] != pdims[map[j]]) return;
+    const std::vector<perm_t> pns = group_perms(G.ngrp, G.n);
     se_part sp2(cp.bis, pdims);
-    index_t i1(n);
+    index_t i1(pdims.n);
     do {
-        index_t p1(i1);
-        pp.apply(p1);
-        if(cp.is_forbidden(i1) && cp.is_forbidden(p1)) {
-            sp2.mark_forbidden(i1);
-            sp2.mark_forbidden(p1);
+        bool allf = true;
+        for(size_t k = 0; k < pns.size() && allf; k++)
+            if(!cp.is_forbidden(permute_groups(i1, pns[k], map, G.ngrp))) allf = false;
+        if(allf) {
+            for(size_t k = 0; k < pns.size(); k++)
+                sp2.mark_forbidden(permute_groups(i1, pns[k], map, G.ngrp));
             continue;
         }
         if(cp.is_forbidden(i1)) continue;
         index_t i2 = cp.get_direct_map(i1);
         bool found = false;
         while(!found && i1 < i2) {
-            if(part_map_both(cp, i1, i2, pp)) found = true;
+            if(part_map_all(cp, i1, i2, pns, map, G.ngrp)) found = true;
             else i2 = cp.get_direct_map(i2);
         }
         if(found) {
-            scalar_tr tr = cp.get_transf(i1, i2);
-            index_t p2(i2);
-            pp.apply(p2);
-            sp2.add_map(i1, i2, tr);
-            sp2.add_map(p1, p2, tr);
+            const scalar_tr tr = cp.get_transf(i1, i2);
+            for(size_t k = 0; k < pns.size(); k++)
+                sp2.add_map(permute_groups(i1, pns[k], map, G.ngrp),
+                    permute_groups(i2, pns[k], map, G.ngrp), tr);
         }
     } while(inc_index(i1, pdims));
     out.insert(sp2);
 }
 
-// se_label handler (so_symmetrize_se_label_impl.h:20-95): the rule of every
-// element united with its copy permuted by P.
-void symmetrize_label(const std::vector<std::shared_ptr<sym_element> > &set,
-    const perm_t &pp, symmetry &out) {
+// se_label handler (so_symmetrize_se_label_impl.h:20-95): every rule united
+// with its copies under all group permutations, kept as the element plus the
+// permutations whose images it also allows.
+void symmetrize_label(const sym_groups &G,
+    const std::vector<std::shared_ptr<sym_element> > &set, symmetry &out) {
 
+    std::vector<size_t> map(G.ngrp * G.nidx, G.n);
+    for(size_t i = 0; i < G.n; i++)
+        if(G.idxgrp[i]) map[(G.idxgrp[i] - 1) * G.nidx + G.symidx[i] - 1] = i;
+    // tensor permutations of the group permutations
+    std::vector<perm_t> gp;
+    const std::vector<perm_t> pns = group_perms(G.ngrp, G.n);
+    for(size_t k = 0; k < pns.size(); k++) {
+        perm_t q(G.n);
+        for(size_t g = 0; g < G.ngrp; g++)
+            for(size_t j = 0; j < G.nidx; j++)
+                q.p[map[g * G.nidx + j]] = (uint8_t)map[pns[k][g] * G.nidx + j];
+        if(!q.is_identity()) gp.push_back(q);
+    }
     for(size_t e = 0; e < set.size(); e++) {
         const se_label &l1 = dynamic_cast<const se_label&>(*set[e]);
-        for(size_t i = 0; i < pp.n; i++) {
-            lt_require(l1.bidims[i] == l1.bidims[pp[i]] && l1.labels[i] == l1.labels[pp[i]],
-                LT_ERR_SYMMETRY, "so_symmetrize: incompatible dimensions");
+        for(size_t j = 0; j < G.nidx; j++) {
+            const size_t d0 = map[j];
+            for(size_t g = 1; g < G.ngrp; g++) {
+                const size_t d = map[g * G.nidx + j];
+                lt_require(l1.bidims[d] == l1.bidims[d0] && l1.labels[d] == l1.labels[d0],
+                    LT_ERR_SYMMETRY, "so_symmetrize: incompatible dimensions");
+            }
+        }
+        std::vector<perm_t> alts(l1.alts);
+        for(size_t k = 0; k < gp.size(); k++) {
+            alts.push_back(gp[k]);
+            for(size_t a = 0; a < l1.alts.size(); a++) {
+                perm_t q(gp[k]);
+                q.permute(l1.alts[a]);
+                alts.push_back(q);
+            }
         }
         se_label l2(l1);
-        std::vector<perm_t> alts(l1.alts);
-        alts.push_back(pp);
-        for(size_t k = 0; k < l1.alts.size(); k++) {
-            perm_t q(pp);
-            q.permute(l1.alts[k]);
-            alts.push_back(q);
-        }
         l2.alts.clear();
         for(size_t k = 0; k < alts.size(); k++) {
             if(alts[k].is_identity()) continue;
@@ -1674,24 +1800,9 @@ void symmetrize_label(const std::vector<std::shared_ptr<sym_element> > &set,
     }
 }
 
-} // unnamed namespace
+symmetry so_symmetrize_groups(const symmetry &sym, const sym_groups &G,
+    const scalar_tr &trp, const scalar_tr &trc) {
 
-symmetry so_symmetrize_pair(const symmetry &sym, const perm_t &perm, double s) {
-    const size_t n = sym.order();
-    perm_t p2(perm);
-    p2.permute(perm);
-    lt_require(perm.n == n && !perm.is_identity() && p2.is_identity(), LT_ERR_PARAM,
-        "symmetrize2: the permutation must be a non-trivial pair permutation");
-    bis_t pb(sym.bis);
-    pb.permute(perm);
-    lt_require(pb == sym.bis, LT_ERR_SYMMETRY,
-        "symmetrize2: the permutation does not map the block index space onto itself");
-    // Index groups as gen_bto_symmetrize2::make_symmetry numbers them: the
-    // k-th pair is (i, perm[i]) for the k-th i with perm[i] > i.
-    std::vector<size_t> gmap;
-    for(size_t i = 0; i < n; i++)
-        if(perm[i] > i) { gmap.push_back(i); gmap.push_back(perm[i]); }
-    const scalar_tr tr(s);
     symmetry r(sym.bis);
     bool perm_done = false;
     for(size_t k = 0; k < sym.subsets.size(); k++) {
@@ -1699,17 +1810,90 @@ symmetry so_symmetrize_pair(const symmetry &sym, const perm_t &perm, double s) {
         const std::vector<std::shared_ptr<sym_element> > &set = sym.subsets[k].second;
         if(t == "perm") {
             perm_done = true;
-            symmetrize_perm(n, &set, perm, tr, r);
+            symmetrize_perm(G, &set, trp, trc, r);
         } else if(t == "part") {
-            symmetrize_part(set, perm, gmap, r);
+            symmetrize_part(G, set, r);
         } else if(t == "label") {
-            symmetrize_label(set, perm, r);
+            symmetrize_label(G, set, r);
         } else {
             throw lt_error(LT_ERR_SYMMETRY, "so_symmetrize: unknown element type " + t);
         }
     }
-    if(!perm_done) symmetrize_perm(n, 0, perm, tr, r);
+    if(!perm_done) symmetrize_perm(G, 0, trp, trc, r);
     return r;
+}
+
+void require_maps_bis(const symmetry &sym, const perm_t &perm, const char *what) {
+    bis_t pb(sym.bis);
+    pb.permute(perm);
+    lt_require(pb == sym.bis, LT_ERR_SYMMETRY, (std::string(what) +
+        ": the permutation does not map the block index space onto itself").c_str());
+}
+
+bool is_pair_perm(const perm_t &p) {
+    perm_t p2(p);
+    p2.permute(p);
+    return !p.is_identity() && p2.is_identity();
+}
+
+} // unnamed namespace
+
+symmetry so_symmetrize_pair(const symmetry &sym, const perm_t &perm, double s) {
+    const size_t n = sym.order();
+    lt_require(perm.n == n && is_pair_perm(perm), LT_ERR_PARAM,
+        "symmetrize2: the permutation must be a non-trivial pair permutation");
+    require_maps_bis(sym, perm, "symmetrize2");
+    // Index groups as gen_bto_symmetrize2::make_symmetry numbers them: the
+    // k-th pair is (i, perm[i]) for the k-th i with perm[i] > i.
+    sym_groups G;
+    G.n = n;
+    G.ngrp = 2;
+    G.nidx = 0;
+    G.idxgrp = seq_t(n, 0);
+    G.symidx = seq_t(n, 0);
+    for(size_t i = 0; i < n; i++) {
+        if(perm[i] <= i) continue;
+        G.nidx++;
+        G.idxgrp[i] = 1;
+        G.idxgrp[perm[i]] = 2;
+        G.symidx[i] = G.symidx[perm[i]] = G.nidx;
+    }
+    return so_symmetrize_groups(sym, G, scalar_tr(s), scalar_tr(1.0));
+}
+
+symmetry so_symmetrize_triple(const symmetry &sym, const perm_t &p1, const perm_t &p2,
+    double s) {
+    const size_t n = sym.order();
+    lt_require(p1.n == n && p2.n == n && is_pair_perm(p1) && is_pair_perm(p2), LT_ERR_PARAM,
+        "symmetrize3: each permutation must be a non-trivial pair permutation");
+    perm_t p12(p1);
+    p12.permute(p2);
+    perm_t p123(p12);
+    p123.permute(p12).permute(p12);
+    lt_require(!p12.is_identity() && p123.is_identity(), LT_ERR_PARAM,
+        "symmetrize3: the two permutations must generate the exchanges of three groups");
+    require_maps_bis(sym, p1, "symmetrize3");
+    require_maps_bis(sym, p2, "symmetrize3");
+    // gen_bto_symmetrize3::make_symmetry: dimensions moved by both
+    // permutations form group 3, by p1 only group 1, by p2 only group 2,
+    // numbered in order within each group.
+    sym_groups G;
+    G.n = n;
+    G.ngrp = 3;
+    G.idxgrp = seq_t(n, 0);
+    G.symidx = seq_t(n, 0);
+    size_t c[4] = { 0, 1, 1, 1 };
+    for(size_t i = 0; i < n; i++) {
+        const bool m1 = p1[i] != i, m2 = p2[i] != i;
+        size_t g = m1 ? (m2 ? 3 : 1) : (m2 ? 2 : 0);
+        if(!g) continue;
+        G.idxgrp[i] = g;
+        G.symidx[i] = c[g]++;
+    }
+    G.nidx = c[1] - 1;
+    lt_require(c[2] - 1 == G.nidx && c[3] - 1 == G.nidx, LT_ERR_PARAM,
+        "symmetrize3: groups of different sizes");
+    return so_symmetrize_groups(sym, G, scalar_tr(s), scalar_tr(1.0));
 }
 
 } // namespace lta

@@ -262,6 +262,13 @@ symmetry so_permute(const symmetry &sym, const perm_t &p);
 //! (gen_block_tensor/impl/gen_bto_symmetrize2_impl.h:131-149).
 symmetry so_symmetrize_pair(const symmetry &sym, const perm_t &perm, double s);
 
+//! Symmetry of the sum over the six exchanges of three index groups, the
+//! pair exchanges weighted by s (gen_bto_symmetrize3::make_symmetry,
+//! gen_block_tensor/impl/gen_bto_symmetrize3_impl.h:132-160): p1 exchanges
+//! groups 1 and 2... as libtensor numbers them, p2 the other pair.
+symmetry so_symmetrize_triple(const symmetry &sym, const perm_t &p1, const perm_t &p2,
+    double s);
+
 //! Symmetry of the result of a contraction
 //! (libtensor/gen_block_tensor/impl/gen_bto_contract2_sym_impl.h:42-86):
 //! so_reduce(so_dirprod(symA, symB)) over the contracted index pairs.

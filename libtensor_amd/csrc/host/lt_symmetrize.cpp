@@ -11,6 +11,7 @@
 // of B in launch k, so no two descriptors of a launch share a destination).
 #include "lt_runtime.h"
 
+#include <algorithm>
 #include <list>
 #include <map>
 #include <set>
@@ -56,54 +57,10 @@ void btod_contract3::compute_block(bool zero, const index_t &id, const tensor_tr
     m_op2->compute_block(zero, id, tr, blk);
 }
 
-btod_symmetrize2::btod_symmetrize2(additive_op &op, const perm_t &perm, bool symm) :
-    m_op(op), m_perm(perm), m_symm(symm), m_last_ms(0.0), m_gather_ms(0.0),
-    m_gather_bytes(0.0) {
+// ---------------------------------------------------------------------------
+// Common gather of the symmetrizations.
 
-    m_sym = so_symmetrize_pair(op.sym(), perm, symm ? 1.0 : -1.0);
-    make_schedule();
-}
-
-btod_symmetrize2::~btod_symmetrize2() { }
-
-void btod_symmetrize2::make_schedule() {
-    const symmetry &s0 = m_op.sym();
-    const dims_t bid = m_sym.bis.block_index_dims();
-    const double sc = m_symm ? 1.0 : -1.0;
-    std::map<size_t, std::vector<rec> > recs;
-    std::set<size_t> insch;
-    const std::vector<size_t> &sch0 = m_op.schedule();
-    for(size_t i = 0; i < sch0.size(); i++) {
-        const size_t ai0 = sch0[i];
-        orbit o(s0, ai0, true);
-        for(size_t j = 0; j < o.orb.size(); j++) {
-            const size_t aj1 = o.orb[j].first;
-            index_t j2 = get_index(aj1, bid);
-            m_perm.apply(j2);
-            const size_t aj2 = abs_index(j2, bid);
-            short_orbit so1(m_sym, aj1, false), so2(m_sym, aj2, false);
-            if(so1.acidx == aj1) {
-                if(insch.insert(aj1).second) m_sched.push_back(aj1);
-                rec r;
-                r.ai = ai0;
-                r.tr = o.get_transf(aj1);
-                recs[aj1].push_back(r);
-            }
-            if(so2.acidx == aj2) {
-                if(insch.insert(aj2).second) m_sched.push_back(aj2);
-                rec r;
-                r.ai = ai0;
-                r.tr = o.get_transf(aj1);
-                r.tr.perm.permute(m_perm);
-                r.tr.st.scale(sc);
-                recs[aj2].push_back(r);
-            }
-        }
-    }
-    m_recs.assign(recs.begin(), recs.end());
-}
-
-const std::vector<btod_symmetrize2::rec> *btod_symmetrize2::records(size_t aib) const {
+const std::vector<symmetrizer::rec> *symmetrizer::records(size_t aib) const {
     std::vector<std::pair<size_t, std::vector<rec> > >::const_iterator it =
         std::lower_bound(m_recs.begin(), m_recs.end(), aib,
             [](const std::pair<size_t, std::vector<rec> > &x, size_t v) { return x.first < v; });
@@ -111,15 +68,15 @@ const std::vector<btod_symmetrize2::rec> *btod_symmetrize2::records(size_t aib) 
     return &it->second;
 }
 
-double btod_symmetrize2::run(btensor &b, bool add, double d, int mode) {
+double symmetrizer::run(btensor &b, bool add, double d, int mode) {
     context &cx = ctx();
-    cx.require_device("btod_symmetrize2");
+    cx.require_device("btod_symmetrize");
     lt_require(&b.ctx() == &cx, LT_ERR_PARAM, "tensors of different contexts");
-    lt_require(b.bis() == m_sym.bis, LT_ERR_DIMS, "btod_symmetrize2: block index space of B");
+    lt_require(b.bis() == m_sym.bis, LT_ERR_DIMS, "btod_symmetrize: block index space of B");
     const dims_t bid = b.bidims();
     if(add) {
         lt_require(same_orbits(m_sym, b.sym(), bid), LT_ERR_SYMMETRY,
-            "btod_symmetrize2: B += ... needs B with the symmetry of the result");
+            "btod_symmetrize: B += ... needs B with the symmetry of the result");
     }
     if(!m_x) m_x.reset(new btensor(cx, m_sym.bis));
     btensor &x = *m_x;
@@ -159,16 +116,55 @@ double btod_symmetrize2::run(btensor &b, bool add, double d, int mode) {
     return ms;
 }
 
-void btod_symmetrize2::perform(btensor &b, int mode) {
-    run(b, false, 1.0, mode);
+// ---------------------------------------------------------------------------
+// btod_symmetrize2
+
+btod_symmetrize2::btod_symmetrize2(additive_op &op, const perm_t &perm, bool symm) :
+    symmetrizer(op), m_perm(perm), m_symm(symm) {
+
+    m_sym = so_symmetrize_pair(op.sym(), perm, symm ? 1.0 : -1.0);
+    make_schedule();
 }
 
-void btod_symmetrize2::perform_add(btensor &b, double d, int mode) {
-    run(b, true, d, mode);
-}
-
-double btod_symmetrize2::perform_into(btensor &t, int mode) {
-    return run(t, false, 1.0, mode);
+void btod_symmetrize2::make_schedule() {
+    // gen_bto_symmetrize2::make_schedule (impl :152-199): for every block j
+    // of the orbit of a scheduled X_ai0, the block j and its image P(j),
+    // where canonical in the new symmetry, receive tr_j(X_ai0) and
+    // (P, s) tr_j(X_ai0).
+    const symmetry &s0 = m_op.sym();
+    const dims_t bid = m_sym.bis.block_index_dims();
+    const double sc = m_symm ? 1.0 : -1.0;
+    std::map<size_t, std::vector<rec> > recs;
+    std::set<size_t> insch;
+    const std::vector<size_t> &sch0 = m_op.schedule();
+    for(size_t i = 0; i < sch0.size(); i++) {
+        const size_t ai0 = sch0[i];
+        orbit o(s0, ai0, true);
+        for(size_t j = 0; j < o.orb.size(); j++) {
+            const size_t aj1 = o.orb[j].first;
+            index_t j2 = get_index(aj1, bid);
+            m_perm.apply(j2);
+            const size_t aj2 = abs_index(j2, bid);
+            short_orbit so1(m_sym, aj1, false), so2(m_sym, aj2, false);
+            if(so1.acidx == aj1) {
+                if(insch.insert(aj1).second) m_sched.push_back(aj1);
+                rec r;
+                r.ai = ai0;
+                r.tr = o.get_transf(aj1);
+                recs[aj1].push_back(r);
+            }
+            if(so2.acidx == aj2) {
+                if(insch.insert(aj2).second) m_sched.push_back(aj2);
+                rec r;
+                r.ai = ai0;
+                r.tr = o.get_transf(aj1);
+                r.tr.perm.permute(m_perm);
+                r.tr.st.scale(sc);
+                recs[aj2].push_back(r);
+            }
+        }
+    }
+    m_recs.assign(recs.begin(), recs.end());
 }
 
 void btod_symmetrize2::compute_block(bool zero, const index_t &ib, const tensor_tr &trb,
@@ -228,6 +224,162 @@ void btod_symmetrize2::compute_block(bool zero, const index_t &ib, const tensor_
             throw;
         }
         cx.free(tmp);
+    }
+}
+
+// ---------------------------------------------------------------------------
+// btod_symmetrize3
+
+btod_symmetrize3::btod_symmetrize3(additive_op &op, const perm_t &p1, const perm_t &p2,
+    bool symm) : symmetrizer(op), m_p1(p1), m_p2(p2), m_symm(symm) {
+
+    m_sym = so_symmetrize_triple(op.sym(), p1, p2, symm ? 1.0 : -1.0);
+    m_sched0 = op.schedule();
+    std::sort(m_sched0.begin(), m_sched0.end());
+    make_schedule();
+}
+
+void btod_symmetrize3::make_schedule() {
+    // gen_bto_symmetrize3::make_schedule (impl :162-290): the canonical
+    // blocks (new symmetry) of the orbits that the six images of the blocks
+    // of every scheduled orbit of X fall in; then the sources of each
+    // (make_schedule_blk).
+    const symmetry &s0 = m_op.sym();
+    const dims_t bid = m_sym.bis.block_index_dims();
+    std::set<size_t> sch;
+    const std::vector<size_t> &sch0 = m_op.schedule();
+    for(size_t i = 0; i < sch0.size(); i++) {
+        std::set<size_t> visited;
+        orbit o(s0, sch0[i], true);
+        for(size_t j = 0; j < o.orb.size(); j++) {
+            const index_t i1 = get_index(o.orb[j].first, bid);
+            index_t im[6] = { i1, i1, i1, i1, i1, i1 };
+            m_p1.apply(im[1]);
+            m_p2.apply(im[2]);
+            m_p1.apply(im[3]); m_p2.apply(im[3]);
+            m_p2.apply(im[4]); m_p1.apply(im[4]);
+            m_p1.apply(im[5]); m_p2.apply(im[5]); m_p1.apply(im[5]);
+            for(size_t k = 0; k < 6; k++) {
+                const size_t a = abs_index(im[k], bid);
+                if(visited.count(a)) continue;
+                orbit ok(m_sym, a, true);
+                sch.insert(ok.acidx);
+                for(size_t q = 0; q < ok.orb.size(); q++) visited.insert(ok.orb[q].first);
+            }
+        }
+    }
+    m_sched.assign(sch.begin(), sch.end());
+    for(size_t i = 0; i < m_sched.size(); i++) {
+        std::vector<rec> rs;
+        block_records(m_sched[i], rs);
+        if(!rs.empty()) m_recs.push_back(std::make_pair(m_sched[i], rs));
+    }
+}
+
+void btod_symmetrize3::block_records(size_t aib, std::vector<rec> &out) const {
+    // gen_bto_symmetrize3::make_schedule_blk (impl :300-380): for each of
+    // the six images of the block, the operand's canonical block (when
+    // scheduled) with the orbit transformation followed by the exchange back
+    // and its sign; sources of one block with one permutation summed.
+    const symmetry &s0 = m_op.sym();
+    const dims_t bid = m_sym.bis.block_index_dims();
+    const double sc = m_symm ? 1.0 : -1.0;
+    const index_t i0 = get_index(aib, bid);
+    index_t im[6] = { i0, i0, i0, i0, i0, i0 };
+    m_p1.apply(im[1]);
+    m_p2.apply(im[2]);
+    m_p1.apply(im[3]); m_p2.apply(im[3]);
+    m_p2.apply(im[4]); m_p1.apply(im[4]);
+    m_p1.apply(im[5]); m_p2.apply(im[5]); m_p1.apply(im[5]);
+    // the exchanges back, applied to the transformation after the orbit's
+    const perm_t *back[6][3] = {
+        { 0, 0, 0 }, { &m_p1, 0, 0 }, { &m_p2, 0, 0 }, { &m_p2, &m_p1, 0 },
+        { &m_p1, &m_p2, 0 }, { &m_p1, &m_p2, &m_p1 } };
+    const bool sign[6] = { false, true, true, false, false, true };
+    std::list<rec> l;
+    for(size_t k = 0; k < 6; k++) {
+        orbit ok(s0, abs_index(im[k], bid), true);
+        if(!std::binary_search(m_sched0.begin(), m_sched0.end(), ok.acidx)) continue;
+        rec r;
+        r.ai = ok.acidx;
+        r.tr = ok.get_transf(abs_index(im[k], bid));
+        for(size_t q = 0; q < 3; q++) if(back[k][q]) r.tr.perm.permute(*back[k][q]);
+        if(sign[k]) r.tr.st.scale(sc);
+        l.push_back(r);
+    }
+    while(!l.empty()) {
+        std::list<rec>::iterator i = l.begin();
+        const size_t ai = i->ai;
+        const perm_t p0 = i->tr.perm;
+        double c = 0.0;
+        while(i != l.end()) {
+            if(i->ai == ai && i->tr.perm == p0) {
+                c += i->tr.st.c;
+                i = l.erase(i);
+            } else {
+                ++i;
+            }
+        }
+        if(c != 0.0) {
+            rec r;
+            r.ai = ai;
+            r.tr = tensor_tr(p0, scalar_tr(c));
+            out.push_back(r);
+        }
+    }
+}
+
+void btod_symmetrize3::compute_block(bool zero, const index_t &ib, const tensor_tr &trb,
+    double *blk) {
+
+    // gen_bto_symmetrize3::compute_block (impl :66-128): as for
+    // btod_symmetrize2, with the shared operand block computed untransformed;
+    // a block without sources is set to zero.
+    context &cx = ctx();
+    cx.require_device("btod_symmetrize3::compute_block");
+    const dims_t bid = m_sym.bis.block_index_dims();
+    lt_require(ib.n == bid.n && trb.perm.n == bid.n, LT_ERR_PARAM, "block index order");
+    std::vector<rec> rs;
+    block_records(abs_index(ib, bid), rs);
+    std::list<rec> l(rs.begin(), rs.end());
+    bool zero1 = zero;
+    while(!l.empty()) {
+        const size_t ai = l.front().ai;
+        size_t n = 0;
+        for(std::list<rec>::const_iterator j = l.begin(); j != l.end(); ++j)
+            if(j->ai == ai) n++;
+        const index_t ia = get_index(ai, bid);
+        if(n == 1) {
+            tensor_tr tri(l.front().tr);
+            tri.transform(trb);
+            m_op.compute_block(zero1, ia, tri, blk);
+            zero1 = false;
+            l.pop_front();
+            continue;
+        }
+        const dims_t dt = m_sym.bis.block_dims(ia);
+        double *tmp = (double*)cx.alloc(std::max<size_t>(dt.size(), 1) * sizeof(double));
+        try {
+            m_op.compute_block(true, ia, tensor_tr(bid.n), tmp);
+            for(std::list<rec>::iterator j = l.begin(); j != l.end();) {
+                if(j->ai != ai) { ++j; continue; }
+                tensor_tr trj(j->tr);
+                trj.transform(trb);
+                dense_permute(cx, tmp, dt, trj, blk, !zero1);
+                zero1 = false;
+                j = l.erase(j);
+            }
+        } catch(...) {
+            cx.free(tmp);
+            throw;
+        }
+        cx.free(tmp);
+    }
+    if(zero1 && zero) {
+        dims_t db = m_sym.bis.block_dims(ib);
+        cx.activate();
+        hip_check(dev::launch_fill(blk, db.size(), 0.0, cx.stream()), "fill");
+        cx.sync();
     }
 }
 

@@ -21,6 +21,7 @@
 #include <libtensor/libtensor.h>
 #include <libtensor/block_tensor/btod_contract2.h>
 #include <libtensor/block_tensor/btod_symmetrize2.h>
+#include <libtensor/block_tensor/btod_symmetrize3.h>
 #include <libtensor/block_tensor/btod_contract3.h>
 #include <libtensor/block_tensor/btod_traits.h>
 #include <libtensor/core/orbit.h>
@@ -211,17 +212,37 @@ void print_symt(FILE *f, const symmetry<NC, double> &sym) {
     }
 }
 
+// btod_symmetrize3 (libtensor instantiates it for orders 3..8 only).
+template<size_t NC>
+additive_gen_bto<NC, btod_traits::bti_traits> *make_symm3(
+    additive_gen_bto<NC, btod_traits::bti_traits> &op, const spec::problem::symm2 &y,
+    std::true_type) {
+    return new btod_symmetrize3<NC>(op, make_perm<NC>(y.perm), make_perm<NC>(y.perm2),
+        y.symm != 0);
+}
+
+template<size_t NC>
+additive_gen_bto<NC, btod_traits::bti_traits> *make_symm3(
+    additive_gen_bto<NC, btod_traits::bti_traits> &, const spec::problem::symm2 &,
+    std::false_type) {
+    throw std::runtime_error("symmetrize3 needs a result of order 3 or more");
+}
+
 // btod_symmetrize2 chain over the contraction (libtensor instantiates it for
 // orders 2..8 only).
 template<size_t NC>
 int run_symm(const spec::problem &p, additive_gen_bto<NC, btod_traits::bti_traits> &op0,
     FILE *f, const std::string &out, const options &opt, std::true_type) {
-    std::vector<btod_symmetrize2<NC>*> chain;
+    std::vector<additive_gen_bto<NC, btod_traits::bti_traits>*> chain;
     additive_gen_bto<NC, btod_traits::bti_traits> *cur = &op0;
     double t0 = wall_ms();
     for(size_t i = 0; i < p.symm.size(); i++) {
-        chain.push_back(new btod_symmetrize2<NC>(*cur, make_perm<NC>(p.symm[i].perm),
-            p.symm[i].symm != 0));
+        if(p.symm[i].perm2.empty())
+            chain.push_back(new btod_symmetrize2<NC>(*cur, make_perm<NC>(p.symm[i].perm),
+                p.symm[i].symm != 0));
+        else
+            chain.push_back(make_symm3<NC>(*cur, p.symm[i],
+                std::integral_constant<bool, (NC >= 3)>()));
         cur = chain.back();
     }
     fprintf(f, "symmetrize_ms %.6f\n", wall_ms() - t0);

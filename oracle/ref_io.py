@@ -92,8 +92,16 @@ def write_spec(prob, path, dense=None, tensors=None):
             c["N"], c["M"], c["K"], " ".join(str(p) for p in permc),
             " ".join("%d %d" % (a, b) for a, b in c["pairs"]),
             c.get("ka", 1.0), c.get("kb", 1.0), c.get("kc", 1.0)))
-    for perm, symm in prob.get("symm", []):
-        lines.append("symmetrize2 %s %d" % (" ".join(str(x) for x in perm), 1 if symm else 0))
+    for entry in prob.get("symm", []):
+        if len(entry) == 3:
+            perm, perm2, symm = entry
+            lines.append("symmetrize3 %s %s %d" % (" ".join(str(x) for x in perm),
+                                                  " ".join(str(x) for x in perm2),
+                                                  1 if symm else 0))
+        else:
+            perm, symm = entry
+            lines.append("symmetrize2 %s %d" % (" ".join(str(x) for x in perm),
+                                                1 if symm else 0))
     if prob.get("self"):
         lines.append("self 1")
     with open(path, "w") as f:

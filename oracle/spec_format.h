@@ -25,6 +25,9 @@
 //             <permd x N1+N2+N3> <ab c x K2> <kd>
 //                                        btod_contract3 of A, B and C
 //                                        (tensor C declared like A and B)
+//   symmetrize3 <perm1 x N+M> <perm2 x N+M> <symm 0|1>
+//                                        wrap it in btod_symmetrize3 (same
+//                                        chain as symmetrize2)
 //   symmetrize2 <perm x N+M> <symm 0|1>  wrap the contraction in
 //                                        btod_symmetrize2 (repeatable: each
 //                                        line wraps the previous operation)
@@ -94,7 +97,7 @@ struct problem {
     size_t N1, N2, N3, K1, K2;
     std::vector<size_t> permab, permd, ka1, kb1, kab2, kc2;
     double kd;
-    struct symm2 { std::vector<size_t> perm; int symm; };
+    struct symm2 { std::vector<size_t> perm, perm2; int symm; };   // perm2: symmetrize3
     std::vector<symm2> symm;        // in file order (innermost first)
     problem() : N(0), M(0), K(0), sa(1), sb(1), sc(1), self(false), c3(false),
         N1(0), N2(0), N3(0), K1(0), K2(0), kd(1) { }
@@ -177,6 +180,10 @@ inline problem parse(const std::string &path) {
             p.permd = read_sz(in, p.N1 + p.N2 + p.N3);
             for(size_t i = 0; i < p.K2; i++) { size_t a, b; in >> a >> b; p.kab2.push_back(a); p.kc2.push_back(b); }
             in >> p.kd;
+        } else if(kw == "symmetrize3") {
+            problem::symm2 y; y.perm = read_sz(in, p.N + p.M); y.perm2 = read_sz(in, p.N + p.M);
+            in >> y.symm;
+            p.symm.push_back(y);
         } else if(kw == "symmetrize2") {
             problem::symm2 y; y.perm = read_sz(in, p.N + p.M); in >> y.symm;
             p.symm.push_back(y);

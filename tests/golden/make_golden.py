@@ -26,6 +26,7 @@ sys.path.insert(0, os.path.join(os.path.dirname(os.path.dirname(HERE)), "oracle"
 
 import problems as P  # noqa: E402
 import libtensor_amd as lt  # noqa: E402
+from libtensor_amd import workloads as W  # noqa: E402
 import numpy as np  # noqa: E402
 
 
@@ -232,22 +233,94 @@ def write_c3(work):
         print("%-22s schedule %4d" % (name, len(ref["schedule"])))
 
 
+def sym3_problem(kind, n_o, splits, spin, labels):
+    """Contractions whose results have three dimensions (or three pairs) of one
+    block structure, with spin partitions and C2v labels on every dimension:
+    "k1": C[i,j,k,l] = A[i,j,k,m] B[m,l]; "pairs": C[i,a,j,b,k,c] =
+    A[i,a,j,m] B[m,b,k,c] with B symmetric under (ia)(jb)... exchange."""
+    blocks = len(splits) + 1
+    labs = [(0, 2)[b % 2] for b in range(blocks)]
+    labv = [(1, 3)[b % 2] for b in range(blocks)]
+
+    def T(kinds, conserve, perms=()):
+        n = len(kinds)
+        t = {"dims": [n_o] * n, "splits": [], "elems": [("perm", list(p), c) for p, c in perms]}
+        for d in range(n):
+            m = [1 if q == d else 0 for q in range(n)]
+            t["splits"] += [(m, p) for p in splits]
+        if spin:
+            t["elems"].append(W.spin_part(n, conserve))
+        if labels:
+            assigns = []
+            for d, k in enumerate(kinds):
+                m = [1 if q == d else 0 for q in range(n)]
+                assigns += [(m, b, l) for b, l in enumerate(labs if k == "o" else labv)]
+            t["elems"].append(("label", "c2v", assigns, [0]))
+        return t
+    if kind == "k1":
+        prob = {"A": T("oooo", lambda p: p[0] + p[1] == p[2] + p[3]),
+                "B": T("oo", lambda p: p[0] == p[1]),
+                "contr": {"N": 3, "M": 1, "K": 1, "permc": None, "pairs": [(3, 0)]}}
+    else:
+        prob = {"A": T("ovoo", lambda p: p[0] + p[2] == p[1] + p[3]),
+                "B": T("ovov", lambda p: p[0] + p[2] == p[1] + p[3], [([2, 3, 0, 1], 1.0)]),
+                "contr": {"N": 3, "M": 3, "K": 1, "permc": None, "pairs": [(3, 0)]}}
+    if labels:
+        prob["tables"] = [W.C2V]
+    return prob
+
+
+def swap(n, i, j):
+    p = list(range(n))
+    p[i], p[j] = p[j], p[i]
+    return p
+
+
+# btod_symmetrize3 (entries (perm1, perm2, symmetric)), over a golden case or
+# over a problem of its own (inputs stored with the fixture).
+SYMM3_CASES = [
+    ("symm3_self_ladder_012", "self_ladder", [(swap(4, 0, 1), swap(4, 0, 2), True)]),
+    ("symm3_self_ladder_123", "self_ladder", [(swap(4, 1, 2), swap(4, 1, 3), False)]),
+    ("symm3_self_ladder_then_2", "self_ladder", [(swap(4, 1, 2), swap(4, 1, 3), False),
+                                                 ([1, 0, 2, 3], True)]),
+    ("symm3_k1_spin_label", ("k1", 10, [2, 5, 7], True, True),
+     [(swap(4, 0, 1), swap(4, 0, 2), True)]),
+    ("symm3_k1_spin", ("k1", 10, [2, 5, 7], True, False),
+     [(swap(4, 1, 2), swap(4, 1, 3), False)]),
+    ("symm3_pairs_spin", ("pairs", 4, [2], True, False),
+     [([2, 3, 0, 1, 4, 5], [4, 5, 2, 3, 0, 1], True)]),
+    ("symm3_err_groups", "self_ladder", [(swap(4, 0, 1), swap(4, 2, 3), True)]),
+]
+
+
 def write_symm(work):
-    """Fixtures symm_*.json (+ .npz with libtensor's result)."""
-    for name, base, symm in SYMM_CASES:
-        prob, dense, meta, result = P.load_case(os.path.join(HERE, base))
+    """Fixtures symm_*.json (+ .npz with libtensor's result; with the inputs
+    too for a problem of its own)."""
+    ctx = lt.Context(-1)
+    for name, base, symm in SYMM_CASES + SYMM3_CASES:
+        if isinstance(base, tuple):
+            prob = sym3_problem(*base)
+            dense = P.make_data(ctx, prob, 5000 + len(name))
+            out = {"prob": prob, "symm": symm}
+        else:
+            prob, dense, meta, result = P.load_case(os.path.join(HERE, base))
+            out = {"base": base, "symm": symm}
         prob = dict(prob)
         prob["symm"] = symm
-        out = {"base": base, "symm": symm}
         try:
             ref = P.run_ref(prob, dense, os.path.join(work, name))
         except RuntimeError:
             out["error"] = True
             ref = None
         if ref is not None:
-            out["schedule"] = ref["schedule"]
+            # the schedule as a set: libtensor's btod_symmetrize3 can list a
+            # block more than once
+            out["schedule"] = sorted(set(ref["schedule"]))
             out["symt"] = {str(k): list(v) for k, v in ref["symt"].items()}
-            np.savez_compressed(os.path.join(HERE, name + ".npz"), result=ref["result"])
+            arrays = {"result": ref["result"]}
+            if "prob" in out:
+                arrays.update(dense)
+            np.savez_compressed(os.path.join(HERE, name + ".npz"), **arrays)
         with open(os.path.join(HERE, name + ".json"), "w") as f:
             json.dump(out, f)
         print("%-26s %s" % (name, "error" if ref is None else

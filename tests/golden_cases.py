@@ -11,10 +11,10 @@ GOLDEN = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
 
 
 def names():
-    """btod_contract2 cases (the btod_symmetrize2 fixtures symm_* and the
+    """btod_contract2 cases (the symmetrization fixtures symm* and the
     btod_contract3 fixtures c3* excluded)."""
     return sorted(os.path.basename(p)[:-5] for p in glob.glob(os.path.join(GOLDEN, "*.json"))
-                  if not os.path.basename(p).startswith(("symm_", "c3")))
+                  if not os.path.basename(p).startswith(("symm", "c3")))
 
 
 def c3_names():
@@ -30,10 +30,10 @@ def load_c3(name):
 
 
 def symm_names(errors=False):
-    """btod_symmetrize2 fixtures: those with a result, or (errors) those
-    libtensor rejects."""
+    """btod_symmetrize2 / btod_symmetrize3 fixtures (symm_*, symm3_*): those
+    with a result, or (errors) those libtensor rejects."""
     out = []
-    for p in sorted(glob.glob(os.path.join(GOLDEN, "symm_*.json"))):
+    for p in sorted(glob.glob(os.path.join(GOLDEN, "symm*.json"))):
         with open(p) as f:
             meta = json.load(f)
         if bool(meta.get("error")) == errors:
@@ -46,11 +46,15 @@ def load_symm(name):
     symmetrized result or None)."""
     with open(os.path.join(GOLDEN, name + ".json")) as f:
         meta = json.load(f)
-    prob, dense, bmeta, bresult = load(meta["base"])
-    result = None
     path = os.path.join(GOLDEN, name + ".npz")
-    if os.path.exists(path):
-        result = np.load(path, allow_pickle=False)["result"]
+    z = np.load(path, allow_pickle=False) if os.path.exists(path) else None
+    if "base" in meta:
+        prob, dense, bmeta, bresult = load(meta["base"])
+    else:
+        # a problem of its own: inputs stored with the fixture, no base result
+        prob, bresult = meta["prob"], None
+        dense = {k: z[k] for k in ("A", "B") if z is not None and k in z.files}
+    result = z["result"] if z is not None else None
     return prob, dense, bresult, meta, result
 
 

@@ -1,7 +1,9 @@
-"""btod_symmetrize2 on the device through the C ABI: B = X + s P(X) for the
-contraction X of a golden case (and symmetrizations of symmetrizations)
-against libtensor's result (fixtures symm_*), with B := result, B += d
-result and direct blocks (compute_block, transformed and accumulated)."""
+"""btod_symmetrize2 / btod_symmetrize3 on the device through the C ABI:
+B = X + s P(X), and the sum over the exchanges of three index groups, for
+the contraction X of a golden case or of a problem of the fixture's own
+(and symmetrizations of symmetrizations) against libtensor's result
+(fixtures symm_*, symm3_*), with B := result, B += d result and direct
+blocks (compute_block, transformed and accumulated)."""
 import sys
 
 import numpy as np
@@ -23,8 +25,11 @@ def _say(*a):
 
 def chain(op, symm):
     ops = [op]
-    for perm, s in symm:
-        ops.append(lt.btod_symmetrize2(ops[-1], perm, s))
+    for entry in symm:
+        if len(entry) == 3:
+            ops.append(lt.btod_symmetrize3(ops[-1], entry[0], entry[1], entry[2]))
+        else:
+            ops.append(lt.btod_symmetrize2(ops[-1], entry[0], entry[1]))
     return ops
 
 
@@ -67,8 +72,11 @@ def test_compute_block(gpu_ctx, name):
         idx = c.index(bi)
         st, dims = c.block_start(idx), c.block_dims(idx)
         ref = result[tuple(slice(s0, s0 + n) for s0, n in zip(st, dims))]
-        if bi not in canon:
-            ref = np.zeros(dims)   # no sources: the block is written as zero
+        if bi not in canon and len(meta["symm"][-1]) == 2:
+            # btod_symmetrize2 has sources for its scheduled blocks only: the
+            # block is written as zero (btod_symmetrize3 gathers the sources
+            # of any block, which gives the block's value)
+            ref = np.zeros(dims)
         out = lt.device_array(gpu_ctx, tuple(dims), np.full(dims, 7.0))
         s.compute_block(bi, out)
         assert np.max(np.abs(out.download() - ref)) <= TOL * scale, bi

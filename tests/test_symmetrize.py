@@ -1,22 +1,30 @@
-"""btod_symmetrize2 host side (no GPU): the result symmetry (canonical block,
-orbit transformation and allowedness of every block) and the schedule of
-B = X + s P(X) against libtensor's (fixtures symm_* written by the real
-reference build, tests/golden/make_golden.py), including symmetrizations of
-symmetrizations and the cases libtensor rejects.  The stored results are
-checked against the closed form X + s P(X) of the base case's golden
-contraction result, which pins what the fixtures mean."""
+"""btod_symmetrize2 / btod_symmetrize3 host side (no GPU): the result
+symmetry (canonical block, orbit transformation and allowedness of every
+block) and the schedule of B = X + s P(X), and of the sum over the six
+exchanges of three index groups, against libtensor's (fixtures symm_* and
+symm3_* written by the real reference build, tests/golden/make_golden.py),
+including symmetrizations of symmetrizations and the cases libtensor
+rejects.  The stored results are checked against the closed forms applied
+to the contraction result X (the base case's golden result, or the dense
+oracle's), which pins what the fixtures mean."""
 import numpy as np
 import pytest
 
+import dense_oracle as O
 import golden_cases as G
 import libtensor_amd as lt
 import problems as P
 
 
 def chain(op, symm):
+    """The chain of symmetrizations of a fixture: (perm, symmetric) entries
+    are btod_symmetrize2, (perm1, perm2, symmetric) btod_symmetrize3."""
     ops = [op]
-    for perm, s in symm:
-        ops.append(lt.btod_symmetrize2(ops[-1], perm, s))
+    for entry in symm:
+        if len(entry) == 3:
+            ops.append(lt.btod_symmetrize3(ops[-1], entry[0], entry[1], entry[2]))
+        else:
+            ops.append(lt.btod_symmetrize2(ops[-1], entry[0], entry[1]))
     return ops
 
 
@@ -40,14 +48,28 @@ def test_rejected_like_libtensor(host_ctx, name):
 
 
 def closed_form(x, symm):
-    for perm, s in symm:
-        x = x + (1.0 if s else -1.0) * np.transpose(x, perm)
+    for entry in symm:
+        s = 1.0 if entry[-1] else -1.0
+        if len(entry) == 2:
+            x = x + s * np.transpose(x, entry[0])
+            continue
+        p1, p2 = entry[0], entry[1]
+
+        def t(y, *ps):
+            for p in ps:
+                y = np.transpose(y, p)
+            return y
+        x = x + s * t(x, p1) + s * t(x, p2) + t(x, p1, p2) + t(x, p2, p1) + \
+            s * t(x, p1, p2, p1)
     return x
 
 
 @pytest.mark.parametrize("name", G.symm_names())
 def test_fixture_is_the_closed_form(name):
     prob, dense, bresult, meta, result = G.load_symm(name)
+    if bresult is None:
+        bresult = O.btod_contract2_dense(prob["contr"], dense["A"],
+                                         dense["A"] if prob.get("self") else dense["B"])
     exp = closed_form(bresult, meta["symm"])
     scale = float(np.max(np.abs(exp))) or 1.0
     assert np.max(np.abs(result - exp)) <= 1e-13 * scale
