@@ -283,6 +283,16 @@ int lt_btod_contract3_create(size_t N1, size_t N2, size_t N3, size_t K1, size_t 
     const size_t *permab, const size_t *ka1, const size_t *kb1, const size_t *permd,
     const size_t *kab2, const size_t *kc2, lt_btensor *a, lt_btensor *b, lt_btensor *c,
     double kd, lt_btod_contract3 **out);
+/* btod_symcontract3 (libtensor/block_tensor/btod_symcontract3.h;
+ * gen_block_tensor/impl/gen_bto_symcontract3_impl.h): as btod_contract3 with
+ * AB replaced by AB + s P(AB) before the second contraction (P =
+ * sympermab, a pair permutation of AB's indexes; s = +1 if symmab), with
+ * the symmetry and schedule of btod_symmetrize2.  Returns a btod_contract3
+ * handle (the lt_btod_contract3_* calls apply). */
+int lt_btod_symcontract3_create(size_t N1, size_t N2, size_t N3, size_t K1, size_t K2,
+    const size_t *permab, const size_t *ka1, const size_t *kb1, const size_t *permd,
+    const size_t *kab2, const size_t *kc2, lt_btensor *a, lt_btensor *b,
+    const size_t *sympermab, int symmab, lt_btensor *c, double kd, lt_btod_contract3 **out);
 int lt_btod_contract3_destroy(lt_btod_contract3 *op);
 int lt_btod_contract3_perform(lt_btod_contract3 *op, lt_btensor *d, int mode);
 int lt_btod_contract3_perform_add(lt_btod_contract3 *op, lt_btensor *d, double k, int mode);

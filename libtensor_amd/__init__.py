@@ -401,7 +401,9 @@ class btod_contract3(object):
     contraction2 with N = N1, M = N2 + K2, K = K1) contracts A with B into the
     intermediate AB, contr2 (N = N1 + N2, M = N3, K = K2) contracts AB with C."""
 
-    def __init__(self, contr1, contr2, bta, btb, btc, kd=1.0):
+    def __init__(self, contr1, contr2, bta, btb, btc, kd=1.0, sympermab=None, symmab=True):
+        """sympermab: btod_symcontract3 (AB replaced by AB + s P(AB), s = +1 if
+        symmab, before the second contraction)."""
         for ct in (contr1, contr2):
             if len(ct.pairs) != ct.K:
                 raise LtError(-1, "contraction is incomplete")
@@ -411,11 +413,15 @@ class btod_contract3(object):
         if n2 < 0 or contr2.N != n1 + n2:
             raise LtError(-1, "contractions do not chain")
         h = ctypes.c_void_p()
-        call("lt_btod_contract3_create", n1, n2, n3, k1, k2, sizes(contr1.permc),
-             sizes([p[0] for p in contr1.pairs]), sizes([p[1] for p in contr1.pairs]),
-             sizes(contr2.permc), sizes([p[0] for p in contr2.pairs]),
-             sizes([p[1] for p in contr2.pairs]), bta._h, btb._h, btc._h, float(kd),
-             ctypes.byref(h))
+        args = (n1, n2, n3, k1, k2, sizes(contr1.permc), sizes([p[0] for p in contr1.pairs]),
+                sizes([p[1] for p in contr1.pairs]), sizes(contr2.permc),
+                sizes([p[0] for p in contr2.pairs]), sizes([p[1] for p in contr2.pairs]),
+                bta._h, btb._h)
+        if sympermab is None:
+            call("lt_btod_contract3_create", *args, btc._h, float(kd), ctypes.byref(h))
+        else:
+            call("lt_btod_symcontract3_create", *args, sizes(sympermab), 1 if symmab else 0,
+                 btc._h, float(kd), ctypes.byref(h))
         self._h = h
         self.n = n1 + n2 + n3
         self.a, self.b, self.c = bta, btb, btc

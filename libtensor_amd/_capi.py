@@ -132,6 +132,8 @@ _SIGS = {
     "lt_btod_contract3_create": [ctypes.c_size_t] * 5 + [c_size_p] * 6 + [vp, vp, vp,
                                                                     ctypes.c_double,
                                                                     ctypes.POINTER(vp)],
+    "lt_btod_symcontract3_create": [ctypes.c_size_t] * 5 + [c_size_p] * 6 + [
+        vp, vp, c_size_p, ctypes.c_int, vp, ctypes.c_double, ctypes.POINTER(vp)],
     "lt_btod_contract3_destroy": [vp],
     "lt_btod_contract3_perform": [vp, vp, ctypes.c_int],
     "lt_btod_contract3_perform_add": [vp, vp, ctypes.c_double, ctypes.c_int],

@@ -852,30 +852,57 @@ int lt_btod_symmetrize2_create_c3(lt_btod_contract3 *op, const size_t *perm, int
     });
 }
 
+namespace {
+
+void c3_create(size_t N1, size_t N2, size_t N3, size_t K1, size_t K2,
+    const size_t *permab, const size_t *ka1, const size_t *kb1, const size_t *permd,
+    const size_t *kab2, const size_t *kc2, lt_btensor *a, lt_btensor *b, lt_btensor *c,
+    double kd, const size_t *sympermab, int symmab, lt_btod_contract3 **out) {
+
+    need(a, "a"); need(b, "b"); need(c, "c"); need(out, "out");
+    if(K1) { need(ka1, "ka1"); need(kb1, "kb1"); }
+    if(K2) { need(kab2, "kab2"); need(kc2, "kc2"); }
+    lt_require(&a->t->ctx() == &b->t->ctx() && &a->t->ctx() == &c->t->ctx(), LT_ERR_PARAM,
+        "tensors of different contexts");
+    lt_require(a->t->bis().order() == N1 + K1 && b->t->bis().order() == N2 + K1 + K2 &&
+        c->t->bis().order() == N3 + K2, LT_ERR_DIMS, "btod_contract3: tensor orders");
+    contraction2 c1 = make_contraction(N1, N2 + K2, K1, permab, ka1, kb1);
+    contraction2 c2 = make_contraction(N1 + N2, N3, K2, permd, kab2, kc2);
+    perm_t ps;
+    if(sympermab) ps = perm_or_identity(N1 + N2 + K2, sympermab);
+    lt_btod_contract3 *h = new lt_btod_contract3;
+    h->ctx = a->ctx;
+    try {
+        h->op = new btod_contract3(c1, c2, *a->t, *b->t, *c->t, kd, sympermab ? &ps : 0,
+            symmab != 0);
+    } catch(...) {
+        delete h;
+        throw;
+    }
+    ctx_retain(h->ctx);
+    *out = h;
+}
+
+} // unnamed namespace
+
+int lt_btod_symcontract3_create(size_t N1, size_t N2, size_t N3, size_t K1, size_t K2,
+    const size_t *permab, const size_t *ka1, const size_t *kb1, const size_t *permd,
+    const size_t *kab2, const size_t *kc2, lt_btensor *a, lt_btensor *b,
+    const size_t *sympermab, int symmab, lt_btensor *c, double kd, lt_btod_contract3 **out) {
+    return guard([&]() {
+        need(sympermab, "sympermab");
+        c3_create(N1, N2, N3, K1, K2, permab, ka1, kb1, permd, kab2, kc2, a, b, c, kd,
+            sympermab, symmab, out);
+    });
+}
+
 int lt_btod_contract3_create(size_t N1, size_t N2, size_t N3, size_t K1, size_t K2,
     const size_t *permab, const size_t *ka1, const size_t *kb1, const size_t *permd,
     const size_t *kab2, const size_t *kc2, lt_btensor *a, lt_btensor *b, lt_btensor *c,
     double kd, lt_btod_contract3 **out) {
     return guard([&]() {
-        need(a, "a"); need(b, "b"); need(c, "c"); need(out, "out");
-        if(K1) { need(ka1, "ka1"); need(kb1, "kb1"); }
-        if(K2) { need(kab2, "kab2"); need(kc2, "kc2"); }
-        lt_require(&a->t->ctx() == &b->t->ctx() && &a->t->ctx() == &c->t->ctx(), LT_ERR_PARAM,
-            "tensors of different contexts");
-        lt_require(a->t->bis().order() == N1 + K1 && b->t->bis().order() == N2 + K1 + K2 &&
-            c->t->bis().order() == N3 + K2, LT_ERR_DIMS, "btod_contract3: tensor orders");
-        contraction2 c1 = make_contraction(N1, N2 + K2, K1, permab, ka1, kb1);
-        contraction2 c2 = make_contraction(N1 + N2, N3, K2, permd, kab2, kc2);
-        lt_btod_contract3 *h = new lt_btod_contract3;
-        h->ctx = a->ctx;
-        try {
-            h->op = new btod_contract3(c1, c2, *a->t, *b->t, *c->t, kd);
-        } catch(...) {
-            delete h;
-            throw;
-        }
-        ctx_retain(h->ctx);
-        *out = h;
+        c3_create(N1, N2, N3, K1, K2, permab, ka1, kb1, permd, kab2, kc2, a, b, c, kd, 0, 1,
+            out);
     });
 }
 

@@ -262,12 +262,10 @@ hipError_t launch_grouped(const contract_plan &pl, int k, hipStream_t s) {
 
 // ---------------------------------------------------------------------------
 
-namespace {
 uint64_t next_op_uid() {
     static std::atomic<uint64_t> next(1);
     return next++;
 }
-} // unnamed namespace
 
 btod_contract2::btod_contract2(const contraction2 &c, btensor &a, double ka,
     btensor &b, double kb, double kc) :

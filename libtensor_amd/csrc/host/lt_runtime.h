@@ -176,6 +176,10 @@ void btod_copy(btensor &a, const tensor_tr &tra, btensor &b, bool add, double d)
 void btod_add(const std::vector<btensor*> &ops, const std::vector<tensor_tr> &trs,
     btensor &b, bool add, double d);
 
+//! A new identity for an operation (tags the symmetry it sets on the
+//! tensors it writes).
+uint64_t next_op_uid();
+
 //! Same canonical block, transformation and allowedness for every block.
 bool same_orbits(const symmetry &x, const symmetry &y, const dims_t &bid);
 
@@ -350,10 +354,15 @@ private:
 //! (gen_bto_contract3::make_schedule, :501-526: the second contraction's
 //! non-zero orbits over AB's schedule).  AB is computed whole into a resident
 //! temporary instead of in batches of B.
+class btod_symmetrize2;
 class btod_contract3 : public additive_op {
 public:
+    //! sympermab: btod_symcontract3 (libtensor/block_tensor/btod_symcontract3.h;
+    //! gen_block_tensor/impl/gen_bto_symcontract3_impl.h): AB is replaced by
+    //! AB + s P(AB) (s = +1 if symmab) before the second contraction, with
+    //! the symmetry and schedule of btod_symmetrize2 (make_schedule :539-590).
     btod_contract3(const contraction2 &c1, const contraction2 &c2, btensor &a, btensor &b,
-        btensor &c, double kd);
+        btensor &c, double kd, const perm_t *sympermab = 0, bool symmab = true);
     ~btod_contract3();
 
     context &ctx() { return m_a.ctx(); }
@@ -376,9 +385,12 @@ public:
 private:
     btensor &m_a;
     std::unique_ptr<btod_contract2> m_op1;
+    std::unique_ptr<additive_op> m_op1x;          //!< m_op1 as an additive_op
+    std::unique_ptr<btod_symmetrize2> m_symab;    //!< symmetrization of AB
     std::unique_ptr<btensor> m_ab;
     std::unique_ptr<btod_contract2> m_op2;
     double m_last_ms;
+    double make_ab(int mode);
 };
 
 //! Sum of transformed copies of another operation's result X
@@ -412,8 +424,8 @@ protected:
         size_t ai;        //!< canonical block of X (in the operand's schedule)
         tensor_tr tr;     //!< contribution tr(X_ai)
     };
-    explicit symmetrizer(additive_op &op) : m_op(op), m_last_ms(0.0), m_gather_ms(0.0),
-        m_gather_bytes(0.0) { }
+    explicit symmetrizer(additive_op &op) : m_op(op), m_uid(next_op_uid()), m_last_ms(0.0),
+        m_gather_ms(0.0), m_gather_bytes(0.0) { }
     const std::vector<rec> *records(size_t aib) const;
 
     additive_op &m_op;
@@ -423,6 +435,7 @@ protected:
 
 private:
     std::unique_ptr<btensor> m_x;
+    uint64_t m_uid;
     double m_last_ms, m_gather_ms, m_gather_bytes;
     double run(btensor &b, bool add, double d, int mode);
 };

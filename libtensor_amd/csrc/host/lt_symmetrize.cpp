@@ -19,31 +19,45 @@
 namespace lta {
 
 btod_contract3::btod_contract3(const contraction2 &c1, const contraction2 &c2, btensor &a,
-    btensor &b, btensor &c, double kd) : m_a(a), m_last_ms(0.0) {
+    btensor &b, btensor &c, double kd, const perm_t *sympermab, bool symmab) :
+    m_a(a), m_last_ms(0.0) {
 
     m_op1.reset(new btod_contract2(c1, a, 1.0, b, 1.0, 1.0));
-    // AB carries its symmetry and the first contraction's schedule as its
-    // non-zero blocks, so the second contraction's schedule is built over
-    // that list as the reference builds it.
-    m_ab.reset(new btensor(a.ctx(), m_op1->symc().bis));
-    m_ab->set_symmetry(m_op1->symc());
-    const std::vector<size_t> &sab = m_op1->schedule();
-    for(size_t i = 0; i < sab.size(); i++) m_ab->mark_nonzero(sab[i]);
+    const symmetry *symab = &m_op1->symc();
+    const std::vector<size_t> *sab = &m_op1->schedule();
+    if(sympermab) {
+        m_op1x.reset(new contract2_op(*m_op1, a));
+        m_symab.reset(new btod_symmetrize2(*m_op1x, *sympermab, symmab));
+        symab = &m_symab->sym();
+        sab = &m_symab->schedule();
+    }
+    // AB carries its symmetry and schedule as its non-zero blocks, so the
+    // second contraction's schedule is built over that list as the
+    // reference builds it.
+    m_ab.reset(new btensor(a.ctx(), symab->bis));
+    m_ab->set_symmetry(*symab);
+    for(size_t i = 0; i < sab->size(); i++) m_ab->mark_nonzero((*sab)[i]);
     m_op2.reset(new btod_contract2(c2, *m_ab, 1.0, c, 1.0, kd));
 }
 
 btod_contract3::~btod_contract3() { }
 
-void btod_contract3::perform(btensor &d, int mode) {
+double btod_contract3::make_ab(int mode) {
+    if(m_symab) return m_symab->perform_into(*m_ab, mode);
     m_op1->perform(*m_ab, mode);
+    return m_op1->stats().t_gpu_ms;
+}
+
+void btod_contract3::perform(btensor &d, int mode) {
+    const double ms = make_ab(mode);
     m_op2->perform(d, mode);
-    m_last_ms = m_op1->stats().t_gpu_ms + m_op2->stats().t_gpu_ms;
+    m_last_ms = ms + m_op2->stats().t_gpu_ms;
 }
 
 void btod_contract3::perform_add(btensor &d, double k, int mode) {
-    m_op1->perform(*m_ab, mode);
+    const double ms = make_ab(mode);
     m_op2->perform_add(d, k, mode);
-    m_last_ms = m_op1->stats().t_gpu_ms + m_op2->stats().t_gpu_ms;
+    m_last_ms = ms + m_op2->stats().t_gpu_ms;
 }
 
 double btod_contract3::perform_into(btensor &t, int mode) {
@@ -53,7 +67,7 @@ double btod_contract3::perform_into(btensor &t, int mode) {
 
 void btod_contract3::compute_block(bool zero, const index_t &id, const tensor_tr &tr,
     double *blk) {
-    m_op1->perform(*m_ab, MODE_GROUPED);
+    make_ab(MODE_GROUPED);
     m_op2->compute_block(zero, id, tr, blk);
 }
 
@@ -81,9 +95,19 @@ double symmetrizer::run(btensor &b, bool add, double d, int mode) {
     if(!m_x) m_x.reset(new btensor(cx, m_sym.bis));
     btensor &x = *m_x;
     double ms = m_op.perform_into(x, mode);
+    std::vector<size_t> written;
+    for(size_t i = 0; i < m_recs.size(); i++) {
+        const std::vector<rec> &rs = m_recs[i].second;
+        for(size_t k = 0; k < rs.size(); k++)
+            if(x.is_nonzero(rs[k].ai)) { written.push_back(m_recs[i].first); break; }
+    }
     if(!add) {
-        b.zero_all();
-        b.set_symmetry(m_sym);
+        // A tensor this operation wrote last, with the same blocks, keeps
+        // its structure and version (plans reading it stay valid).
+        if(b.sym_tag() != m_uid || b.nonzero_blocks() != written) {
+            b.zero_all();
+            b.set_symmetry(m_sym, m_uid);
+        }
     }
     std::vector<std::vector<dev::perm_desc> > rounds;
     double bytes = 0.0;

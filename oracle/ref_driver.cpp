@@ -23,6 +23,7 @@
 #include <libtensor/block_tensor/btod_symmetrize2.h>
 #include <libtensor/block_tensor/btod_symmetrize3.h>
 #include <libtensor/block_tensor/btod_contract3.h>
+#include <libtensor/block_tensor/btod_symcontract3.h>
 #include <libtensor/block_tensor/btod_traits.h>
 #include <libtensor/core/orbit.h>
 #include <libtensor/core/orbit_list.h>
@@ -456,6 +457,56 @@ int run3(const spec::problem &p, const std::string &out, const options &opt) {
     return 0;
 }
 
+// btod_symcontract3<N1, N2, N3, K1, K2>: it exposes no symmetry or schedule,
+// so the result tensor's symmetry ("symt") and non-zero blocks ("schedule")
+// are written after the perform.
+template<size_t N1, size_t N2, size_t N3, size_t K1, size_t K2>
+int run3s(const spec::problem &p, const std::string &out, const options &opt) {
+    enum { NA = N1 + K1, NB = N2 + K1 + K2, NC = N3 + K2, NAB = N1 + N2 + K2,
+           ND = N1 + N2 + N3 };
+    block_tensor<NA, double, allocator<double> > bta(make_bis<NA>(p.a));
+    fill_tensor<NA>(p.a, bta);
+    block_tensor<NB, double, allocator<double> > btb(make_bis<NB>(p.b));
+    fill_tensor<NB>(p.b, btb);
+    block_tensor<NC, double, allocator<double> > btc(make_bis<NC>(p.c));
+    fill_tensor<NC>(p.c, btc);
+    contraction2<N1, N2 + K2, K1> contr1(make_perm<NAB>(p.permab));
+    for(size_t i = 0; i < K1; i++) contr1.contract(p.ka1[i], p.kb1[i]);
+    contraction2<N1 + N2, N3, K2> contr2(make_perm<ND>(p.permd));
+    for(size_t i = 0; i < K2; i++) contr2.contract(p.kab2[i], p.kc2[i]);
+    FILE *f = fopen((out + ".txt").c_str(), "w");
+    btod_symcontract3<N1, N2, N3, K1, K2> op(contr1, contr2, bta, btb,
+        make_perm<NAB>(p.symab), p.symab_symm != 0, btc, p.kd);
+    // the result's block index space is that of the plain contract3
+    btod_contract3<N1, N2, N3, K1, K2> op3(contr1, contr2, bta, btb, btc, p.kd);
+    block_tensor<ND, double, allocator<double> > btd(op3.get_bis());
+    double t0 = wall_ms();
+    op.perform(btd);
+    fprintf(f, "perform_ms %.6f\n", wall_ms() - t0);
+    {
+        block_tensor_ctrl<ND, double> cd(btd);
+        print_symt<ND>(f, cd.req_const_symmetry());
+        std::vector<size_t> nz;
+        gen_block_tensor_rd_ctrl<ND, block_tensor_i_traits<double> >(btd).req_nonzero_blocks(nz);
+        std::sort(nz.begin(), nz.end());
+        fprintf(f, "schedule %zu\n", nz.size());
+        for(size_t i = 0; i < nz.size(); i++) fprintf(f, "%zu\n", nz[i]);
+    }
+    if(opt.result) {
+        dimensions<ND> dd = op3.get_bis().get_dims();
+        dense_tensor<ND, double, allocator<double> > t(dd);
+        tod_btconv<ND>(btd).perform(t);
+        dense_tensor_rd_ctrl<ND, double> ctl(t);
+        const double *pd = ctl.req_const_dataptr();
+        FILE *g = fopen((out + ".result.bin").c_str(), "wb");
+        fwrite(pd, 8, dd.get_size(), g);
+        fclose(g);
+        ctl.ret_const_dataptr(pd);
+    }
+    fclose(f);
+    return 0;
+}
+
 typedef int (*runner)(const spec::problem &, const std::string &, const options &);
 
 struct entry { size_t n, m, k; runner r; };
@@ -474,11 +525,18 @@ struct entry3 { size_t n1, n2, n3, k1, k2; runner r; };
 
 // The orders libtensor instantiates (block_tensor/impl/btod_contract3.C).
 #define E3(a, b, c, d, e) { a, b, c, d, e, &run3<a, b, c, d, e> }
+#define E3S(a, b, c, d, e) { a, b, c, d, e, &run3s<a, b, c, d, e> }
 const entry3 k_table3[] = {
     E3(1, 0, 1, 1, 1), E3(1, 1, 1, 1, 1), E3(1, 1, 2, 0, 0), E3(1, 1, 2, 1, 2),
     E3(2, 0, 0, 0, 2), E3(2, 0, 1, 1, 2), E3(2, 0, 2, 1, 2)
 };
+
+// ... and btod_symcontract3 (block_tensor/impl/btod_symcontract3.C).
+const entry3 k_table3s[] = {
+    E3S(1, 0, 1, 1, 1), E3S(1, 1, 1, 1, 1), E3S(2, 0, 2, 1, 2)
+};
 #undef E3
+#undef E3S
 
 } // unnamed namespace
 
@@ -515,8 +573,11 @@ int main(int argc, char **argv) {
         allocator<double>::init(4, 16, 16777216, 16777216);
         batching_policy_base::set_batch_size(size_t(1) << 40);
         if(p.c3) {
-            for(size_t i = 0; i < sizeof(k_table3) / sizeof(k_table3[0]); i++) {
-                const entry3 &e = k_table3[i];
+            const entry3 *tb = p.symab.empty() ? k_table3 : k_table3s;
+            const size_t nt = p.symab.empty() ? sizeof(k_table3) / sizeof(k_table3[0]) :
+                sizeof(k_table3s) / sizeof(k_table3s[0]);
+            for(size_t i = 0; i < nt; i++) {
+                const entry3 &e = tb[i];
                 if(e.n1 == p.N1 && e.n2 == p.N2 && e.n3 == p.N3 && e.k1 == p.K1 &&
                     e.k2 == p.K2) return e.r(p, argv[2], opt);
             }

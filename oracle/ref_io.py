@@ -85,6 +85,9 @@ def write_spec(prob, path, dense=None, tensors=None):
             " ".join("%d %d" % (a, b) for a, b in c["pairs1"]),
             " ".join(str(p) for p in c["permd"]),
             " ".join("%d %d" % (a, b) for a, b in c["pairs2"]), c.get("kd", 1.0)))
+        if c.get("symab"):
+            lines.append("symab %s %d" % (" ".join(str(x) for x in c["symab"][0]),
+                                          1 if c["symab"][1] else 0))
     else:
         c = prob["contr"]
         permc = c.get("permc") or list(range(c["N"] + c["M"]))

@@ -25,6 +25,8 @@
 //             <permd x N1+N2+N3> <ab c x K2> <kd>
 //                                        btod_contract3 of A, B and C
 //                                        (tensor C declared like A and B)
+//   symab <perm x N1+N2+K2> <symm 0|1>   btod_symcontract3: symmetrize AB
+//                                        (after the contract3 line)
 //   symmetrize3 <perm1 x N+M> <perm2 x N+M> <symm 0|1>
 //                                        wrap it in btod_symmetrize3 (same
 //                                        chain as symmetrize2)
@@ -97,10 +99,12 @@ struct problem {
     size_t N1, N2, N3, K1, K2;
     std::vector<size_t> permab, permd, ka1, kb1, kab2, kc2;
     double kd;
+    std::vector<size_t> symab;      // btod_symcontract3 when not empty
+    int symab_symm;
     struct symm2 { std::vector<size_t> perm, perm2; int symm; };   // perm2: symmetrize3
     std::vector<symm2> symm;        // in file order (innermost first)
     problem() : N(0), M(0), K(0), sa(1), sb(1), sc(1), self(false), c3(false),
-        N1(0), N2(0), N3(0), K1(0), K2(0), kd(1) { }
+        N1(0), N2(0), N3(0), K1(0), K2(0), kd(1), symab_symm(1) { }
 };
 
 inline std::vector<size_t> read_sz(std::istringstream &in, size_t n) {
@@ -180,6 +184,9 @@ inline problem parse(const std::string &path) {
             p.permd = read_sz(in, p.N1 + p.N2 + p.N3);
             for(size_t i = 0; i < p.K2; i++) { size_t a, b; in >> a >> b; p.kab2.push_back(a); p.kc2.push_back(b); }
             in >> p.kd;
+        } else if(kw == "symab") {
+            p.symab = read_sz(in, p.N1 + p.N2 + p.K2);
+            in >> p.symab_symm;
         } else if(kw == "symmetrize3") {
             problem::symm2 y; y.perm = read_sz(in, p.N + p.M); y.perm2 = read_sz(in, p.N + p.M);
             in >> y.symm;

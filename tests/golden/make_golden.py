@@ -144,7 +144,13 @@ SYMM_CASES = [
 ]
 
 
-def random_problem3(rng, N1, N2, N3, K1, K2, sym=False):
+def swap(n, i, j):
+    p = list(range(n))
+    p[i], p[j] = p[j], p[i]
+    return p
+
+
+def random_problem3(rng, N1, N2, N3, K1, K2, sym=False, symab=None):
     """btod_contract3 problem with random extents, splits and index orders.
     sym: the two K2 indexes (and the two N3 indexes when N3 == 2) get equal
     extents and splits, and B and C antisymmetry in them."""
@@ -171,6 +177,10 @@ def random_problem3(rng, N1, N2, N3, K1, K2, sym=False):
     permab = list(range(len(rem1)))
     rng.shuffle(permab)
     iab = [rem1[permab[i]] for i in range(len(rem1))]
+    if symab is not None:
+        # btod_symcontract3: the first two indexes of AB get one block
+        # structure and AB is (anti)symmetrized in them
+        size[rem1[1]], split[rem1[1]] = size[rem1[0]], split[rem1[0]]
     rem2 = [i for i in iab if i not in k2] + [i for i in ic if i not in k2]
     permd = list(range(len(rem2)))
     rng.shuffle(permd)
@@ -188,6 +198,10 @@ def random_problem3(rng, N1, N2, N3, K1, K2, sym=False):
                        "pairs1": [(ia.index(k), ib.index(k)) for k in k1],
                        "permd": permd,
                        "pairs2": [(iab.index(k), ic.index(k)) for k in k2], "kd": -1.5}}
+    prob["idx"]["AB"] = iab
+    if symab is not None:
+        prob["contr3"]["symab"] = (swap(len(iab), iab.index(rem1[0]), iab.index(rem1[1])),
+                                   symab)
     if sym and K2 == 2:
         for name, ix in (("B", ib), ("C", ic)):
             p = list(range(len(ix)))
@@ -213,6 +227,12 @@ def c3_cases():
     for (n1, n2, n3, k1, k2) in [(1, 1, 2, 1, 2), (2, 0, 2, 1, 2)]:
         out.append(("c3sym_%d%d%d%d%d" % (n1, n2, n3, k1, k2),
                     random_problem3(rng, n1, n2, n3, k1, k2, sym=True)))
+    # btod_symcontract3, the orders libtensor instantiates
+    rng = random.Random(20261204)
+    for (n1, n2, n3, k1, k2), symm in [((1, 0, 1, 1, 1), True), ((1, 1, 1, 1, 1), False),
+                                       ((2, 0, 2, 1, 2), True), ((2, 0, 2, 1, 2), False)]:
+        out.append(("c3s_%d%d%d%d%d_%s" % (n1, n2, n3, k1, k2, "s" if symm else "a"),
+                    random_problem3(rng, n1, n2, n3, k1, k2, sym=(k2 == 2), symab=symm)))
     return out
 
 
@@ -268,12 +288,6 @@ def sym3_problem(kind, n_o, splits, spin, labels):
     if labels:
         prob["tables"] = [W.C2V]
     return prob
-
-
-def swap(n, i, j):
-    p = list(range(n))
-    p[i], p[j] = p[j], p[i]
-    return p
 
 
 # btod_symmetrize3 (entries (perm1, perm2, symmetric)), over a golden case or

@@ -129,7 +129,10 @@ def build3(ctx, prob, dense):
     c2 = lt.contraction2(c["N1"] + c["N2"], c["N3"], c["K2"], c["permd"])
     for ia, ib in c["pairs2"]:
         c2.contract(ia, ib)
-    op = lt.btod_contract3(c1, c2, ts[0], ts[1], ts[2], c.get("kd", 1.0))
+    sym = c.get("symab")
+    op = lt.btod_contract3(c1, c2, ts[0], ts[1], ts[2], c.get("kd", 1.0),
+                           sympermab=None if sym is None else sym[0],
+                           symmab=True if sym is None else sym[1])
     return ts, op
 
 
@@ -138,9 +141,15 @@ def result_tensor3(ctx, prob):
 
 
 def einsum3(prob, dense):
-    """kd (A B) C of the dense inputs."""
+    """kd (A B) C of the dense inputs; with "symab" (btod_symcontract3) AB is
+    replaced by AB + s P(AB) first."""
     ix = prob["idx"]
-    expr = "%s,%s,%s->%s" % ("".join(ix["A"]), "".join(ix["B"]), "".join(ix["C"]),
-                             "".join(ix["D"]))
-    return prob["contr3"].get("kd", 1.0) * np.einsum(expr, dense["A"], dense["B"], dense["C"],
-                                                     optimize=True)
+    c = prob["contr3"]
+    ab = np.einsum("%s,%s->%s" % ("".join(ix["A"]), "".join(ix["B"]), "".join(ix["AB"])),
+                   dense["A"], dense["B"], optimize=True)
+    if c.get("symab"):
+        perm, symm = c["symab"]
+        ab = ab + (1.0 if symm else -1.0) * np.transpose(ab, perm)
+    return c.get("kd", 1.0) * np.einsum("%s,%s->%s" % ("".join(ix["AB"]), "".join(ix["C"]),
+                                                       "".join(ix["D"])),
+                                        ab, dense["C"], optimize=True)

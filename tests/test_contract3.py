@@ -2,8 +2,9 @@
 orbit transformation and allowedness of every block) and the schedule of
 D = kd (A B) C against libtensor's (fixtures c3_* written by the real
 reference build for every order combination libtensor instantiates, two of
-them with antisymmetric operands), and the stored results against a numpy
-einsum of the dense inputs."""
+them with antisymmetric operands; c3s_*: btod_symcontract3, the intermediate
+symmetrized), and the stored results against a numpy einsum of the dense
+inputs."""
 import numpy as np
 import pytest
 
@@ -15,7 +16,12 @@ import problems as P
 def test_symmetry_and_schedule(host_ctx, name):
     prob, dense, meta, result = G.load_c3(name)
     ts, op = P.build3(host_ctx, prob, dense)
-    assert sorted(op.get_schedule()) == meta["schedule"]
+    if prob["contr3"].get("symab"):
+        # btod_symcontract3 exposes no schedule: the fixture holds the blocks
+        # its perform wrote, all of which must be scheduled
+        assert set(meta["schedule"]) <= set(op.get_schedule())
+    else:
+        assert sorted(op.get_schedule()) == meta["schedule"]
     for aidx, (ac, al, perm, c) in meta["symt"].items():
         got = op.sym_orbit(int(aidx))
         assert (got[0], got[3], list(got[1]), got[2]) == (ac, bool(al), perm, c), aidx

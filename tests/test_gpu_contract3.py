@@ -1,5 +1,6 @@
-"""btod_contract3 on the device through the C ABI: D = kd (A B) C for every
-fixture c3_* against libtensor's result, with D := result in both launch
+"""btod_contract3 / btod_symcontract3 on the device through the C ABI:
+D = kd (A B) C (AB symmetrized first for btod_symcontract3) for every
+fixture c3* against libtensor's result, with D := result in both launch
 modes, D += k result, repeated performs (the intermediate and both plans
 reused), and a symmetrization of the result (btod_symmetrize2 over
 btod_contract3) against the closed form D + s P(D)."""
@@ -33,7 +34,7 @@ def test_perform(gpu_ctx, name, mode):
     scale = float(np.max(np.abs(result))) or 1.0
     op.perform(d, mode)
     assert np.max(np.abs(d.to_dense() - result)) <= TOL * scale
-    assert set(d.req_nonzero_blocks()) <= set(meta["schedule"])
+    assert set(d.req_nonzero_blocks()) <= set(op.get_schedule())
     op.perform_add(d, 2.0, mode)
     assert np.max(np.abs(d.to_dense() - 3.0 * result)) <= TOL * scale
     op.perform(d, mode)
