@@ -324,7 +324,7 @@ se_label::se_label(const dims_t &bid, const std::string &tid) :
 se_label::se_label(const se_label &o) :
     sym_element(o), table_id(o.table_id), bidims(o.bidims), labels(o.labels),
     rule(o.rule), src(o.src), keep(o.keep), rdims(o.rdims), rlabels(o.rlabels),
-    alts(o.alts) {
+    alts(o.alts), factors(o.factors) {
 }
 
 void se_label::set_label(size_t dim, size_t blk, size_t label) {
@@ -381,6 +381,15 @@ bool se_label::eval_rule(const size_t *lab) const {
 }
 
 bool se_label::eval_one(const size_t *lab) const {
+    if(!factors.empty()) {
+        for(size_t f = 0; f < factors.size(); f++) {
+            size_t sub[kMaxOrder];
+            const std::vector<size_t> &m = factors[f].map;
+            for(size_t k = 0; k < m.size(); k++) sub[k] = lab[m[k]];
+            if(!factors[f].el->eval_labels(sub)) return false;
+        }
+        return true;
+    }
     if(src) return eval_reduced(lab);
     return eval_rule(lab);
 }
@@ -457,6 +466,9 @@ void se_label::permute(const perm_t &p) {
     }
     // new index y = p(x): alternative q becomes p^-1, then q, then p
     const perm_t pinv(p.inverse());
+    for(size_t f = 0; f < factors.size(); f++)
+        for(size_t k = 0; k < factors[f].map.size(); k++)
+            factors[f].map[k] = pinv[factors[f].map[k]];
     for(size_t k = 0; k < alts.size(); k++) {
         perm_t q(pinv);
         q.permute(alts[k]).permute(p);
@@ -1069,6 +1081,38 @@ void dirprod_label(size_t na, size_t nb, const perm_t &perm, const bis_t &bis3,
         always.intrinsic = k_label_invalid;
         e3.rule.assign(1, label_product(1, always));
 
+        // Elements without a rule of their own (the result of a contraction,
+        // a symmetrization or a product of those) enter as factors: the
+        // allowed blocks are the same conjunction of both sides.
+        bool product = false;
+        for(int side = 0; side < 2; side++) {
+            const std::vector<std::shared_ptr<sym_element> > *s = side ? s2 : s1;
+            if(s) for(size_t e = 0; e < s->size(); e++) {
+                const se_label &l = dynamic_cast<const se_label&>(*(*s)[e]);
+                if(l.table_id == tables[t] && (l.src || !l.alts.empty() || !l.factors.empty()))
+                    product = true;
+            }
+        }
+        if(product) {
+            for(int side = 0; side < 2; side++) {
+                const std::vector<std::shared_ptr<sym_element> > *s = side ? s2 : s1;
+                const size_t *map = side ? map2 : map1;
+                const size_t ns = side ? nb : na;
+                if(!s) continue;
+                for(size_t e = 0; e < s->size(); e++) {
+                    const se_label &l = dynamic_cast<const se_label&>(*(*s)[e]);
+                    if(l.table_id != tables[t]) continue;
+                    for(size_t i = 0; i < ns; i++) e3.labels[map[i]] = l.labels[i];
+                    se_label::factor f;
+                    f.el.reset(new se_label(l));
+                    f.map.assign(map, map + ns);
+                    e3.factors.push_back(f);
+                }
+            }
+            out.insert(e3);
+            continue;
+        }
+
         for(int side = 0; side < 2; side++) {
             const std::vector<std::shared_ptr<sym_element> > *s = side ? s2 : s1;
             const size_t *map = side ? map2 : map1;
@@ -1076,9 +1120,6 @@ void dirprod_label(size_t na, size_t nb, const perm_t &perm, const bis_t &bis3,
             if(!s) continue;
             for(size_t e = 0; e < s->size(); e++) {
                 const se_label &l = dynamic_cast<const se_label&>(*(*s)[e]);
-                lt_require(!l.src, LT_ERR_INTERNAL, "dirprod of a reduced label");
-                lt_require(l.alts.empty(), LT_ERR_SYMMETRY,
-                    "contraction of an operand with a symmetrized label element");
                 if(l.table_id != tables[t]) continue;
                 for(size_t i = 0; i < ns; i++) e3.labels[map[i]] = l.labels[i];
                 label_rule nr;

@@ -148,6 +148,16 @@ struct se_label : public sym_element {
     //! so_symmetrize builds).
     std::vector<perm_t> alts;
 
+    //! Product form (the direct product of elements that are reduced,
+    //! symmetrized or products themselves): a block is allowed if every
+    //! factor allows the labels of its dimensions (factor dimension k is
+    //! dimension map[k] of this element).
+    struct factor {
+        std::shared_ptr<const se_label> el;
+        std::vector<size_t> map;
+    };
+    std::vector<factor> factors;
+
     se_label(const dims_t &bid, const std::string &tid);
     const char *type() const { return "label"; }
     sym_element *clone() const { return new se_label(*this); }

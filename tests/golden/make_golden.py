@@ -150,7 +150,7 @@ def swap(n, i, j):
     return p
 
 
-def random_problem3(rng, N1, N2, N3, K1, K2, sym=False, symab=None):
+def random_problem3(rng, N1, N2, N3, K1, K2, sym=False, symab=None, labels=False):
     """btod_contract3 problem with random extents, splits and index orders.
     sym: the two K2 indexes (and the two N3 indexes when N3 == 2) get equal
     extents and splits, and B and C antisymmetry in them."""
@@ -186,19 +186,42 @@ def random_problem3(rng, N1, N2, N3, K1, K2, sym=False, symab=None):
     rng.shuffle(permd)
     idd = [rem2[permd[i]] for i in range(len(rem2))]
 
-    def tensor(ix):
+    # C2v labels of the blocks of every index (labels: on A, B and C, the
+    # totally symmetric product allowed; irreps A1 and A2 only, so that about
+    # half the blocks are allowed)
+    lab = {}
+    if labels:
+        lab = {i: [rng.randrange(2) for _ in range(1 if split[i] is None else 2)]
+               for i in sorted(size)}
+        if sym:
+            for grp in (k2, o3):
+                if len(grp) == 2:
+                    lab[grp[1]] = list(lab[grp[0]])
+        if symab is not None:
+            lab[rem1[1]] = list(lab[rem1[0]])
+
+    def tensor(ix, with_labels=False):
         t = {"dims": [size[i] for i in ix], "splits": []}
         for p, i in enumerate(ix):
             if split[i] is not None:
                 t["splits"].append(([1 if q == p else 0 for q in range(len(ix))], split[i]))
+        if with_labels:
+            assigns = []
+            for p, i in enumerate(ix):
+                m = [1 if q == p else 0 for q in range(len(ix))]
+                assigns += [(m, b, l) for b, l in enumerate(lab[i])]
+            t["elems"] = [("label", "c2v", assigns, [0])]
         return t
-    prob = {"A": tensor(ia), "B": tensor(ib), "C": tensor(ic), "D": tensor(idd),
+    prob = {"A": tensor(ia, labels), "B": tensor(ib, labels), "C": tensor(ic, labels),
+            "D": tensor(idd),
             "idx": {"A": ia, "B": ib, "C": ic, "D": idd},
             "contr3": {"N1": N1, "N2": N2, "N3": N3, "K1": K1, "K2": K2, "permab": permab,
                        "pairs1": [(ia.index(k), ib.index(k)) for k in k1],
                        "permd": permd,
                        "pairs2": [(iab.index(k), ic.index(k)) for k in k2], "kd": -1.5}}
     prob["idx"]["AB"] = iab
+    if labels:
+        prob["tables"] = [W.C2V]
     if symab is not None:
         prob["contr3"]["symab"] = (swap(len(iab), iab.index(rem1[0]), iab.index(rem1[1])),
                                    symab)
@@ -207,12 +230,12 @@ def random_problem3(rng, N1, N2, N3, K1, K2, sym=False, symab=None):
             p = list(range(len(ix)))
             a, b = ix.index(k2[0]), ix.index(k2[1])
             p[a], p[b] = b, a
-            prob[name]["elems"] = [("perm", p, -1.0)]
+            prob[name]["elems"] = [("perm", p, -1.0)] + prob[name].get("elems", [])
         if N3 == 2:
             p = list(range(len(ic)))
             a, b = ic.index(o3[0]), ic.index(o3[1])
             p[a], p[b] = b, a
-            prob["C"]["elems"].append(("perm", p, -1.0))
+            prob["C"]["elems"].insert(1, ("perm", p, -1.0))
     return prob
 
 
@@ -233,6 +256,13 @@ def c3_cases():
                                        ((2, 0, 2, 1, 2), True), ((2, 0, 2, 1, 2), False)]:
         out.append(("c3s_%d%d%d%d%d_%s" % (n1, n2, n3, k1, k2, "s" if symm else "a"),
                     random_problem3(rng, n1, n2, n3, k1, k2, sym=(k2 == 2), symab=symm)))
+    # with C2v labels on every operand: the second contraction takes the
+    # intermediate's reduced labels as an operand's
+    rng = random.Random(20261205)
+    for (n1, n2, n3, k1, k2), sym in [((1, 1, 1, 1, 1), False), ((1, 1, 2, 1, 2), False),
+                                      ((2, 0, 2, 1, 2), True), ((2, 0, 1, 1, 2), False)]:
+        out.append(("c3lab_%d%d%d%d%d" % (n1, n2, n3, k1, k2),
+                    random_problem3(rng, n1, n2, n3, k1, k2, sym=sym, labels=True)))
     return out
 
 

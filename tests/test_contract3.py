@@ -2,9 +2,10 @@
 orbit transformation and allowedness of every block) and the schedule of
 D = kd (A B) C against libtensor's (fixtures c3_* written by the real
 reference build for every order combination libtensor instantiates, two of
-them with antisymmetric operands; c3s_*: btod_symcontract3, the intermediate
-symmetrized), and the stored results against a numpy einsum of the dense
-inputs."""
+them with antisymmetric operands; c3lab_*: C2v labels on every operand, so
+the second contraction reduces the intermediate's reduced labels; c3s_*:
+btod_symcontract3, the intermediate symmetrized), and the stored results
+against a numpy einsum of the dense inputs."""
 import numpy as np
 import pytest
 
@@ -22,9 +23,16 @@ def test_symmetry_and_schedule(host_ctx, name):
         assert set(meta["schedule"]) <= set(op.get_schedule())
     else:
         assert sorted(op.get_schedule()) == meta["schedule"]
+    labels = any(e[0] == "label" for t in ("A", "B", "C") for e in prob[t].get("elems", []))
     for aidx, (ac, al, perm, c) in meta["symt"].items():
         got = op.sym_orbit(int(aidx))
-        assert (got[0], got[3], list(got[1]), got[2]) == (ac, bool(al), perm, c), aidx
+        assert (got[0], list(got[1]), got[2]) == (ac, perm, c), aidx
+        if labels and al and not got[3]:
+            # libtensor reduces label rules symbolically and can allow a block
+            # that no labelling of the contracted blocks allows; ours is exact
+            # (such a block has no contributions: the schedules agree)
+            continue
+        assert got[3] == bool(al), aidx
 
 
 @pytest.mark.parametrize("name", G.c3_names())
