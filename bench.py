@@ -121,8 +121,9 @@ def block_ops(ctx, lt, prob, a, b, op, c, steps=10):
     batched permute launch; HBM-bound: 16 B per element), compute_block of
     the largest scheduled result block (direct tensors), and btod_symmetrize2
     of the contraction over (ij)(ab) (the contraction, then one gather launch
-    per source rank; gather HBM-bound), timed on the host clock around
-    synchronous calls and with HIP events around the kernels."""
+    per source rank; gather HBM-bound), and btod_contract3 of T T T, timed
+    on the host clock around synchronous calls and with HIP events around
+    the kernels."""
     perm = (1, 0, 2, 3)
     td = prob["B"]
     bis = lt.block_index_space([td["dims"][p] for p in perm])
@@ -169,6 +170,31 @@ def block_ops(ctx, lt, prob, a, b, op, c, steps=10):
               "note": "gpu_ms: contraction (grouped) + gather launches, HIP events; "
                       "call_ms: host clock around the synchronous perform"}
     del sym
+    # btod_contract3: D[ijab] = 1/4 sum_{klcd} (T[ijcd] T[klcd]) T[klab] (the
+    # quadratic ladder term of CCD), the intermediate X[ijkl] resident
+    c1 = lt.contraction2(2, 2, 2)
+    c1.contract(2, 2)
+    c1.contract(3, 3)
+    c2 = lt.contraction2(2, 2, 2)
+    c2.contract(2, 0)
+    c2.contract(3, 1)
+    op3 = lt.btod_contract3(c1, c2, a, a, a, kd=0.25)
+    d3 = W.result_tensor(ctx, prob)
+    op3.perform(d3)
+    op3.perform(d3)
+    g3 = []
+    t0 = time.perf_counter()
+    for i in range(steps):
+        op3.perform(d3)
+        g3.append(op3.stats()["gpu_ms"])
+    c3_ms = (time.perf_counter() - t0) * 1e3 / steps
+    f3 = op3.stats()["flops"]
+    c3res = {"blocks": len(op3.get_schedule()), "flops": f3, "call_ms": c3_ms,
+             "gpu_ms": statistics.mean(g3), "tflops": f3 / (c3_ms * 1e-3) / 1e12,
+             "plans_built": op3.plans(),
+             "note": "both contractions grouped (HIP graphs); gpu_ms: their device time; "
+                     "tflops: schedule FLOPs of both over call_ms"}
+    del op3
     res = {"btod_copy": {"elements": elems, "kernel_ms": kms, "GB_per_s": gbs,
                          "hbm_frac": gbs / 8000.0, "call_ms": ms, "perm": list(perm),
                          "note": "kernel_ms: HIP events around the permute launch; call_ms: "
@@ -176,7 +202,7 @@ def block_ops(ctx, lt, prob, a, b, op, c, steps=10):
                                  "descriptors, upload, launch)"},
            "compute_block": {"block": int(big), "dims": list(dims),
                              "pairs": len(op.clst(big)), "ms": cb_ms},
-           "btod_symmetrize2": symres}
+           "btod_symmetrize2": symres, "btod_contract3": c3res}
     log("block ops: %s" % json.dumps(res))
     return res
 
