@@ -263,6 +263,14 @@ def c3_cases():
                                       ((2, 0, 2, 1, 2), True), ((2, 0, 1, 1, 2), False)]:
         out.append(("c3lab_%d%d%d%d%d" % (n1, n2, n3, k1, k2),
                     random_problem3(rng, n1, n2, n3, k1, k2, sym=sym, labels=True)))
+    # btod_symcontract3 with labels: the symmetrized intermediate's label
+    # element (a reduced element with its permuted copy) feeds the second
+    # contraction
+    rng = random.Random(20261206)
+    for (n1, n2, n3, k1, k2), symm in [((1, 1, 1, 1, 1), True), ((2, 0, 2, 1, 2), False)]:
+        out.append(("c3slab_%d%d%d%d%d" % (n1, n2, n3, k1, k2),
+                    random_problem3(rng, n1, n2, n3, k1, k2, sym=(k2 == 2), symab=symm,
+                                    labels=True)))
     return out
 
 
