@@ -239,6 +239,10 @@ int lt_btod_contract2_set_graph(lt_btod_contract2 *op, int on);
  * 1..8 (0 = 7).  No counterpart in the reference (all of its arithmetic is
  * FP64 BLAS, linalg_cblas_level3.C:10-60). */
 int lt_btod_contract2_set_ozaki(lt_btod_contract2 *op, int slices);
+/* C tile shape of the split-precision product kernel: -1 (default) = each
+ * shape timed once on the plan's work list, the fastest kept; 0 = 16x32,
+ * 1 = 16x64, 2 = 32x32 (rows x columns).  No counterpart in the reference. */
+int lt_btod_contract2_set_ozaki_tile(lt_btod_contract2 *op, int tile);
 
 /* ---- btod_dotprod (libtensor/block_tensor/btod_dotprod.h:35-66:
  *      btod_dotprod(bta, tra, btb, trb).calculate()): sum over all elements

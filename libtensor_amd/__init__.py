@@ -395,6 +395,11 @@ class btod_contract2(object):
     def set_ozaki(self, slices):
         call("lt_btod_contract2_set_ozaki", self._h, int(slices))
 
+    def set_ozaki_tile(self, tile):
+        """Split-precision tile shape: -1 measured (default), 0 16x32,
+        1 16x64, 2 32x32."""
+        call("lt_btod_contract2_set_ozaki_tile", self._h, int(tile))
+
 
 class btod_contract3(object):
     """D = kd (A B) C (libtensor/block_tensor/btod_contract3.h): contr1 (a

@@ -189,9 +189,12 @@ hipError_t launch_gemm(int cfg, const gemm_cblock *cb,
 
 //! Splitting of one operand matrix into int8 slices with one power-of-two
 //! scale per row: row r = 2^exps[r] * sum_t slice_t[r][.] * 2^(-7 t),
-//! t = 1..slices.  Element (r, k) of the source is src[r * rs + k * ks];
-//! slice t - 1 is stored at dst + (t - 1) * sstride as rows x kp bytes
-//! (k contiguous, zero beyond k).
+//! t = 1..slices.  Element (r, k) of the source is src[r * rs + k * ks].
+//! Slice t - 1 starts at dst + (t - 1) * sstride and is fragment-major: rows
+//! form 16-row panels (the last one zero-padded), k forms 64-k chunks (zero
+//! beyond k), panel p, chunk c is 1024 contiguous bytes at
+//! (p * kp / 64 + c) * 1024, and within it byte j of lane l (row 16 p + l % 16,
+//! k = 64 c + 16 (l / 16) + j) sits at l * 16 + j.  sstride = panels * 16 * kp.
 struct oz_split_desc {
     const double *src;
     int8_t *dst;
@@ -201,8 +204,8 @@ struct oz_split_desc {
 };
 
 //! One product of a C block in split form: A slices (rows of C) and B
-//! slices (columns of C), both k-contiguous with kp (multiple of 32) bytes
-//! per row.
+//! slices (columns of C) in the fragment-major layout of oz_split_desc, kp
+//! (multiple of 64) padded k.
 struct oz_pair {
     const int8_t *a;
     const int8_t *b;
@@ -212,15 +215,17 @@ struct oz_pair {
     double d;
 };
 
-static const uint32_t kOzTileM = 16, kOzTileN = 32;
+//! C tile shapes of the split-precision product kernel (rows x columns).
+enum oz_tile_cfg { OZ_TILE_16x32 = 0, OZ_TILE_16x64 = 1, OZ_TILE_32x32 = 2, OZ_TILE_COUNT = 3 };
+void oz_tile_shape(int tile, uint32_t &bm, uint32_t &bn);
 
 //! items: nitems pairs (descriptor, row).
 hipError_t launch_oz_split(const oz_split_desc *descs, const uint32_t *items,
     uint32_t nitems, hipStream_t s);
-//! C tiles of kOzTileM x kOzTileN; products of slices t, u with
+//! C tiles of shape `tile` (oz_tile_cfg); products of slices t, u with
 //! t + u <= slices + 1, exact in int32, recombined in FP64.
 hipError_t launch_oz_gemm(const gemm_cblock *cb, const oz_pair *pairs,
-    const gemm_tile *tiles, uint32_t ntiles, int slices, hipStream_t s);
+    const gemm_tile *tiles, uint32_t ntiles, int slices, int tile, hipStream_t s);
 
 //! Fill n doubles with v.
 hipError_t launch_fill(double *p, size_t n, double v, hipStream_t s);

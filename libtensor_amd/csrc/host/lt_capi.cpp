@@ -1158,7 +1158,15 @@ int lt_btod_contract2_set_ozaki(lt_btod_contract2 *op, int slices) {
     return guard([&]() {
         need(op, "op");
         lt_require(slices >= 0 && slices <= 8, LT_ERR_PARAM, "slices");
-        op->op->set_ozaki(slices, 1);
+        op->op->set_ozaki(slices);
+    });
+}
+
+int lt_btod_contract2_set_ozaki_tile(lt_btod_contract2 *op, int tile) {
+    return guard([&]() {
+        need(op, "op");
+        lt_require(tile >= -1 && tile < dev::OZ_TILE_COUNT, LT_ERR_PARAM, "tile");
+        op->op->set_ozaki_tile(tile);
     });
 }
 

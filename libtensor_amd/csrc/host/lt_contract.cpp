@@ -223,6 +223,8 @@ struct contract_plan {
 
     // split-precision form (built on the first perform in that mode)
     int oz_slices;
+    int oz_tile;                      //!< dev::oz_tile_cfg of the product launch
+    int oz_tile_set;                  //!< the requested shape it was built for (-1 measured)
     dev_lists oz_lists, oz_mem;
     dev::oz_split_desc *d_oz_split;
     uint32_t *d_oz_items;
@@ -238,7 +240,7 @@ struct contract_plan {
         scratch_elems(0), ctx(0), events(false), graph(0), gexec(0), graph_launches(0),
         d_stacks(0), d_spairs(0), d_arows(0), d_crows(0), d_stiles(0), d_vtiles(0),
         n_prep_st_chunks(0), d_prep_st(0), d_pst_cd(0), d_pst_co(0),
-        oz_slices(0), d_oz_split(0), d_oz_items(0), n_oz_items(0), d_oz_pairs(0),
+        oz_slices(0), oz_tile(0), oz_tile_set(-1), d_oz_split(0), d_oz_items(0), n_oz_items(0), d_oz_pairs(0),
         d_oz_tiles(0), n_oz_tiles(0) { }
     ~contract_plan() {
         if(gexec) hipGraphExecDestroy(gexec);
@@ -270,7 +272,7 @@ uint64_t next_op_uid() {
 btod_contract2::btod_contract2(const contraction2 &c, btensor &a, double ka,
     btensor &b, double kb, double kc) :
     m_contr(c), m_a(a), m_b(b), m_ka(ka), m_kb(kb), m_kc(kc), m_uid(next_op_uid()),
-    m_tile_cfg(-2), m_tuned_cfg(-1), m_tuned_order(-1), m_tuned_va(0), m_tuned_vb(0), m_use_graph(true), m_oz_slices(0), m_oz_int8(1) {
+    m_tile_cfg(-2), m_tuned_cfg(-1), m_tuned_order(-1), m_tuned_va(0), m_tuned_vb(0), m_use_graph(true), m_oz_slices(0), m_oz_tile(-1) {
 
     lt_require(c.is_complete(), LT_ERR_PARAM, "contraction is incomplete");
     lt_require(a.bis().order() == c.NA() && b.bis().order() == c.NB(),
@@ -1258,11 +1260,11 @@ void btod_contract2::build_ozaki(int slices) {
             return i->second;
         }
         op_info o;
-        o.src = src; o.rows = rows; o.k = k; o.kp = (k + 31) & ~31u;
+        o.src = src; o.rows = rows; o.k = k; o.kp = (k + 63) & ~63u;
         o.rs = kmajor ? 1 : ld;
         o.ks = kmajor ? ld : 1;
         o.off = bytes; o.eoff = nexp;
-        bytes += size_t(slices) * o.rows * o.kp;
+        bytes += size_t(slices) * ((rows + 15) & ~15u) * o.kp;
         bytes = (bytes + 255) & ~size_t(255);
         nexp += rows;
         ops.push_back(o);
@@ -1300,8 +1302,9 @@ void btod_contract2::build_ozaki(int slices) {
         d.rows = ops[i].rows; d.k = ops[i].k; d.kp = ops[i].kp;
         d.slices = (uint32_t)slices;
         d.rs = ops[i].rs; d.ks = ops[i].ks;
-        d.sstride = ops[i].rows * ops[i].kp;
-        for(uint32_t r = 0; r < ops[i].rows; r++) {
+        const uint32_t prows = (ops[i].rows + 15) & ~15u;   // whole 16-row panels
+        d.sstride = prows * ops[i].kp;
+        for(uint32_t r = 0; r < prows; r++) {
             items.push_back((uint32_t)i);
             items.push_back(r);
         }
@@ -1313,32 +1316,42 @@ void btod_contract2::build_ozaki(int slices) {
         std::memset(&o, 0, sizeof(o));
         o.a = sl + a.off; o.b = sl + b.off;
         o.ea = ex + a.eoff; o.eb = ex + b.eoff;
-        o.astride = a.rows * a.kp; o.bstride = b.rows * b.kp;
+        o.astride = ((a.rows + 15) & ~15u) * a.kp; o.bstride = ((b.rows + 15) & ~15u) * b.kp;
         o.kp = a.kp;
         o.d = pl.pairs[p].d;
     }
-    std::vector<dev::gemm_tile> tiles;
-    for(size_t b = 0; b < pl.cblocks.size(); b++) {
-        const dev::gemm_cblock &cb = pl.cblocks[b];
-        uint32_t tm = (cb.m + dev::kOzTileM - 1) / dev::kOzTileM;
-        uint32_t tn = (cb.n + dev::kOzTileN - 1) / dev::kOzTileN;
-        lt_require(tm < 65536 && tn < 65536, LT_ERR_PARAM, "C block too large");
-        for(uint32_t i = 0; i < tm; i++) for(uint32_t j = 0; j < tn; j++) {
-            dev::gemm_tile t;
-            t.cblock = (uint32_t)b; t.tm = (uint16_t)i; t.tn = (uint16_t)j;
-            tiles.push_back(t);
+    // C tiles of every product-kernel shape
+    std::vector<dev::gemm_tile> tl[dev::OZ_TILE_COUNT];
+    for(int k = 0; k < dev::OZ_TILE_COUNT; k++) {
+        uint32_t bm, bn;
+        dev::oz_tile_shape(k, bm, bn);
+        for(size_t b = 0; b < pl.cblocks.size(); b++) {
+            const dev::gemm_cblock &cb = pl.cblocks[b];
+            uint32_t tm = (cb.m + bm - 1) / bm, tn = (cb.n + bn - 1) / bn;
+            lt_require(tm < 65536 && tn < 65536, LT_ERR_PARAM, "C block too large");
+            for(uint32_t i = 0; i < tm; i++) for(uint32_t j = 0; j < tn; j++) {
+                dev::gemm_tile t;
+                t.cblock = (uint32_t)b; t.tm = (uint16_t)i; t.tn = (uint16_t)j;
+                tl[k].push_back(t);
+            }
         }
     }
     size_t sz = 0;
     size_t o_d = sz; sz = align256(sz + descs.size() * sizeof(dev::oz_split_desc));
     size_t o_i = sz; sz = align256(sz + items.size() * 4);
     size_t o_p = sz; sz = align256(sz + ozp.size() * sizeof(dev::oz_pair));
-    size_t o_t = sz; sz = align256(sz + tiles.size() * sizeof(dev::gemm_tile));
+    size_t o_t[dev::OZ_TILE_COUNT];
+    for(int k = 0; k < dev::OZ_TILE_COUNT; k++) {
+        o_t[k] = sz;
+        sz = align256(sz + tl[k].size() * sizeof(dev::gemm_tile));
+    }
     std::vector<char> host(sz ? sz : 256, 0);
     if(!descs.empty()) std::memcpy(&host[o_d], descs.data(), descs.size() * sizeof(dev::oz_split_desc));
     if(!items.empty()) std::memcpy(&host[o_i], items.data(), items.size() * 4);
     if(!ozp.empty()) std::memcpy(&host[o_p], ozp.data(), ozp.size() * sizeof(dev::oz_pair));
-    if(!tiles.empty()) std::memcpy(&host[o_t], tiles.data(), tiles.size() * sizeof(dev::gemm_tile));
+    for(int k = 0; k < dev::OZ_TILE_COUNT; k++)
+        if(!tl[k].empty())
+            std::memcpy(&host[o_t[k]], tl[k].data(), tl[k].size() * sizeof(dev::gemm_tile));
     pl.oz_lists.ctx = &ctx;
     pl.oz_lists.mem = ctx.alloc(host.size());
     pl.oz_lists.bytes = host.size();
@@ -1348,8 +1361,55 @@ void btod_contract2::build_ozaki(int slices) {
     pl.d_oz_items = (uint32_t*)(base + o_i);
     pl.n_oz_items = (uint32_t)(items.size() / 2);
     pl.d_oz_pairs = (dev::oz_pair*)(base + o_p);
-    pl.d_oz_tiles = (dev::gemm_tile*)(base + o_t);
-    pl.n_oz_tiles = (uint32_t)tiles.size();
+
+    // Tile shape: each shape's product launch is timed once on this plan's
+    // own work list (after one prepare and split) and the fastest kept.
+    // Launches that would accumulate into C are never run for this: then
+    // the shape reading the fewest operand bytes (tiles x chunks x
+    // fragments per chunk) is taken.
+    bool all_zero = true;
+    for(size_t b = 0; b < pl.cblocks.size(); b++) all_zero = all_zero && pl.cblocks[b].zero;
+    int best = dev::OZ_TILE_16x64;
+    if(m_oz_tile >= 0) {
+        best = m_oz_tile;
+    } else if(all_zero && pl.events && !pl.cblocks.empty()) {
+        hipStream_t s = ctx.stream();
+        ctx.activate();
+        if(pl.n_prep_chunks)
+            hip_check(dev::launch_permute(pl.d_prep, pl.d_prep_cd, pl.d_prep_co,
+                pl.n_prep_chunks, s), "permute (prepare)");
+        hip_check(dev::launch_oz_split(pl.d_oz_split, pl.d_oz_items, pl.n_oz_items, s),
+            "split (ozaki)");
+        float bestms = 1e30f;
+        for(int k = 0; k < dev::OZ_TILE_COUNT; k++) {
+            float ms = 0.f;
+            hip_check(hipEventRecord(pl.ev[0], s), "hipEventRecord");
+            hip_check(dev::launch_oz_gemm(pl.d_cb, pl.d_oz_pairs,
+                (const dev::gemm_tile*)(base + o_t[k]), (uint32_t)tl[k].size(), slices, k, s),
+                "gemm (ozaki, tile shapes)");
+            hip_check(hipEventRecord(pl.ev[1], s), "hipEventRecord");
+            hip_check(hipStreamSynchronize(s), "hipStreamSynchronize");
+            hip_check(hipEventElapsedTime(&ms, pl.ev[0], pl.ev[1]), "hipEventElapsedTime");
+            if(ms < bestms) { bestms = ms; best = k; }
+        }
+    } else {
+        double bestb = 0.0;
+        for(int k = 0; k < dev::OZ_TILE_COUNT; k++) {
+            uint32_t bm, bn;
+            dev::oz_tile_shape(k, bm, bn);
+            double bytes = 0.0;
+            for(size_t t = 0; t < tl[k].size(); t++) {
+                const dev::gemm_cblock &cb = pl.cblocks[tl[k][t].cblock];
+                for(uint32_t p = cb.pair_begin; p < cb.pair_end; p++)
+                    bytes += double((pl.pairs[p].k + 63) / 64) * double(bm + bn);
+            }
+            if(k == 0 || bytes < bestb) { bestb = bytes; best = k; }
+        }
+    }
+    pl.oz_tile = best;
+    pl.oz_tile_set = m_oz_tile;
+    pl.d_oz_tiles = (dev::gemm_tile*)(base + o_t[best]);
+    pl.n_oz_tiles = (uint32_t)tl[best].size();
 }
 
 void btod_contract2::enqueue(int mode, hipStream_t s, bool timed, size_t &launches) {
@@ -1370,7 +1430,7 @@ void btod_contract2::enqueue(int mode, hipStream_t s, bool timed, size_t &launch
         hip_check(dev::launch_oz_split(pl.d_oz_split, pl.d_oz_items, pl.n_oz_items, s),
             "split (ozaki)");
         hip_check(dev::launch_oz_gemm(pl.d_cb, pl.d_oz_pairs, pl.d_oz_tiles, pl.n_oz_tiles,
-            pl.oz_slices, s), "gemm (ozaki)");
+            pl.oz_slices, pl.oz_tile, s), "gemm (ozaki)");
         launches += 2;
     } else if(mode == MODE_PER_BLOCK) {
         for(size_t b = 0; b + 1 < pl.block_tile_begin.size(); b++) {
@@ -1415,7 +1475,7 @@ void btod_contract2::run(btensor &c, int mode) {
     }
     if(mode == MODE_OZAKI) {
         int sl = m_oz_slices > 0 ? m_oz_slices : 7;
-        if(pl.oz_slices != sl) build_ozaki(sl);
+        if(pl.oz_slices != sl || pl.oz_tile_set != m_oz_tile) build_ozaki(sl);
         m_stats.ozaki_slices = sl;
     }
     // The grouped perform runs as a HIP graph (prepare permute, grouped

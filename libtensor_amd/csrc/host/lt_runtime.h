@@ -283,7 +283,9 @@ public:
     void set_tile_cfg(int cfg) { m_tile_cfg = cfg; }
     //! Grouped performs run as a captured HIP graph (default) or eagerly.
     void set_graph(bool on) { m_use_graph = on; }
-    void set_ozaki(int slices, int use_int8) { m_oz_slices = slices; m_oz_int8 = use_int8; }
+    void set_ozaki(int slices) { m_oz_slices = slices; }
+    //! split-precision tile shape (dev::oz_tile_cfg), -1 = measured per plan
+    void set_ozaki_tile(int tile) { m_oz_tile = tile; }
     double flops() const { return m_stats.flops; }
 
 private:
@@ -299,7 +301,7 @@ private:
     int m_tuned_order;      //!< and its grouped-launch tile order
     uint64_t m_tuned_va, m_tuned_vb;   //!< operand versions it was found for
     bool m_use_graph;
-    int m_oz_slices, m_oz_int8;
+    int m_oz_slices, m_oz_tile;
     std::unique_ptr<contract_plan> m_plan;
 
     void build_plan(btensor &c, bool add, double d);

@@ -71,3 +71,42 @@ def test_perform_add(gpu_ctx):
     op.perform(c, lt.LT_MODE_OZAKI)
     op.perform_add(c, -0.5, lt.LT_MODE_OZAKI)
     assert O.rel_err(c.to_dense(), 0.5 * result) < TOL
+
+
+@pytest.mark.parametrize("tile", [0, 1, 2])
+def test_every_tile_shape_exact_on_small_integers(gpu_ctx, tile):
+    """Each product-kernel tile shape (16x32, 16x64, 32x32), forced, gives the
+    exact integer product (C blocks of 60 rows and columns: the last row and
+    column panels of every shape are partial)."""
+    prob = P.prob_ring(4, 12, so=(2,), sv=(5,))
+    rng = np.random.default_rng(11)
+    dense = {"A": rng.integers(-60, 61, size=tuple(prob["A"]["dims"])).astype(float),
+             "B": rng.integers(-60, 61, size=tuple(prob["B"]["dims"])).astype(float)}
+    a, b, op = P.build(gpu_ctx, prob, dense)
+    c = P.make_result_tensor(gpu_ctx, prob)
+    op.perform(c, lt.LT_MODE_GROUPED)
+    exact = c.to_dense()
+    op.set_ozaki(3)
+    op.set_ozaki_tile(tile)
+    op.perform(c, lt.LT_MODE_OZAKI)
+    assert np.array_equal(c.to_dense(), exact), tile
+
+
+@pytest.mark.parametrize("name", ["ladder_full", "ladder_spin", "rand_221"])
+def test_tile_shapes_agree(gpu_ctx, name):
+    """Forced tile shapes and the measured choice give the same sums (the
+    integer slice products are exact, the FP64 recombination per element runs
+    in the same order): within 1e-14 of each other and 1e-12 of the FP64
+    reference, with 7 slices."""
+    prob, dense, meta, result = G.load(name)
+    a, b, op = P.build(gpu_ctx, prob, dense)
+    op.set_ozaki(7)
+    outs = {}
+    for tile in (-1, 0, 1, 2):
+        op.set_ozaki_tile(tile)
+        c = P.make_result_tensor(gpu_ctx, prob)
+        op.perform(c, lt.LT_MODE_OZAKI)
+        outs[tile] = c.to_dense()
+        assert O.rel_err(outs[tile], result) < TOL, tile
+    for tile in (0, 1, 2):
+        assert O.rel_err(outs[tile], outs[-1]) < 1e-14, tile
