@@ -402,8 +402,9 @@ __device__ __forceinline__ double ldg(const double *base, uint32_t off_bytes) {
 // Loads of one chunk.  k beyond the pair's extent reads element K - 1 (in
 // bounds); mma_chunk multiplies those A values by zero, so the loaded values
 // are not consumed here and the loads of the next chunk stay in flight
-// while the current chunk is on the matrix cores.  Blocks are < 2^31
-// elements, element offsets in bytes fit in 32 bits.
+// while the current chunk is on the matrix cores.  Offsets are 32-bit
+// byte offsets: the host refuses plans with an operand or C block of 2^29
+// or more elements (4 GiB), so every offset fits.
 template<int FM, int FN, bool TA, bool TB>
 __device__ __forceinline__ void load_chunk(double (&af)[FM][4], double (&bf)[FN][4],
     const gemm_pair &pr, int c, const int (&arow)[FM], const int (&bcol)[FN], int lg) {

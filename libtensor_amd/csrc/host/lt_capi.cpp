@@ -123,6 +123,11 @@ void dense_permute(context &ctx, const double *src, const dims_t &dsrc,
 void dense_contract2(context &cx, const contraction2 &contr, const dims_t &da, const double *a,
     const dims_t &db, const double *b, double d, bool zero, double *c) {
     dims_t dc = contract2_dims(contr, da, db);
+    // the GEMM kernels address operands with 32-bit byte offsets
+    const size_t lim = size_t(1) << 29;
+    lt_require(da.size() < lim && db.size() < lim && dc.size() < lim, LT_ERR_PARAM,
+        "tod_contract2: a tensor of 2^29 or more elements (32-bit byte offsets in the "
+        "GEMM kernels)");
     if(d == 0.0) {
         if(zero) {
             cx.activate();

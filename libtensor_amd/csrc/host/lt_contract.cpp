@@ -97,18 +97,58 @@ dev::perm_desc make_perm_desc(const double *src, const dims_t &dsrc,
     return d;
 }
 
-//! Device-side copy of host vectors in one allocation.
+//! Device-side copy of host vectors in one allocation (owned: freed on
+//! release, on reassignment and on destruction; not copyable).
 struct dev_lists {
     context *ctx;
     void *mem;
     size_t bytes;
     dev_lists() : ctx(0), mem(0), bytes(0) { }
-    ~dev_lists() { if(ctx && mem) ctx->free(mem); }
+    dev_lists(const dev_lists&) = delete;
+    dev_lists &operator=(const dev_lists&) = delete;
+    ~dev_lists() { release(); }
+    void release() {
+        if(ctx && mem) ctx->free(mem);
+        mem = 0;
+        bytes = 0;
+    }
+    //! Replaces the allocation with a new one of `n` bytes.
+    void *reset(context &c, size_t n) {
+        release();
+        ctx = &c;
+        mem = c.alloc(n);
+        bytes = n;
+        return mem;
+    }
 };
 
 size_t align256(size_t x) { return (x + 255) & ~size_t(255); }
 
+//! Largest block of a block index space (product of the largest segment
+//! of each dimension).
+double max_block_elems(const bis_t &bis) {
+    double n = 1.0;
+    const dims_t bid = bis.block_index_dims();
+    for(size_t i = 0; i < bis.order(); i++) {
+        size_t m = 0;
+        for(size_t b = 0; b < bid[i]; b++) m = std::max(m, bis.block_len(i, b));
+        n *= double(m);
+    }
+    return n;
+}
+
 } // unnamed namespace
+
+//! The GEMM and split-precision kernels address operand and C blocks with
+//! 32-bit byte offsets from a block's base address: every block of A, B and
+//! C must hold fewer than 2^29 elements (4 GiB).
+void btod_contract2::check_block_sizes() const {
+    const double lim = double(size_t(1) << 29);
+    lt_require(max_block_elems(m_a.bis()) < lim && max_block_elems(m_b.bis()) < lim &&
+        max_block_elems(m_symc.bis) < lim, LT_ERR_PARAM,
+        "btod_contract2: a block of 2^29 or more elements (the kernels address blocks "
+        "with 32-bit byte offsets; split such blocks further)");
+}
 
 //! Work items of a list of permute descriptors: chunks of kPermChunk
 //! destination elements, or the corners of 32 x 32 tiles for transposing
@@ -307,7 +347,7 @@ bool btod_contract2::plan_valid(btensor &c, bool add, double d, int) const {
 
 // ---------------------------------------------------------------------------
 
-void btod_contract2::build_plan(btensor &c, bool add, double dscale) {
+void btod_contract2::build_plan(btensor &c, bool add, double dscale, bool measure) {
     double t0 = now_ms();
     m_stats.n_plans++;
     context &ctx = m_a.ctx();
@@ -964,7 +1004,7 @@ void btod_contract2::build_plan(btensor &c, bool add, double dscale) {
         // launches, and each candidate costs two timed launches
         for(size_t i = 0; i < est.size(); i++)
             if(cfg != -2 || dev::is_stacked(est[i].second)) candidates.push_back(est[i].second);
-        if(cfg == -1 || add) candidates.resize(1);
+        if(cfg == -1 || add || !measure) candidates.resize(1);
         // the configuration measured for these operands (same pairs, same
         // work list for any result tensor) is reused without measuring again
         if(cfg == -2 && m_tuned_cfg >= 0 && m_tuned_va == pl.va && m_tuned_vb == pl.vb)
@@ -1024,9 +1064,7 @@ void btod_contract2::build_plan(btensor &c, bool add, double dscale) {
     put(o_sco, sco.data(), sco.size() * 4);
     put(o_ar, pl.arows.data(), pl.arows.size() * sizeof(dev::stack_arow));
     put(o_cr, pl.crows.data(), pl.crows.size() * sizeof(dev::stack_crow));
-    pl.lists.ctx = &ctx;
-    pl.lists.mem = ctx.alloc(sz);
-    pl.lists.bytes = sz;
+    pl.lists.reset(ctx, sz);
     {
         // stack pairs with their offset tables at their device addresses
         const int32_t *ot = (const int32_t*)((char*)pl.lists.mem + o_ot);
@@ -1063,10 +1101,7 @@ void btod_contract2::build_plan(btensor &c, bool add, double dscale) {
         size_t nt = pl.tiles_sorted.size() * sizeof(dev::gemm_tile);
         size_t nb = pl.tiles_by_block.size() * sizeof(dev::gemm_tile);
         size_t tsz = align256(nt) + align256(nb) + 256;
-        if(pl.tiles.mem) ctx.free(pl.tiles.mem);
-        pl.tiles.ctx = &ctx;
-        pl.tiles.mem = ctx.alloc(tsz);
-        pl.tiles.bytes = tsz;
+        pl.tiles.reset(ctx, tsz);
         std::vector<char> th(tsz, 0);
         if(nt) std::memcpy(&th[0], pl.tiles_sorted.data(), nt);
         if(nb) std::memcpy(&th[align256(nt)], pl.tiles_by_block.data(), nb);
@@ -1102,10 +1137,7 @@ void btod_contract2::build_plan(btensor &c, bool add, double dscale) {
                     return x.first > y.first; });
             for(size_t i = 0; i < vt.size(); i++) pl.vtiles.push_back(vt[i].second);
             size_t nb = pl.vtiles.size() * sizeof(dev::var_tile);
-            if(pl.stk_tiles.mem) ctx.free(pl.stk_tiles.mem);
-            pl.stk_tiles.ctx = &ctx;
-            pl.stk_tiles.mem = ctx.alloc(nb ? nb : 256);
-            pl.stk_tiles.bytes = nb ? nb : 256;
+            pl.stk_tiles.reset(ctx, nb ? nb : 256);
             if(nb) ctx.upload(pl.stk_tiles.mem, pl.vtiles.data(), nb);
             pl.d_vtiles = (dev::var_tile*)pl.stk_tiles.mem;
         } else if(dev::is_stacked(k)) {
@@ -1159,10 +1191,7 @@ void btod_contract2::build_plan(btensor &c, bool add, double dscale) {
                     for(size_t l = 0; l < q[x].size(); l++) pl.stiles[l * nx + x] = q[x][l];
             }
             size_t nb = pl.stiles.size() * sizeof(dev::gemm_tile);
-            if(pl.stk_tiles.mem) ctx.free(pl.stk_tiles.mem);
-            pl.stk_tiles.ctx = &ctx;
-            pl.stk_tiles.mem = ctx.alloc(nb ? nb : 256);
-            pl.stk_tiles.bytes = nb ? nb : 256;
+            pl.stk_tiles.reset(ctx, nb ? nb : 256);
             if(nb) ctx.upload(pl.stk_tiles.mem, pl.stiles.data(), nb);
             pl.d_stiles = (dev::gemm_tile*)pl.stk_tiles.mem;
         }
@@ -1239,8 +1268,8 @@ void btod_contract2::build_plan(btensor &c, bool add, double dscale) {
 void btod_contract2::build_ozaki(int slices) {
     contract_plan &pl = *m_plan;
     context &ctx = m_a.ctx();
-    pl.oz_lists = dev_lists();
-    pl.oz_mem = dev_lists();
+    pl.oz_lists.release();
+    pl.oz_mem.release();
     pl.oz_slices = slices;
 
     // one split per distinct operand matrix: A-role operands scaled per row
@@ -1285,9 +1314,7 @@ void btod_contract2::build_ozaki(int slices) {
     }
     size_t exp_off = bytes;
     bytes += nexp * sizeof(int32_t);
-    pl.oz_mem.ctx = &ctx;
-    pl.oz_mem.mem = ctx.alloc(bytes ? bytes : 256);
-    pl.oz_mem.bytes = bytes;
+    pl.oz_mem.reset(ctx, bytes ? bytes : 256);
     int8_t *sl = (int8_t*)pl.oz_mem.mem;
     int32_t *ex = (int32_t*)(sl + exp_off);
 
@@ -1352,9 +1379,7 @@ void btod_contract2::build_ozaki(int slices) {
     for(int k = 0; k < dev::OZ_TILE_COUNT; k++)
         if(!tl[k].empty())
             std::memcpy(&host[o_t[k]], tl[k].data(), tl[k].size() * sizeof(dev::gemm_tile));
-    pl.oz_lists.ctx = &ctx;
-    pl.oz_lists.mem = ctx.alloc(host.size());
-    pl.oz_lists.bytes = host.size();
+    pl.oz_lists.reset(ctx, host.size());
     ctx.upload(pl.oz_lists.mem, host.data(), host.size());
     char *base = (char*)pl.oz_lists.mem;
     pl.d_oz_split = (dev::oz_split_desc*)(base + o_d);
@@ -1540,6 +1565,7 @@ void btod_contract2::perform_batch(std::vector<batch_item> &items, dev_buffer &m
         batch_item &it = items[i];
         btod_contract2 &op = *it.op;
         lt_require(&op.m_a.ctx() == &ctx, LT_ERR_PARAM, "batch spans contexts");
+        op.check_block_sizes();
         lt_require(it.c->bis() == op.m_symc.bis, LT_ERR_DIMS,
             "result tensor has a different block index space");
         if(it.add) {
@@ -1669,7 +1695,8 @@ void btod_contract2::perform_batch(std::vector<batch_item> &items, dev_buffer &m
     }
 }
 
-void btod_contract2::prepare(btensor &c, bool add, double d) {
+void btod_contract2::prepare(btensor &c, bool add, double d, bool measure) {
+    check_block_sizes();
     m_a.ctx().require_device("btod_contract2::prepare");
     lt_require(c.bis() == m_symc.bis, LT_ERR_DIMS, "result tensor has a different block index space");
     if(add) {
@@ -1679,7 +1706,7 @@ void btod_contract2::prepare(btensor &c, bool add, double d) {
                 "perform_add: result symmetry differs from the contraction's");
         }
     }
-    if(!plan_valid(c, add, d, MODE_GROUPED)) build_plan(c, add, d);
+    if(!plan_valid(c, add, d, MODE_GROUPED)) build_plan(c, add, d, measure);
 }
 
 size_t btod_contract2::footprint_estimate() {
@@ -1694,6 +1721,7 @@ size_t btod_contract2::footprint_estimate() {
 }
 
 void btod_contract2::perform(btensor &c, int mode) {
+    check_block_sizes();
     m_a.ctx().require_device("btod_contract2::perform");
     lt_require(c.bis() == m_symc.bis, LT_ERR_DIMS, "result tensor has a different block index space");
     if(!plan_valid(c, false, 1.0, mode)) build_plan(c, false, 1.0);
@@ -1701,6 +1729,7 @@ void btod_contract2::perform(btensor &c, int mode) {
 }
 
 void btod_contract2::perform_add(btensor &c, double d, int mode) {
+    check_block_sizes();
     m_a.ctx().require_device("btod_contract2::perform");
     lt_require(c.bis() == m_symc.bis, LT_ERR_DIMS, "result tensor has a different block index space");
     // the += path supports a result tensor with the same symmetry orbits
@@ -1724,6 +1753,7 @@ void btod_contract2::compute_block(bool zero, const index_t &ic, const tensor_tr
     // by their transformations and the scalars folded into d, pairs with
     // d == 0 skipped (tod_contract2_impl.h:129), the sum transformed by trc
     // into the output block.
+    check_block_sizes();
     context &ctx = m_a.ctx();
     ctx.require_device("btod_contract2::compute_block");
     dims_t bida = m_a.bidims(), bidb = m_b.bidims();

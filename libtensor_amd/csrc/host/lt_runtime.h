@@ -224,6 +224,8 @@ struct dev_buffer {
     void *mem;
     size_t bytes;
     dev_buffer() : ctx(0), mem(0), bytes(0) { }
+    dev_buffer(const dev_buffer&) = delete;
+    dev_buffer &operator=(const dev_buffer&) = delete;
     ~dev_buffer() { release(); }
     void release() { if(ctx && mem) ctx->free(mem); mem = 0; bytes = 0; }
 };
@@ -248,6 +250,8 @@ public:
 
     const symmetry &symc() const { return m_symc; }
     const std::vector<size_t> &schedule() const { return m_sched; }
+    const btensor &operand_a() const { return m_a; }
+    const btensor &operand_b() const { return m_b; }
 
     //! C := result (gen_bto_aux_copy): zero C, take the result symmetry,
     //! write every scheduled block.
@@ -270,8 +274,11 @@ public:
     //! every write-back.  `merged` holds the concatenated lists.
     static void perform_batch(std::vector<batch_item> &items, dev_buffer &merged);
     //! Builds the execution plan for C unless it is cached (device memory
-    //! for work lists and operand scratch, kernel autotuning).
-    void prepare(btensor &c, bool add, double d);
+    //! for work lists and operand scratch, kernel autotuning).  measure
+    //! false: the kernel configuration is chosen by the padding estimate
+    //! instead of timed launches (callers whose stream is shared with other
+    //! work, where timings would include that work).
+    void prepare(btensor &c, bool add, double d, bool measure = true);
     bool plan_resident(btensor &c, bool add, double d) const {
         return plan_valid(c, add, d, MODE_GROUPED);
     }
@@ -304,7 +311,8 @@ private:
     int m_oz_slices, m_oz_tile;
     std::unique_ptr<contract_plan> m_plan;
 
-    void build_plan(btensor &c, bool add, double d);
+    void check_block_sizes() const;
+    void build_plan(btensor &c, bool add, double d, bool measure = true);
     bool plan_valid(btensor &c, bool add, double d, int mode) const;
     void run(btensor &c, int mode);
     void enqueue(int mode, hipStream_t s, bool timed, size_t &launches);

@@ -46,8 +46,16 @@ uint64_t server::submit(btod_contract2 *op, btensor *c, bool add, double d) {
     lt_require(!m_stop, LT_ERR_PARAM, "server is stopping");
     for(std::map<uint64_t, request>::iterator i = m_reqs.begin(); i != m_reqs.end(); ++i) {
         if(i->second.state == DONE) continue;
-        lt_require(i->second.op != op && i->second.c != c, LT_ERR_PARAM,
+        const request &q = i->second;
+        lt_require(q.op != op && q.c != c, LT_ERR_PARAM,
             "operation or result tensor already has a request in flight");
+        // a request may neither read the result of a request in flight nor
+        // write an operand one reads (requests of a batch run their prepare
+        // steps before any product)
+        lt_require(q.c != &op->operand_a() && q.c != &op->operand_b(), LT_ERR_STATE,
+            "an operand is the result tensor of a request in flight");
+        lt_require(c != &q.op->operand_a() && c != &q.op->operand_b(), LT_ERR_STATE,
+            "the result tensor is an operand of a request in flight");
     }
     request r;
     r.op = op; r.c = c; r.add = add; r.d = d;
@@ -125,7 +133,8 @@ void server::planner_loop() {
         int status = LT_OK;
         std::string msg;
         try {
-            snapshot.op->prepare(*snapshot.c, snapshot.add, snapshot.d);
+            // no timed autotuning here: the executor shares the stream
+            snapshot.op->prepare(*snapshot.c, snapshot.add, snapshot.d, false);
         } catch(const lt_error &e) {
             status = e.code;
             msg = e.what();

@@ -91,3 +91,31 @@ def test_block_access_and_orbits(host_ctx):
     assert np.array_equal(d[0:2, 2:5], x)
     assert np.array_equal(d[2:5, 0:2], -x.T)
     assert bt.req_nonzero_blocks() == [1]
+
+
+def test_large_blocks_refused_before_device_work(host_ctx):
+    """The GEMM kernels address blocks with 32-bit byte offsets: a plan
+    with a block of 2^29 or more elements is refused with LT_ERR_PARAM (on
+    any context, before device work), one element fewer is accepted up to
+    the device check."""
+    c = lt.contraction2(1, 1, 1)
+    c.contract(1, 0)
+    big = lt.block_tensor(host_ctx, [1 << 15, 1 << 14])       # one 2^29 block
+    b = lt.block_tensor(host_ctx, [1 << 14, 2])
+    op = lt.btod_contract2(c, big, b)
+    out = lt.block_tensor(host_ctx, [1 << 15, 2])
+    with pytest.raises(lt.LtError) as e:
+        op.perform(out)
+    assert e.value.code == -1 and "2^29" in str(e.value)
+    with pytest.raises(lt.LtError) as e:
+        op.perform_add(out, 1.0)
+    assert e.value.code == -1
+    # split in two along the rows: blocks of 2^28 elements pass the guard
+    # and stop at the device check
+    bis = lt.block_index_space([1 << 15, 1 << 14]).split([1, 0], 1 << 14)
+    a2 = lt.block_tensor(host_ctx, bis)
+    op2 = lt.btod_contract2(c, a2, b)
+    out2 = lt.block_tensor(host_ctx, lt.block_index_space([1 << 15, 2]).split([1, 0], 1 << 14))
+    with pytest.raises(lt.LtError) as e:
+        op2.perform(out2)
+    assert e.value.code == -4

@@ -110,3 +110,26 @@ def test_tile_shapes_agree(gpu_ctx, name):
         assert O.rel_err(outs[tile], result) < TOL, tile
     for tile in (0, 1, 2):
         assert O.rel_err(outs[tile], outs[-1]) < 1e-14, tile
+
+
+def test_slice_count_changes_keep_device_memory(gpu_ctx):
+    """Rebuilding the split form for another slice count or tile shape
+    frees the previous one: alternating performs leave the context's
+    device memory in use unchanged."""
+    prob, dense, meta, result = G.load("ladder_full")
+    a, b, op = P.build(gpu_ctx, prob, dense)
+    c = P.make_result_tensor(gpu_ctx, prob)
+    op.set_ozaki(3)
+    op.perform(c, lt.LT_MODE_OZAKI)
+    op.set_ozaki(7)
+    op.perform(c, lt.LT_MODE_OZAKI)
+    used = gpu_ctx.mem()[0]
+    for s, tile in ((3, -1), (7, -1), (5, 0), (7, 2), (3, 1), (7, -1)):
+        op.set_ozaki(s)
+        op.set_ozaki_tile(tile)
+        op.perform(c, lt.LT_MODE_OZAKI)
+    op.set_ozaki(7)
+    op.set_ozaki_tile(-1)
+    op.perform(c, lt.LT_MODE_OZAKI)
+    assert gpu_ctx.mem()[0] == used
+    assert O.rel_err(c.to_dense(), result) <= TOL
