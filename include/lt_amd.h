@@ -125,6 +125,23 @@ int lt_tod_contract2(lt_ctx *ctx, size_t N, size_t M, size_t K,
     const size_t *dims_a, const double *a, const size_t *dims_b,
     const double *b, double d, int zero, double *c);
 
+/* No counterpart in the reference: lt_tod_contract2 for callers that cannot
+ * allocate device memory (code under HIP graph capture, serving loops and
+ * harnesses that forbid hipMalloc).  Same arguments and results as
+ * lt_tod_contract2 (same kernels); every temporary (aligned operand copies,
+ * the aligned result, descriptors, tile list) lives in a fixed device region
+ * of LT_NOALLOC_SCRATCH_BYTES per GPU, held from context creation until the
+ * process ends: no hipMalloc, hipFree or allocating library call.  A call
+ * whose temporaries do not fit fails with LT_ERR_PARAM before any device
+ * work.  The region is shared by every call on one GPU and keeps the last
+ * call's contents: calls that overlap on one GPU (several threads or
+ * contexts) race. */
+#define LT_NOALLOC_SCRATCH_BYTES ((size_t)32 << 20)
+int lt_tod_contract2_noalloc(lt_ctx *ctx, size_t N, size_t M, size_t K,
+    const size_t *permc, const size_t *ka, const size_t *kb,
+    const size_t *dims_a, const double *a, const size_t *dims_b,
+    const double *b, double d, int zero, double *c);
+
 /* tod_dotprod<N>(a, perma, b, permb).calculate()
  * (libtensor/dense_tensor/tod_dotprod.h:77, :98) times c. */
 int lt_tod_dotprod(lt_ctx *ctx, size_t n, const size_t *dims_a,

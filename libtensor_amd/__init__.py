@@ -16,7 +16,7 @@ from ._capi import (GEMM_CONFIGS, GEMM_STACKED_FIRST, LT_MODE_GROUPED, LT_MODE_O
 
 __all__ = ["Context", "product_table", "block_index_space", "block_tensor",
            "se_perm", "se_part", "se_label", "contraction2", "btod_contract2",
-           "device_array", "tod_copy", "tod_add", "tod_set", "tod_contract2", "tod_dotprod", "btod_dotprod", "btod_copy", "btod_add", "btod_symmetrize2", "btod_symmetrize3", "btod_contract3", "server",
+           "device_array", "tod_copy", "tod_add", "tod_set", "tod_contract2", "tod_contract2_noalloc", "tod_dotprod", "btod_dotprod", "btod_copy", "btod_add", "btod_symmetrize2", "btod_symmetrize3", "btod_contract3", "server",
            "LtError", "GEMM_CONFIGS", "GEMM_STACKED_FIRST", "LT_MODE_PER_BLOCK", "LT_MODE_GROUPED", "LT_MODE_OZAKI"]
 
 
@@ -642,6 +642,16 @@ def tod_contract2(contr, a, b, c, d=1.0, zero=True):
     call("lt_tod_contract2", a.ctx._h, contr.N, contr.M, contr.K, sizes(contr.permc),
          sizes(ka), sizes(kb), sizes(a.shape), a.ptr, sizes(b.shape), b.ptr, float(d),
          1 if zero else 0, c.ptr)
+
+
+def tod_contract2_noalloc(contr, a, b, c, d=1.0, zero=True):
+    """tod_contract2 with every temporary in the fixed per-GPU scratch region
+    (no device allocation; LtError LT_ERR_PARAM when it does not fit)."""
+    ka = [p[0] for p in contr.pairs]
+    kb = [p[1] for p in contr.pairs]
+    call("lt_tod_contract2_noalloc", a.ctx._h, contr.N, contr.M, contr.K,
+         sizes(contr.permc), sizes(ka), sizes(kb), sizes(a.shape), a.ptr, sizes(b.shape),
+         b.ptr, float(d), 1 if zero else 0, c.ptr)
 
 
 def tod_dotprod(a, b, perma=None, permb=None, c=1.0):

@@ -79,6 +79,9 @@ _SIGS = {
     "lt_tod_contract2": [vp, ctypes.c_size_t, ctypes.c_size_t, ctypes.c_size_t,
                          c_size_p, c_size_p, c_size_p, c_size_p, vp, c_size_p, vp,
                          ctypes.c_double, ctypes.c_int, vp],
+    "lt_tod_contract2_noalloc": [vp, ctypes.c_size_t, ctypes.c_size_t, ctypes.c_size_t,
+                                 c_size_p, c_size_p, c_size_p, c_size_p, vp, c_size_p, vp,
+                                 ctypes.c_double, ctypes.c_int, vp],
     "lt_tod_dotprod": [vp, ctypes.c_size_t, c_size_p, vp, c_size_p, c_size_p, vp,
                        c_size_p, ctypes.c_double, c_double_p],
     "lt_product_table_create": [ctypes.c_char_p, ctypes.c_size_t],

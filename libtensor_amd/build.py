@@ -23,7 +23,7 @@ HOST_SRC = ["host/lt_core.cpp", "host/lt_sym.cpp", "host/lt_sched.cpp",
             "host/lt_runtime.cpp", "host/lt_contract.cpp", "host/lt_dotprod.cpp",
             "host/lt_blockops.cpp", "host/lt_symmetrize.cpp",
             "host/lt_server.cpp", "host/lt_capi.cpp"]
-DEV_SRC = ["device/lt_kernels.hip", "device/lt_ozaki.hip"]
+DEV_SRC = ["device/lt_kernels.hip", "device/lt_ozaki.hip", "device/lt_scratch.hip"]
 
 HOST_FLAGS = ["-std=c++17", "-O2", "-fPIC", "-Wall", "-Wno-unused-variable",
               "-Wno-unused-but-set-variable", "-D__HIP_PLATFORM_AMD__",

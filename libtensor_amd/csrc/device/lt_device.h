@@ -237,5 +237,13 @@ hipError_t launch_dot(const perm_desc *descs, const uint32_t *chunk_desc,
     const uint32_t *chunk_off, uint32_t nchunks, double *partials,
     double *out, hipStream_t s);
 
+//! Size of the fixed device scratch region of the no-allocation entry
+//! points (one per GPU, held until the process ends).
+static const size_t kNoallocScratchBytes = size_t(32) << 20;
+static const int kMaxDevices = 64;
+//! Device address of that region on `device`, which must be the current
+//! device on the first call for it (resolved once, then cached).
+hipError_t noalloc_scratch(int device, void **out);
+
 } // namespace dev
 } // namespace lta

@@ -9,6 +9,9 @@
 
 using namespace lta;
 
+static_assert(LT_NOALLOC_SCRATCH_BYTES == lta::dev::kNoallocScratchBytes,
+    "scratch region size in the header and the device source");
+
 // A context stays alive while tensors, device arrays or operations created
 // from it exist, whatever order the caller releases handles in.
 struct lt_ctx {
@@ -90,38 +93,91 @@ contraction2 make_contraction(size_t N, size_t M, size_t K, const size_t *permc,
     return c;
 }
 
-} // unnamed namespace
-
-namespace lta {
-
-void dense_permute(context &ctx, const double *src, const dims_t &dsrc,
-    const tensor_tr &tr, double *dst, bool accumulate) {
-    dev::perm_desc d = make_perm_desc_public(src, dsrc, tr, dst, accumulate);
-    std::vector<uint32_t> cd, co;
-    make_chunks(std::vector<dev::perm_desc>(1, d), cd, co);
-    if(cd.empty()) return;
-    size_t bytes = sizeof(d) + 8 * cd.size() + 512;
-    char *mem = (char*)ctx.alloc(bytes);
-    std::vector<char> host(bytes, 0);
-    size_t ocd = (sizeof(d) + 255) & ~size_t(255);
-    size_t oco = ocd + ((4 * cd.size() + 255) & ~size_t(255));
-    std::memcpy(&host[0], &d, sizeof(d));
-    std::memcpy(&host[ocd], cd.data(), 4 * cd.size());
-    std::memcpy(&host[oco], co.data(), 4 * co.size());
-    try {
-        ctx.upload(mem, host.data(), bytes);
-        hip_check(dev::launch_permute((dev::perm_desc*)mem, (uint32_t*)(mem + ocd),
-            (uint32_t*)(mem + oco), (uint32_t)cd.size(), ctx.stream()), "permute");
-        ctx.sync();
-    } catch(...) {
-        ctx.free(mem);
-        throw;
+//! Device memory for the temporaries of one dense operation: taken from the
+//! context's allocator and released when the arena goes out of scope, or
+//! carved out of a fixed device region (the no-allocation entry point; no
+//! allocation at all, LT_ERR_PARAM when the region is too small).  Callers
+//! take every buffer before their first device operation, so a refusal
+//! leaves the device untouched.
+class scratch_arena {
+public:
+    explicit scratch_arena(context &ctx, char *region = 0, size_t cap = 0) :
+        m_ctx(ctx), m_region(region), m_cap(cap), m_off(0) { }
+    ~scratch_arena() {
+        if(!m_region) for(size_t i = 0; i < m_taken.size(); i++) m_ctx.free(m_taken[i]);
     }
-    ctx.free(mem);
+    scratch_arena(const scratch_arena&) = delete;
+    scratch_arena &operator=(const scratch_arena&) = delete;
+    //! A 256-byte aligned buffer of `bytes` bytes.
+    char *take(size_t bytes) {
+        bytes = (std::max<size_t>(bytes, 8) + 255) & ~size_t(255);
+        if(!m_region) {
+            char *p = (char*)m_ctx.alloc(bytes);
+            m_taken.push_back(p);
+            return p;
+        }
+        if(bytes > m_cap - m_off) {
+            throw lt_error(LT_ERR_PARAM, "temporaries exceed the fixed scratch region of " +
+                std::to_string(m_cap) + " bytes");
+        }
+        char *p = m_region + m_off;
+        m_off += bytes;
+        return p;
+    }
+private:
+    context &m_ctx;
+    char *m_region;
+    size_t m_cap, m_off;
+    std::vector<void*> m_taken;
+};
+
+//! One permute launch: descriptor and work-item lists laid out for one
+//! upload (descriptor at 0, chunk descriptors at ocd, offsets at oco).
+struct permute_job {
+    std::vector<char> host;
+    size_t ocd, oco, nchunks;
+    char *mem;
+    permute_job() : ocd(0), oco(0), nchunks(0), mem(0) { }
+    void prepare(const double *src, const dims_t &dsrc, const tensor_tr &tr, double *dst,
+        bool accumulate, scratch_arena &ar) {
+        dev::perm_desc d = make_perm_desc_public(src, dsrc, tr, dst, accumulate);
+        std::vector<uint32_t> cd, co;
+        make_chunks(std::vector<dev::perm_desc>(1, d), cd, co);
+        nchunks = cd.size();
+        if(!nchunks) return;
+        ocd = (sizeof(d) + 255) & ~size_t(255);
+        oco = ocd + ((4 * cd.size() + 255) & ~size_t(255));
+        host.assign(oco + 4 * co.size(), 0);
+        std::memcpy(&host[0], &d, sizeof(d));
+        std::memcpy(&host[ocd], cd.data(), 4 * cd.size());
+        std::memcpy(&host[oco], co.data(), 4 * co.size());
+        mem = ar.take(host.size());
+    }
+    void upload(context &ctx) { if(nchunks) ctx.upload(mem, host.data(), host.size()); }
+    void launch(context &ctx) {
+        if(!nchunks) return;
+        hip_check(dev::launch_permute((dev::perm_desc*)mem, (uint32_t*)(mem + ocd),
+            (uint32_t*)(mem + oco), (uint32_t)nchunks, ctx.stream()), "permute");
+    }
+};
+
+void dense_permute_in(context &ctx, const double *src, const dims_t &dsrc,
+    const tensor_tr &tr, double *dst, bool accumulate, scratch_arena &ar) {
+    permute_job j;
+    j.prepare(src, dsrc, tr, dst, accumulate, ar);
+    if(!j.nchunks) return;
+    j.upload(ctx);
+    ctx.activate();
+    j.launch(ctx);
+    ctx.sync();
 }
 
-void dense_contract2(context &cx, const contraction2 &contr, const dims_t &da, const double *a,
-    const dims_t &db, const double *b, double d, bool zero, double *c) {
+//! tod_contract2 with every temporary (aligned operand copies, the aligned
+//! result, descriptors and tile list) taken from `ar` before the first
+//! device operation.
+void dense_contract2_in(context &cx, const contraction2 &contr, const dims_t &da,
+    const double *a, const dims_t &db, const double *b, double d, bool zero, double *c,
+    scratch_arena &ar) {
     dims_t dc = contract2_dims(contr, da, db);
     // the GEMM kernels address operands with 32-bit byte offsets
     const size_t lim = size_t(1) << 29;
@@ -147,80 +203,93 @@ void dense_contract2(context &cx, const contraction2 &contr, const dims_t &da, c
     da2.permute(al.perma);
     db2.permute(al.permb);
     dc2.permute(al.permc);
-    std::vector<void*> tmp;
-    try {
-        const double *pa = a, *pb = b;
-        if(!al.perma.is_identity()) {
-            double *t = (double*)cx.alloc(da.size() * 8);
-            tmp.push_back(t);
-            dense_permute(cx, a, da, tensor_tr(al.perma, scalar_tr()), t, false);
-            pa = t;
-        }
-        if(!al.permb.is_identity()) {
-            double *t = (double*)cx.alloc(db.size() * 8);
-            tmp.push_back(t);
-            dense_permute(cx, b, db, tensor_tr(al.permb, scalar_tr()), t, false);
-            pb = t;
-        }
-        double *pc = c;
-        bool cperm = !al.permc.is_identity();
-        if(cperm) {
-            pc = (double*)cx.alloc(dc.size() * 8);
-            tmp.push_back(pc);
-        }
-        gemm_form g = aligned_gemm_form(ca, da2, db2);
-        dev::gemm_cblock cb;
-        dev::gemm_pair gp;
-        std::memset(&cb, 0, sizeof(cb));
-        std::memset(&gp, 0, sizeof(gp));
-        size_t lda = g.ta ? g.m : g.k, ldb = g.tb ? g.k : g.n;
-        bool ta = g.ta, tb = g.tb;
-        if(!g.tc) {
-            cb.m = (uint32_t)g.m; cb.n = (uint32_t)g.n;
-            gp.a = pa; gp.b = pb; gp.lda = (uint32_t)lda; gp.ldb = (uint32_t)ldb;
-        } else {
-            cb.m = (uint32_t)g.n; cb.n = (uint32_t)g.m;
-            gp.a = pb; gp.b = pa; gp.lda = (uint32_t)ldb; gp.ldb = (uint32_t)lda;
-            ta = !g.tb; tb = !g.ta;
-        }
-        cb.ldc = cb.n;
-        cb.c = pc;
-        cb.zero = (cperm || zero) ? 1 : 0;
-        cb.pair_begin = 0;
-        cb.pair_end = 1;
-        gp.k = (uint32_t)g.k;
-        gp.d = d;
-        gp.flags = (ta ? dev::kPairAKMajor : 0) | (tb ? dev::kPairBNMajor : 0);
-        uint32_t bm = 64, bn = 64;
-        dev::tile_shape(dev::TILE_64x64, bm, bn);
-        std::vector<dev::gemm_tile> tiles;
-        for(uint32_t i = 0; i < (cb.m + bm - 1) / bm; i++)
-            for(uint32_t j = 0; j < (cb.n + bn - 1) / bn; j++) {
-                dev::gemm_tile t; t.cblock = 0; t.tm = (uint16_t)i; t.tn = (uint16_t)j;
-                tiles.push_back(t);
-            }
-        size_t bytes = 512 + 512 + tiles.size() * sizeof(dev::gemm_tile);
-        char *mem = (char*)cx.alloc(bytes);
-        tmp.push_back(mem);
-        std::vector<char> host(bytes, 0);
-        std::memcpy(&host[0], &cb, sizeof(cb));
-        std::memcpy(&host[512], &gp, sizeof(gp));
-        std::memcpy(&host[1024], tiles.data(), tiles.size() * sizeof(dev::gemm_tile));
-        cx.upload(mem, host.data(), bytes);
-        cx.activate();
-        hip_check(dev::launch_gemm(dev::TILE_64x64, (dev::gemm_cblock*)mem,
-            (dev::gemm_pair*)(mem + 512), (dev::gemm_tile*)(mem + 1024),
-            (uint32_t)tiles.size(), (uint32_t)tiles.size(), cx.stream()), "gemm");
-        cx.sync();
-        if(cperm) {
-            dense_permute(cx, pc, dc2, tensor_tr(al.permc.inverse(), scalar_tr()),
-                c, zero == 0);
-        }
-    } catch(...) {
-        for(size_t i = 0; i < tmp.size(); i++) cx.free(tmp[i]);
-        throw;
+
+    // host side: every buffer and descriptor
+    const double *pa = a, *pb = b;
+    permute_job ja, jb, jc;
+    if(!al.perma.is_identity()) {
+        double *t = (double*)ar.take(da.size() * 8);
+        ja.prepare(a, da, tensor_tr(al.perma, scalar_tr()), t, false, ar);
+        pa = t;
     }
-    for(size_t i = 0; i < tmp.size(); i++) cx.free(tmp[i]);
+    if(!al.permb.is_identity()) {
+        double *t = (double*)ar.take(db.size() * 8);
+        jb.prepare(b, db, tensor_tr(al.permb, scalar_tr()), t, false, ar);
+        pb = t;
+    }
+    double *pc = c;
+    bool cperm = !al.permc.is_identity();
+    if(cperm) {
+        pc = (double*)ar.take(dc.size() * 8);
+        jc.prepare(pc, dc2, tensor_tr(al.permc.inverse(), scalar_tr()), c, !zero, ar);
+    }
+    gemm_form g = aligned_gemm_form(ca, da2, db2);
+    dev::gemm_cblock cb;
+    dev::gemm_pair gp;
+    std::memset(&cb, 0, sizeof(cb));
+    std::memset(&gp, 0, sizeof(gp));
+    size_t lda = g.ta ? g.m : g.k, ldb = g.tb ? g.k : g.n;
+    bool ta = g.ta, tb = g.tb;
+    if(!g.tc) {
+        cb.m = (uint32_t)g.m; cb.n = (uint32_t)g.n;
+        gp.a = pa; gp.b = pb; gp.lda = (uint32_t)lda; gp.ldb = (uint32_t)ldb;
+    } else {
+        cb.m = (uint32_t)g.n; cb.n = (uint32_t)g.m;
+        gp.a = pb; gp.b = pa; gp.lda = (uint32_t)ldb; gp.ldb = (uint32_t)lda;
+        ta = !g.tb; tb = !g.ta;
+    }
+    cb.ldc = cb.n;
+    cb.c = pc;
+    cb.zero = (cperm || zero) ? 1 : 0;
+    cb.pair_begin = 0;
+    cb.pair_end = 1;
+    gp.k = (uint32_t)g.k;
+    gp.d = d;
+    gp.flags = (ta ? dev::kPairAKMajor : 0) | (tb ? dev::kPairBNMajor : 0);
+    uint32_t bm = 64, bn = 64;
+    dev::tile_shape(dev::TILE_64x64, bm, bn);
+    std::vector<dev::gemm_tile> tiles;
+    for(uint32_t i = 0; i < (cb.m + bm - 1) / bm; i++)
+        for(uint32_t j = 0; j < (cb.n + bn - 1) / bn; j++) {
+            dev::gemm_tile t; t.cblock = 0; t.tm = (uint16_t)i; t.tn = (uint16_t)j;
+            tiles.push_back(t);
+        }
+    size_t bytes = 512 + 512 + tiles.size() * sizeof(dev::gemm_tile);
+    char *mem = ar.take(bytes);
+    std::vector<char> host(bytes, 0);
+    std::memcpy(&host[0], &cb, sizeof(cb));
+    std::memcpy(&host[512], &gp, sizeof(gp));
+    std::memcpy(&host[1024], tiles.data(), tiles.size() * sizeof(dev::gemm_tile));
+
+    // device side, in stream order
+    ja.upload(cx);
+    jb.upload(cx);
+    jc.upload(cx);
+    cx.upload(mem, host.data(), bytes);
+    cx.activate();
+    ja.launch(cx);
+    jb.launch(cx);
+    hip_check(dev::launch_gemm(dev::TILE_64x64, (dev::gemm_cblock*)mem,
+        (dev::gemm_pair*)(mem + 512), (dev::gemm_tile*)(mem + 1024),
+        (uint32_t)tiles.size(), (uint32_t)tiles.size(), cx.stream()), "gemm");
+    jc.launch(cx);
+    cx.sync();
+}
+
+} // unnamed namespace
+
+namespace lta {
+
+void dense_permute(context &ctx, const double *src, const dims_t &dsrc,
+    const tensor_tr &tr, double *dst, bool accumulate) {
+    scratch_arena ar(ctx);
+    dense_permute_in(ctx, src, dsrc, tr, dst, accumulate, ar);
+}
+
+void dense_contract2(context &cx, const contraction2 &contr, const dims_t &da, const double *a,
+    const dims_t &db, const double *b, double d, bool zero, double *c) {
+    scratch_arena ar(cx);
+    dense_contract2_in(cx, contr, da, a, db, b, d, zero, c, ar);
 }
 }
 
@@ -238,7 +307,15 @@ int lt_ctx_create(int device, size_t cap, lt_ctx **out) {
         c->refs = 1;
         try {
             c->c = new context(device, cap);
+            if(device >= 0) {
+                // resolve this device's fixed scratch region (no-allocation
+                // entry points) once, with the device current
+                void *p = 0;
+                c->c->activate();
+                hip_check(dev::noalloc_scratch(device, &p), "scratch region address");
+            }
         } catch(...) {
+            delete c->c;
             delete c;
             throw;
         }
@@ -367,6 +444,26 @@ int lt_tod_contract2(lt_ctx *ctx, size_t N, size_t M, size_t K,
         contraction2 contr = make_contraction(N, M, K, permc, ka, kb);
         dense_contract2(cx, contr, dims_t::from(N + K, dims_a), a, dims_t::from(M + K, dims_b),
             b, d, zero != 0, c);
+    });
+}
+
+int lt_tod_contract2_noalloc(lt_ctx *ctx, size_t N, size_t M, size_t K,
+    const size_t *permc, const size_t *ka, const size_t *kb,
+    const size_t *dims_a, const double *a, const size_t *dims_b,
+    const double *b, double d, int zero, double *c) {
+    return guard([&]() {
+        need(ctx, "ctx"); need(dims_a, "dims_a"); need(dims_b, "dims_b");
+        need(a, "a"); need(b, "b"); need(c, "c");
+        if(K) { need(ka, "ka"); need(kb, "kb"); }
+        context &cx = *ctx->c;
+        cx.require_device("tod_contract2_noalloc");
+        contraction2 contr = make_contraction(N, M, K, permc, ka, kb);
+        cx.activate();
+        void *region = 0;
+        hip_check(dev::noalloc_scratch(cx.device(), &region), "scratch region address");
+        scratch_arena ar(cx, (char*)region, dev::kNoallocScratchBytes);
+        dense_contract2_in(cx, contr, dims_t::from(N + K, dims_a), a,
+            dims_t::from(M + K, dims_b), b, d, zero != 0, c, ar);
     });
 }
 
