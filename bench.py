@@ -38,24 +38,60 @@ def log(*a):
 
 
 def cpu_baseline(prob, a, b, flops, shape, threads):
-    """The real libtensor (oracle/_ref/ref_driver) on the host cores: the FP64
-    reference result and the reference's own perform() time."""
+    """The host-CPU baseline on this machine's cores, and the FP64 reference
+    result.
+
+    libtensor itself (oracle/_ref/ref_driver, built from its sources with its
+    default linalg_generic loops: the image has no cblas.h for its BLAS
+    backend) runs the contraction on the same inputs: its schedule and
+    contraction lists, its result (the parity reference) and its own perform
+    time (thread_pool of `threads` workers, median of 5 after a warm-up).  The
+    timed baseline is the oracle's restatement of libtensor's per-block work
+    over those lists with a BLAS GEMM per block pair (oracle/cpu_contract.py:
+    numpy's OpenBLAS, one thread per worker, `threads` workers over C blocks
+    as libtensor's thread_pool(n, n)), median of 5 after a warm-up, and on
+    one core (median of 3)."""
     sys.path.insert(0, os.path.join(HERE, "oracle"))
     import ref_io
+    import cpu_contract as CC
     if not ref_io.ref_available():
         return None, None
     with tempfile.TemporaryDirectory() as d:
         t0 = time.time()
         out = ref_io.run_ref(prob, d, shape, tensors={"A": a, "B": b},
-                             extra=["--threads", str(threads), "--repeat", "2",
-                                    "--no-symc", "--no-clst"], timeout=1500)
+                             extra=["--threads", str(threads), "--repeat", "5", "--no-symc"],
+                             timeout=1500)
         log("cpu reference done in %.1fs" % (time.time() - t0))
-    ms = out.get("perform_median_ms", out.get("perform_ms"))
-    base = {"value": flops / (ms * 1e-3) / 1e12, "unit": "TFLOP/s", "cores": threads,
-            "kind": "reference",
-            "sample": "full pp-ladder contraction (libtensor btod_contract2, linalg_generic "
-                      "FP64 loops, thread_pool of %d workers), median of 2 after 1 warm-up; "
-                      "%.1f ms per perform" % (threads, ms)}
+    ref_ms = out.get("perform_median_ms", out.get("perform_ms"))
+    clst = out["clst"]
+    na = prob["contr"]["N"] + prob["contr"]["K"]
+    ia, ib = CC.needed_blocks(clst, na)
+    ba = {x: a.get_block(x) for x in ia if x in set(a.req_nonzero_blocks())}
+    nzb = set(b.req_nonzero_blocks())
+    bb = {x: b.get_block(x) for x in ib if x in nzb}
+    bc = CC.BlockContraction(prob, ba, bb, out["schedule"], clst, shape)
+    t0 = time.time()
+    med, times, blocks = CC.time_perform(bc, threads, 5)
+    med1, times1, _ = CC.time_perform(bc, 1, 3)
+    err = bc.max_rel_err(blocks, out["result"]) if "result" in out else None
+    log("cpu restatement: %.1f ms (%d threads), %.1f ms (1 thread), err %s, %.1fs" % (
+        med, threads, med1, err, time.time() - t0))
+    info = CC.host_info()
+    base = {"value": flops / (med * 1e-3) / 1e12, "unit": "TFLOP/s", "cores": threads,
+            "kind": "port",
+            "sample": "full pp-ladder contraction: libtensor's schedule and contraction lists, "
+                      "per block pair a %s GEMM (numpy.tensordot, single-threaded BLAS), "
+                      "%d workers over C blocks; median of 5 after 1 warm-up: %.1f ms; "
+                      "1 core: %.1f ms (%.4f TFLOP/s); libtensor itself (linalg_generic "
+                      "loops, thread_pool of %d): %.1f ms (%.4f TFLOP/s); host %s, nproc %d, "
+                      "affinity %d" % (CC.blas_info(), threads, med, med1,
+                                       flops / (med1 * 1e-3) / 1e12, threads, ref_ms,
+                                       flops / (ref_ms * 1e-3) / 1e12, info["cpu_model"],
+                                       info["nproc"], info["affinity"]),
+            "one_core_value": flops / (med1 * 1e-3) / 1e12,
+            "reference_value": flops / (ref_ms * 1e-3) / 1e12,
+            "reference_ms": ref_ms, "port_ms": med, "port_one_core_ms": med1,
+            "port_max_rel_err": err, "cpu_model": info["cpu_model"], "nproc": info["nproc"]}
     return base, out.get("result")
 
 
@@ -361,7 +397,11 @@ def main():
 
     base, ref = None, None
     if not args.no_cpu:
-        threads = args.cpu_threads or min(16, os.cpu_count() or 1)
+        try:
+            ncpu = len(os.sched_getaffinity(0))
+        except AttributeError:
+            ncpu = os.cpu_count() or 1
+        threads = args.cpu_threads or min(16, ncpu)
         base, ref = cpu_baseline(prob, a, b, flops, result.shape, threads)
     rel = None
     if ref is not None:

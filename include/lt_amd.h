@@ -190,6 +190,13 @@ int lt_btensor_export(lt_btensor *bt, double *host);
 int lt_btensor_import(lt_btensor *bt, const double *host);
 /* Copy the symmetry of src (so_copy). */
 int lt_btensor_copy_symmetry(lt_btensor *dst, lt_btensor *src);
+/* Lower bt's symmetry to the symmetry bt shares with `other`, as a sum of
+ * the two would have it (gen_bto_aux_add::open, gen_block_tensor/impl/
+ * gen_bto_aux_add_impl.h:40-66: so_merge(so_dirsum(...)); addition_schedule
+ * touches the orbits it splits, :115-145): canonical blocks of the new
+ * symmetry that bt held only through an orbit of its old one are written out
+ * on the device.  A tensor without blocks may be on a host-only context. */
+int lt_btensor_merge_symmetry(lt_btensor *bt, lt_btensor *other);
 
 /* ---- symmetry elements (libtensor/symmetry/se_perm.h:28, se_part.h:32,
  *      se_label.h:37); build, then lt_btensor_insert (symmetry::insert) ---- */
@@ -202,6 +209,14 @@ int lt_se_label_create(lt_btensor *bt, const char *table_id, lt_symel **out);
 int lt_se_label_assign(lt_symel *e, const int *mask, size_t blk, size_t label);
 /* se_label::set_rule(label_set) */
 int lt_se_label_set_rule(lt_symel *e, const size_t *intrinsic, size_t nintr);
+/* Evaluation rule term by term (libtensor/symmetry/evaluation_rule.h:56
+ * new_product, product_rule.h add(seq, intr); evaluation_rule::clear): a
+ * block is allowed if, for one product, every term's labels (dimension i
+ * counted seqs[t * n + i] times) have the intrinsic label intr[t] in their
+ * product; (size_t)-1 as intr[t] accepts any label. */
+int lt_se_label_clear_rule(lt_symel *e);
+int lt_se_label_add_product(lt_symel *e, size_t nterms, const size_t *seqs,
+    const size_t *intr);
 int lt_symel_destroy(lt_symel *e);
 int lt_btensor_insert(lt_btensor *bt, const lt_symel *e);
 
@@ -215,12 +230,18 @@ int lt_btod_contract2_create(size_t N, size_t M, size_t K, const size_t *permc,
 int lt_btod_contract2_destroy(lt_btod_contract2 *op);
 /* perform(btc): C := result.  mode: LT_MODE_*. */
 int lt_btod_contract2_perform(lt_btod_contract2 *op, lt_btensor *c, int mode);
-/* perform(btc, d): C += d * result (C with the result's symmetry). */
+/* perform(btc, d): C += d * result.  C of another symmetry is first lowered
+ * to the symmetry it shares with the result (lt_btensor_merge_symmetry), as
+ * addition_schedule + gen_bto_aux_add do (block_tensor/impl/
+ * btod_contract2_impl.h:70-90). */
 int lt_btod_contract2_perform_add(lt_btod_contract2 *op, lt_btensor *c,
     double d, int mode);
 /* get_schedule(): sorted non-zero canonical C blocks. */
 int lt_btod_contract2_schedule(lt_btod_contract2 *op, size_t *out, size_t cap,
     size_t *count);
+/* get_symmetry() (libtensor/gen_block_tensor/direct_gen_bto.h:38): dst (of
+ * C's block index space, holding no blocks) takes the result symmetry. */
+int lt_btod_contract2_get_symmetry(lt_btod_contract2 *op, lt_btensor *dst);
 /* Result symmetry: canonical index / allowedness of a C block. */
 int lt_btod_contract2_symc_orbit(lt_btod_contract2 *op, size_t aidx,
     size_t *acidx, size_t *perm, double *coeff, int *allowed);
@@ -272,8 +293,9 @@ int lt_btod_dotprod(lt_btensor *a, const size_t *perma, double ca, lt_btensor *b
  *      btod_copy(bta, perm, c).perform(btb) and .perform(btb, d);
  *      gen_block_tensor/impl/gen_bto_copy_impl.h:150-163, :302-330) ----
  * add == 0: B := c P(A); B (block index space of A permuted by perm) takes
- * the permuted symmetry of A.  add != 0: B += d c P(A); B must already have
- * that symmetry.  perm may be NULL (identity). */
+ * the permuted symmetry of A.  add != 0: B += d c P(A); a B of another
+ * symmetry is first lowered to the symmetry both share (through
+ * gen_bto_aux_add in the reference).  perm may be NULL (identity). */
 int lt_btod_copy(lt_btensor *a, const size_t *perm, double c, lt_btensor *b, int add,
     double d);
 
@@ -281,11 +303,12 @@ int lt_btod_copy(lt_btensor *a, const size_t *perm, double c, lt_btensor *b, int
  *      add_op(bta, perm, c), then perform(btb) / perform(btb, d);
  *      gen_block_tensor/impl/gen_bto_add_impl.h) ----
  * B := sum_i c[i] P_i(A_i) (add == 0) or B += d sum_i ... (add != 0).  perms
- * holds nops permutations of the tensor order (NULL: identities).  The
- * operands' permuted symmetries must give every block the same orbit
- * (LT_ERR_SYMMETRY otherwise; the reference also adds operands of different
- * symmetry, into their intersection); B takes that symmetry, or must have it
- * for add != 0. */
+ * holds nops permutations of the tensor order (NULL: identities).  B takes
+ * the symmetry of the sum (gen_bto_add::add_operand, gen_bto_add_impl.h:
+ * 226-270: so_dirsum + so_merge operand by operand); with add != 0, B is
+ * first lowered to the symmetry it shares with that.  Every block of the sum
+ * is computed (the reference's schedule can miss blocks where the sum's
+ * symmetry is lower than an operand's). */
 int lt_btod_add(size_t nops, lt_btensor *const *a, const size_t *perms, const double *c,
     lt_btensor *b, int add, double d);
 

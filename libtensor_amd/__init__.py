@@ -278,6 +278,12 @@ class block_tensor(object):
     def copy_symmetry_from(self, other):
         call("lt_btensor_copy_symmetry", self._h, other._h)
 
+    def merge_symmetry_with(self, other):
+        """Lowers this tensor's symmetry to the one it shares with `other`
+        (the symmetry of a sum; canonical blocks that split off are written
+        out on the device)."""
+        call("lt_btensor_merge_symmetry", self._h, other._h)
+
 
 class contraction2(object):
     """Connection of the indexes of A, B and C (libtensor/core/contraction2.h)."""
@@ -330,6 +336,10 @@ class btod_contract2(object):
         out = (ctypes.c_size_t * max(1, cnt.value))()
         call("lt_btod_contract2_schedule", self._h, out, cnt.value, ctypes.byref(cnt))
         return [out[i] for i in range(cnt.value)]
+
+    def get_symmetry(self, dst):
+        """dst (C's block index space, no blocks) takes the result symmetry."""
+        call("lt_btod_contract2_get_symmetry", self._h, dst._h)
 
     def symc_orbit(self, aidx):
         nc = self.contr.N + self.contr.M

@@ -105,6 +105,7 @@ _SIGS = {
     "lt_btensor_export": [vp, c_double_p],
     "lt_btensor_import": [vp, c_double_p],
     "lt_btensor_copy_symmetry": [vp, vp],
+    "lt_btensor_merge_symmetry": [vp, vp],
     "lt_se_perm_create": [ctypes.c_size_t, c_size_p, ctypes.c_double, ctypes.POINTER(vp)],
     "lt_se_part_create": [vp, c_size_p, ctypes.POINTER(vp)],
     "lt_se_part_add_map": [vp, c_size_p, c_size_p, ctypes.c_double],
@@ -112,6 +113,8 @@ _SIGS = {
     "lt_se_label_create": [vp, ctypes.c_char_p, ctypes.POINTER(vp)],
     "lt_se_label_assign": [vp, c_int_p, ctypes.c_size_t, ctypes.c_size_t],
     "lt_se_label_set_rule": [vp, c_size_p, ctypes.c_size_t],
+    "lt_se_label_clear_rule": [vp],
+    "lt_se_label_add_product": [vp, ctypes.c_size_t, c_size_p, c_size_p],
     "lt_symel_destroy": [vp],
     "lt_btensor_insert": [vp, vp],
     "lt_btod_contract2_create": [ctypes.c_size_t, ctypes.c_size_t, ctypes.c_size_t,
@@ -122,6 +125,7 @@ _SIGS = {
     "lt_btod_contract2_perform": [vp, vp, ctypes.c_int],
     "lt_btod_contract2_perform_add": [vp, vp, ctypes.c_double, ctypes.c_int],
     "lt_btod_contract2_schedule": [vp, c_size_p, ctypes.c_size_t, c_size_p],
+    "lt_btod_contract2_get_symmetry": [vp, vp],
     "lt_btod_contract2_symc_orbit": [vp, ctypes.c_size_t, c_size_p, c_size_p,
                                      c_double_p, c_int_p],
     "lt_btod_contract2_clst": [vp, ctypes.c_size_t, c_double_p, ctypes.c_size_t, c_size_p],

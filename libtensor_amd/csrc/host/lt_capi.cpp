@@ -731,6 +731,17 @@ int lt_btensor_copy_symmetry(lt_btensor *dst, lt_btensor *src) {
     });
 }
 
+int lt_btensor_merge_symmetry(lt_btensor *bt, lt_btensor *other) {
+    return guard([&]() {
+        need(bt, "bt"); need(other, "other");
+        lt_require(bt->t->bis() == other->t->bis(), LT_ERR_DIMS,
+            "merge_symmetry: different block index spaces");
+        if(!bt->t->nonzero_blocks().empty())
+            bt->ctx->c->require_device("merge_symmetry of a tensor holding blocks");
+        merge_symmetry(*bt->t, other->t->sym());
+    });
+}
+
 // ---- symmetry elements ----
 
 int lt_se_perm_create(size_t n, const size_t *perm, double coeff, lt_symel **out) {
@@ -812,6 +823,38 @@ int lt_se_label_set_rule(lt_symel *e, const size_t *intr, size_t nintr) {
         std::vector<size_t> v;
         for(size_t i = 0; i < nintr; i++) v.push_back(intr[i]);
         l->set_rule_set(v);
+    });
+}
+
+int lt_se_label_clear_rule(lt_symel *e) {
+    return guard([&]() {
+        need(e, "e");
+        se_label *l = dynamic_cast<se_label*>(e->e.get());
+        lt_require(l != 0, LT_ERR_PARAM, "not an se_label");
+        l->rule.clear();
+    });
+}
+
+int lt_se_label_add_product(lt_symel *e, size_t nterms, const size_t *seqs,
+    const size_t *intr) {
+    return guard([&]() {
+        need(e, "e");
+        se_label *l = dynamic_cast<se_label*>(e->e.get());
+        lt_require(l != 0, LT_ERR_PARAM, "not an se_label");
+        if(nterms) { need(seqs, "seqs"); need(intr, "intr"); }
+        const size_t n = l->bidims.n;
+        const product_table &pt = product_table_registry::instance().get(l->table_id);
+        label_product pr;
+        for(size_t t = 0; t < nterms; t++) {
+            label_term term;
+            term.seq = seq_t(n);
+            for(size_t i = 0; i < n; i++) term.seq[i] = seqs[t * n + i];
+            term.intrinsic = intr[t];
+            lt_require(term.intrinsic == k_label_invalid || term.intrinsic < pt.nlabels,
+                LT_ERR_PARAM, "se_label: invalid intrinsic label");
+            pr.push_back(term);
+        }
+        l->rule.push_back(pr);
     });
 }
 
@@ -909,6 +952,17 @@ int lt_btod_contract2_symc_orbit(lt_btod_contract2 *op, size_t aidx, size_t *aci
         if(acidx) *acidx = o.acidx;
         fill_transf(o.get_transf(aidx), s.bis.order(), perm, coeff);
         if(allowed) *allowed = o.allowed ? 1 : 0;
+    });
+}
+
+int lt_btod_contract2_get_symmetry(lt_btod_contract2 *op, lt_btensor *dst) {
+    return guard([&]() {
+        need(op, "op"); need(dst, "dst");
+        lt_require(dst->t->bis() == op->op->symc().bis, LT_ERR_DIMS,
+            "get_symmetry: tensor of another block index space");
+        lt_require(dst->t->nonzero_blocks().empty(), LT_ERR_STATE,
+            "get_symmetry: the tensor holds blocks");
+        dst->t->set_symmetry(op->op->symc());
     });
 }
 

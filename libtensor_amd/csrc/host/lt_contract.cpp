@@ -312,7 +312,8 @@ uint64_t next_op_uid() {
 btod_contract2::btod_contract2(const contraction2 &c, btensor &a, double ka,
     btensor &b, double kb, double kc) :
     m_contr(c), m_a(a), m_b(b), m_ka(ka), m_kb(kb), m_kc(kc), m_uid(next_op_uid()),
-    m_tile_cfg(-2), m_tuned_cfg(-1), m_tuned_order(-1), m_tuned_va(0), m_tuned_vb(0), m_use_graph(true), m_oz_slices(0), m_oz_tile(-1) {
+    m_tile_cfg(-2), m_tuned_cfg(-1), m_tuned_order(-1), m_tuned_va(0), m_tuned_vb(0), m_use_graph(true), m_oz_slices(0), m_oz_tile(-1),
+    m_add_uid(0), m_add_ver(0), m_add_direct(false) {
 
     lt_require(c.is_complete(), LT_ERR_PARAM, "contraction is incomplete");
     lt_require(a.bis().order() == c.NA() && b.bis().order() == c.NB(),
@@ -1569,11 +1570,9 @@ void btod_contract2::perform_batch(std::vector<batch_item> &items, dev_buffer &m
         lt_require(it.c->bis() == op.m_symc.bis, LT_ERR_DIMS,
             "result tensor has a different block index space");
         if(it.add) {
-            for(size_t j = 0; j < op.m_sched.size(); j++) {
-                short_orbit so(it.c->sym(), op.m_sched[j], false);
-                lt_require(so.acidx == op.m_sched[j], LT_ERR_SYMMETRY,
-                    "perform_add: result symmetry differs from the contraction's");
-            }
+            lt_require(op.add_direct(*it.c), LT_ERR_SYMMETRY,
+                "perform_add: result symmetry differs from the contraction's (the request "
+                "loop adds into tensors of the result's symmetry)");
         }
         if(!op.plan_valid(*it.c, it.add, it.d, MODE_GROUPED))
             op.build_plan(*it.c, it.add, it.d);
@@ -1700,11 +1699,9 @@ void btod_contract2::prepare(btensor &c, bool add, double d, bool measure) {
     m_a.ctx().require_device("btod_contract2::prepare");
     lt_require(c.bis() == m_symc.bis, LT_ERR_DIMS, "result tensor has a different block index space");
     if(add) {
-        for(size_t i = 0; i < m_sched.size(); i++) {
-            short_orbit so(c.sym(), m_sched[i], false);
-            lt_require(so.acidx == m_sched[i], LT_ERR_SYMMETRY,
-                "perform_add: result symmetry differs from the contraction's");
-        }
+        lt_require(add_direct(c), LT_ERR_SYMMETRY,
+            "perform_add: result symmetry differs from the contraction's (the request "
+            "loop adds into tensors of the result's symmetry)");
     }
     if(!plan_valid(c, add, d, MODE_GROUPED)) build_plan(c, add, d, measure);
 }
@@ -1728,15 +1725,31 @@ void btod_contract2::perform(btensor &c, int mode) {
     run(c, mode);
 }
 
+bool btod_contract2::add_direct(btensor &c) {
+    if(c.sym_tag() == m_uid) return true;
+    if(c.uid() != m_add_uid || c.version() != m_add_ver) {
+        m_add_direct = same_orbits(m_symc, c.sym(), c.bidims());
+        m_add_uid = c.uid();
+        m_add_ver = c.version();
+    }
+    return m_add_direct;
+}
+
 void btod_contract2::perform_add(btensor &c, double d, int mode) {
     check_block_sizes();
     m_a.ctx().require_device("btod_contract2::perform");
     lt_require(c.bis() == m_symc.bis, LT_ERR_DIMS, "result tensor has a different block index space");
-    // the += path supports a result tensor with the same symmetry orbits
-    for(size_t i = 0; i < m_sched.size(); i++) {
-        short_orbit so(c.sym(), m_sched[i], false);
-        lt_require(so.acidx == m_sched[i], LT_ERR_SYMMETRY,
-            "perform_add: result symmetry differs from the contraction's");
+    if(!add_direct(c)) {
+        // C of another symmetry (btod_contract2_impl.h:70-90: addition_schedule
+        // + gen_bto_aux_add): C is lowered to the symmetry it shares with the
+        // result, the result is computed into a resident temporary and every
+        // canonical block of it added to each canonical block of C in its orbit
+        merge_symmetry(c, m_symc);
+        if(!m_addtmp) m_addtmp.reset(new btensor(m_a.ctx(), m_symc.bis));
+        perform(*m_addtmp, mode);
+        tensor_tr id(c.bis().order());
+        add_into(c, *m_addtmp, id, d);
+        return;
     }
     if(!plan_valid(c, true, d, mode)) build_plan(c, true, d);
     run(c, mode);

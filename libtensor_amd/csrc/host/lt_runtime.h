@@ -163,6 +163,18 @@ void make_chunks(const std::vector<dev::perm_desc> &ds, std::vector<uint32_t> &c
 dev::perm_desc make_perm_desc_public(const double *src, const dims_t &dsrc,
     const tensor_tr &tr, double *dst, bool accumulate);
 
+//! Lowers b's symmetry to the symmetry of a sum of b and a term of symmetry
+//! `other` (sum_symmetry): canonical blocks of the new symmetry that b held
+//! only through an orbit of the old one are written out on the device.
+//! No-op when the orbits stay the same.  ms (optional) accumulates the
+//! device time.
+void merge_symmetry(btensor &b, const symmetry &other, double *ms = 0);
+
+//! b += d tr(s) for a resident tensor s of any symmetry whose orbits, after
+//! tr, contain b's (b already merged): every non-zero canonical block of s
+//! goes to each canonical block of b in its orbit (one permute launch).
+void add_into(btensor &b, btensor &s, const tensor_tr &tr, double d, double *ms = 0);
+
 //! btod_dotprod(A, tra, B, trb).calculate() on resident block tensors.
 double btod_dotprod(btensor &a, const tensor_tr &tra, btensor &b, const tensor_tr &trb);
 
@@ -170,9 +182,9 @@ double btod_dotprod(btensor &a, const tensor_tr &tra, btensor &b, const tensor_t
 //! symmetry of A) or perform(b, d) (add: B += d tra(A), B with that symmetry).
 void btod_copy(btensor &a, const tensor_tr &tra, btensor &b, bool add, double d);
 
-//! btod_add: B := sum_i trs[i](ops[i]) (add false; B takes the operands'
-//! permuted symmetry, which must give every block the same orbit for all
-//! operands) or B += d sum_i trs[i](ops[i]) (add; B with that symmetry).
+//! btod_add: B := sum_i trs[i](ops[i]) (add false; B takes the symmetry all
+//! permuted operands have) or B += d sum_i trs[i](ops[i]) (add; B lowered to
+//! the symmetry it shares with them first).
 void btod_add(const std::vector<btensor*> &ops, const std::vector<tensor_tr> &trs,
     btensor &b, bool add, double d);
 
@@ -310,8 +322,16 @@ private:
     bool m_use_graph;
     int m_oz_slices, m_oz_tile;
     std::unique_ptr<contract_plan> m_plan;
+    //! += into a tensor whose orbits differ from the result's: the result
+    //! goes to this resident temporary first
+    std::unique_ptr<btensor> m_addtmp;
+    uint64_t m_add_uid, m_add_ver;   //!< tensor (uid, version) last checked
+    bool m_add_direct;               //!< ... and whether its orbits agree
 
     void check_block_sizes() const;
+    //! True if c's orbits are the result's (the result adds into c's
+    //! canonical blocks directly); cached per (uid, structure version).
+    bool add_direct(btensor &c);
     void build_plan(btensor &c, bool add, double d, bool measure = true);
     bool plan_valid(btensor &c, bool add, double d, int mode) const;
     void run(btensor &c, int mode);

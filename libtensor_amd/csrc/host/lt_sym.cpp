@@ -324,7 +324,7 @@ se_label::se_label(const dims_t &bid, const std::string &tid) :
 se_label::se_label(const se_label &o) :
     sym_element(o), table_id(o.table_id), bidims(o.bidims), labels(o.labels),
     rule(o.rule), src(o.src), keep(o.keep), rdims(o.rdims), rlabels(o.rlabels),
-    alts(o.alts), factors(o.factors) {
+    alts(o.alts), factors(o.factors), any_of(o.any_of) {
 }
 
 void se_label::set_label(size_t dim, size_t blk, size_t label) {
@@ -381,6 +381,15 @@ bool se_label::eval_rule(const size_t *lab) const {
 }
 
 bool se_label::eval_one(const size_t *lab) const {
+    if(!any_of.empty()) {
+        for(size_t k = 0; k < any_of.size(); k++) {
+            bool all = true;
+            for(size_t e = 0; e < any_of[k].size() && all; e++)
+                all = any_of[k][e]->eval_labels(lab);
+            if(all) return true;
+        }
+        return false;
+    }
     if(!factors.empty()) {
         for(size_t f = 0; f < factors.size(); f++) {
             size_t sub[kMaxOrder];
@@ -464,6 +473,12 @@ void se_label::permute(const perm_t &p) {
             for(size_t it = 0; it < rule[ip].size(); it++)
                 p.apply(rule[ip][it].seq);
     }
+    for(size_t k = 0; k < any_of.size(); k++)
+        for(size_t e = 0; e < any_of[k].size(); e++) {
+            std::shared_ptr<se_label> x(new se_label(*any_of[k][e]));
+            x->permute(p);
+            any_of[k][e] = x;
+        }
     // new index y = p(x): alternative q becomes p^-1, then q, then p
     const perm_t pinv(p.inverse());
     for(size_t f = 0; f < factors.size(); f++)
@@ -1935,6 +1950,219 @@ symmetry so_symmetrize_triple(const symmetry &sym, const perm_t &p1, const perm_
     lt_require(c[2] - 1 == G.nidx && c[3] - 1 == G.nidx, LT_ERR_PARAM,
         "symmetrize3: groups of different sizes");
     return so_symmetrize_groups(sym, G, scalar_tr(s), scalar_tr(1.0));
+}
+
+
+// ---------------------------------------------------------------------------
+// Symmetry of a sum of two tensors.
+
+namespace {
+
+//! The se_perm elements of a subset as (permutation, scalar) pairs.
+std::vector<tensor_tr> perm_gens(const std::vector<std::shared_ptr<sym_element> > *set) {
+    std::vector<tensor_tr> out;
+    if(set) for(size_t i = 0; i < set->size(); i++) {
+        const se_perm &e = dynamic_cast<const se_perm&>(*(*set)[i]);
+        lt_require(e.tr.st.c == 1.0 || e.tr.st.c == -1.0, LT_ERR_SYMMETRY,
+            "sum of tensors: permutational symmetry with a scalar other than +1 or -1");
+        out.push_back(e.tr);
+    }
+    return out;
+}
+
+size_t perm_order(const perm_t &p) {
+    perm_t q(p);
+    size_t k = 1;
+    while(!q.is_identity()) { q.permute(p); k++; }
+    return k;
+}
+
+//! Adds to grp (over na + nb indexes) the elements libtensor's so_dirsum
+//! forms from one fixed side (perm `fixed`, scalar t) and the generators of
+//! the other side (inst/so_dirsum_se_perm_impl.h, combine()): with t = +1,
+//! fixed (+) g for the generators g of scalar +1; with t = -1, fixed (+) g
+//! for the generators of scalar -1 when the other side has no generator of
+//! scalar +1, else fixed (+) h^j g h^(i-j) for every generator h of scalar
+//! +1, 0 <= j < i < order(h), and g of scalar -1.  (For scalars +1 and -1 the
+//! reference's search for products of the remaining generators contributes
+//! only the identity.)  left: the fixed side is the first na indexes.
+void dirsum_combine(size_t na, size_t nb, bool left, const perm_t &fixed, double t,
+    const std::vector<tensor_tr> &other, permutation_group &grp) {
+    if(other.empty()) return;
+    std::vector<perm_t> p1, p2;   // scalar == t, scalar +1 (when t != +1)
+    for(size_t i = 0; i < other.size(); i++) {
+        if(other[i].st.c == t) p1.push_back(other[i].perm);
+        else if(other[i].st.c == 1.0) p2.push_back(other[i].perm);
+    }
+    const size_t n = na + nb;
+    std::vector<perm_t> add;
+    if(!p2.empty()) {
+        for(size_t a = 0; a < p2.size(); a++) {
+            const size_t ord = perm_order(p2[a]);
+            for(size_t i = 1; i < ord; i++)
+                for(size_t j = 0; j < i; j++)
+                    for(size_t b = 0; b < p1.size(); b++) {
+                        perm_t px(left ? nb : na);
+                        size_t k = 0;
+                        for(; k < j; k++) px.permute(p2[a]);
+                        px.permute(p1[b]);
+                        for(; k < i; k++) px.permute(p2[a]);
+                        add.push_back(px);
+                    }
+        }
+    } else {
+        add = p1;
+    }
+    for(size_t x = 0; x < add.size(); x++) {
+        // sequence of the combined permutation: the fixed side's images,
+        // the other side's images shifted past the first na indexes
+        size_t seqa[2 * kMaxOrder], seqb[2 * kMaxOrder];
+        for(size_t i = 0; i < n; i++) seqa[i] = i;
+        for(size_t i = 0; i < na; i++) seqb[i] = left ? fixed[i] : add[x][i];
+        for(size_t i = 0; i < nb; i++) seqb[na + i] = (left ? add[x][i] : fixed[i]) + na;
+        grp.add_orbit(scalar_tr(t), permutation_builder(n, seqb, seqa));
+    }
+}
+
+//! libtensor's so_dirsum of two se_perm sets followed by so_merge of index
+//! i with index i + n (gen_bto_aux_add_impl.h:40-66,
+//! addition_schedule_impl.h:27-45, gen_bto_add_impl.h:238-267): the
+//! combined group is built from the generators as the reference does
+//! (inst/so_dirsum_se_perm_impl.h:24-60), permuted by `perm` (the second
+//! operand's permutation in btod_add), and its stabilizer of the diagonal
+//! pairs mapped back to n indexes (inst/so_merge_se_perm_impl.h:26-97).  The
+//! resulting group can be smaller than the intersection of the two groups:
+//! the reference combines generators, not group elements.
+void sum_perm(const symmetry &a, const symmetry &b, const perm_t &bperm, symmetry &out) {
+    const std::vector<tensor_tr> g1 = perm_gens(a.subset("perm")),
+        g2 = perm_gens(b.subset("perm"));
+    if(g1.empty() && g2.empty()) return;
+    const size_t n = a.order(), n2 = 2 * n;
+    permutation_group grp(n2);
+    const perm_t id(n);
+    dirsum_combine(n, n, false, id, 1.0, g1, grp);
+    dirsum_combine(n, n, true, id, 1.0, g2, grp);
+    for(size_t i = 0; i < g1.size(); i++)
+        if(!g1[i].st.is_identity()) dirsum_combine(n, n, true, g1[i].perm, g1[i].st.c, g2, grp);
+    for(size_t i = 0; i < g2.size(); i++)
+        if(!g2[i].st.is_identity()) dirsum_combine(n, n, false, g2[i].perm, g2[i].st.c, g1, grp);
+    // the second operand's permutation acts on the upper n indexes
+    perm_t pfull(n2);
+    {
+        size_t s1[2 * kMaxOrder], s2[2 * kMaxOrder];
+        seq_t sa(n);
+        for(size_t i = 0; i < n; i++) sa[i] = i + n;
+        bperm.apply(sa);
+        for(size_t i = 0; i < n; i++) { s1[i] = s2[i] = i; }
+        for(size_t i = n; i < n2; i++) { s1[i] = i; s2[i] = sa[i - n]; }
+        pfull = permutation_builder(n2, s2, s1);
+    }
+    grp.permute(pfull);
+    std::vector<permutation_group::gen_t> gs;
+    grp.convert(gs);
+    // so_merge: the group again from those generators, its stabilizer of
+    // the index pairs (i, i + n), each element restricted to n indexes
+    std::vector<std::shared_ptr<sym_element> > set;
+    for(size_t i = 0; i < gs.size(); i++)
+        set.push_back(std::shared_ptr<sym_element>(new se_perm(gs[i].first, gs[i].second)));
+    permutation_group g1x(n2, &set), g2x(n2);
+    seq_t sseq(n2);
+    for(size_t i = 0; i < n2; i++) sseq[i] = (i % n) + 1;
+    g1x.stabilize(sseq, g2x);
+    std::vector<permutation_group::gen_t> st;
+    g2x.convert(st);
+    for(size_t e = 0; e < st.size(); e++) {
+        seq_t img(n2);
+        for(size_t i = 0; i < n2; i++) img[i] = i;
+        st[e].first.apply(img);
+        size_t s1[kMaxOrder], s2[kMaxOrder];
+        for(size_t i = 0; i < n; i++) {
+            s1[i] = i;
+            s2[i] = std::min(img[i], img[i + n]);
+        }
+        const perm_t pm = permutation_builder(n, s2, s1);
+        if(pm.is_identity()) {
+            lt_require(st[e].second.is_identity(), LT_ERR_SYMMETRY,
+                "sum of tensors: contradictory permutational symmetry");
+            continue;
+        }
+        out.insert(se_perm(pm, st[e].second));
+    }
+}
+
+void sum_part(const symmetry &a, const symmetry &b, symmetry &out) {
+    const std::vector<std::shared_ptr<sym_element> > *sa = a.subset("part"),
+        *sb = b.subset("part");
+    if(!sa || !sb) return;
+    for(size_t i = 0; i < sa->size(); i++) {
+        const se_part &pa = dynamic_cast<const se_part&>(*(*sa)[i]);
+        for(size_t j = 0; j < sb->size(); j++) {
+            const se_part &pb = dynamic_cast<const se_part&>(*(*sb)[j]);
+            if(!(pa.pdims == pb.pdims)) continue;
+            se_part pc(a.bis, pa.pdims);
+            const size_t np = pa.pdims.size();
+            // forbidden in both terms, else related to the smallest partition
+            // both relate it to with the same scalar
+            for(size_t x = 0; x < np; x++) {
+                const index_t ix = get_index(x, pa.pdims);
+                if(pa.is_forbidden(ix) && pb.is_forbidden(ix)) {
+                    pc.mark_forbidden(ix);
+                    continue;
+                }
+                if(pa.is_forbidden(ix) || pb.is_forbidden(ix)) continue;
+                for(size_t y = 0; y < x; y++) {
+                    const index_t iy = get_index(y, pa.pdims);
+                    if(pa.is_forbidden(iy) || pb.is_forbidden(iy)) continue;
+                    if(!pa.map_exists(iy, ix) || !pb.map_exists(iy, ix)) continue;
+                    const scalar_tr ta = pa.get_transf(iy, ix), tb = pb.get_transf(iy, ix);
+                    if(ta != tb) continue;
+                    pc.add_map(iy, ix, ta);
+                    break;
+                }
+            }
+            out.insert(pc);
+        }
+    }
+}
+
+void sum_label(const symmetry &a, const symmetry &b, symmetry &out) {
+    const std::vector<std::shared_ptr<sym_element> > *sa = a.subset("label"),
+        *sb = b.subset("label");
+    if(!sa || !sb || sa->empty() || sb->empty()) return;
+    const se_label &l0 = dynamic_cast<const se_label&>(*(*sa)[0]);
+    std::vector<std::shared_ptr<const se_label> > ca, cb;
+    for(int t = 0; t < 2; t++) {
+        const std::vector<std::shared_ptr<sym_element> > &set = t ? *sb : *sa;
+        for(size_t i = 0; i < set.size(); i++) {
+            std::shared_ptr<const se_label> e =
+                std::dynamic_pointer_cast<const se_label>(set[i]);
+            lt_require(e != 0, LT_ERR_INTERNAL, "label subset holds another element");
+            // one product table and one block labelling for both terms
+            if(e->table_id != l0.table_id || e->labels != l0.labels) return;
+            (t ? cb : ca).push_back(e);
+        }
+    }
+    se_label u(l0.bidims, l0.table_id);
+    u.labels = l0.labels;
+    u.any_of.push_back(ca);
+    u.any_of.push_back(cb);
+    out.insert(u);
+}
+
+} // unnamed namespace
+
+symmetry sum_symmetry(const symmetry &a, const symmetry &b, const perm_t *pb) {
+    const perm_t id(a.order());
+    const perm_t &p = pb ? *pb : id;
+    bis_t bisb(b.bis);
+    bisb.permute(p);
+    lt_require(a.bis == bisb, LT_ERR_SYMMETRY, "sum of tensors of different block index spaces");
+    symmetry out(a.bis);
+    sum_perm(a, b, p, out);
+    const symmetry bp = p.is_identity() ? b : so_permute(b, p);
+    sum_part(a, bp, out);
+    sum_label(a, bp, out);
+    return out;
 }
 
 } // namespace lta

@@ -158,6 +158,11 @@ struct se_label : public sym_element {
     };
     std::vector<factor> factors;
 
+    //! Union form (the symmetry of a sum): a block is allowed if, for one
+    //! of the lists, every element of that list allows it.  The elements
+    //! share this element's dimensions and block labels.
+    std::vector<std::vector<std::shared_ptr<const se_label> > > any_of;
+
     se_label(const dims_t &bid, const std::string &tid);
     const char *type() const { return "label"; }
     sym_element *clone() const { return new se_label(*this); }
@@ -278,6 +283,24 @@ symmetry so_symmetrize_pair(const symmetry &sym, const perm_t &perm, double s);
 //! groups 1 and 2... as libtensor numbers them, p2 the other pair.
 symmetry so_symmetrize_triple(const symmetry &sym, const perm_t &p1, const perm_t &p2,
     double s);
+
+//! Symmetry of a sum of two tensors on one block index space, the symmetry
+//! both terms have (libtensor forms it as so_merge(so_dirsum(a, b)) over the
+//! diagonal: gen_block_tensor/impl/gen_bto_aux_add_impl.h:40-66,
+//! addition_schedule_impl.h:27-45, gen_bto_add_impl.h:240-267):
+//!  - perm: the group libtensor's so_dirsum builds from the two generating
+//!    sets, restricted to the diagonal (it can be smaller than the
+//!    intersection of the two groups);
+//!  - part: for elements of equal partition dimensions, the maps both have
+//!    with the same scalar; a partition is forbidden only if both forbid it;
+//!  - label: for elements of one product table and equal block labels, a
+//!    block is allowed if either term's label elements allow it.
+//! Elements with no counterpart of the same kind in the other term drop out.
+//! pb (optional): b enters permuted by pb (btod_add's operand permutation).
+//! The order of a and b matters for the generators of the result: a is the
+//! operation's result (or the sum so far), b the target (or the next
+//! operand), as libtensor passes them.
+symmetry sum_symmetry(const symmetry &a, const symmetry &b, const perm_t *pb = 0);
 
 //! Symmetry of the result of a contraction
 //! (libtensor/gen_block_tensor/impl/gen_bto_contract2_sym_impl.h:42-86):

@@ -19,6 +19,7 @@
 // Options: --threads T (libutil::thread_pool of T workers), --repeat R
 // (timed performs after one warm-up), --no-symc, --no-clst, --no-result.
 #include <libtensor/libtensor.h>
+#include <libtensor/block_tensor/btod_add.h>
 #include <libtensor/block_tensor/btod_contract2.h>
 #include <libtensor/block_tensor/btod_symmetrize2.h>
 #include <libtensor/block_tensor/btod_symmetrize3.h>
@@ -47,8 +48,13 @@
 #include <vector>
 
 #include "spec_format.h"
+#include "ref_build.h"
 
 using namespace libtensor;
+using ref_build::make_perm;
+using ref_build::make_index;
+using ref_build::make_bis;
+using ref_build::fill_tensor;
 
 namespace {
 
@@ -62,137 +68,6 @@ double wall_ms() {
     struct timeval tv;
     gettimeofday(&tv, 0);
     return tv.tv_sec * 1e3 + tv.tv_usec * 1e-3;
-}
-
-template<size_t N>
-permutation<N> make_perm(const std::vector<size_t> &p) {
-    sequence<N, size_t> s1(0), s2(0);
-    for(size_t i = 0; i < N; i++) { s1[i] = p[i]; s2[i] = i; }
-    return permutation_builder<N>(s1, s2).get_perm();
-}
-
-template<size_t N>
-libtensor::index<N> make_index(const std::vector<size_t> &v) {
-    libtensor::index<N> i;
-    for(size_t k = 0; k < N; k++) i[k] = v[k];
-    return i;
-}
-
-template<size_t N>
-block_index_space<N> make_bis(const spec::tensor &t) {
-    libtensor::index<N> i1, i2;
-    for(size_t i = 0; i < N; i++) i2[i] = t.dims[i] - 1;
-    block_index_space<N> bis(dimensions<N>(index_range<N>(i1, i2)));
-    for(size_t s = 0; s < t.splits.size(); s++) {
-        mask<N> m;
-        for(size_t i = 0; i < N; i++) m[i] = t.splits[s].mask[i] != 0;
-        bis.split(m, t.splits[s].pos);
-    }
-    return bis;
-}
-
-template<size_t N>
-void fill_tensor(const spec::tensor &t, block_tensor<N, double, allocator<double> > &bt) {
-    block_tensor_ctrl<N, double> ctrl(bt);
-    const block_index_space<N> &bis = bt.get_bis();
-    symmetry<N, double> &sym = ctrl.req_symmetry();
-    for(size_t e = 0; e < t.elems.size(); e++) {
-        const spec::element &el = t.elems[e];
-        if(el.kind == 0) {
-            sym.insert(se_perm<N, double>(make_perm<N>(el.perm),
-                scalar_transf<double>(el.coeff)));
-        } else if(el.kind == 1) {
-            libtensor::index<N> i1, i2;
-            for(size_t i = 0; i < N; i++) i2[i] = el.part.pdims[i] - 1;
-            se_part<N, double> sp(bis, dimensions<N>(index_range<N>(i1, i2)));
-            size_t im = 0, ifb = 0;
-            for(size_t k = 0; k < el.part.order.size(); k++) {
-                if(el.part.order[k] == 0) {
-                    const spec::part_el::mp &m = el.part.maps[im++];
-                    sp.add_map(make_index<N>(m.i1), make_index<N>(m.i2),
-                        scalar_transf<double>(m.c));
-                } else {
-                    sp.mark_forbidden(make_index<N>(el.part.forbid[ifb++]));
-                }
-            }
-            sym.insert(sp);
-        } else {
-            se_label<N, double> sl(bis.get_block_index_dims(), el.label.table);
-            for(size_t a = 0; a < el.label.assigns.size(); a++) {
-                mask<N> m;
-                for(size_t i = 0; i < N; i++) m[i] = el.label.assigns[a].mask[i] != 0;
-                sl.get_labeling().assign(m, el.label.assigns[a].blk,
-                    el.label.assigns[a].label);
-            }
-            if(el.label.has_rule) {
-                product_table_i::label_set_t ls;
-                for(size_t r = 0; r < el.label.rule.size(); r++) ls.insert(el.label.rule[r]);
-                sl.set_rule(ls);
-            }
-            sym.insert(sl);
-        }
-    }
-
-    dimensions<N> bidims = bis.get_block_index_dims();
-    const dimensions<N> &dims = bis.get_dims();
-    if(!t.dense_file.empty()) {
-        std::vector<double> dense(dims.get_size());
-        FILE *f = fopen(t.dense_file.c_str(), "rb");
-        if(!f || fread(&dense[0], 8, dense.size(), f) != dense.size())
-            throw std::runtime_error("cannot read " + t.dense_file);
-        fclose(f);
-        orbit_list<N, double> ol(sym);
-        for(typename orbit_list<N, double>::iterator i = ol.begin(); i != ol.end(); ++i) {
-            libtensor::index<N> bi;
-            ol.get_index(i, bi);
-            libtensor::index<N> st = bis.get_block_start(bi);
-            dimensions<N> bd = bis.get_block_dims(bi);
-            std::vector<double> blk(bd.get_size());
-            bool nz = false;
-            abs_index<N> ai(bd);
-            do {
-                const libtensor::index<N> &e = ai.get_index();
-                size_t off = 0;
-                for(size_t k = 0; k < N; k++) off = off * dims[k] + st[k] + e[k];
-                blk[ai.get_abs_index()] = dense[off];
-                if(dense[off] != 0.0) nz = true;
-            } while(ai.inc());
-            if(!nz) continue;
-            dense_tensor_wr_i<N, double> &b = ctrl.req_block(bi);
-            {
-                dense_tensor_wr_ctrl<N, double> dc(b);
-                double *p = dc.req_dataptr();
-                std::memcpy(p, &blk[0], 8 * blk.size());
-                dc.ret_dataptr(p);
-            }
-            ctrl.ret_block(bi);
-        }
-    }
-    if(!t.blocks_file.empty()) {
-        FILE *f = fopen(t.blocks_file.c_str(), "rb");
-        if(!f) throw std::runtime_error("cannot read " + t.blocks_file);
-        unsigned long long cnt = 0;
-        if(fread(&cnt, 8, 1, f) != 1) throw std::runtime_error("blocks: header");
-        for(unsigned long long k = 0; k < cnt; k++) {
-            unsigned long long aidx = 0, ne = 0;
-            if(fread(&aidx, 8, 1, f) != 1 || fread(&ne, 8, 1, f) != 1)
-                throw std::runtime_error("blocks: record");
-            std::vector<double> blk(ne);
-            if(ne && fread(&blk[0], 8, ne, f) != ne) throw std::runtime_error("blocks: data");
-            libtensor::index<N> bi;
-            abs_index<N>::get_index(aidx, bidims, bi);
-            dense_tensor_wr_i<N, double> &b = ctrl.req_block(bi);
-            {
-                dense_tensor_wr_ctrl<N, double> dc(b);
-                double *p = dc.req_dataptr();
-                if(ne != b.get_dims().get_size() && ne) throw std::runtime_error("blocks: size");
-                std::memcpy(p, &blk[0], 8 * ne);
-                dc.ret_dataptr(p);
-            }
-            ctrl.ret_block(bi);
-        }
-        fclose(f);
-    }
 }
 
 template<size_t N>
@@ -211,6 +86,19 @@ void print_symt(FILE *f, const symmetry<NC, double> &sym) {
         print_tr<NC>(f, o.get_transf(a));
         fprintf(f, "\n");
     }
+}
+
+template<size_t N>
+void write_dense(block_tensor<N, double, allocator<double> > &bt, const std::string &path) {
+    dimensions<N> dc = bt.get_bis().get_dims();
+    dense_tensor<N, double, allocator<double> > t(dc);
+    tod_btconv<N>(bt).perform(t);
+    dense_tensor_rd_ctrl<N, double> ctl(t);
+    const double *pd = ctl.req_const_dataptr();
+    FILE *g = fopen(path.c_str(), "wb");
+    fwrite(pd, 8, dc.get_size(), g);
+    fclose(g);
+    ctl.ret_const_dataptr(pd);
 }
 
 // btod_symmetrize3 (libtensor instantiates it for orders 3..8 only).
@@ -393,19 +281,49 @@ int run(const spec::problem &p, const std::string &out, const options &opt) {
     }
     fprintf(f, "threads %zu\n", opt.threads);
 
-    if(opt.result) {
-        dimensions<NC> dc = op.get_bis().get_dims();
-        dense_tensor<NC, double, allocator<double> > t(dc);
-        tod_btconv<NC>(btc).perform(t);
-        dense_tensor_rd_ctrl<NC, double> ctl(t);
-        const double *pd = ctl.req_const_dataptr();
-        FILE *g = fopen((out + ".result.bin").c_str(), "wb");
-        fwrite(pd, 8, dc.get_size(), g);
-        fclose(g);
-        ctl.ret_const_dataptr(pd);
+    if(opt.result) write_dense<NC>(btc, out + ".result.bin");
+    if(p.has_add) {
+        // perform(btc, d) into tensor C of the spec (its own symmetry and
+        // data): gen_bto_aux_add merges the symmetries
+        // (gen_bto_aux_add_impl.h:40-66, addition_schedule)
+        if(p.c.n != NC) throw std::runtime_error("add: tensor C of order N + M needed");
+        block_tensor<NC, double, allocator<double> > btt(make_bis<NC>(p.c));
+        fill_tensor<NC>(p.c, btt);
+        op.perform(btt, scalar_transf<double>(p.add_d));
+        block_tensor_ctrl<NC, double> ct(btt);
+        fprintf(f, "add\n");
+        print_symt<NC>(f, ct.req_const_symmetry());
+        write_dense<NC>(btt, out + ".add.bin");
     }
     fclose(f);
     delete btbp;
+    return 0;
+}
+
+// btod_add<n>(A, ca) + add_op(B, perm, cb), perform(out): the symmetry of out
+// (every orbit: "symt") and its dense result.
+template<size_t N>
+int run_add(const spec::problem &p, const std::string &out, const options &) {
+    block_tensor<N, double, allocator<double> > bta(make_bis<N>(p.a));
+    fill_tensor<N>(p.a, bta);
+    block_tensor<N, double, allocator<double> > btb(make_bis<N>(p.b));
+    fill_tensor<N>(p.b, btb);
+    btod_add<N> op(bta, p.add_ca);
+    op.add_op(btb, make_perm<N>(p.add_perm), p.add_cb);
+    block_tensor<N, double, allocator<double> > bto(op.get_bis());
+    op.perform(bto);
+    FILE *f = fopen((out + ".txt").c_str(), "w");
+    block_tensor_ctrl<N, double> ct(bto);
+    print_symt<N>(f, ct.req_const_symmetry());
+    const assignment_schedule<N, double> &sch = op.get_schedule();
+    std::vector<size_t> sched;
+    for(typename assignment_schedule<N, double>::iterator i = sch.begin(); i != sch.end(); ++i)
+        sched.push_back(sch.get_abs_index(i));
+    std::sort(sched.begin(), sched.end());
+    fprintf(f, "schedule %zu\n", sched.size());
+    for(size_t i = 0; i < sched.size(); i++) fprintf(f, "%zu\n", sched[i]);
+    fclose(f);
+    write_dense<N>(bto, out + ".result.bin");
     return 0;
 }
 
@@ -557,19 +475,7 @@ int main(int argc, char **argv) {
     }
     try {
         spec::problem p = spec::parse(argv[1]);
-        for(size_t t = 0; t < p.tables.size(); t++) {
-            const spec::table &tb = p.tables[t];
-            std::vector<std::string> irreps;
-            char buf[32];
-            for(size_t l = 0; l < tb.nlabels; l++) {
-                snprintf(buf, sizeof(buf), "L%zu", l);
-                irreps.push_back(buf);
-            }
-            point_group_table pg(tb.id, irreps, irreps[0]);
-            for(size_t k = 0; k < tb.products.size(); k++)
-                pg.add_product(tb.products[k].l1, tb.products[k].l2, tb.products[k].lr);
-            product_table_container::get_instance().add(pg);
-        }
+        for(size_t t = 0; t < p.tables.size(); t++) ref_build::add_table(p.tables[t]);
         allocator<double>::init(4, 16, 16777216, 16777216);
         batching_policy_base::set_batch_size(size_t(1) << 40);
         if(p.c3) {
@@ -583,6 +489,15 @@ int main(int argc, char **argv) {
             }
             fprintf(stderr, "unsupported contract3 orders\n");
             return 3;
+        }
+        if(p.btod_add_n) {
+            switch(p.btod_add_n) {
+            case 1: return run_add<1>(p, argv[2], opt);
+            case 2: return run_add<2>(p, argv[2], opt);
+            case 3: return run_add<3>(p, argv[2], opt);
+            case 4: return run_add<4>(p, argv[2], opt);
+            default: fprintf(stderr, "btod_add: order 1..4\n"); return 3;
+            }
         }
         for(size_t i = 0; i < sizeof(k_table) / sizeof(k_table[0]); i++) {
             const entry &e = k_table[i];

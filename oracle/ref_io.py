@@ -32,7 +32,10 @@ def write_blocks(bt, path):
 
 
 def write_spec(prob, path, dense=None, tensors=None):
-    """dense: {"A": ndarray, "B": ndarray}, or tensors: {"A": bt, "B": bt}."""
+    """dense: {"A": ndarray, "B": ndarray[, "C": ndarray]}, or tensors:
+    {"A": bt, "B": bt}.  prob["C"] (with dense["C"]) is the target of
+    prob["add"] (perform(btc, d) after perform) or the third operand of a
+    btod_contract3; prob["btod_add"] makes the problem btod_add(A) + add_op(B)."""
     d = os.path.dirname(path)
     lines = []
     for tid, n, prods in prob.get("tables", []):
@@ -77,7 +80,13 @@ def write_spec(prob, path, dense=None, tensors=None):
             f = os.path.join(d, "blocks_%s.bin" % name)
             write_blocks(tensors[name], f)
             lines.append("blocks %s %s" % (name, f))
-    if "contr3" in prob:
+    if "btod_add" in prob:
+        ba = prob["btod_add"]
+        n = len(prob["A"]["dims"])
+        lines.append("btod_add %d %s %.17g %.17g" % (
+            n, " ".join(str(x) for x in ba.get("perm") or range(n)), ba.get("ca", 1.0),
+            ba.get("cb", 1.0)))
+    elif "contr3" in prob:
         c = prob["contr3"]
         lines.append("contract3 %d %d %d %d %d %s %s %s %s %.17g" % (
             c["N1"], c["N2"], c["N3"], c["K1"], c["K2"],
@@ -107,12 +116,15 @@ def write_spec(prob, path, dense=None, tensors=None):
                                                 1 if symm else 0))
     if prob.get("self"):
         lines.append("self 1")
+    if "add" in prob:
+        lines.append("add %.17g" % prob["add"])
     with open(path, "w") as f:
         f.write("\n".join(lines) + "\n")
 
 
 def parse_output(prefix, result_shape=None):
-    out = {"symc": {}, "symt": {}, "schedule": [], "clst": {}}
+    out = {"symc": {}, "symt": {}, "schedule": [], "clst": {}, "symt_add": {}}
+    symt_key = "symt"
     with open(prefix + ".txt") as f:
         lines = f.read().split("\n")
     i = 0
@@ -127,12 +139,15 @@ def parse_output(prefix, result_shape=None):
                 a, ac, al = lines[i + k].split()
                 out["symc"][int(a)] = (int(ac), int(al))
             i += n
+        elif l[0] == "add":
+            # the symmetry of the target after perform(btc, d) follows
+            symt_key = "symt_add"
         elif l[0] == "symt":
             n = int(l[1])
             for k in range(n):
                 r = lines[i + k].split()
-                out["symt"][int(r[0])] = (int(r[1]), int(r[2]), [int(x) for x in r[3:-1]],
-                                          float(r[-1]))
+                out[symt_key][int(r[0])] = (int(r[1]), int(r[2]), [int(x) for x in r[3:-1]],
+                                            float(r[-1]))
             i += n
         elif l[0] == "schedule":
             n = int(l[1])
@@ -147,6 +162,9 @@ def parse_output(prefix, result_shape=None):
     if result_shape is not None and os.path.exists(prefix + ".result.bin"):
         out["result"] = np.fromfile(prefix + ".result.bin",
                                     dtype=np.float64).reshape(result_shape)
+    if result_shape is not None and os.path.exists(prefix + ".add.bin"):
+        out["add_result"] = np.fromfile(prefix + ".add.bin",
+                                        dtype=np.float64).reshape(result_shape)
     return out
 
 

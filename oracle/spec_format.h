@@ -30,6 +30,13 @@
 //   symmetrize3 <perm1 x N+M> <perm2 x N+M> <symm 0|1>
 //                                        wrap it in btod_symmetrize3 (same
 //                                        chain as symmetrize2)
+//   add <d>                              also perform(btc, d) into tensor C
+//                                        (declared like A and B: its own
+//                                        symmetry and data) after the plain
+//                                        perform
+//   btod_add <n> <perm x n> <ca> <cb>    instead of a contraction:
+//                                        btod_add(A, ca) + add_op(B, perm, cb),
+//                                        perform(out) (A, B of order n)
 //   symmetrize2 <perm x N+M> <symm 0|1>  wrap the contraction in
 //                                        btod_symmetrize2 (repeatable: each
 //                                        line wraps the previous operation)
@@ -103,8 +110,14 @@ struct problem {
     int symab_symm;
     struct symm2 { std::vector<size_t> perm, perm2; int symm; };   // perm2: symmetrize3
     std::vector<symm2> symm;        // in file order (innermost first)
+    bool has_add;                   // perform(C, add_d) after perform
+    double add_d;
+    size_t btod_add_n;              // > 0: btod_add of A and P(B) instead
+    std::vector<size_t> add_perm;
+    double add_ca, add_cb;
     problem() : N(0), M(0), K(0), sa(1), sb(1), sc(1), self(false), c3(false),
-        N1(0), N2(0), N3(0), K1(0), K2(0), kd(1), symab_symm(1) { }
+        N1(0), N2(0), N3(0), K1(0), K2(0), kd(1), symab_symm(1), has_add(false),
+        add_d(1), btod_add_n(0), add_ca(1), add_cb(1) { }
 };
 
 inline std::vector<size_t> read_sz(std::istringstream &in, size_t n) {
@@ -194,6 +207,11 @@ inline problem parse(const std::string &path) {
         } else if(kw == "symmetrize2") {
             problem::symm2 y; y.perm = read_sz(in, p.N + p.M); in >> y.symm;
             p.symm.push_back(y);
+        } else if(kw == "add") {
+            p.has_add = true; in >> p.add_d;
+        } else if(kw == "btod_add") {
+            in >> p.btod_add_n; p.add_perm = read_sz(in, p.btod_add_n);
+            in >> p.add_ca >> p.add_cb;
         } else if(kw == "self") {
             int s; in >> s; p.self = s != 0;
         } else {

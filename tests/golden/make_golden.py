@@ -10,8 +10,11 @@ coalesced contraction list of every scheduled block as printed by libtensor;
 The inputs are generated with libtensor_amd's block_tensor.random on a
 host-only context (no GPU needed).  Run:  python tests/golden/make_golden.py
 (--symm-only: only the btod_symmetrize2 fixtures symm_*, from the committed
-inputs of their base cases; --c3-only: only the btod_contract3 fixtures c3_*).
+inputs of their base cases; --c3-only: only the btod_contract3 fixtures c3_*;
+--add-only: only the fixtures of += into a target of another symmetry, add_*,
+and of btod_add across symmetries, badd_*).
 """
+import copy
 import json
 import os
 import random
@@ -379,17 +382,114 @@ def write_symm(work):
                             "schedule %d" % len(ref["schedule"])))
 
 
+def target_like(td, kind):
+    """A tensor description with td's block index space and the symmetry
+    `kind`: "same", "none", "ij" ((01) antisymmetry only), "pair" ((01)(23)
+    symmetric only), "nolabel", "nopart", "label01" (label rule {0, 1})."""
+    t = copy.deepcopy(td)
+    for k in ("scale", "scale_dims", "dist"):
+        t.pop(k, None)
+    el = t.get("elems", [])
+    if kind == "none":
+        t["elems"] = []
+    elif kind == "ij":
+        t["elems"] = [("perm", [1, 0, 2, 3], -1.0)] + [e for e in el if e[0] != "perm"]
+    elif kind == "pair":
+        t["elems"] = [("perm", [1, 0, 3, 2], 1.0)] + [e for e in el if e[0] != "perm"]
+    elif kind == "nolabel":
+        t["elems"] = [e for e in el if e[0] != "label"]
+    elif kind == "nopart":
+        t["elems"] = [e for e in el if e[0] != "part"]
+    elif kind == "label01":
+        t["elems"] = [e if e[0] != "label" else (e[0], e[1], e[2], [0, 1]) for e in el]
+    return t
+
+
+# perform(btc, d) into a target of another symmetry: (name, base problem,
+# target symmetry, d).  The target has the result's block index space (the
+# ladder's T and the ring's A share it) and random data of its own symmetry.
+ADD_CASES = [
+    ("add_ladder_same", "ladder_full", "same", 0.5),
+    ("add_ladder_none", "ladder_full", "none", 0.5),
+    ("add_ladder_ij", "ladder_full", "ij", -1.5),
+    ("add_ladder_pair", "ladder_full", "pair", 0.5),
+    ("add_ladder_nolabel", "ladder_full", "nolabel", 2.0),
+    ("add_ladder_nopart", "ladder_full", "nopart", 0.5),
+    ("add_ladder_label01", "ladder_full", "label01", 1.0),
+    ("add_ladder_perm_none", "ladder_perm", "none", 0.5),
+    ("add_ring_ij", "ring_small", "ij", 0.25),
+]
+
+# btod_add(A, ca) + add_op(B, perm, cb) of two tensors with the ladder T's
+# block index space: (name, symmetry of A, symmetry of B, perm, ca, cb).
+BADD_CASES = [
+    ("badd_full_none", "same", "none", [0, 1, 2, 3], 1.0, 1.0),
+    ("badd_ij_pair_perm", "ij", "pair", [1, 0, 2, 3], 0.5, -2.0),
+    ("badd_full_nolabel", "same", "nolabel", [0, 1, 3, 2], 1.0, 0.5),
+    ("badd_label01_nopart", "label01", "nopart", [0, 1, 2, 3], -1.0, 1.0),
+    ("badd_full_full_perm", "same", "same", [1, 0, 3, 2], 1.0, 1.0),
+]
+
+
+def write_add(work):
+    """Fixtures add_*.json / .npz (target symmetry after perform(btc, d),
+    every orbit; dense inputs, target and result) and badd_*."""
+    ctx = lt.Context(-1)
+    for i, (name, base, kind, d) in enumerate(ADD_CASES):
+        prob, dense, meta, result = P.load_case(os.path.join(HERE, base))
+        prob = copy.deepcopy(prob)
+        prob["C"] = target_like(prob["A"], kind)
+        prob["add"] = d
+        W.register_tables(prob)
+        bt = W.build_tensor(ctx, prob["C"])
+        W.fill_random(bt, prob["C"], 7000 + i)
+        dense = dict(dense)
+        dense["C"] = bt.to_dense()
+        ref = P.run_ref(prob, dense, os.path.join(work, name))
+        out = {"prob": prob, "symt_add": {str(k): list(v) for k, v in ref["symt_add"].items()}}
+        with open(os.path.join(HERE, name + ".json"), "w") as f:
+            json.dump(out, f)
+        arrays = dict(dense)
+        arrays["result"] = ref["add_result"]
+        np.savez_compressed(os.path.join(HERE, name + ".npz"), **arrays)
+        print("%-26s canonical allowed %d" % (name, sum(
+            1 for k, v in ref["symt_add"].items() if v[0] == k and v[1])))
+    base = P.prob_ladder()
+    for i, (name, ka, kb, perm, ca, cb) in enumerate(BADD_CASES):
+        prob = {"A": target_like(base["A"], ka), "B": target_like(base["A"], kb),
+                "tables": base.get("tables", []),
+                "btod_add": {"perm": perm, "ca": ca, "cb": cb}}
+        W.register_tables(prob)
+        dense = {}
+        for k, nm in enumerate(("A", "B")):
+            bt = W.build_tensor(ctx, prob[nm])
+            W.fill_random(bt, prob[nm], 7100 + 10 * i + k)
+            dense[nm] = bt.to_dense()
+        shape = tuple(prob["A"]["dims"])
+        ref = P.ref_io.run_ref(prob, os.path.join(work, name), shape, dense=dense)
+        out = {"prob": prob, "schedule": ref["schedule"],
+               "symt": {str(k): list(v) for k, v in ref["symt"].items()}}
+        with open(os.path.join(HERE, name + ".json"), "w") as f:
+            json.dump(out, f)
+        arrays = dict(dense)
+        arrays["result"] = ref["result"]
+        np.savez_compressed(os.path.join(HERE, name + ".npz"), **arrays)
+        print("%-26s schedule %d" % (name, len(ref["schedule"])))
+
+
 def main():
     if not P.ref_available():
         print("oracle/_ref/ref_driver is not built (make -C oracle ref)")
         return 1
-    if "--symm-only" in sys.argv or "--c3-only" in sys.argv:
+    if "--symm-only" in sys.argv or "--c3-only" in sys.argv or "--add-only" in sys.argv:
         work = tempfile.mkdtemp(prefix="lt_golden_")
         try:
             if "--symm-only" in sys.argv:
                 write_symm(work)
             if "--c3-only" in sys.argv:
                 write_c3(work)
+            if "--add-only" in sys.argv:
+                write_add(work)
         finally:
             shutil.rmtree(work, ignore_errors=True)
         return 0
@@ -403,6 +503,7 @@ def main():
             print("%-22s schedule %4d" % (name, len(ref["schedule"])))
         write_symm(work)
         write_c3(work)
+        write_add(work)
     finally:
         shutil.rmtree(work, ignore_errors=True)
     return 0

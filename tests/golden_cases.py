@@ -14,7 +14,7 @@ def names():
     """btod_contract2 cases (the symmetrization fixtures symm* and the
     btod_contract3 fixtures c3* excluded)."""
     return sorted(os.path.basename(p)[:-5] for p in glob.glob(os.path.join(GOLDEN, "*.json"))
-                  if not os.path.basename(p).startswith(("symm", "c3")))
+                  if not os.path.basename(p).startswith(("symm", "c3", "add_", "badd_")))
 
 
 def c3_names():
@@ -56,6 +56,26 @@ def load_symm(name):
         dense = {k: z[k] for k in ("A", "B") if z is not None and k in z.files}
     result = z["result"] if z is not None else None
     return prob, dense, bresult, meta, result
+
+
+def add_names():
+    """perform(btc, d) into a target of another symmetry (add_*)."""
+    return sorted(os.path.basename(p)[:-5] for p in glob.glob(os.path.join(GOLDEN, "add_*.json")))
+
+
+def badd_names():
+    """btod_add of operands with different symmetries (badd_*)."""
+    return sorted(os.path.basename(p)[:-5] for p in glob.glob(os.path.join(GOLDEN, "badd_*.json")))
+
+
+def load_add(name):
+    """(problem with target "C" and "add", dense A, B, C, fixture, libtensor's
+    target after perform(btc, d) or btod_add's result)."""
+    with open(os.path.join(GOLDEN, name + ".json")) as f:
+        meta = json.load(f)
+    z = np.load(os.path.join(GOLDEN, name + ".npz"), allow_pickle=False)
+    dense = {k: z[k] for k in ("A", "B", "C") if k in z.files}
+    return meta["prob"], dense, meta, z["result"]
 
 
 def load(name):

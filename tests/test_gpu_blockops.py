@@ -2,7 +2,8 @@
 B += d c P(A) on the inputs of every golden case (every symmetry kind:
 se_perm with signs, se_part, se_label), for several permutations, against
 numpy transposes of the dense inputs; B takes the permuted symmetry of A
-(its dense expansion must equal the permuted dense A)."""
+(its dense expansion must equal the permuted dense A).  btod_add and += into
+tensors of other symmetries take the symmetry all terms have."""
 import sys
 
 import numpy as np
@@ -47,12 +48,16 @@ def test_btod_copy(gpu_ctx, name):
         assert np.max(np.abs(bt.to_dense() - 3.0 * exp)) <= 1e-14 * scale, perm
 
 
-def test_btod_copy_add_needs_the_result_symmetry(gpu_ctx):
+def test_btod_copy_add_into_other_symmetry(gpu_ctx):
+    """B += P(A) into a B without symmetry (libtensor's gen_bto_aux_add): B
+    keeps no symmetry and holds A's full expansion."""
     prob, dense, meta, result = G.load("ladder_full")
     a, b, op = P.build(gpu_ctx, prob, dense)
     bt = permuted_tensor(gpu_ctx, prob["A"], tuple(range(dense["A"].ndim)))
-    with pytest.raises(lt.LtError):
-        lt.btod_copy(a, bt, None, add=True)
+    lt.btod_copy(a, bt, None, add=True, d=0.5)
+    scale = float(np.max(np.abs(dense["A"])))
+    assert np.max(np.abs(bt.to_dense() - 0.5 * dense["A"])) <= 1e-15 * scale
+    assert len(bt.req_nonzero_blocks()) >= len(a.req_nonzero_blocks())
 
 
 @pytest.mark.parametrize("name", G.names())
@@ -80,7 +85,9 @@ def test_btod_add(gpu_ctx, name):
     assert np.max(np.abs(bt.to_dense() - exp)) <= 1e-14 * scale
 
 
-def test_btod_add_rejects_different_symmetries(gpu_ctx):
+def test_btod_add_different_symmetries(gpu_ctx):
+    """Operands of different symmetries (libtensor: so_dirsum + so_merge,
+    gen_bto_add_impl.h:238-267): the sum takes the symmetry both have."""
     prob, dense, meta, result = G.load("ladder_full")
     a, b, op = P.build(gpu_ctx, prob, dense)
     plain = permuted_tensor(gpu_ctx, prob["A"], tuple(range(dense["A"].ndim)))
@@ -88,6 +95,11 @@ def test_btod_add_rejects_different_symmetries(gpu_ctx):
     bare = permuted_tensor(gpu_ctx, prob["A"], tuple(range(dense["A"].ndim)))
     bare.from_dense(dense["A"])         # same data, no symmetry
     out = permuted_tensor(gpu_ctx, prob["A"], tuple(range(dense["A"].ndim)))
+    scale = float(np.max(np.abs(dense["A"])))
     lt.btod_add([(a, None, 1.0), (plain, None, 1.0)], out)
-    with pytest.raises(lt.LtError):
-        lt.btod_add([(a, None, 1.0), (bare, None, 1.0)], out)
+    assert np.max(np.abs(out.to_dense() - 2.0 * dense["A"])) <= 1e-15 * scale
+    lt.btod_add([(a, None, 1.0), (bare, None, -3.0)], out)
+    assert np.max(np.abs(out.to_dense() + 2.0 * dense["A"])) <= 1e-15 * scale
+    # every block of the result is its own orbit (bare has no symmetry)
+    nblk = int(np.prod(out.bidims))
+    assert all(out.orbit(out.index(x))[0] == x for x in range(nblk))

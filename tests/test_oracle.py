@@ -79,3 +79,21 @@ def test_copy_and_dotprod():
         for j in range(4):
             s += x[i, j] * y[j, i]
     assert abs(O.dotprod(x, y, None, [1, 0]) - s) < 1e-12
+
+
+@pytest.mark.parametrize("name", [n for n in G.names() if not n.startswith("self")])
+def test_cpu_baseline_restatement(name):
+    """The benchmark's host baseline (per-pair GEMMs over libtensor's own
+    contraction lists) reproduces libtensor's result on every scheduled
+    block."""
+    import cpu_contract as CC
+    prob, dense, meta, result = G.load(name)
+    sched = meta["schedule"]
+    clst = {int(k): v for k, v in meta["clst"].items()}
+    c = prob["contr"]
+    ia, ib = CC.needed_blocks(clst, c["N"] + c["K"])
+    ba = CC.blocks_of_dense(dense["A"], prob["A"], ia)
+    bb = CC.blocks_of_dense(dense["B"], prob["B"], ib)
+    bc = CC.BlockContraction(prob, ba, bb, sched, clst, result.shape)
+    blocks = bc.perform()
+    assert bc.max_rel_err(blocks, result) <= 1e-13
