@@ -81,7 +81,7 @@ def cpu_baseline(prob, a, b, flops, shape, threads):
             "kind": "port",
             "sample": "full pp-ladder contraction: libtensor's schedule and contraction lists, "
                       "per block pair a %s GEMM (numpy.tensordot, single-threaded BLAS), "
-                      "%d workers over C blocks; median of 5 after 1 warm-up: %.1f ms; "
+                      "%d worker processes over C blocks; median of 5 after 1 warm-up: %.1f ms; "
                       "1 core: %.1f ms (%.4f TFLOP/s); libtensor itself (linalg_generic "
                       "loops, thread_pool of %d): %.1f ms (%.4f TFLOP/s); host %s, nproc %d, "
                       "affinity %d" % (CC.blas_info(), threads, med, med1,
@@ -99,7 +99,8 @@ def serving_mix(ctx, lt, W, n_clients, rounds, rate, seed=1005):
     """The serving mix: n_clients calculations (o ~ U{8..48}, v ~ U{40..240},
     ring / ladder), each submitting `rounds` contractions with Poisson
     arrivals (aggregate `rate` per second) to one request loop.  Every result
-    is compared bit for bit with the same contraction run alone."""
+    is compared bit for bit with the same contraction run alone, and, where the
+    dense operands are small enough, with the dense FP64 oracle."""
     probs = W.serving_clients(n_clients, seed)
     clients = []
     t0 = time.time()
@@ -137,6 +138,25 @@ def serving_mix(ctx, lt, W, n_clients, rounds, rate, seed=1005):
     for name, a, b, op, c, digest, fl in clients:
         if hashlib.sha1(c.to_dense().tobytes()).hexdigest() != digest:
             bad.append(name)
+    # the results against the FP64 oracle (dense contraction of the unfolded
+    # inputs, oracle/dense_oracle.py) for every client whose dense operands
+    # and dense contraction stay within a bound (the result in the mix equals
+    # the one checked, bit for bit, above)
+    sys.path.insert(0, os.path.join(HERE, "oracle"))
+    import dense_oracle as O
+    checked, worst, t0 = 0, 0.0, time.time()
+    for (name, a, b, op, c, digest, fl), p in zip(clients, probs):
+        na = int(np.prod(p["A"]["dims"]))
+        nb = int(np.prod(p["B"]["dims"]))
+        nc = int(np.prod(c.to_dense().shape))
+        kd = int(np.prod([p["A"]["dims"][ia] for ia, ib in p["contr"]["pairs"]]))
+        if max(na, nb) > (1 << 26) or 2.0 * nc * kd > 4e10:
+            continue
+        ref = O.btod_contract2_dense(p["contr"], a.to_dense(), b.to_dense())
+        worst = max(worst, O.rel_err(c.to_dense(), ref))
+        checked += 1
+    log("serving: %d of %d results checked against the oracle in %.1fs, max rel err %.2e"
+        % (checked, len(clients), time.time() - t0, worst))
     flops = rounds * sum(x[6] for x in clients)
     out = {"clients": len(clients), "requests": int(st["requests"]),
            "batches": int(st["batches"]), "mean_batch": st["mean_batch"],
@@ -147,7 +167,11 @@ def serving_mix(ctx, lt, W, n_clients, rounds, rate, seed=1005):
            "queue_p99_ms": float(np.percentile(qlat, 99)),
            "request_gflop_min": min(x[6] for x in clients) / 1e9,
            "request_gflop_max": max(x[6] for x in clients) / 1e9,
-           "results_same_as_alone": not bad, "differing": bad}
+           "results_same_as_alone": not bad, "differing": bad,
+           "oracle_checked": checked, "oracle_max_rel_err": worst,
+           "oracle_note": "dense FP64 oracle for the clients whose dense operands have at "
+                          "most 2^26 elements and whose dense contraction is at most 4e10 "
+                          "FLOPs; every other result equals its run alone bit for bit"}
     log("serving: %s" % json.dumps(out))
     return out
 

@@ -57,7 +57,9 @@ typedef struct lt_stats {
     uint64_t n_cblocks, n_pairs, n_tiles, n_unfold, n_launches;
     int32_t tile_cfg;
     int32_t ozaki_slices;
-    double ozaki_max_rel_err;
+    double ozaki_max_rel_err;  /* split-precision mode: the stated a-priori bound
+                                of max |error| / max |C| over the C blocks of the
+                                last perform */
     double tune_ms[32];      /* autotuner: grouped GEMM time per kernel
                                configuration (0 = not measured) */
     int32_t graph;           /* 1: the last perform ran as a HIP graph
@@ -67,6 +69,10 @@ typedef struct lt_stats {
     uint64_t n_plans;        /* execution plans built so far (a plan is reused
                                 while the operands' structure, the result
                                 tensor and the mode of addition stay the same) */
+    uint64_t n_batches;      /* batches of C blocks of the last perform: 1, or
+                                more when the plan of the whole schedule does
+                                not fit the context's device memory budget
+                                (gen_bto_contract2_batching_policy.h:48-70) */
 } lt_stats;
 
 /* Message of the calling thread's last failed call. */
@@ -84,6 +90,11 @@ int lt_ctx_create(int device, size_t hbm_cap_bytes, lt_ctx **out);
 int lt_ctx_destroy(lt_ctx *ctx);
 int lt_ctx_sync(lt_ctx *ctx);
 int lt_ctx_mem(lt_ctx *ctx, size_t *used, size_t *cap);
+/* The context's device memory budget, the counterpart of the memory limit
+ * of allocator<double>::init (libtensor/core/allocator.h:84-85) that the
+ * contraction's batching policy reads (gen_bto_contract2_batching_policy.h:
+ * 48-70): a contraction whose plan does not fit runs in batches of C blocks. */
+int lt_ctx_set_cap(lt_ctx *ctx, size_t cap);
 int lt_ctx_set_threads(lt_ctx *ctx, size_t nthreads);
 /* Device time (HIP events) of the kernels of the last block operation
  * (lt_btod_copy, lt_btod_add, lt_btod_symmetrize2_perform) on this context.
@@ -188,6 +199,12 @@ int lt_btensor_export(lt_btensor *bt, double *host);
 /* btod_import_raw: take the canonical allowed blocks of a dense host array
  * (blocks that are exactly zero are left zero). */
 int lt_btensor_import(lt_btensor *bt, const double *host);
+/* block_tensor::set_immutable / is_immutable (libtensor/core/immutable.h):
+ * an immutable tensor refuses every later write (LT_ERR_STATE); operations
+ * may keep data derived from it (the split-precision mode keeps its int8
+ * slices across performs). */
+int lt_btensor_set_immutable(lt_btensor *bt);
+int lt_btensor_is_immutable(lt_btensor *bt, int *out);
 /* Copy the symmetry of src (so_copy). */
 int lt_btensor_copy_symmetry(lt_btensor *dst, lt_btensor *src);
 /* Lower bt's symmetry to the symmetry bt shares with `other`, as a sum of
@@ -277,6 +294,16 @@ int lt_btod_contract2_set_graph(lt_btod_contract2 *op, int on);
  * 1..8 (0 = 7).  No counterpart in the reference (all of its arithmetic is
  * FP64 BLAS, linalg_cblas_level3.C:10-60). */
 int lt_btod_contract2_set_ozaki(lt_btod_contract2 *op, int slices);
+/* Stated error of the last split-precision perform, per C block: the
+ * canonical block, an a-priori bound of max |error| over its elements
+ * (sum over its pairs of |d| K (s + 2) 2^(e_max + f_max - 7 s), e / f the
+ * largest row / column exponents of the pair's operands, s the slices) and
+ * the largest |C| of the block; count = number of blocks (cap 0: count
+ * only).  lt_stats.ozaki_max_rel_err = max bound / max |C|.  A NaN or an
+ * infinity in an operand row or column makes its outputs NaN and its bound
+ * infinite.  No counterpart in the reference. */
+int lt_btod_contract2_ozaki_bounds(lt_btod_contract2 *op, size_t *aidx, double *bound,
+    double *cmax, size_t cap, size_t *count);
 /* C tile shape of the split-precision product kernel: -1 (default) = each
  * shape timed once on the plan's work list, the fastest kept; 0 = 16x32,
  * 1 = 16x64, 2 = 32x32 (rows x columns).  No counterpart in the reference. */

@@ -65,6 +65,11 @@ class Context(object):
         call("lt_ctx_mem", self._h, ctypes.byref(u), ctypes.byref(c))
         return u.value, c.value
 
+    def set_cap(self, cap):
+        """Device memory budget (a contraction whose plan does not fit runs
+        in batches of C blocks)."""
+        call("lt_ctx_set_cap", self._h, int(cap))
+
     def set_threads(self, n):
         call("lt_ctx_set_threads", self._h, int(n))
 
@@ -275,6 +280,15 @@ class block_tensor(object):
             raise ValueError("shape")
         call("lt_btensor_import", self._h, dptr(arr))
 
+    def set_immutable(self):
+        """block_tensor::set_immutable: later writes are refused."""
+        call("lt_btensor_set_immutable", self._h)
+
+    def is_immutable(self):
+        x = ctypes.c_int()
+        call("lt_btensor_is_immutable", self._h, ctypes.byref(x))
+        return bool(x.value)
+
     def copy_symmetry_from(self, other):
         call("lt_btensor_copy_symmetry", self._h, other._h)
 
@@ -316,11 +330,15 @@ class btod_contract2(object):
         self.contr = contr
         self.a, self.b = bta, btb
 
+    def close(self):
+        """Releases the operation (and its execution plan's device memory)."""
+        if self._h:
+            call("lt_btod_contract2_destroy", self._h)
+            self._h = None
+
     def __del__(self):
         try:
-            if self._h:
-                call("lt_btod_contract2_destroy", self._h)
-                self._h = None
+            self.close()
         except Exception:
             pass
 
@@ -336,6 +354,18 @@ class btod_contract2(object):
         out = (ctypes.c_size_t * max(1, cnt.value))()
         call("lt_btod_contract2_schedule", self._h, out, cnt.value, ctypes.byref(cnt))
         return [out[i] for i in range(cnt.value)]
+
+    def ozaki_bounds(self):
+        """Stated error of the last split-precision perform:
+        [(canonical C block, bound of max |error|, max |C|)]."""
+        n = ctypes.c_size_t()
+        call("lt_btod_contract2_ozaki_bounds", self._h, None, None, None, 0, ctypes.byref(n))
+        k = max(1, n.value)
+        ai = (ctypes.c_size_t * k)()
+        bd = (ctypes.c_double * k)()
+        cm = (ctypes.c_double * k)()
+        call("lt_btod_contract2_ozaki_bounds", self._h, ai, bd, cm, k, ctypes.byref(n))
+        return [(ai[i], bd[i], cm[i]) for i in range(n.value)]
 
     def get_symmetry(self, dst):
         """dst (C's block index space, no blocks) takes the result symmetry."""

@@ -43,6 +43,7 @@ class LtStats(ctypes.Structure):
         ("graph", ctypes.c_int32),
         ("tile_order", ctypes.c_int32),
         ("n_plans", ctypes.c_uint64),
+        ("n_batches", ctypes.c_uint64),
     ]
 
 
@@ -65,6 +66,7 @@ _SIGS = {
     "lt_ctx_destroy": [vp],
     "lt_ctx_sync": [vp],
     "lt_ctx_mem": [vp, c_size_p, c_size_p],
+    "lt_ctx_set_cap": [vp, ctypes.c_size_t],
     "lt_ctx_set_threads": [vp, ctypes.c_size_t],
     "lt_ctx_last_op_ms": [vp, c_double_p],
     "lt_dev_alloc": [vp, ctypes.c_size_t, ctypes.POINTER(vp)],
@@ -105,6 +107,8 @@ _SIGS = {
     "lt_btensor_export": [vp, c_double_p],
     "lt_btensor_import": [vp, c_double_p],
     "lt_btensor_copy_symmetry": [vp, vp],
+    "lt_btensor_set_immutable": [vp],
+    "lt_btensor_is_immutable": [vp, c_int_p],
     "lt_btensor_merge_symmetry": [vp, vp],
     "lt_se_perm_create": [ctypes.c_size_t, c_size_p, ctypes.c_double, ctypes.POINTER(vp)],
     "lt_se_part_create": [vp, c_size_p, ctypes.POINTER(vp)],
@@ -126,6 +130,8 @@ _SIGS = {
     "lt_btod_contract2_perform_add": [vp, vp, ctypes.c_double, ctypes.c_int],
     "lt_btod_contract2_schedule": [vp, c_size_p, ctypes.c_size_t, c_size_p],
     "lt_btod_contract2_get_symmetry": [vp, vp],
+    "lt_btod_contract2_ozaki_bounds": [vp, c_size_p, c_double_p, c_double_p, ctypes.c_size_t,
+                                       c_size_p],
     "lt_btod_contract2_symc_orbit": [vp, ctypes.c_size_t, c_size_p, c_size_p,
                                      c_double_p, c_int_p],
     "lt_btod_contract2_clst": [vp, ctypes.c_size_t, c_double_p, ctypes.c_size_t, c_size_p],

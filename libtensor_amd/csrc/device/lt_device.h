@@ -227,6 +227,15 @@ hipError_t launch_oz_split(const oz_split_desc *descs, const uint32_t *items,
 hipError_t launch_oz_gemm(const gemm_cblock *cb, const oz_pair *pairs,
     const gemm_tile *tiles, uint32_t ntiles, int slices, int tile, hipStream_t s);
 
+//! Exponent of a split-precision row or column that holds a NaN or an
+//! infinity (its products are NaN).
+static const int32_t kOzNonFinite = 0x3fffffff;
+
+//! Stated error of the split-precision products: out[2 b] = bound of
+//! max |error| of C block b, out[2 b + 1] = max |C| of the block as written.
+hipError_t launch_oz_bound(const gemm_cblock *cb, const gemm_pair *pairs, const oz_pair *ozp,
+    uint32_t ncb, int slices, double *out, hipStream_t s);
+
 //! Fill n doubles with v.
 hipError_t launch_fill(double *p, size_t n, double v, hipStream_t s);
 

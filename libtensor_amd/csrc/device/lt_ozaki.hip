@@ -4,8 +4,12 @@
 // digits (int8 slices, exact), slice products run on v_mfma_i32_16x16x64_i8
 // with exact int32 accumulation, and the products of equal weight
 // 2^(-7 (t + u)) are scaled and summed in FP64.  Products with
-// t + u > slices + 1 are dropped (triangular scheme); the truncation error
-// per element is below 2^(e_i + f_j) * K * (slices + 1) * 2^(-7 (slices + 1)).
+// t + u > slices + 1 are dropped (triangular scheme).  Error per element of
+// one product of extent K (s = slices): each operand keeps a remainder
+// below 2^(e - 7 s), and a dropped slice product t + u >= s + 2 is below
+// 127^2 2^(e + f - 7 (t + u)); summed, |error| < K (s + 2) 2^(e_i + f_j - 7 s)
+// (oz_bound_kernel states it per C block).  A row or column holding a NaN
+// or an infinity gets the exponent kNonFinite and its outputs are NaN.
 // References: K. Ozaki, T. Ogita, S. Oishi, S. M. Rump, "Error-free
 // transformations of matrix multiplication by using fast routines of
 // matrix multiplication and its applications", Numer. Algorithms 59 (2012);
@@ -48,10 +52,19 @@ oz_split_kernel(const oz_split_desc *__restrict__ descs,
     int e = 0;
     if(live) {
         double mx = 0.0;
-        for(uint32_t k = lane; k < d.k; k += 64) mx = fmax(mx, fabs(row[size_t(k) * d.ks]));
+        int nonfinite = 0;
+        for(uint32_t k = lane; k < d.k; k += 64) {
+            const double v = fabs(row[size_t(k) * d.ks]);
+            nonfinite |= !(v <= 1.7976931348623157e308);    // NaN or infinity
+            mx = fmax(mx, v);
+        }
 #pragma unroll
-        for(int off = 32; off > 0; off >>= 1) mx = fmax(mx, __shfl_xor(mx, off));
-        if(mx > 0.0) (void)frexp(mx, &e);  // mx = f 2^e, 0.5 <= f < 1
+        for(int off = 32; off > 0; off >>= 1) {
+            mx = fmax(mx, __shfl_xor(mx, off));
+            nonfinite |= __shfl_xor(nonfinite, off);
+        }
+        if(nonfinite) e = kOzNonFinite;            // slices all zero, outputs NaN
+        else if(mx > 0.0) (void)frexp(mx, &e);     // mx = f 2^e, 0.5 <= f < 1
         if(lane == 0) d.exps[r] = e;
     }
 
@@ -183,9 +196,11 @@ oz_gemm_kernel(const gemm_cblock *__restrict__ cbs,
 #pragma unroll
                 for(int f = 0; f < FN; f++)
 #pragma unroll
-                    for(int v = 0; v < 4; v++)
-                        cacc[i][f][v] += pr.d * ldexp(double(acc[l][i][f][v]),
-                            ea[v] + eb[f] - 7 * (l + 2));
+                    for(int v = 0; v < 4; v++) {
+                        const bool nf = ea[v] == kOzNonFinite || eb[f] == kOzNonFinite;
+                        cacc[i][f][v] += nf ? __builtin_nan("") : pr.d * ldexp(
+                            double(acc[l][i][f][v]), ea[v] + eb[f] - 7 * (l + 2));
+                    }
         }
     }
 
@@ -205,6 +220,70 @@ oz_gemm_kernel(const gemm_cblock *__restrict__ cbs,
                 }
             }
         }
+}
+
+// Stated error of every C block of a split-precision product: one
+// workgroup per C block; bound = sum over the block's pairs (in list order)
+// of |d| K (s + 2) 2^(max_i e_i + max_j f_j - 7 s) (the per-element bound
+// above with the largest row and column exponents of the pair), and the
+// largest |C| of the block as the product kernel wrote it (NaN propagates).
+// out[2 b] = bound, out[2 b + 1] = max |C|.
+__global__ void __launch_bounds__(256)
+oz_bound_kernel(const gemm_cblock *__restrict__ cbs, const gemm_pair *__restrict__ pairs,
+    const oz_pair *__restrict__ ozp, int slices, double *__restrict__ out) {
+
+    __shared__ int ra[256], rb[256];
+    __shared__ double rc[256];
+    const gemm_cblock cb = cbs[blockIdx.x];
+    const int tid = threadIdx.x;
+    double bound = 0.0;
+    for(uint32_t p = cb.pair_begin; p < cb.pair_end; p++) {
+        const oz_pair o = ozp[p];
+        int ma = -2147483647, mb = -2147483647;
+        for(uint32_t r = tid; r < cb.m; r += 256) ma = max(ma, o.ea[r]);
+        for(uint32_t c = tid; c < cb.n; c += 256) mb = max(mb, o.eb[c]);
+        ra[tid] = ma; rb[tid] = mb;
+        __syncthreads();
+        for(int w = 128; w > 0; w >>= 1) {
+            if(tid < w) {
+                ra[tid] = max(ra[tid], ra[tid + w]);
+                rb[tid] = max(rb[tid], rb[tid + w]);
+            }
+            __syncthreads();
+        }
+        if(tid == 0 && ra[0] > -1000000 && rb[0] > -1000000) {
+            const bool nf = ra[0] == kOzNonFinite || rb[0] == kOzNonFinite;
+            bound += nf ? __builtin_inf() : fabs(pairs[p].d) * double(pairs[p].k) *
+                double(slices + 2) * ldexp(1.0, ra[0] + rb[0] - 7 * slices);
+        }
+        __syncthreads();
+    }
+    double cm = 0.0;
+    const uint32_t total = cb.m * cb.n;
+    for(uint32_t x = tid; x < total; x += 256) {
+        const double v = fabs(cb.c[size_t(x / cb.n) * cb.ldc + (x % cb.n)]);
+        cm = (v > cm || v != v) ? v : cm;
+    }
+    rc[tid] = cm;
+    __syncthreads();
+    for(int w = 128; w > 0; w >>= 1) {
+        if(tid < w) {
+            const double v = rc[tid + w];
+            rc[tid] = (v > rc[tid] || v != v) ? v : rc[tid];
+        }
+        __syncthreads();
+    }
+    if(tid == 0) {
+        out[2 * blockIdx.x] = bound;
+        out[2 * blockIdx.x + 1] = rc[0];
+    }
+}
+
+hipError_t launch_oz_bound(const gemm_cblock *cb, const gemm_pair *pairs, const oz_pair *ozp,
+    uint32_t ncb, int slices, double *out, hipStream_t s) {
+    if(ncb == 0) return hipSuccess;
+    hipLaunchKernelGGL(oz_bound_kernel, dim3(ncb), dim3(256), 0, s, cb, pairs, ozp, slices, out);
+    return hipGetLastError();
 }
 
 template<int S>

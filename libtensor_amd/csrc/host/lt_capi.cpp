@@ -342,6 +342,14 @@ int lt_ctx_mem(lt_ctx *ctx, size_t *used, size_t *cap) {
     });
 }
 
+int lt_ctx_set_cap(lt_ctx *ctx, size_t cap) {
+    return guard([&]() {
+        need(ctx, "ctx");
+        lt_require(cap > 0, LT_ERR_PARAM, "device memory budget of 0 bytes");
+        ctx->c->set_cap(cap);
+    });
+}
+
 int lt_ctx_set_threads(lt_ctx *ctx, size_t n) {
     return guard([&]() { need(ctx, "ctx"); ctx->c->set_nthreads(n); });
 }
@@ -699,6 +707,7 @@ int lt_btensor_import(lt_btensor *bt, const double *host) {
     return guard([&]() {
         need(bt, "bt"); need(host, "host");
         btensor &t = *bt->t;
+        t.require_mutable("import");
         const bis_t &bis = t.bis();
         dims_t dims = bis.dims, bid = t.bidims();
         const size_t n = dims.n;
@@ -722,6 +731,14 @@ int lt_btensor_import(lt_btensor *bt, const double *host) {
             if(nz) t.set_block(a, blk.data());
         }
     });
+}
+
+int lt_btensor_set_immutable(lt_btensor *bt) {
+    return guard([&]() { need(bt, "bt"); bt->t->set_immutable(); });
+}
+
+int lt_btensor_is_immutable(lt_btensor *bt, int *out) {
+    return guard([&]() { need(bt, "bt"); need(out, "out"); *out = bt->t->is_immutable() ? 1 : 0; });
 }
 
 int lt_btensor_copy_symmetry(lt_btensor *dst, lt_btensor *src) {
@@ -952,6 +969,22 @@ int lt_btod_contract2_symc_orbit(lt_btod_contract2 *op, size_t aidx, size_t *aci
         if(acidx) *acidx = o.acidx;
         fill_transf(o.get_transf(aidx), s.bis.order(), perm, coeff);
         if(allowed) *allowed = o.allowed ? 1 : 0;
+    });
+}
+
+int lt_btod_contract2_ozaki_bounds(lt_btod_contract2 *op, size_t *aidx, double *bound,
+    double *cmax, size_t cap, size_t *count) {
+    return guard([&]() {
+        need(op, "op"); need(count, "count");
+        const std::vector<btod_contract2::oz_bound> &v = op->op->ozaki_bounds();
+        *count = v.size();
+        if(cap == 0) return;
+        lt_require(cap >= v.size(), LT_ERR_PARAM, "ozaki_bounds: buffer too small");
+        for(size_t i = 0; i < v.size(); i++) {
+            if(aidx) aidx[i] = v[i].aidx;
+            if(bound) bound[i] = v[i].bound;
+            if(cmax) cmax[i] = v[i].cmax;
+        }
     });
 }
 
@@ -1292,6 +1325,7 @@ int lt_btod_contract2_stats(lt_btod_contract2 *op, lt_stats *out) {
         out->graph = s.graph;
         out->tile_order = s.tile_order;
         out->n_plans = s.n_plans;
+        out->n_batches = s.n_batches;
     });
 }
 

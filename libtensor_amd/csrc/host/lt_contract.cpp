@@ -18,6 +18,8 @@
 //      permuted into the canonical blocks of C.
 #include "lt_runtime.h"
 
+#include <functional>
+
 #include <atomic>
 
 #include <map>
@@ -217,6 +219,7 @@ struct contract_plan {
     std::vector<uint32_t> block_tile_begin;
     std::vector<double> cb_ksum;                      // sum of k over a C block's pairs
     std::vector<size_t> c_out;                        // canonical C blocks written
+    bool batch;                                       // built for a batch of the schedule
     std::vector<std::pair<double*, size_t> > c_out_mem; // their storage
     bool ta, tb;
     int tile_cfg, tile_order;
@@ -272,8 +275,10 @@ struct contract_plan {
     dev::oz_pair *d_oz_pairs;
     dev::gemm_tile *d_oz_tiles;
     uint32_t n_oz_tiles;
+    uint32_t n_oz_items_mut;          //!< split items of operands split every perform
+    double *d_oz_bound;               //!< stated error per C block (bound, max |C|)
 
-    contract_plan() : va(0), vb(0), cuid(0), c(0), add(false), d(1.0), empty(false),
+    contract_plan() : batch(false), va(0), vb(0), cuid(0), c(0), add(false), d(1.0), empty(false),
         ta(false), tb(false), tile_cfg(0), tile_order(0), d_prep(0), d_wback(0), d_prep_cd(0),
         d_prep_co(0), d_wb_cd(0), d_wb_co(0), n_prep_chunks(0), n_wb_chunks(0),
         d_cb(0), d_pairs(0), d_tiles_sorted(0), d_tiles_block(0), scratch(0),
@@ -281,7 +286,7 @@ struct contract_plan {
         d_stacks(0), d_spairs(0), d_arows(0), d_crows(0), d_stiles(0), d_vtiles(0),
         n_prep_st_chunks(0), d_prep_st(0), d_pst_cd(0), d_pst_co(0),
         oz_slices(0), oz_tile(0), oz_tile_set(-1), d_oz_split(0), d_oz_items(0), n_oz_items(0), d_oz_pairs(0),
-        d_oz_tiles(0), n_oz_tiles(0) { }
+        d_oz_tiles(0), n_oz_tiles(0), n_oz_items_mut(0), d_oz_bound(0) { }
     ~contract_plan() {
         if(gexec) hipGraphExecDestroy(gexec);
         if(graph) hipGraphDestroy(graph);
@@ -313,7 +318,8 @@ btod_contract2::btod_contract2(const contraction2 &c, btensor &a, double ka,
     btensor &b, double kb, double kc) :
     m_contr(c), m_a(a), m_b(b), m_ka(ka), m_kb(kb), m_kc(kc), m_uid(next_op_uid()),
     m_tile_cfg(-2), m_tuned_cfg(-1), m_tuned_order(-1), m_tuned_va(0), m_tuned_vb(0), m_use_graph(true), m_oz_slices(0), m_oz_tile(-1),
-    m_add_uid(0), m_add_ver(0), m_add_direct(false) {
+    m_add_uid(0), m_add_ver(0), m_add_direct(false), m_sched_subset(0),
+    m_use_graph_saved(true) {
 
     lt_require(c.is_complete(), LT_ERR_PARAM, "contraction is incomplete");
     lt_require(a.bis().order() == c.NA() && b.bis().order() == c.NB(),
@@ -341,6 +347,7 @@ void btod_contract2::clst(const index_t &ic, contr_list &out) {
 }
 
 bool btod_contract2::plan_valid(btensor &c, bool add, double d, int) const {
+    if(m_plan && m_plan->batch) return false;
     if(!m_plan) return false;
     return m_plan->va == m_a.version() && m_plan->vb == m_b.version() &&
         m_plan->cuid == c.uid() && m_plan->add == add && m_plan->d == d;
@@ -362,8 +369,11 @@ void btod_contract2::build_plan(btensor &c, bool add, double dscale, bool measur
     pl.d = dscale;
     pl.ctx = &ctx;
 
+    // the whole schedule, or one batch of it (memory-bounded performs)
+    const std::vector<size_t> &sched = m_sched_subset ? *m_sched_subset : m_sched;
+    pl.batch = m_sched_subset != 0;
     std::vector<size_t> nza = m_a.nonzero_blocks(), nzb = m_b.nonzero_blocks();
-    if(nza.empty() || nzb.empty() || m_sched.empty()) {
+    if(nza.empty() || nzb.empty() || sched.empty()) {
         pl.empty = true;
         return;
     }
@@ -398,8 +408,8 @@ void btod_contract2::build_plan(btensor &c, bool add, double dscale, bool measur
         permb.apply(ib);
         blb2.push_back(short_orbit(symb2, abs_index(ib, bidb2), true).acidx);
     }
-    for(size_t i = 0; i < m_sched.size(); i++) {
-        index_t ic = get_index(m_sched[i], bidc);
+    for(size_t i = 0; i < sched.size(); i++) {
+        index_t ic = get_index(sched[i], bidc);
         permc.apply(ic);
         batchc.push_back(short_orbit(symct, abs_index(ic, bidct), true).acidx);
     }
@@ -1275,7 +1285,7 @@ void btod_contract2::build_ozaki(int slices) {
 
     // one split per distinct operand matrix: A-role operands scaled per row
     // of C, B-role operands per column of C
-    struct op_info { const double *src; uint32_t rows, k, kp, rs, ks; size_t off, eoff; };
+    struct op_info { const double *src; uint32_t rows, k, kp, rs, ks; size_t off, eoff; bool cached; };
     std::vector<op_info> ops;
     typedef std::map<std::pair<const double*, bool>, size_t> op_index;
     op_index ida, idb;
@@ -1294,6 +1304,9 @@ void btod_contract2::build_ozaki(int slices) {
         o.rs = kmajor ? 1 : ld;
         o.ks = kmajor ? ld : 1;
         o.off = bytes; o.eoff = nexp;
+        // an operand read in place from an immutable tensor is split once
+        // per plan, the others on every perform
+        o.cached = (m_a.is_immutable() && m_a.owns(src)) || (m_b.is_immutable() && m_b.owns(src));
         bytes += size_t(slices) * ((rows + 15) & ~15u) * o.kp;
         bytes = (bytes + 255) & ~size_t(255);
         nexp += rows;
@@ -1320,7 +1333,7 @@ void btod_contract2::build_ozaki(int slices) {
     int32_t *ex = (int32_t*)(sl + exp_off);
 
     std::vector<dev::oz_split_desc> descs(ops.size());
-    std::vector<uint32_t> items;
+    std::vector<uint32_t> items, items_cached;
     for(size_t i = 0; i < ops.size(); i++) {
         dev::oz_split_desc &d = descs[i];
         std::memset(&d, 0, sizeof(d));
@@ -1332,11 +1345,15 @@ void btod_contract2::build_ozaki(int slices) {
         d.rs = ops[i].rs; d.ks = ops[i].ks;
         const uint32_t prows = (ops[i].rows + 15) & ~15u;   // whole 16-row panels
         d.sstride = prows * ops[i].kp;
+        std::vector<uint32_t> &it = ops[i].cached ? items_cached : items;
         for(uint32_t r = 0; r < prows; r++) {
-            items.push_back((uint32_t)i);
-            items.push_back(r);
+            it.push_back((uint32_t)i);
+            it.push_back(r);
         }
     }
+    // items of the operands split every perform first, then the cached ones
+    const uint32_t n_mut = (uint32_t)(items.size() / 2);
+    items.insert(items.end(), items_cached.begin(), items_cached.end());
     std::vector<dev::oz_pair> ozp(pl.pairs.size());
     for(size_t p = 0; p < pl.pairs.size(); p++) {
         const op_info &a = ops[pa[p]], &b = ops[pb[p]];
@@ -1368,6 +1385,7 @@ void btod_contract2::build_ozaki(int slices) {
     size_t o_d = sz; sz = align256(sz + descs.size() * sizeof(dev::oz_split_desc));
     size_t o_i = sz; sz = align256(sz + items.size() * 4);
     size_t o_p = sz; sz = align256(sz + ozp.size() * sizeof(dev::oz_pair));
+    size_t o_bd = sz; sz = align256(sz + 2 * pl.cblocks.size() * sizeof(double));
     size_t o_t[dev::OZ_TILE_COUNT];
     for(int k = 0; k < dev::OZ_TILE_COUNT; k++) {
         o_t[k] = sz;
@@ -1387,6 +1405,14 @@ void btod_contract2::build_ozaki(int slices) {
     pl.d_oz_items = (uint32_t*)(base + o_i);
     pl.n_oz_items = (uint32_t)(items.size() / 2);
     pl.d_oz_pairs = (dev::oz_pair*)(base + o_p);
+    pl.d_oz_bound = (double*)(base + o_bd);
+    pl.n_oz_items_mut = n_mut;
+    if(pl.n_oz_items > n_mut) {
+        // operands of immutable tensors: split now, once for this plan
+        ctx.activate();
+        hip_check(dev::launch_oz_split(pl.d_oz_split, pl.d_oz_items + 2 * n_mut,
+            pl.n_oz_items - n_mut, ctx.stream()), "split (ozaki, immutable operands)");
+    }
 
     // Tile shape: each shape's product launch is timed once on this plan's
     // own work list (after one prepare and split) and the fastest kept.
@@ -1404,7 +1430,7 @@ void btod_contract2::build_ozaki(int slices) {
         if(pl.n_prep_chunks)
             hip_check(dev::launch_permute(pl.d_prep, pl.d_prep_cd, pl.d_prep_co,
                 pl.n_prep_chunks, s), "permute (prepare)");
-        hip_check(dev::launch_oz_split(pl.d_oz_split, pl.d_oz_items, pl.n_oz_items, s),
+        hip_check(dev::launch_oz_split(pl.d_oz_split, pl.d_oz_items, pl.n_oz_items_mut, s),
             "split (ozaki)");
         float bestms = 1e30f;
         for(int k = 0; k < dev::OZ_TILE_COUNT; k++) {
@@ -1453,11 +1479,15 @@ void btod_contract2::enqueue(int mode, hipStream_t s, bool timed, size_t &launch
     }
     if(timed) hip_check(hipEventRecord(pl.ev[1], s), "hipEventRecord");
     if(mode == MODE_OZAKI) {
-        hip_check(dev::launch_oz_split(pl.d_oz_split, pl.d_oz_items, pl.n_oz_items, s),
+        hip_check(dev::launch_oz_split(pl.d_oz_split, pl.d_oz_items, pl.n_oz_items_mut, s),
             "split (ozaki)");
         hip_check(dev::launch_oz_gemm(pl.d_cb, pl.d_oz_pairs, pl.d_oz_tiles, pl.n_oz_tiles,
             pl.oz_slices, pl.oz_tile, s), "gemm (ozaki)");
         launches += 2;
+        if(timed) hip_check(hipEventRecord(pl.ev[2], s), "hipEventRecord");
+        hip_check(dev::launch_oz_bound(pl.d_cb, pl.d_pairs, pl.d_oz_pairs,
+            (uint32_t)pl.c_out.size(), pl.oz_slices, pl.d_oz_bound, s), "bound (ozaki)");
+        launches++;
     } else if(mode == MODE_PER_BLOCK) {
         for(size_t b = 0; b + 1 < pl.block_tile_begin.size(); b++) {
             uint32_t t0 = pl.block_tile_begin[b], t1 = pl.block_tile_begin[b + 1];
@@ -1470,7 +1500,7 @@ void btod_contract2::enqueue(int mode, hipStream_t s, bool timed, size_t &launch
         hip_check(launch_grouped(pl, pl.tile_cfg, s), "gemm (grouped)");
         launches++;
     }
-    if(timed) hip_check(hipEventRecord(pl.ev[2], s), "hipEventRecord");
+    if(timed && mode != MODE_OZAKI) hip_check(hipEventRecord(pl.ev[2], s), "hipEventRecord");
     if(pl.n_wb_chunks) {
         hip_check(dev::launch_permute(pl.d_wback, pl.d_wb_cd, pl.d_wb_co,
             pl.n_wb_chunks, s), "permute (write back)");
@@ -1483,7 +1513,7 @@ void btod_contract2::run(btensor &c, int mode) {
     context &ctx = m_a.ctx();
     ctx.activate();
     hipStream_t s = ctx.stream();
-    if(!pl.add) {
+    if(!pl.add && !pl.batch) {
         // A result tensor this operation wrote last keeps its structure, and
         // so its version (plans of operations that read it stay valid):
         // every block of c_out is overwritten below.
@@ -1550,41 +1580,162 @@ void btod_contract2::run(btensor &c, int mode) {
     m_stats.t_gemm_ms = msg;
     m_stats.n_launches = launches;
     m_stats.graph = graph ? 1 : 0;
+    if(mode == MODE_OZAKI) m_oz_bounds.clear();
+    if(mode == MODE_OZAKI && !pl.c_out.empty()) {
+        // the stated error of this perform: per C block the bound and the
+        // largest |C|; overall max bound / max |C| (the relative error as
+        // the metric measures it)
+        // (the C block list ends with the empty block of the padding tiles)
+        std::vector<double> bd(2 * pl.c_out.size());
+        ctx.download(bd.data(), pl.d_oz_bound, bd.size() * sizeof(double));
+        m_oz_bounds.clear();
+        double bmax = 0.0, cmax = 0.0;
+        for(size_t b = 0; b < pl.c_out.size(); b++) {
+            m_oz_bounds.push_back(oz_bound{pl.c_out[b], bd[2 * b], bd[2 * b + 1]});
+            bmax = std::max(bmax, bd[2 * b]);
+            cmax = std::max(cmax, bd[2 * b + 1]);
+        }
+        m_stats.ozaki_max_rel_err = cmax > 0.0 ? bmax / cmax : bmax;
+    }
     for(size_t i = 0; i < pl.c_out.size(); i++) c.mark_nonzero(pl.c_out[i]);
 }
 
-void btod_contract2::perform_batch(std::vector<batch_item> &items, dev_buffer &merged) {
-    if(items.empty()) return;
-    context &ctx = items[0].op->m_a.ctx();
+void btod_contract2::perform_sliced(btensor &c, bool add, double d, double slice_flops,
+    const std::function<void()> &between) {
+    // run() in grouped mode without the graph, with the products launched in
+    // slices of the plan's own tile list and `between` called after each
+    // slice has finished
+    c.require_mutable("btod_contract2::perform");
+    check_block_sizes();
+    m_a.ctx().require_device("btod_contract2::perform");
+    lt_require(c.bis() == m_symc.bis, LT_ERR_DIMS, "result tensor has a different block index space");
+    if(add) {
+        lt_require(add_direct(c), LT_ERR_SYMMETRY,
+            "perform_add: result symmetry differs from the contraction's (the request "
+            "loop adds into tensors of the result's symmetry)");
+    }
+    if(!plan_valid(c, add, d, MODE_GROUPED)) build_plan(c, add, d, false);
+    contract_plan &pl = *m_plan;
+    context &ctx = m_a.ctx();
+    ctx.activate();
+    hipStream_t s = ctx.stream();
+    if(!pl.add) {
+        std::vector<size_t> co(pl.c_out);
+        std::sort(co.begin(), co.end());
+        if(c.sym_tag() != m_uid || c.nonzero_blocks() != co) {
+            c.zero_all();
+            c.set_symmetry(m_symc, m_uid);
+        }
+    }
+    if(pl.empty) return;
+    hip_check(hipEventRecord(pl.ev[0], s), "hipEventRecord");
+    if(pl.add) {
+        for(size_t i = 0; i < pl.c_out.size(); i++) {
+            if(c.is_nonzero(pl.c_out[i])) continue;
+            hip_check(dev::launch_fill(pl.c_out_mem[i].first, pl.c_out_mem[i].second,
+                0.0, s), "fill");
+        }
+    }
+    const int k = pl.tile_cfg;
+    if(dev::is_stacked(k)) {
+        if(pl.n_prep_st_chunks)
+            hip_check(dev::launch_permute(pl.d_prep_st, pl.d_pst_cd, pl.d_pst_co,
+                pl.n_prep_st_chunks, s), "permute (prepare)");
+    } else if(pl.n_prep_chunks) {
+        hip_check(dev::launch_permute(pl.d_prep, pl.d_prep_cd, pl.d_prep_co,
+            pl.n_prep_chunks, s), "permute (prepare)");
+    }
+    hip_check(hipEventRecord(pl.ev[1], s), "hipEventRecord");
+    const uint32_t nt = dev::is_var(k) ? (uint32_t)pl.vtiles.size() :
+        dev::is_stacked(k) ? (uint32_t)pl.stiles.size() : (uint32_t)pl.tiles_sorted.size();
+    const uint32_t nslices = (uint32_t)std::max(1.0, std::min(double(nt),
+        std::ceil(m_stats.flops / slice_flops)));
+    const uint32_t per = (nt + nslices - 1) / nslices;
+    for(uint32_t t0 = 0; t0 < nt; t0 += per) {
+        const uint32_t n = std::min(per, nt - t0);
+        if(dev::is_var(k))
+            hip_check(dev::launch_gemm_stacked_var(k, pl.d_stacks, pl.d_spairs, pl.d_arows,
+                pl.d_crows, pl.d_vtiles + t0, n, s), "gemm (slice)");
+        else if(dev::is_stacked(k))
+            hip_check(dev::launch_gemm_stacked(k, pl.d_stacks, pl.d_spairs, pl.d_arows,
+                pl.d_crows, pl.d_stiles + t0, n, s), "gemm (slice)");
+        else
+            hip_check(dev::launch_gemm(k, pl.d_cb, pl.d_pairs, pl.d_tiles_sorted + t0, n, n, s),
+                "gemm (slice)");
+        if(t0 + n < nt) {
+            hip_check(hipStreamSynchronize(s), "hipStreamSynchronize");
+            between();
+        }
+    }
+    hip_check(hipEventRecord(pl.ev[2], s), "hipEventRecord");
+    if(pl.n_wb_chunks)
+        hip_check(dev::launch_permute(pl.d_wback, pl.d_wb_cd, pl.d_wb_co, pl.n_wb_chunks, s),
+            "permute (write back)");
+    hip_check(hipEventRecord(pl.ev[3], s), "hipEventRecord");
+    hip_check(hipStreamSynchronize(s), "hipStreamSynchronize");
+    float ms = 0.f, msg = 0.f;
+    hip_check(hipEventElapsedTime(&ms, pl.ev[0], pl.ev[3]), "hipEventElapsedTime");
+    hip_check(hipEventElapsedTime(&msg, pl.ev[1], pl.ev[2]), "hipEventElapsedTime");
+    m_stats.t_gpu_ms = ms;
+    m_stats.t_gemm_ms = msg;
+    m_stats.n_launches = nslices + 2;
+    m_stats.graph = 0;
+    m_stats.n_batches = 1;
+    for(size_t i = 0; i < pl.c_out.size(); i++) c.mark_nonzero(pl.c_out[i]);
+}
+
+batch_run::batch_run(std::vector<batch_item> &items, dev_buffer &merged) :
+    m_items(items), m_ctx(0), m_base(0), m_launch(0), m_tile(0), m_flops(0.0),
+    m_finished(false) {
+
+    // every item checked and planned on its own: one that fails keeps its
+    // error, the others run
+    for(size_t i = 0; i < items.size(); i++) {
+        batch_item &it = items[i];
+        it.status = LT_OK;
+        it.message.clear();
+        try {
+            lt_require(it.op && it.c, LT_ERR_PARAM, "batch item without operation or tensor");
+            it.c->require_mutable("btod_contract2::perform");
+            btod_contract2 &op = *it.op;
+            if(!m_ctx) m_ctx = &op.m_a.ctx();
+            lt_require(&op.m_a.ctx() == m_ctx, LT_ERR_PARAM, "batch spans contexts");
+            op.check_block_sizes();
+            lt_require(it.c->bis() == op.m_symc.bis, LT_ERR_DIMS,
+                "result tensor has a different block index space");
+            if(it.add) {
+                lt_require(op.add_direct(*it.c), LT_ERR_SYMMETRY,
+                    "perform_add: result symmetry differs from the contraction's (the request "
+                    "loop adds into tensors of the result's symmetry)");
+            }
+            // no timed autotuning: the stream may be shared with other work
+            if(!op.plan_valid(*it.c, it.add, it.d, MODE_GROUPED))
+                op.build_plan(*it.c, it.add, it.d, false);
+            m_live.push_back(i);
+            m_plans.push_back(op.m_plan.get());
+        } catch(const lt_error &e) {
+            it.status = e.code;
+            it.message = e.what();
+        } catch(const std::bad_alloc &) {
+            it.status = LT_ERR_NOMEM;
+            it.message = "host out of memory";
+        }
+    }
+    if(m_live.empty()) { m_finished = true; return; }
+    context &ctx = *m_ctx;
     ctx.require_device("btod_contract2::perform_batch");
     ctx.activate();
     hipStream_t s = ctx.stream();
 
-    // plans (cached per operation), result preparation, prepare permutes
-    std::vector<contract_plan*> plans(items.size(), (contract_plan*)0);
-    for(size_t i = 0; i < items.size(); i++) {
-        batch_item &it = items[i];
-        btod_contract2 &op = *it.op;
-        lt_require(&op.m_a.ctx() == &ctx, LT_ERR_PARAM, "batch spans contexts");
-        op.check_block_sizes();
-        lt_require(it.c->bis() == op.m_symc.bis, LT_ERR_DIMS,
-            "result tensor has a different block index space");
-        if(it.add) {
-            lt_require(op.add_direct(*it.c), LT_ERR_SYMMETRY,
-                "perform_add: result symmetry differs from the contraction's (the request "
-                "loop adds into tensors of the result's symmetry)");
-        }
-        if(!op.plan_valid(*it.c, it.add, it.d, MODE_GROUPED))
-            op.build_plan(*it.c, it.add, it.d);
-        plans[i] = op.m_plan.get();
-    }
-    hip_check(hipEventRecord(plans[0]->ev[0], s), "hipEventRecord");
-    for(size_t i = 0; i < items.size(); i++) {
-        contract_plan &pl = *plans[i];
-        btensor &c = *items[i].c;
+    // result preparation and prepare permutes
+    hip_check(hipEventRecord(m_plans[0]->ev[0], s), "hipEventRecord");
+    for(size_t q = 0; q < m_live.size(); q++) {
+        contract_plan &pl = *m_plans[q];
+        batch_item &it = items[m_live[q]];
+        btensor &c = *it.c;
         if(!pl.add) {
             c.zero_all();
-            c.set_symmetry(items[i].op->m_symc);
+            c.set_symmetry(it.op->m_symc);
         } else {
             for(size_t j = 0; j < pl.c_out.size(); j++) {
                 if(c.is_nonzero(pl.c_out[j])) continue;
@@ -1597,28 +1748,25 @@ void btod_contract2::perform_batch(std::vector<batch_item> &items, dev_buffer &m
                 pl.n_prep_chunks, s), "permute (prepare)");
     }
 
-    // one grouped GEMM per kernel configuration: the work lists of
-    // the plans are concatenated (C block and pair indexes shifted) and the
-    // tiles ordered longest first across all requests
-    std::map<std::pair<int, int>, std::vector<size_t> > groups;
-    for(size_t i = 0; i < items.size(); i++) {
-        if(plans[i]->empty) continue;
-        groups[std::make_pair(0, plans[i]->tile_cfg)].push_back(i);
+    // one grouped GEMM per kernel configuration: the work lists of the
+    // plans concatenated (C block and pair indexes shifted), the tiles
+    // ordered longest first across all requests
+    std::map<int, std::vector<size_t> > groups;
+    for(size_t q = 0; q < m_live.size(); q++) {
+        if(m_plans[q]->empty) continue;
+        groups[m_plans[q]->tile_cfg].push_back(q);
     }
     std::vector<dev::gemm_cblock> cbs;
     std::vector<dev::gemm_pair> prs;
     std::vector<std::pair<double, dev::gemm_tile> > tl;
-    std::vector<std::pair<size_t, size_t> > spans;     // (cblock offset, tile offset) per group
     std::vector<char> host;
-    struct launch { int cfg; size_t ocb, opr, otl; uint32_t nt; };
-    std::vector<launch> launches;
     size_t sz = 0;
-    std::vector<std::vector<char> > parts;
-    for(std::map<std::pair<int, int>, std::vector<size_t> >::iterator g = groups.begin();
-        g != groups.end(); ++g) {
+    for(std::map<int, std::vector<size_t> >::iterator g = groups.begin(); g != groups.end(); ++g) {
         cbs.clear(); prs.clear(); tl.clear();
-        for(size_t q = 0; q < g->second.size(); q++) {
-            contract_plan &pl = *plans[g->second[q]];
+        uint32_t bm = 64, bn = 64;
+        dev::tile_shape(g->first, bm, bn);
+        for(size_t x = 0; x < g->second.size(); x++) {
+            contract_plan &pl = *m_plans[g->second[x]];
             uint32_t cb0 = (uint32_t)cbs.size(), pr0 = (uint32_t)prs.size();
             for(size_t b = 0; b < pl.cblocks.size(); b++) {
                 dev::gemm_cblock cb = pl.cblocks[b];
@@ -1628,18 +1776,24 @@ void btod_contract2::perform_batch(std::vector<batch_item> &items, dev_buffer &m
             }
             prs.insert(prs.end(), pl.pairs.begin(), pl.pairs.end());
             for(size_t t = 0; t < pl.tiles_sorted.size(); t++) {
-                dev::gemm_tile x = pl.tiles_sorted[t];
-                if(x.cblock >= pl.cb_ksum.size()) continue;     // padding tile
-                double cost = pl.cb_ksum[x.cblock];
-                x.cblock += cb0;
-                tl.push_back(std::make_pair(cost, x));
+                dev::gemm_tile y = pl.tiles_sorted[t];
+                if(y.cblock >= pl.cb_ksum.size()) continue;     // padding tile
+                // FLOPs of the tile: its rows x columns (clipped to the
+                // block) x 2 x the k extents of the block's pairs
+                const dev::gemm_cblock &cb = pl.cblocks[y.cblock];
+                const double rows = std::min<double>(bm, double(cb.m) - double(y.tm) * bm);
+                const double cols = std::min<double>(bn, double(cb.n) - double(y.tn) * bn);
+                const double cost = 2.0 * rows * cols * pl.cb_ksum[y.cblock];
+                y.cblock += cb0;
+                tl.push_back(std::make_pair(cost, y));
+                m_flops += cost;
             }
         }
         std::stable_sort(tl.begin(), tl.end(),
             [](const std::pair<double, dev::gemm_tile> &x, const std::pair<double, dev::gemm_tile> &y) {
                 return x.first > y.first; });
         launch L;
-        L.cfg = g->first.second;
+        L.cfg = g->first;
         L.ocb = sz; sz = align256(sz + cbs.size() * sizeof(dev::gemm_cblock));
         L.opr = sz; sz = align256(sz + prs.size() * sizeof(dev::gemm_pair));
         L.otl = sz; sz = align256(sz + tl.size() * sizeof(dev::gemm_tile));
@@ -1647,9 +1801,12 @@ void btod_contract2::perform_batch(std::vector<batch_item> &items, dev_buffer &m
         host.resize(sz, 0);
         if(!cbs.empty()) std::memcpy(&host[L.ocb], cbs.data(), cbs.size() * sizeof(dev::gemm_cblock));
         if(!prs.empty()) std::memcpy(&host[L.opr], prs.data(), prs.size() * sizeof(dev::gemm_pair));
-        for(size_t t = 0; t < tl.size(); t++)
-            std::memcpy(&host[L.otl + t * sizeof(dev::gemm_tile)], &tl[t].second, sizeof(dev::gemm_tile));
-        launches.push_back(L);
+        for(size_t t = 0; t < tl.size(); t++) {
+            std::memcpy(&host[L.otl + t * sizeof(dev::gemm_tile)], &tl[t].second,
+                sizeof(dev::gemm_tile));
+            L.tcost.push_back(tl[t].first);
+        }
+        m_launches.push_back(L);
     }
     if(sz) {
         if(merged.bytes < sz) {
@@ -1660,38 +1817,68 @@ void btod_contract2::perform_batch(std::vector<batch_item> &items, dev_buffer &m
         }
         hip_check(hipMemcpyAsync(merged.mem, host.data(), sz, hipMemcpyHostToDevice, s),
             "hipMemcpy H2D");
+        // the host buffer goes out of scope: the copy must have read it
+        hip_check(hipStreamSynchronize(s), "hipStreamSynchronize");
     }
-    char *base = (char*)merged.mem;
-    hip_check(hipEventRecord(plans[0]->ev[1], s), "hipEventRecord");
-    for(size_t l = 0; l < launches.size(); l++) {
-        const launch &L = launches[l];
-        hip_check(dev::launch_gemm(L.cfg, (dev::gemm_cblock*)(base + L.ocb),
-            (dev::gemm_pair*)(base + L.opr), (dev::gemm_tile*)(base + L.otl), L.nt, L.nt, s),
-            "gemm (batch)");
+    m_base = (char*)merged.mem;
+    hip_check(hipEventRecord(m_plans[0]->ev[1], s), "hipEventRecord");
+}
+
+bool batch_run::step(double flop_budget) {
+    if(m_finished) return false;
+    hipStream_t s = m_ctx->stream();
+    while(m_launch < m_launches.size()) {
+        const launch &L = m_launches[m_launch];
+        if(m_tile >= L.nt) { m_launch++; m_tile = 0; continue; }
+        // at least one tile, then as many as the budget allows
+        uint32_t t1 = (uint32_t)m_tile;
+        double f = 0.0;
+        do { f += L.tcost[t1]; t1++; } while(t1 < L.nt && f + L.tcost[t1] <= flop_budget);
+        const uint32_t n = t1 - (uint32_t)m_tile;
+        hip_check(dev::launch_gemm(L.cfg, (dev::gemm_cblock*)(m_base + L.ocb),
+            (dev::gemm_pair*)(m_base + L.opr),
+            (dev::gemm_tile*)(m_base + L.otl) + m_tile, n, n, s), "gemm (batch)");
+        m_tile = t1;
+        return m_launch < m_launches.size() &&
+            (m_tile < L.nt || m_launch + 1 < m_launches.size());
     }
-    hip_check(hipEventRecord(plans[0]->ev[2], s), "hipEventRecord");
-    for(size_t i = 0; i < items.size(); i++) {
-        contract_plan &pl = *plans[i];
+    return false;
+}
+
+void batch_run::finish() {
+    if(m_finished) return;
+    while(step(1e300)) { }
+    hipStream_t s = m_ctx->stream();
+    hip_check(hipEventRecord(m_plans[0]->ev[2], s), "hipEventRecord");
+    for(size_t q = 0; q < m_live.size(); q++) {
+        contract_plan &pl = *m_plans[q];
         if(!pl.empty && pl.n_wb_chunks)
             hip_check(dev::launch_permute(pl.d_wback, pl.d_wb_cd, pl.d_wb_co,
                 pl.n_wb_chunks, s), "permute (write back)");
     }
-    hip_check(hipEventRecord(plans[0]->ev[3], s), "hipEventRecord");
+    hip_check(hipEventRecord(m_plans[0]->ev[3], s), "hipEventRecord");
     hip_check(hipStreamSynchronize(s), "hipStreamSynchronize");
     float ms = 0.f, msg = 0.f;
-    hip_check(hipEventElapsedTime(&ms, plans[0]->ev[0], plans[0]->ev[3]), "hipEventElapsedTime");
-    hip_check(hipEventElapsedTime(&msg, plans[0]->ev[1], plans[0]->ev[2]), "hipEventElapsedTime");
-    for(size_t i = 0; i < items.size(); i++) {
-        contract_plan &pl = *plans[i];
-        for(size_t j = 0; j < pl.c_out.size(); j++) items[i].c->mark_nonzero(pl.c_out[j]);
-        exec_stats &st = items[i].op->m_stats;
+    hip_check(hipEventElapsedTime(&ms, m_plans[0]->ev[0], m_plans[0]->ev[3]), "hipEventElapsedTime");
+    hip_check(hipEventElapsedTime(&msg, m_plans[0]->ev[1], m_plans[0]->ev[2]), "hipEventElapsedTime");
+    for(size_t q = 0; q < m_live.size(); q++) {
+        contract_plan &pl = *m_plans[q];
+        batch_item &it = m_items[m_live[q]];
+        for(size_t j = 0; j < pl.c_out.size(); j++) it.c->mark_nonzero(pl.c_out[j]);
+        exec_stats &st = it.op->m_stats;
         st.t_gpu_ms = ms;
         st.t_gemm_ms = msg;
-        st.n_launches = launches.size();
+        st.n_launches = m_launches.size();
         st.graph = 0;
-        items[i].batch_gpu_ms = ms;
-        items[i].batch_gemm_launches = launches.size();
+        it.batch_gpu_ms = ms;
+        it.batch_gemm_launches = m_launches.size();
     }
+    m_finished = true;
+}
+
+void btod_contract2::perform_batch(std::vector<batch_item> &items, dev_buffer &merged) {
+    batch_run run(items, merged);
+    run.finish();
 }
 
 void btod_contract2::prepare(btensor &c, bool add, double d, bool measure) {
@@ -1717,11 +1904,76 @@ size_t btod_contract2::footprint_estimate() {
     return n * sizeof(double) + (size_t(1) << 20);
 }
 
+bool btod_contract2::try_plan(btensor &c, bool add, double d, bool measure) {
+    try {
+        build_plan(c, add, d, measure);
+        return true;
+    } catch(const lt_error &e) {
+        if(e.code != LT_ERR_NOMEM) throw;
+        m_plan.reset();           // frees what the plan took
+        return false;
+    }
+}
+
+void btod_contract2::run_batched(btensor &c, bool add, double d, int mode) {
+    // gen_bto_contract2::perform (gen_bto_contract2_impl.h:122-199,
+    // gen_bto_contract2_batching_policy.h:48-70) runs the contraction in
+    // batches of C blocks whose temporaries fit the memory limit.  Here the
+    // schedule is cut into 2, 4, 8, ... batches of consecutive blocks until
+    // the plan of every batch (prepared operand copies, work lists) fits the
+    // context's budget; each batch is planned, run and released in turn.
+    // The probe pass plans every batch once before any device work, so an
+    // accumulating perform never stops half way.
+    const size_t n = m_sched.size();
+    std::vector<std::vector<size_t> > parts;
+    for(size_t nb = 2; ; nb *= 2) {
+        lt_require(nb / 2 < n, LT_ERR_NOMEM,
+            "btod_contract2: a single C block's plan exceeds the device memory budget");
+        parts.assign(std::min(nb, n), std::vector<size_t>());
+        for(size_t i = 0; i < n; i++) parts[i * parts.size() / n].push_back(m_sched[i]);
+        bool ok = true;
+        for(size_t k = 0; k < parts.size() && ok; k++) {
+            m_sched_subset = &parts[k];
+            ok = try_plan(c, add, d, false);
+            m_sched_subset = 0;
+            m_plan.reset();
+        }
+        if(ok) break;
+    }
+    if(!add) {
+        c.zero_all();
+        c.set_symmetry(m_symc, m_uid);
+    }
+    for(size_t k = 0; k < parts.size(); k++) {
+        m_sched_subset = &parts[k];
+        try {
+            build_plan(c, add, d, false);
+            m_use_graph_saved = m_use_graph;
+            m_use_graph = false;        // a batch plan runs once: no capture
+            run(c, mode);
+            m_use_graph = m_use_graph_saved;
+        } catch(...) {
+            m_sched_subset = 0;
+            m_use_graph = m_use_graph_saved;
+            m_plan.reset();
+            throw;
+        }
+        m_sched_subset = 0;
+        m_plan.reset();
+    }
+    m_stats.n_batches = parts.size();
+}
+
 void btod_contract2::perform(btensor &c, int mode) {
+    c.require_mutable("btod_contract2::perform");
     check_block_sizes();
     m_a.ctx().require_device("btod_contract2::perform");
     lt_require(c.bis() == m_symc.bis, LT_ERR_DIMS, "result tensor has a different block index space");
-    if(!plan_valid(c, false, 1.0, mode)) build_plan(c, false, 1.0);
+    if(!plan_valid(c, false, 1.0, mode) && !try_plan(c, false, 1.0)) {
+        run_batched(c, false, 1.0, mode);
+        return;
+    }
+    m_stats.n_batches = 1;
     run(c, mode);
 }
 
@@ -1736,6 +1988,7 @@ bool btod_contract2::add_direct(btensor &c) {
 }
 
 void btod_contract2::perform_add(btensor &c, double d, int mode) {
+    c.require_mutable("btod_contract2::perform");
     check_block_sizes();
     m_a.ctx().require_device("btod_contract2::perform");
     lt_require(c.bis() == m_symc.bis, LT_ERR_DIMS, "result tensor has a different block index space");
@@ -1751,7 +2004,11 @@ void btod_contract2::perform_add(btensor &c, double d, int mode) {
         add_into(c, *m_addtmp, id, d);
         return;
     }
-    if(!plan_valid(c, true, d, mode)) build_plan(c, true, d);
+    if(!plan_valid(c, true, d, mode) && !try_plan(c, true, d)) {
+        run_batched(c, true, d, mode);
+        return;
+    }
+    m_stats.n_batches = 1;
     run(c, mode);
 }
 

@@ -18,7 +18,7 @@ void hip_check(hipError_t e, const char *what) {
 // ---------------------------------------------------------------------------
 
 context::context(int device, size_t hbm_cap) :
-    m_device(device), m_stream(0), m_cap(hbm_cap), m_used(0), m_nthreads(1) {
+    m_device(device), m_stream(0), m_xfer(0), m_cap(hbm_cap), m_used(0), m_nthreads(1) {
 
     unsigned hc = std::thread::hardware_concurrency();
     m_nthreads = hc ? (hc > 16 ? 16 : hc) : 1;
@@ -32,6 +32,11 @@ context::context(int device, size_t hbm_cap) :
     hip_check(hipSetDevice(device), "hipSetDevice");
     hip_check(hipStreamCreateWithFlags(&m_stream, hipStreamNonBlocking),
         "hipStreamCreate");
+    // uploads go through a stream of their own, so a host thread that plans
+    // (uploads work lists) does not wait for kernels queued on the compute
+    // stream by another thread; every upload completes before the host
+    // launches anything that reads it
+    hip_check(hipStreamCreateWithFlags(&m_xfer, hipStreamNonBlocking), "hipStreamCreate");
     if(m_cap == 0) {
         size_t fr = 0, tot = 0;
         hip_check(hipMemGetInfo(&fr, &tot), "hipMemGetInfo");
@@ -54,14 +59,37 @@ context::~context() {
     }
     hipSetDevice(m_device);
     if(m_stream) hipStreamSynchronize(m_stream);
+    m_launch_cache.clear();      // their memory goes with the loop below
     for(std::map<void*, size_t>::iterator i = m_sizes.begin(); i != m_sizes.end(); ++i)
         hipFree(i->first);
     if(m_stream) hipStreamDestroy(m_stream);
+    if(m_xfer) hipStreamDestroy(m_xfer);
 }
 
 void context::activate() const {
     if(m_device < 0) return;
     hip_check(hipSetDevice(m_device), "hipSetDevice");
+}
+
+std::shared_ptr<context::cached_launch> context::find_launch(const std::string &key) {
+    std::lock_guard<std::mutex> lk(m_cache_mtx);
+    std::map<std::string, std::shared_ptr<cached_launch> >::iterator i = m_launch_cache.find(key);
+    return i == m_launch_cache.end() ? std::shared_ptr<cached_launch>() : i->second;
+}
+
+void context::keep_launch(const std::string &key, std::shared_ptr<cached_launch> l) {
+    std::vector<std::shared_ptr<cached_launch> > drop;
+    {
+        std::lock_guard<std::mutex> lk(m_cache_mtx);
+        if(!m_launch_cache.count(key)) m_launch_order.push_back(key);
+        m_launch_cache[key] = l;
+        while(m_launch_order.size() > kLaunchCache) {
+            drop.push_back(m_launch_cache[m_launch_order.front()]);
+            m_launch_cache.erase(m_launch_order.front());
+            m_launch_order.erase(m_launch_order.begin());
+        }
+    }
+    for(size_t i = 0; i < drop.size(); i++) if(drop[i] && drop[i]->mem) free(drop[i]->mem);
 }
 
 void *context::staging(size_t bytes) {
@@ -121,9 +149,9 @@ void context::free(void *p) {
 void context::upload(void *dst, const void *src, size_t bytes) {
     if(m_device < 0) { std::memcpy(dst, src, bytes); return; }
     activate();
-    hip_check(hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, m_stream),
+    hip_check(hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, m_xfer),
         "hipMemcpy H2D");
-    hip_check(hipStreamSynchronize(m_stream), "hipStreamSynchronize");
+    hip_check(hipStreamSynchronize(m_xfer), "hipStreamSynchronize");
 }
 
 void context::download(void *dst, const void *src, size_t bytes) {
@@ -158,6 +186,13 @@ double *slab::take(size_t n) {
     return r;
 }
 
+bool slab::owns(const void *p) const {
+    const double *x = (const double*)p;
+    for(size_t i = 0; i < m_chunks.size(); i++)
+        if(x >= m_chunks[i].first && x < m_chunks[i].first + m_chunks[i].second) return true;
+    return false;
+}
+
 size_t slab::bytes() const {
     size_t b = 0;
     for(size_t i = 0; i < m_chunks.size(); i++) b += m_chunks[i].second * sizeof(double);
@@ -182,13 +217,20 @@ btensor::btensor(context &ctx, const bis_t &bis) :
         "block tensor order must be 1..8");
 }
 
+void btensor::require_mutable(const char *what) const {
+    if(m_immutable)
+        throw lt_error(LT_ERR_STATE, std::string(what) + ": the block tensor is immutable");
+}
+
 symmetry &btensor::sym_rw() {
+    require_mutable("symmetry change");
     touch();
     m_sym_tag = 0;
     return m_sym;
 }
 
 void btensor::split(const bool *mask, size_t pos) {
+    require_mutable("split");
     lt_require(m_store.empty(), LT_ERR_STATE, "split after blocks were allocated");
     lt_require(m_sym.subsets.empty(), LT_ERR_STATE, "split after symmetry was set");
     for(size_t i = 0; i < m_sym.bis.order(); i++)
@@ -199,6 +241,7 @@ void btensor::split(const bool *mask, size_t pos) {
 }
 
 void btensor::set_symmetry(const symmetry &s, uint64_t tag) {
+    require_mutable("set_symmetry");
     lt_require(s.bis == m_sym.bis, LT_ERR_SYMMETRY, "symmetry of a different block index space");
     m_sym = s;
     m_sym_tag = tag;
@@ -223,6 +266,7 @@ bool btensor::is_nonzero(size_t aidx) const {
 }
 
 void btensor::mark_nonzero(size_t aidx) {
+    require_mutable("block write");
     std::vector<size_t>::iterator i = std::lower_bound(m_nonzero.begin(), m_nonzero.end(), aidx);
     if(i != m_nonzero.end() && *i == aidx) return;
     m_nonzero.insert(i, aidx);
@@ -230,6 +274,7 @@ void btensor::mark_nonzero(size_t aidx) {
 }
 
 void btensor::zero_block(size_t aidx) {
+    require_mutable("zero_block");
     std::vector<size_t>::iterator i = std::lower_bound(m_nonzero.begin(), m_nonzero.end(), aidx);
     if(i != m_nonzero.end() && *i == aidx) {
         m_nonzero.erase(i);
@@ -238,6 +283,7 @@ void btensor::zero_block(size_t aidx) {
 }
 
 void btensor::zero_all() {
+    require_mutable("zero_all");
     if(!m_nonzero.empty()) touch();
     m_nonzero.clear();
 }
@@ -245,6 +291,7 @@ void btensor::zero_all() {
 std::vector<size_t> btensor::nonzero_blocks() const { return m_nonzero; }
 
 void btensor::set_block(size_t aidx, const double *host) {
+    require_mutable("set_block");
     short_orbit so(m_sym, aidx, true);
     lt_require(so.acidx == aidx, LT_ERR_SYMMETRY, "block is not canonical");
     lt_require(so.allowed, LT_ERR_SYMMETRY, "block is forbidden by symmetry");
@@ -338,6 +385,7 @@ std::vector<tensor_tr> block_stabilizer(const symmetry &sym, size_t aidx) {
 }
 
 void btensor::fill_random(uint64_t seed, int dist, double scale, int sk_d0, int sk_d1) {
+    require_mutable("random fill");
     const size_t n = m_sym.bis.order();
     const size_t nblk = m_bidims.size();
     std::vector<size_t> canon;

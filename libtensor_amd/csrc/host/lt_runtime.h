@@ -4,6 +4,7 @@
 #pragma once
 
 #include <atomic>
+#include <functional>
 #include <map>
 #include <memory>
 #include <mutex>
@@ -39,6 +40,9 @@ public:
     bool fits(size_t bytes) const { return m_used + bytes <= m_cap; }
     size_t used() const { return m_used; }
     size_t cap() const { return m_cap; }
+    //! New device memory budget (allocations beyond it fail with
+    //! LT_ERR_NOMEM; memory in use is not released).
+    void set_cap(size_t cap) { std::lock_guard<std::mutex> lk(m_mtx); m_cap = cap; }
 
     void upload(void *dst, const void *src, size_t bytes);
     void download(void *dst, const void *src, size_t bytes);
@@ -60,9 +64,30 @@ public:
     void *staging(size_t bytes);
     std::mutex &staging_lock() { return m_stage_mtx; }
 
+    //! Uploaded descriptor lists of block operations, kept per key (the
+    //! operation, its operands' identities and structure versions, its
+    //! scalars): a repeated call launches without rebuilding them.  At most
+    //! kLaunchCache entries, the oldest dropped first.
+    struct cached_launch {
+        //! one permute launch: descriptors at o_d, work items at o_cd / o_co
+        struct segment { size_t o_d, o_cd, o_co, nchunks; };
+        void *mem;
+        std::vector<segment> segs;       //!< launched in order
+        std::vector<size_t> touched;     //!< blocks the launches write
+        symmetry sym;                    //!< symmetry the result takes
+        cached_launch() : mem(0) { }
+    };
+    std::shared_ptr<cached_launch> find_launch(const std::string &key);
+    void keep_launch(const std::string &key, std::shared_ptr<cached_launch> l);
+    static const size_t kLaunchCache = 32;
+
 private:
     int m_device;
     hipStream_t m_stream;
+    hipStream_t m_xfer;   //!< host-to-device copies (default priority)
+    std::mutex m_cache_mtx;
+    std::map<std::string, std::shared_ptr<cached_launch> > m_launch_cache;
+    std::vector<std::string> m_launch_order;
     size_t m_cap;
     std::atomic<size_t> m_used;
     std::mutex m_mtx;
@@ -83,6 +108,8 @@ public:
     double *take(size_t n);
     size_t bytes() const;
     void release_all();
+    //! True if p lies in one of the slab's chunks.
+    bool owns(const void *p) const;
 private:
     context &m_ctx;
     std::vector<std::pair<double*, size_t> > m_chunks;
@@ -134,6 +161,16 @@ public:
     //! each block's stabilizer (btod_random semantics).
     void fill_random(uint64_t seed, int dist, double scale, int sk_d0, int sk_d1);
 
+    //! block_tensor::set_immutable (libtensor/core/immutable.h): from now on
+    //! the tensor's structure and data do not change (every write is
+    //! refused with LT_ERR_STATE), so derived data may be cached.
+    void set_immutable() { m_immutable = true; }
+    bool is_immutable() const { return m_immutable; }
+    //! Throws LT_ERR_STATE if the tensor is immutable.
+    void require_mutable(const char *what) const;
+    //! True if p points into the tensor's block storage.
+    bool owns(const void *p) const { return m_slab.owns(p); }
+
 private:
     context &m_ctx;
     symmetry m_sym;
@@ -143,6 +180,7 @@ private:
     std::vector<size_t> m_nonzero;   // sorted
     uint64_t m_version, m_uid;
     uint64_t m_sym_tag = 0;
+    bool m_immutable = false;
     static std::atomic<uint64_t> s_version;
 };
 
@@ -211,12 +249,13 @@ struct exec_stats {
     double bytes_perm;      //!< permute traffic of the last perform (bytes)
     size_t n_cblocks, n_pairs, n_tiles, n_unfold, n_launches;
     int tile_cfg;
-    double ozaki_max_rel_err; //!< set when mode == MODE_OZAKI and checked
+    double ozaki_max_rel_err; //!< split-precision: stated bound of max |error| / max |C|
     int ozaki_slices;
     double tune_ms[dev::TILE_COUNT]; //!< autotuner timings (0 = not measured)
     int graph;              //!< 1 if the last perform ran as a HIP graph
     int tile_order;         //!< grouped tile order (0 longest first, 1 XCD groups)
     size_t n_plans;         //!< execution plans built so far
+    size_t n_batches;       //!< batches of C blocks of the last perform
 };
 
 //! Dense helpers on a context's stream (both synchronous): dst = or +=
@@ -242,7 +281,7 @@ struct dev_buffer {
     void release() { if(ctx && mem) ctx->free(mem); mem = 0; bytes = 0; }
 };
 
-//! One perform of a batch (btod_contract2::perform_batch).
+//! One perform of a batch (btod_contract2::perform_batch, batch_run).
 struct batch_item {
     btod_contract2 *op;
     btensor *c;
@@ -250,6 +289,43 @@ struct batch_item {
     double d;
     double batch_gpu_ms;      //!< out: device time of the whole batch
     size_t batch_gemm_launches;
+    int status;               //!< out: LT_OK, or the error of this item alone
+    std::string message;
+    batch_item() : op(0), c(0), add(false), d(1.0), batch_gpu_ms(0.0),
+        batch_gemm_launches(0), status(LT_OK) { }
+};
+
+struct contract_plan;
+
+//! One batch of performs on the context's stream, run in steps so that
+//! other work can be interleaved between its GEMM slices.  The constructor
+//! checks and plans every item on its own (an item that fails is left out of
+//! the batch with its status and message set; the others go on), launches
+//! the result preparation and prepare permutes and uploads the merged work
+//! lists; step() launches the next slice of the grouped GEMM tiles (tiles
+//! in the order longest first, at least one tile and at most `flop_budget`
+//! FLOPs beyond it); finish() launches the write-backs and waits for the
+//! stream.  Every C tile is computed whole by one launch, so the results do
+//! not depend on how the tiles are sliced.
+class batch_run {
+public:
+    batch_run(std::vector<batch_item> &items, dev_buffer &merged);
+    bool step(double flop_budget);
+    void finish();
+    double flops() const { return m_flops; }
+    size_t live() const { return m_live.size(); }
+
+private:
+    struct launch { int cfg; size_t ocb, opr, otl; uint32_t nt; std::vector<double> tcost; };
+    std::vector<batch_item> &m_items;
+    std::vector<size_t> m_live;
+    std::vector<contract_plan*> m_plans;
+    std::vector<launch> m_launches;
+    context *m_ctx;
+    char *m_base;
+    size_t m_launch, m_tile;
+    double m_flops;
+    bool m_finished;
 };
 
 //! btod_contract2<N, M, K> (libtensor/block_tensor/btod_contract2.h:29):
@@ -285,6 +361,13 @@ public:
     //! of the work lists per operand layout and kernel configuration, then
     //! every write-back.  `merged` holds the concatenated lists.
     static void perform_batch(std::vector<batch_item> &items, dev_buffer &merged);
+    friend class batch_run;
+    //! perform (add false) or perform_add in grouped mode with the plan's own
+    //! kernel, the products launched in slices of about slice_flops FLOPs;
+    //! `between` runs on the calling thread after each slice has finished
+    //! (other work on the same stream goes in there).
+    void perform_sliced(btensor &c, bool add, double d, double slice_flops,
+        const std::function<void()> &between);
     //! Builds the execution plan for C unless it is cached (device memory
     //! for work lists and operand scratch, kernel autotuning).  measure
     //! false: the kernel configuration is chosen by the padding estimate
@@ -299,6 +382,9 @@ public:
     size_t footprint_estimate();
 
     const exec_stats &stats() const { return m_stats; }
+    //! Stated error of the last split-precision perform, per C block.
+    struct oz_bound { size_t aidx; double bound, cmax; };
+    const std::vector<oz_bound> &ozaki_bounds() const { return m_oz_bounds; }
     void set_tile_cfg(int cfg) { m_tile_cfg = cfg; }
     //! Grouped performs run as a captured HIP graph (default) or eagerly.
     void set_graph(bool on) { m_use_graph = on; }
@@ -322,11 +408,21 @@ private:
     bool m_use_graph;
     int m_oz_slices, m_oz_tile;
     std::unique_ptr<contract_plan> m_plan;
+    std::vector<oz_bound> m_oz_bounds;
     //! += into a tensor whose orbits differ from the result's: the result
     //! goes to this resident temporary first
     std::unique_ptr<btensor> m_addtmp;
     uint64_t m_add_uid, m_add_ver;   //!< tensor (uid, version) last checked
     bool m_add_direct;               //!< ... and whether its orbits agree
+    //! The batch of the schedule the next plan is built for (null: all).
+    const std::vector<size_t> *m_sched_subset;
+    bool m_use_graph_saved;
+
+    //! Builds the plan; false (and no plan) if it does not fit the device
+    //! memory budget.
+    bool try_plan(btensor &c, bool add, double d, bool measure = true);
+    //! Perform in batches of C blocks whose plans fit the budget.
+    void run_batched(btensor &c, bool add, double d, int mode);
 
     void check_block_sizes() const;
     //! True if c's orbits are the result's (the result adds into c's

@@ -1,10 +1,10 @@
 // Request loop for many concurrent calculations on one GPU: block tensors stay
 // resident in HBM, contractions are admitted by free device memory, planned
-// on a host thread while the GPU runs the previous batch, and every batch
-// runs all ready requests together (btod_contract2::perform_batch: one
-// grouped GEMM launch over the union of their work lists).  Requests that
-// arrive while a batch is on the GPU join the next batch (continuous
-// batching at perform granularity).
+// on a host thread while the GPU runs the previous batch, and a batch runs
+// ready requests together (batch_run: one grouped GEMM launch over the union
+// of their work lists).  A large request's products run in slices, and
+// requests that arrive meanwhile run between two slices (continuous
+// batching below the perform).
 #include "lt_server.h"
 
 #include <algorithm>
@@ -160,9 +160,114 @@ void server::planner_loop() {
     }
 }
 
-// Executor: every ready request joins the next batch (up to max_batch).
+// Executor.  Ready requests below kLargeFlops form batches of up to
+// max_batch (one grouped launch); a larger request runs as a batch of its
+// own whose GEMM tiles are launched in slices of about kSliceFlops, and
+// between two slices every request that became ready meanwhile runs as a
+// complete batch (its prepare, products and write-back), so a small request
+// does not wait for a large one to end.  A request that fails its checks
+// fails alone; the others of its batch complete.
+namespace {
+const double kLargeFlops = 2e10;
+const double kSliceFlops = 4e10;
+}
+
+bool server::take_small(std::vector<uint64_t> &ids, std::vector<batch_item> &items,
+    bool small_only) {
+    // called with m_mtx held
+    double t = now_ms();
+    for(std::deque<uint64_t>::iterator i = m_ready.begin();
+        i != m_ready.end() && ids.size() < m_max_batch;) {
+        request &r = m_reqs[*i];
+        const bool large = r.op->flops() > kLargeFlops;
+        if(large && (small_only || !ids.empty())) { ++i; continue; }
+        r.state = RUNNING;
+        r.t_start = t;
+        ids.push_back(*i);
+        batch_item it;
+        it.op = r.op; it.c = r.c; it.add = r.add; it.d = r.d;
+        items.push_back(it);
+        i = m_ready.erase(i);
+        if(large) break;         // a large request runs alone
+    }
+    return !ids.empty();
+}
+
+void server::complete(const std::vector<uint64_t> &ids, const std::vector<batch_item> &items,
+    int status, const std::string &msg) {
+    std::lock_guard<std::mutex> lk(m_mtx);
+    double t = now_ms();
+    m_batches++;
+    m_batched_requests += ids.size();
+    m_max_seen = std::max(m_max_seen, ids.size());
+    for(size_t i = 0; i < ids.size(); i++) {
+        request &r = m_reqs[ids[i]];
+        r.t_end = t;
+        r.batch = m_batches;
+        r.batch_size = ids.size();
+        r.status = status != LT_OK ? status : items[i].status;
+        r.message = status != LT_OK ? msg : items[i].message;
+        if(r.status != LT_OK) m_rejected++;
+        r.state = DONE;
+        m_done++;
+    }
+    m_cv.notify_all();
+}
+
+void server::run_batch(std::vector<uint64_t> &ids, std::vector<batch_item> &items,
+    dev_buffer &merged, dev_buffer &side) {
+    int status = LT_OK;
+    std::string msg;
+    try {
+        if(items.size() == 1 && items[0].op->flops() > kLargeFlops) {
+            // a large request alone: its own plan and kernel, products in
+            // slices; requests that became ready meanwhile run as a complete
+            // batch of their own between two slices
+            batch_item &it = items[0];
+            auto between = [&]() {
+                std::vector<uint64_t> sids;
+                std::vector<batch_item> sitems;
+                {
+                    std::lock_guard<std::mutex> lk(m_mtx);
+                    take_small(sids, sitems, true);
+                }
+                if(sids.empty()) return;
+                int sst = LT_OK;
+                std::string smsg;
+                try {
+                    batch_run srun(sitems, side);
+                    srun.finish();
+                } catch(const lt_error &e) {
+                    sst = e.code;
+                    smsg = e.what();
+                } catch(const std::exception &e) {
+                    sst = LT_ERR_INTERNAL;
+                    smsg = e.what();
+                }
+                complete(sids, sitems, sst, smsg);
+            };
+            try {
+                it.op->perform_sliced(*it.c, it.add, it.d, kSliceFlops, between);
+            } catch(const lt_error &e) {
+                it.status = e.code;
+                it.message = e.what();
+            }
+        } else {
+            batch_run run(items, merged);
+            run.finish();
+        }
+    } catch(const lt_error &e) {
+        status = e.code;
+        msg = e.what();
+    } catch(const std::exception &e) {
+        status = LT_ERR_INTERNAL;
+        msg = e.what();
+    }
+    complete(ids, items, status, msg);
+}
+
 void server::executor_loop() {
-    dev_buffer merged;
+    dev_buffer merged, side;
     while(true) {
         std::vector<uint64_t> ids;
         std::vector<batch_item> items;
@@ -172,49 +277,9 @@ void server::executor_loop() {
                 return !m_ready.empty() || (m_stop && m_queue.empty() && !planning());
             });
             if(m_ready.empty()) break;
-            double t = now_ms();
-            while(!m_ready.empty() && ids.size() < m_max_batch) {
-                uint64_t id = m_ready.front();
-                m_ready.pop_front();
-                request &r = m_reqs[id];
-                r.state = RUNNING;
-                r.t_start = t;
-                ids.push_back(id);
-                batch_item it;
-                it.op = r.op; it.c = r.c; it.add = r.add; it.d = r.d;
-                it.batch_gpu_ms = 0.0; it.batch_gemm_launches = 0;
-                items.push_back(it);
-            }
+            take_small(ids, items, false);
         }
-        int status = LT_OK;
-        std::string msg;
-        try {
-            btod_contract2::perform_batch(items, merged);
-        } catch(const lt_error &e) {
-            status = e.code;
-            msg = e.what();
-        } catch(const std::exception &e) {
-            status = LT_ERR_INTERNAL;
-            msg = e.what();
-        }
-        {
-            std::lock_guard<std::mutex> lk(m_mtx);
-            double t = now_ms();
-            m_batches++;
-            m_batched_requests += ids.size();
-            m_max_seen = std::max(m_max_seen, ids.size());
-            for(size_t i = 0; i < ids.size(); i++) {
-                request &r = m_reqs[ids[i]];
-                r.t_end = t;
-                r.batch = m_batches;
-                r.batch_size = ids.size();
-                r.status = status;
-                r.message = msg;
-                r.state = DONE;
-                m_done++;
-            }
-        }
-        m_cv.notify_all();
+        run_batch(ids, items, merged, side);
     }
 }
 

@@ -72,6 +72,14 @@ private:
 
     void planner_loop();
     void executor_loop();
+    //! Moves ready requests into a batch (small ones up to max_batch, or
+    //! one large request alone; small_only: small ones only).  m_mtx held.
+    bool take_small(std::vector<uint64_t> &ids, std::vector<batch_item> &items,
+        bool small_only);
+    void run_batch(std::vector<uint64_t> &ids, std::vector<batch_item> &items,
+        dev_buffer &merged, dev_buffer &side);
+    void complete(const std::vector<uint64_t> &ids, const std::vector<batch_item> &items,
+        int status, const std::string &msg);
     bool planning() const;
     bool in_flight() const;
 };

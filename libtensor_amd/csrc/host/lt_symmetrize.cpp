@@ -49,12 +49,14 @@ double btod_contract3::make_ab(int mode) {
 }
 
 void btod_contract3::perform(btensor &d, int mode) {
+    d.require_mutable("btod_contract3::perform");
     const double ms = make_ab(mode);
     m_op2->perform(d, mode);
     m_last_ms = ms + m_op2->stats().t_gpu_ms;
 }
 
 void btod_contract3::perform_add(btensor &d, double k, int mode) {
+    d.require_mutable("btod_contract3::perform");
     const double ms = make_ab(mode);
     m_op2->perform_add(d, k, mode);
     m_last_ms = ms + m_op2->stats().t_gpu_ms;
@@ -83,6 +85,7 @@ const std::vector<symmetrizer::rec> *symmetrizer::records(size_t aib) const {
 }
 
 double symmetrizer::run(btensor &b, bool add, double d, int mode) {
+    b.require_mutable("btod_symmetrize::perform");
     context &cx = ctx();
     cx.require_device("btod_symmetrize");
     lt_require(&b.ctx() == &cx, LT_ERR_PARAM, "tensors of different contexts");

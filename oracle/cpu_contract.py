@@ -12,9 +12,11 @@ tod_contract2 (libtensor/dense_tensor/impl/tod_contract2_impl.h:124-240)
 contracts them with one GEMM: here numpy.tensordot, which copies the
 operands into matrix layout and calls the BLAS numpy links (OpenBLAS) -- the
 reference's linalg_cblas path (linalg/cblas/linalg_cblas_level3.C:10-60).
-Blocks run as tasks of a thread pool, one C block per task with
-single-threaded BLAS: libtensor's thread_pool(n, n) model
-(gen_bto_contract2_batch_impl.h:273-276).
+Blocks run in parallel workers, one BLAS thread each, C blocks dealt out
+longest first: libtensor's thread_pool(n, n) model with single-threaded BLAS
+(gen_bto_contract2_batch_impl.h:273-276); the workers are forked processes
+(numpy's per-call overhead holds the interpreter lock, so threads would not
+scale on small blocks).
 
 Pinning: tests/test_oracle.py checks the result of this restatement against
 the golden results of the reference build.
@@ -22,7 +24,6 @@ the golden results of the reference build.
 import os
 import statistics
 import time
-from concurrent.futures import ThreadPoolExecutor
 
 import numpy as np
 
@@ -108,11 +109,9 @@ class BlockContraction:
                 acc += d * r
         return ic, acc
 
-    def perform(self, pool=None):
+    def perform(self):
         """Every scheduled block; returns {aidx: block (or None: zero)}."""
-        if pool is None:
-            return dict(self.compute_block(w) for w in self.work)
-        return dict(pool.map(self.compute_block, self.work))
+        return dict(self.compute_block(w) for w in self.work)
 
     def max_rel_err(self, blocks, result):
         """max |block - result block| / max |result| over the scheduled
@@ -171,25 +170,82 @@ def blas_info():
     return "numpy BLAS"
 
 
-def time_perform(bc, threads, runs):
-    """One warm-up, then `runs` timed performs with `threads` workers and
-    single-threaded BLAS; returns (median ms, all times, blocks)."""
+# Worker processes of time_perform (forked: they see the contraction and the
+# shared result buffer without copying them).
+_BC = None
+_OUT = None
+
+
+def _run_chunk(chunk):
+    for k in chunk:
+        ic, blk = _BC.compute_block(_BC.work[k])
+        off, n = _BC.offsets[k]
+        if blk is None:
+            _OUT[off:off + n] = 0.0
+        else:
+            _OUT[off:off + n] = blk.reshape(-1)
+    return len(chunk)
+
+
+def time_perform(bc, workers, runs):
+    """One warm-up, then `runs` timed performs with `workers` forked worker
+    processes (one BLAS thread each; the C blocks dealt out longest first, the
+    results written into one shared buffer) or, with workers == 1, in this
+    process.  Returns (median ms, all times, {aidx: block})."""
+    global _BC, _OUT
+    import multiprocessing as mp
     try:
         import threadpoolctl
         limit = threadpoolctl.threadpool_limits(1, user_api="blas")
     except Exception:
         limit = None
     try:
-        pool = ThreadPoolExecutor(max_workers=threads) if threads > 1 else None
-        blocks = bc.perform(pool)
-        times = []
-        for _ in range(runs):
-            t0 = time.perf_counter()
-            blocks = bc.perform(pool)
-            times.append((time.perf_counter() - t0) * 1e3)
-        if pool is not None:
-            pool.shutdown()
+        # result layout: the scheduled blocks back to back
+        sizes = []
+        for ic, pairs in bc.work:
+            idx = block_index(ic, bc.bc)
+            sizes.append(int(np.prod([bc.bc[i][k][1] for i, k in enumerate(idx)])))
+        bc.offsets = []
+        off = 0
+        for n in sizes:
+            bc.offsets.append((off, n))
+            off += n
+        shm = mp.RawArray("d", max(1, off))
+        _OUT = np.frombuffer(shm, dtype=np.float64)
+        _BC = bc
+        # cost of a block: sum over its pairs of the block sizes (balanced
+        # chunks, longest first, dealt round robin)
+        cost = [(sizes[k] * max(1, len(bc.work[k][1])), k) for k in range(len(bc.work))]
+        cost.sort(reverse=True)
+        chunks = [[] for _ in range(max(1, workers))]
+        for j, (cst, k) in enumerate(cost):
+            chunks[j % len(chunks)].append(k)
+        pool = mp.get_context("fork").Pool(workers) if workers > 1 else None
+        try:
+            def once():
+                if pool is None:
+                    _run_chunk(list(range(len(bc.work))))
+                else:
+                    pool.map(_run_chunk, chunks, chunksize=1)
+            once()
+            times = []
+            for _ in range(runs):
+                t0 = time.perf_counter()
+                once()
+                times.append((time.perf_counter() - t0) * 1e3)
+        finally:
+            if pool is not None:
+                pool.close()
+                pool.join()
+        blocks = {}
+        for k, (ic, pairs) in enumerate(bc.work):
+            idx = block_index(ic, bc.bc)
+            shape = tuple(bc.bc[i][kk][1] for i, kk in enumerate(idx))
+            o, n = bc.offsets[k]
+            blocks[ic] = _OUT[o:o + n].reshape(shape).copy()
     finally:
+        _BC = None
+        _OUT = None
         if limit is not None:
             limit.restore_original_limits()
     return statistics.median(times), times, blocks

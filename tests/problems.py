@@ -49,6 +49,13 @@ def build(ctx, prob, dense):
     return a, b, op
 
 
+def build_op(ctx, prob, a, b):
+    """The btod_contract2 op of a problem on existing block tensors."""
+    c = prob["contr"]
+    return lt.btod_contract2(W.make_contraction(prob), a, b if not prob.get("self") else a,
+                             c.get("ka", 1.0), c.get("kb", 1.0), c.get("kc", 1.0))
+
+
 def result_dims(prob):
     order = W.result_order(prob)
     dims = []

@@ -103,3 +103,30 @@ def test_btod_add_different_symmetries(gpu_ctx):
     # every block of the result is its own orbit (bare has no symmetry)
     nblk = int(np.prod(out.bidims))
     assert all(out.orbit(out.index(x))[0] == x for x in range(nblk))
+
+
+def test_repeated_calls_reuse_descriptors_and_follow_changes(gpu_ctx):
+    """btod_copy / btod_add keep their uploaded descriptor lists per operand
+    structure version: repeated calls give the same result, a change of the
+    source's non-zero blocks or of B's blocks (for +=) gives a fresh list."""
+    prob, dense, meta, result = G.load("ladder_full")
+    a, b, op = P.build(gpu_ctx, prob, dense)
+    A = dense["A"]
+    n = A.ndim
+    bt = permuted_tensor(gpu_ctx, prob["A"], tuple(range(n)))
+    for _ in range(3):
+        lt.btod_copy(a, bt, None, c=2.0)
+        assert np.array_equal(bt.to_dense(), 2.0 * A)
+    for k in range(3):
+        lt.btod_copy(a, bt, None, c=1.0, add=True, d=1.0)
+        assert np.allclose(bt.to_dense(), (3.0 + k) * A, rtol=0, atol=1e-14 * np.abs(A).max())
+    # the source loses a block: the next copy sees it
+    nz = a.req_nonzero_blocks()
+    a.zero_block(a.index(nz[0]))
+    lt.btod_copy(a, bt, None, c=1.0)
+    exp = a.to_dense()
+    assert np.array_equal(bt.to_dense(), exp)
+    out = permuted_tensor(gpu_ctx, prob["A"], tuple(range(n)))
+    for _ in range(2):
+        lt.btod_add([(a, None, 1.0), (bt, None, -0.5)], out)
+        assert np.allclose(out.to_dense(), 0.5 * exp, rtol=0, atol=1e-14 * np.abs(A).max())

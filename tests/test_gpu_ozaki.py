@@ -133,3 +133,77 @@ def test_slice_count_changes_keep_device_memory(gpu_ctx):
     op.perform(c, lt.LT_MODE_OZAKI)
     assert gpu_ctx.mem()[0] == used
     assert O.rel_err(c.to_dense(), result) <= TOL
+
+
+def _block(dense, t, aidx):
+    idx = t.index(aidx)
+    st, bd = t.block_start(idx), t.block_dims(idx)
+    return dense[tuple(slice(s0, s0 + d) for s0, d in zip(st, bd))]
+
+
+@pytest.mark.parametrize("name", G.names())
+def test_stated_bound_covers_the_error(gpu_ctx, name):
+    """The per-block a-priori bound the library states for a split-precision
+    perform covers the measured error of every C block (against libtensor's
+    FP64 result), and the overall stated relative bound covers the overall
+    relative error, at 3, 5 and 7 slices."""
+    prob, dense, meta, result = G.load(name)
+    a, b, op = P.build(gpu_ctx, prob, dense)
+    for s in (3, 5, 7):
+        op.set_ozaki(s)
+        c = P.make_result_tensor(gpu_ctx, prob)
+        op.perform(c, lt.LT_MODE_OZAKI)
+        got = c.to_dense()
+        bounds = op.ozaki_bounds()
+        assert sorted(x[0] for x in bounds) == meta["schedule"]
+        for aidx, bound, cmax in bounds:
+            err = np.max(np.abs(_block(got, c, aidx) - _block(result, c, aidx)), initial=0.0)
+            assert err <= bound, (s, aidx, err, bound)
+            assert cmax == np.max(np.abs(_block(got, c, aidx)), initial=0.0)
+        assert O.rel_err(got, result) <= op.stats()["ozaki_max_rel_err"], s
+
+
+def test_immutable_operand_split_once(gpu_ctx):
+    """Slices of an operand read in place from an immutable tensor are cut
+    once per plan: results equal (bit for bit) those of the same perform with
+    a mutable operand, on repeated performs; writes to the immutable tensor
+    are refused."""
+    prob, dense, meta, result = G.load("ladder_full")
+    a, b, op = P.build(gpu_ctx, prob, dense)
+    op.set_ozaki(7)
+    c = P.make_result_tensor(gpu_ctx, prob)
+    op.perform(c, lt.LT_MODE_OZAKI)
+    ref = c.to_dense()
+    a2, b2, op2 = P.build(gpu_ctx, prob, dense)
+    b2.set_immutable()
+    assert b2.is_immutable()
+    op2.set_ozaki(7)
+    c2 = P.make_result_tensor(gpu_ctx, prob)
+    for _ in range(3):
+        op2.perform(c2, lt.LT_MODE_OZAKI)
+        assert np.array_equal(c2.to_dense(), ref)
+    with pytest.raises(lt.LtError) as e:
+        b2.from_dense(dense["B"])
+    assert e.value.code == -6       # LT_ERR_STATE
+    with pytest.raises(lt.LtError):
+        lt.btod_copy(b2, b2)            # B := P(B) writes into B
+
+
+def test_nonfinite_input_gives_nan(gpu_ctx):
+    """A NaN in an operand makes the products of its row NaN (as in FP64)
+    and the stated bound infinite, instead of finite wrong values."""
+    prob, dense, meta, result = G.load("rand_222")
+    A = dense["A"].copy()
+    A.flat[np.flatnonzero(A)[0]] = np.nan
+    d2 = dict(dense, A=A)
+    a, b, op = P.build(gpu_ctx, prob, d2)
+    c = P.make_result_tensor(gpu_ctx, prob)
+    op.perform(c, lt.LT_MODE_GROUPED)
+    fp64 = c.to_dense()
+    op.set_ozaki(7)
+    op.perform(c, lt.LT_MODE_OZAKI)
+    oz = c.to_dense()
+    assert np.isnan(fp64).any()
+    # every element FP64 makes NaN is NaN in the split form too
+    assert np.all(np.isnan(oz[np.isnan(fp64)]))
+    assert op.stats()["ozaki_max_rel_err"] == np.inf or np.isnan(op.stats()["ozaki_max_rel_err"])
