@@ -103,17 +103,29 @@ def serving_mix(ctx, lt, W, n_clients, rounds, rate, seed=1005):
     dense operands are small enough, with the dense FP64 oracle."""
     probs = W.serving_clients(n_clients, seed)
     clients = []
+    solo = {}
     t0 = time.time()
     for i, p in enumerate(probs):
         a, b, op = W.build_random(ctx, p, 2000 + i)
-        op.set_tile(-1)      # configuration from the padding estimate (no per-client tuning)
+        op.set_tile(-1)      # configuration from the padding estimate
         c = W.result_tensor(ctx, p)
         op.perform(c, lt.LT_MODE_GROUPED)
+        if op.stats()["flops"] > 1e11:
+            # a large client's kernel is timed on its own work list once,
+            # before the mix (as a calculation tunes its first iteration)
+            op.set_tile(-2)
+            op.perform(c, lt.LT_MODE_GROUPED)
+        # time of the same perform alone (plan built: one perform, synchronized)
+        ctx.sync()
+        ts = time.perf_counter()
+        op.perform(c, lt.LT_MODE_GROUPED)
+        ctx.sync()
+        solo[p["name"]] = (time.perf_counter() - ts) * 1e3
         digest = hashlib.sha1(c.to_dense().tobytes()).hexdigest()
         clients.append((p["name"], a, b, op, c, digest, op.stats()["flops"]))
     log("serving: %d clients ready in %.1fs" % (len(clients), time.time() - t0))
     srv = lt.server(ctx, max_batch=64)
-    lat, qlat, bad = [], [], []
+    lat, qlat, bad, reqs = [], [], [], []
     lock = threading.Lock()
 
     def client(i):
@@ -125,6 +137,7 @@ def serving_mix(ctx, lt, W, n_clients, rounds, rate, seed=1005):
             with lock:
                 lat.append(st["t_total_ms"])
                 qlat.append(st["t_queue_ms"])
+                reqs.append((st["t_total_ms"], st["t_queue_ms"], solo[name], fl / 1e9))
 
     t1 = time.perf_counter()
     th = [threading.Thread(target=client, args=(i,)) for i in range(len(clients))]
@@ -167,6 +180,10 @@ def serving_mix(ctx, lt, W, n_clients, rounds, rate, seed=1005):
            "queue_p99_ms": float(np.percentile(qlat, 99)),
            "request_gflop_min": min(x[6] for x in clients) / 1e9,
            "request_gflop_max": max(x[6] for x in clients) / 1e9,
+           "largest_solo_ms": max(solo.values()),
+           "p99_over_largest_solo": float(np.percentile(lat, 99)) / max(solo.values()),
+           "slowest": [{"ms": r[0], "queue_ms": r[1], "solo_ms": r[2], "gflop": r[3]}
+                       for r in sorted(reqs, reverse=True)[:6]],
            "results_same_as_alone": not bad, "differing": bad,
            "oracle_checked": checked, "oracle_max_rel_err": worst,
            "oracle_note": "dense FP64 oracle for the clients whose dense operands have at "
@@ -267,10 +284,11 @@ def block_ops(ctx, lt, prob, a, b, op, c, steps=10):
     return res
 
 
-def ozaki_sweep(ctx, lt, W, prob, op, ref, flops, slices=(3, 4, 5, 6, 7), steps=5):
+def ozaki_sweep(ctx, lt, W, prob, op, ref, flops, a, b, slices=(3, 4, 5, 6, 7), steps=5):
     """Split-precision mode on the benchmark contraction: error against the
-    FP64 reference per block and overall, and time per perform, for each
-    slice count."""
+    FP64 reference per block and overall, the error bound the library states,
+    and time per perform, for each slice count; then 7 slices with the
+    integrals V in an immutable tensor (their slices cut once per plan)."""
     out = []
     c = W.result_tensor(ctx, prob)
     sched = op.get_schedule()
@@ -297,13 +315,40 @@ def ozaki_sweep(ctx, lt, W, prob, op, ref, flops, slices=(3, 4, 5, 6, 7), steps=
             sl, r = ref_blocks[bi]
             m = float(np.max(np.abs(r)))
             blk.append(float(np.max(np.abs(got[sl] - r)) / m) if m > 0 else 0.0)
+        st = op.stats()
         out.append({"slices": s, "ms_per_step": ms, "tflops_equiv": flops / (ms * 1e-3) / 1e12,
-                    "max_rel_err": err, "block_rel_err_max": max(blk),
+                    "max_rel_err": err, "stated_rel_err_bound": st["ozaki_max_rel_err"],
+                    "block_rel_err_max": max(blk),
                     "block_rel_err_median": float(np.median(blk)),
-                    "gemm_ms": op.stats()["t_gemm_ms"]})
+                    "gemm_ms": st["t_gemm_ms"]})
         log("ozaki: %s" % json.dumps(out[-1]))
     need = [x["slices"] for x in out if x["max_rel_err"] <= 1e-12]
-    return {"sweep": out, "slices_for_1e-12": min(need) if need else None}
+    res = {"sweep": out, "slices_for_1e-12": min(need) if need else None}
+    # V immutable: a copy of the integrals marked immutable, a new operation
+    # on it; its slices are cut at the plan's first perform only
+    vi = W.build_tensor(ctx, prob["B"])
+    lt.btod_copy(b, vi)
+    vi.set_immutable()
+    cc = prob["contr"]
+    opi = lt.btod_contract2(W.make_contraction(prob), a, vi, cc.get("ka", 1.0),
+                            cc.get("kb", 1.0), cc.get("kc", 1.0))
+    opi.set_ozaki(7)
+    opi.perform(c, lt.LT_MODE_OZAKI)
+    ctx.sync()
+    t0 = time.perf_counter()
+    for i in range(steps):
+        opi.perform(c, lt.LT_MODE_OZAKI)
+    ctx.sync()
+    ms = (time.perf_counter() - t0) * 1e3 / steps
+    err = float(np.max(np.abs(c.to_dense() - ref)) / den)
+    res["immutable_v_7"] = {"ms_per_step": ms, "tflops_equiv": flops / (ms * 1e-3) / 1e12,
+                            "max_rel_err": err,
+                            "stated_rel_err_bound": opi.stats()["ozaki_max_rel_err"],
+                            "gemm_ms": opi.stats()["t_gemm_ms"]}
+    log("ozaki, V immutable: %s" % json.dumps(res["immutable_v_7"]))
+    opi.close()
+    del vi
+    return res
 
 
 # name -> (problem builder, input seed, description)
@@ -439,7 +484,7 @@ def main():
     # when it ran, else this FP64 result)
     oz = None
     if not args.no_ozaki:
-        oz = ozaki_sweep(ctx, lt, W, prob, op, ref if ref is not None else result, flops)
+        oz = ozaki_sweep(ctx, lt, W, prob, op, ref if ref is not None else result, flops, a, b)
         if ref is None:
             oz["reference"] = "FP64 grouped result on the GPU"
         else:

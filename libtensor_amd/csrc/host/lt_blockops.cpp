@@ -152,6 +152,8 @@ void add_descs(btensor &b, btensor &s, const tensor_tr &tr, double d,
     }
 }
 
+} // unnamed namespace
+
 //! Uploads descriptor lists (one launch each) into one device buffer.
 std::shared_ptr<context::cached_launch> upload_launches(context &ctx,
     const std::vector<std::vector<dev::perm_desc> > &lists) {
@@ -237,8 +239,6 @@ std::string key_tensor(const btensor &t, bool version) {
         version ? (unsigned long long)t.version() : 0ULL);
     return buf;
 }
-
-} // unnamed namespace
 
 void add_into(btensor &b, btensor &s, const tensor_tr &tr, double d, double *ms) {
     b.require_mutable("add");

@@ -346,6 +346,11 @@ void btod_contract2::clst(const index_t &ic, contr_list &out) {
     build_clst(m_contr, oca, ocb, blax, blbx, cbl, ic, out);
 }
 
+void btod_contract2::set_tile_cfg(int cfg) {
+    if(cfg != m_tile_cfg) m_plan.reset();
+    m_tile_cfg = cfg;
+}
+
 bool btod_contract2::plan_valid(btensor &c, bool add, double d, int) const {
     if(m_plan && m_plan->batch) return false;
     if(!m_plan) return false;

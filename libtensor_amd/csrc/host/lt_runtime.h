@@ -75,7 +75,8 @@ public:
         std::vector<segment> segs;       //!< launched in order
         std::vector<size_t> touched;     //!< blocks the launches write
         symmetry sym;                    //!< symmetry the result takes
-        cached_launch() : mem(0) { }
+        double bytes;                    //!< bytes the launches move
+        cached_launch() : mem(0), bytes(0.0) { }
     };
     std::shared_ptr<cached_launch> find_launch(const std::string &key);
     void keep_launch(const std::string &key, std::shared_ptr<cached_launch> l);
@@ -237,6 +238,17 @@ bool same_orbits(const symmetry &x, const symmetry &y, const dims_t &bid);
 //! kernel on the context's stream and waits for it; adds the launch's device
 //! time (HIP events) to *ms when ms is not null.
 void run_permutes(context &ctx, const std::vector<dev::perm_desc> &descs, double *ms = 0);
+//! Descriptor lists (one launch each, in order) uploaded into one device
+//! buffer, for the context's launch cache.
+std::shared_ptr<context::cached_launch> upload_launches(context &ctx,
+    const std::vector<std::vector<dev::perm_desc> > &lists);
+//! Runs the launches of a cached entry in order and waits; ms (optional)
+//! accumulates their device time.
+void run_launches(context &ctx, const context::cached_launch &l, double *ms);
+//! Launch-cache key parts: a transformation; a tensor's identity and (with
+//! version) its structure version.
+std::string key_part(const tensor_tr &tr);
+std::string key_tensor(const btensor &t, bool version);
 
 enum exec_mode { MODE_PER_BLOCK = 0, MODE_GROUPED = 1, MODE_OZAKI = 2 };
 
@@ -385,7 +397,9 @@ public:
     //! Stated error of the last split-precision perform, per C block.
     struct oz_bound { size_t aidx; double bound, cmax; };
     const std::vector<oz_bound> &ozaki_bounds() const { return m_oz_bounds; }
-    void set_tile_cfg(int cfg) { m_tile_cfg = cfg; }
+    //! Kernel configuration of the next plan (a plan built with another one
+    //! is dropped, so the next perform uses it).
+    void set_tile_cfg(int cfg);
     //! Grouped performs run as a captured HIP graph (default) or eagerly.
     void set_graph(bool on) { m_use_graph = on; }
     void set_ozaki(int slices) { m_oz_slices = slices; }

@@ -214,40 +214,48 @@ void server::complete(const std::vector<uint64_t> &ids, const std::vector<batch_
     m_cv.notify_all();
 }
 
+void server::run_small(dev_buffer &side) {
+    std::vector<uint64_t> sids;
+    std::vector<batch_item> sitems;
+    {
+        std::lock_guard<std::mutex> lk(m_mtx);
+        take_small(sids, sitems, true);
+    }
+    if(sids.empty()) return;
+    int sst = LT_OK;
+    std::string smsg;
+    try {
+        batch_run srun(sitems, side);
+        srun.finish();
+    } catch(const lt_error &e) {
+        sst = e.code;
+        smsg = e.what();
+    } catch(const std::exception &e) {
+        sst = LT_ERR_INTERNAL;
+        smsg = e.what();
+    }
+    complete(sids, sitems, sst, smsg);
+}
+
+void server::run_large(batch_item &it, dev_buffer &side) {
+    // the large request alone: its own plan and kernel, products in slices;
+    // between two slices the small requests that became ready run as a
+    // complete batch.  Large requests run in arrival order (first come,
+    // first served: the largest requests set the latency tail, and letting
+    // smaller large ones overtake them lengthens it)
+    auto between = [&]() { run_small(side); };
+    it.op->perform_sliced(*it.c, it.add, it.d, kSliceFlops, between);
+}
+
 void server::run_batch(std::vector<uint64_t> &ids, std::vector<batch_item> &items,
     dev_buffer &merged, dev_buffer &side) {
     int status = LT_OK;
     std::string msg;
     try {
         if(items.size() == 1 && items[0].op->flops() > kLargeFlops) {
-            // a large request alone: its own plan and kernel, products in
-            // slices; requests that became ready meanwhile run as a complete
-            // batch of their own between two slices
             batch_item &it = items[0];
-            auto between = [&]() {
-                std::vector<uint64_t> sids;
-                std::vector<batch_item> sitems;
-                {
-                    std::lock_guard<std::mutex> lk(m_mtx);
-                    take_small(sids, sitems, true);
-                }
-                if(sids.empty()) return;
-                int sst = LT_OK;
-                std::string smsg;
-                try {
-                    batch_run srun(sitems, side);
-                    srun.finish();
-                } catch(const lt_error &e) {
-                    sst = e.code;
-                    smsg = e.what();
-                } catch(const std::exception &e) {
-                    sst = LT_ERR_INTERNAL;
-                    smsg = e.what();
-                }
-                complete(sids, sitems, sst, smsg);
-            };
             try {
-                it.op->perform_sliced(*it.c, it.add, it.d, kSliceFlops, between);
+                run_large(it, side);
             } catch(const lt_error &e) {
                 it.status = e.code;
                 it.message = e.what();

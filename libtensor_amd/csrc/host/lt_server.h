@@ -78,6 +78,10 @@ private:
         bool small_only);
     void run_batch(std::vector<uint64_t> &ids, std::vector<batch_item> &items,
         dev_buffer &merged, dev_buffer &side);
+    //! The ready small requests as one batch (between two slices).
+    void run_small(dev_buffer &side);
+    //! A large request in slices, small batches between the slices.
+    void run_large(batch_item &it, dev_buffer &side);
     void complete(const std::vector<uint64_t> &ids, const std::vector<batch_item> &items,
         int status, const std::string &msg);
     bool planning() const;

@@ -112,29 +112,44 @@ double symmetrizer::run(btensor &b, bool add, double d, int mode) {
             b.set_symmetry(m_sym, m_uid);
         }
     }
-    std::vector<std::vector<dev::perm_desc> > rounds;
-    double bytes = 0.0;
-    for(size_t i = 0; i < m_recs.size(); i++) {
-        const size_t ib = m_recs[i].first;
-        const std::vector<rec> &rs = m_recs[i].second;
-        const bool had = add && b.is_nonzero(ib);
-        size_t r = 0;
-        for(size_t k = 0; k < rs.size(); k++) {
-            if(!x.is_nonzero(rs[k].ai)) continue;
-            tensor_tr tr(rs[k].tr);
-            if(add) tr.st.scale(d);
-            if(rounds.size() <= r) rounds.resize(r + 1);
-            const dims_t dx = x.bis().block_dims(get_index(rs[k].ai, bid));
-            const bool acc = r > 0 || had;
-            rounds[r].push_back(make_perm_desc_public(x.block_ptr(rs[k].ai), dx, tr,
-                b.block_ptr(ib), acc));
-            bytes += 8.0 * dx.size() * (acc ? 3 : 2);
-            r++;
+    // the gather launches are kept per structure versions of X and B (and
+    // the scalar of +=): a repeated perform uploads nothing
+    const std::string key = std::string(add ? "symm+" : "symm") + key_tensor(x, true) +
+        key_tensor(b, true) + (add ? "|" + key_part(tensor_tr(perm_t(0), scalar_tr(d))) :
+        std::string());
+    std::shared_ptr<context::cached_launch> l = cx.find_launch(key);
+    if(!l) {
+        std::vector<std::vector<dev::perm_desc> > rounds;
+        std::vector<size_t> touched;
+        double bytes = 0.0;
+        for(size_t i = 0; i < m_recs.size(); i++) {
+            const size_t ib = m_recs[i].first;
+            const std::vector<rec> &rs = m_recs[i].second;
+            const bool had = add && b.is_nonzero(ib);
+            size_t r = 0;
+            for(size_t k = 0; k < rs.size(); k++) {
+                if(!x.is_nonzero(rs[k].ai)) continue;
+                tensor_tr tr(rs[k].tr);
+                if(add) tr.st.scale(d);
+                if(rounds.size() <= r) rounds.resize(r + 1);
+                const dims_t dx = x.bis().block_dims(get_index(rs[k].ai, bid));
+                const bool acc = r > 0 || had;
+                rounds[r].push_back(make_perm_desc_public(x.block_ptr(rs[k].ai), dx, tr,
+                    b.block_ptr(ib), acc));
+                bytes += 8.0 * dx.size() * (acc ? 3 : 2);
+                r++;
+            }
+            if(r) touched.push_back(ib);
         }
-        if(r) b.mark_nonzero(ib);
+        l = upload_launches(cx, rounds);
+        l->touched = touched;
+        l->bytes = bytes;
+        cx.keep_launch(key, l);
     }
     double gms = 0.0;
-    for(size_t r = 0; r < rounds.size(); r++) run_permutes(cx, rounds[r], &gms);
+    run_launches(cx, *l, &gms);
+    for(size_t i = 0; i < l->touched.size(); i++) b.mark_nonzero(l->touched[i]);
+    const double bytes = l->bytes;
     ms += gms;
     m_gather_ms = gms;
     m_gather_bytes = bytes;
