@@ -213,6 +213,7 @@ struct oz_pair {
     const int32_t *eb;
     uint32_t astride, bstride, kp, pad;
     double d;
+    uint32_t ia, ib;                  //!< split descriptors of the two operands
 };
 
 //! C tile shapes of the split-precision product kernel (rows x columns).
@@ -234,7 +235,12 @@ static const int32_t kOzNonFinite = 0x3fffffff;
 //! Stated error of the split-precision products: out[2 b] = bound of
 //! max |error| of C block b, out[2 b + 1] = max |C| of the block as written.
 hipError_t launch_oz_bound(const gemm_cblock *cb, const gemm_pair *pairs, const oz_pair *ozp,
-    uint32_t ncb, int slices, double *out, hipStream_t s);
+    const int32_t *opmax, uint32_t ncb, int slices, double *out, hipStream_t s);
+//! Largest exponent of every split operand (opmax[i] over the rows of
+//! descs[i]; kOzNonFinite when a row holds a NaN or an infinity), read by
+//! the bound kernel.
+hipError_t launch_oz_opmax(const oz_split_desc *descs, uint32_t nops, int32_t *opmax,
+    hipStream_t s);
 
 //! Fill n doubles with v.
 hipError_t launch_fill(double *p, size_t n, double v, hipStream_t s);

@@ -222,47 +222,72 @@ oz_gemm_kernel(const gemm_cblock *__restrict__ cbs,
         }
 }
 
+// Largest exponent of every split operand: one wave per operand, its rows'
+// exponents (as the split kernel wrote them) reduced with max (a row with a
+// NaN or an infinity holds kOzNonFinite, larger than any finite exponent).
+__global__ void __launch_bounds__(256)
+oz_opmax_kernel(const oz_split_desc *__restrict__ descs, uint32_t nops,
+    int32_t *__restrict__ opmax) {
+    const uint32_t i = blockIdx.x * 4 + (threadIdx.x >> 6);
+    if(i >= nops) return;             // whole wave: no barrier in this kernel
+    const int lane = threadIdx.x & 63;
+    const oz_split_desc d = descs[i];
+    int m = -2147483647;
+    for(uint32_t r = lane; r < d.rows; r += 64) m = max(m, d.exps[r]);
+#pragma unroll
+    for(int off = 32; off > 0; off >>= 1) m = max(m, __shfl_xor(m, off));
+    if(lane == 0) opmax[i] = m;
+}
+
 // Stated error of every C block of a split-precision product: one
 // workgroup per C block; bound = sum over the block's pairs (in list order)
 // of |d| K (s + 2) 2^(max_i e_i + max_j f_j - 7 s) (the per-element bound
-// above with the largest row and column exponents of the pair), and the
-// largest |C| of the block as the product kernel wrote it (NaN propagates).
-// out[2 b] = bound, out[2 b + 1] = max |C|.
+// above with the largest row and column exponents of the pair's operands),
+// and the largest |C| of the block as the product kernel wrote it (NaN
+// propagates).  out[2 b] = bound, out[2 b + 1] = max |C|.
 __global__ void __launch_bounds__(256)
 oz_bound_kernel(const gemm_cblock *__restrict__ cbs, const gemm_pair *__restrict__ pairs,
-    const oz_pair *__restrict__ ozp, int slices, double *__restrict__ out) {
+    const oz_pair *__restrict__ ozp, const int32_t *__restrict__ opmax, int slices,
+    double *__restrict__ out) {
 
-    __shared__ int ra[256], rb[256];
     __shared__ double rc[256];
+    __shared__ double rt[256];
     const gemm_cblock cb = cbs[blockIdx.x];
     const int tid = threadIdx.x;
+    // the terms of up to 256 pairs at a time, one per thread, then added by
+    // thread 0 in list order
     double bound = 0.0;
-    for(uint32_t p = cb.pair_begin; p < cb.pair_end; p++) {
-        const oz_pair o = ozp[p];
-        int ma = -2147483647, mb = -2147483647;
-        for(uint32_t r = tid; r < cb.m; r += 256) ma = max(ma, o.ea[r]);
-        for(uint32_t c = tid; c < cb.n; c += 256) mb = max(mb, o.eb[c]);
-        ra[tid] = ma; rb[tid] = mb;
-        __syncthreads();
-        for(int w = 128; w > 0; w >>= 1) {
-            if(tid < w) {
-                ra[tid] = max(ra[tid], ra[tid + w]);
-                rb[tid] = max(rb[tid], rb[tid + w]);
+    for(uint32_t p0 = cb.pair_begin; p0 < cb.pair_end; p0 += 256) {
+        const uint32_t p = p0 + tid;
+        double term = 0.0;
+        if(p < cb.pair_end) {
+            const oz_pair o = ozp[p];
+            const int ma = opmax[o.ia], mb = opmax[o.ib];
+            if(ma > -1000000 && mb > -1000000) {             // rows and columns present
+                const bool nf = ma == kOzNonFinite || mb == kOzNonFinite;
+                term = nf ? __builtin_inf() : fabs(pairs[p].d) * double(pairs[p].k) *
+                    double(slices + 2) * ldexp(1.0, ma + mb - 7 * slices);
             }
-            __syncthreads();
         }
-        if(tid == 0 && ra[0] > -1000000 && rb[0] > -1000000) {
-            const bool nf = ra[0] == kOzNonFinite || rb[0] == kOzNonFinite;
-            bound += nf ? __builtin_inf() : fabs(pairs[p].d) * double(pairs[p].k) *
-                double(slices + 2) * ldexp(1.0, ra[0] + rb[0] - 7 * slices);
+        rt[tid] = term;
+        __syncthreads();
+        if(tid == 0) {
+            const uint32_t n = min(256u, cb.pair_end - p0);
+            for(uint32_t q = 0; q < n; q++) bound += rt[q];
         }
         __syncthreads();
     }
+    // max |C|: wave w rows w, w + 4, ...; lanes columns, four loads in flight
     double cm = 0.0;
-    const uint32_t total = cb.m * cb.n;
-    for(uint32_t x = tid; x < total; x += 256) {
-        const double v = fabs(cb.c[size_t(x / cb.n) * cb.ldc + (x % cb.n)]);
-        cm = (v > cm || v != v) ? v : cm;
+    for(uint32_t r = tid >> 6; r < cb.m; r += 4) {
+        const double *row = cb.c + size_t(r) * cb.ldc;
+        for(uint32_t x = tid & 63; x < cb.n; x += 256) {
+            double v[4];
+#pragma unroll
+            for(int u = 0; u < 4; u++) v[u] = x + 64 * u < cb.n ? fabs(row[x + 64 * u]) : 0.0;
+#pragma unroll
+            for(int u = 0; u < 4; u++) cm = (v[u] > cm || v[u] != v[u]) ? v[u] : cm;
+        }
     }
     rc[tid] = cm;
     __syncthreads();
@@ -279,10 +304,18 @@ oz_bound_kernel(const gemm_cblock *__restrict__ cbs, const gemm_pair *__restrict
     }
 }
 
+hipError_t launch_oz_opmax(const oz_split_desc *descs, uint32_t nops, int32_t *opmax,
+    hipStream_t s) {
+    if(nops == 0) return hipSuccess;
+    hipLaunchKernelGGL(oz_opmax_kernel, dim3((nops + 3) / 4), dim3(256), 0, s, descs, nops, opmax);
+    return hipGetLastError();
+}
+
 hipError_t launch_oz_bound(const gemm_cblock *cb, const gemm_pair *pairs, const oz_pair *ozp,
-    uint32_t ncb, int slices, double *out, hipStream_t s) {
+    const int32_t *opmax, uint32_t ncb, int slices, double *out, hipStream_t s) {
     if(ncb == 0) return hipSuccess;
-    hipLaunchKernelGGL(oz_bound_kernel, dim3(ncb), dim3(256), 0, s, cb, pairs, ozp, slices, out);
+    hipLaunchKernelGGL(oz_bound_kernel, dim3(ncb), dim3(256), 0, s, cb, pairs, ozp, opmax,
+        slices, out);
     return hipGetLastError();
 }
 

@@ -277,6 +277,8 @@ struct contract_plan {
     uint32_t n_oz_tiles;
     uint32_t n_oz_items_mut;          //!< split items of operands split every perform
     double *d_oz_bound;               //!< stated error per C block (bound, max |C|)
+    int32_t *d_oz_opmax;              //!< largest exponent per split operand
+    uint32_t n_oz_ops;
 
     contract_plan() : batch(false), va(0), vb(0), cuid(0), c(0), add(false), d(1.0), empty(false),
         ta(false), tb(false), tile_cfg(0), tile_order(0), d_prep(0), d_wback(0), d_prep_cd(0),
@@ -286,7 +288,8 @@ struct contract_plan {
         d_stacks(0), d_spairs(0), d_arows(0), d_crows(0), d_stiles(0), d_vtiles(0),
         n_prep_st_chunks(0), d_prep_st(0), d_pst_cd(0), d_pst_co(0),
         oz_slices(0), oz_tile(0), oz_tile_set(-1), d_oz_split(0), d_oz_items(0), n_oz_items(0), d_oz_pairs(0),
-        d_oz_tiles(0), n_oz_tiles(0), n_oz_items_mut(0), d_oz_bound(0) { }
+        d_oz_tiles(0), n_oz_tiles(0), n_oz_items_mut(0), d_oz_bound(0), d_oz_opmax(0),
+        n_oz_ops(0) { }
     ~contract_plan() {
         if(gexec) hipGraphExecDestroy(gexec);
         if(graph) hipGraphDestroy(graph);
@@ -1369,6 +1372,7 @@ void btod_contract2::build_ozaki(int slices) {
         o.astride = ((a.rows + 15) & ~15u) * a.kp; o.bstride = ((b.rows + 15) & ~15u) * b.kp;
         o.kp = a.kp;
         o.d = pl.pairs[p].d;
+        o.ia = (uint32_t)pa[p]; o.ib = (uint32_t)pb[p];
     }
     // C tiles of every product-kernel shape
     std::vector<dev::gemm_tile> tl[dev::OZ_TILE_COUNT];
@@ -1391,6 +1395,7 @@ void btod_contract2::build_ozaki(int slices) {
     size_t o_i = sz; sz = align256(sz + items.size() * 4);
     size_t o_p = sz; sz = align256(sz + ozp.size() * sizeof(dev::oz_pair));
     size_t o_bd = sz; sz = align256(sz + 2 * pl.cblocks.size() * sizeof(double));
+    size_t o_om = sz; sz = align256(sz + ops.size() * sizeof(int32_t));
     size_t o_t[dev::OZ_TILE_COUNT];
     for(int k = 0; k < dev::OZ_TILE_COUNT; k++) {
         o_t[k] = sz;
@@ -1411,6 +1416,8 @@ void btod_contract2::build_ozaki(int slices) {
     pl.n_oz_items = (uint32_t)(items.size() / 2);
     pl.d_oz_pairs = (dev::oz_pair*)(base + o_p);
     pl.d_oz_bound = (double*)(base + o_bd);
+    pl.d_oz_opmax = (int32_t*)(base + o_om);
+    pl.n_oz_ops = (uint32_t)ops.size();
     pl.n_oz_items_mut = n_mut;
     if(pl.n_oz_items > n_mut) {
         // operands of immutable tensors: split now, once for this plan
@@ -1490,9 +1497,11 @@ void btod_contract2::enqueue(int mode, hipStream_t s, bool timed, size_t &launch
             pl.oz_slices, pl.oz_tile, s), "gemm (ozaki)");
         launches += 2;
         if(timed) hip_check(hipEventRecord(pl.ev[2], s), "hipEventRecord");
-        hip_check(dev::launch_oz_bound(pl.d_cb, pl.d_pairs, pl.d_oz_pairs,
+        hip_check(dev::launch_oz_opmax(pl.d_oz_split, pl.n_oz_ops, pl.d_oz_opmax, s),
+            "exponents (ozaki)");
+        hip_check(dev::launch_oz_bound(pl.d_cb, pl.d_pairs, pl.d_oz_pairs, pl.d_oz_opmax,
             (uint32_t)pl.c_out.size(), pl.oz_slices, pl.d_oz_bound, s), "bound (ozaki)");
-        launches++;
+        launches += 2;
     } else if(mode == MODE_PER_BLOCK) {
         for(size_t b = 0; b + 1 < pl.block_tile_begin.size(); b++) {
             uint32_t t0 = pl.block_tile_begin[b], t1 = pl.block_tile_begin[b + 1];

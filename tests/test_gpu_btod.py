@@ -137,6 +137,27 @@ def test_kernel_configurations(gpu_ctx, name, tile):
         assert O.rel_err(c.to_dense(), result) < TOL
 
 
+def test_set_tile_applies_to_the_next_perform(gpu_ctx):
+    """A configuration set after a plan was built with another one takes
+    effect at the next perform (the stale plan is dropped); results agree."""
+    prob, dense, meta, result = G.load("ladder_spin")
+    a, b, op = P.build(gpu_ctx, prob, dense)
+    c = P.make_result_tensor(gpu_ctx, prob)
+    op.set_tile(lt.GEMM_STACKED_FIRST)
+    op.perform(c, lt.LT_MODE_GROUPED)
+    assert op.stats()["tile_cfg"] == lt.GEMM_STACKED_FIRST
+    plans = op.stats()["n_plans"]
+    op.set_tile(lt.GEMM_STACKED_FIRST + 3)
+    op.perform(c, lt.LT_MODE_GROUPED)
+    st = op.stats()
+    assert st["tile_cfg"] == lt.GEMM_STACKED_FIRST + 3
+    assert st["n_plans"] == plans + 1
+    assert O.rel_err(c.to_dense(), result) < TOL
+    op.set_tile(lt.GEMM_STACKED_FIRST + 3)          # unchanged: the plan stays
+    op.perform(c, lt.LT_MODE_GROUPED)
+    assert op.stats()["n_plans"] == plans + 1
+
+
 def test_autotuner_records_timings(gpu_ctx):
     prob, dense, meta, result = G.load("ladder_full")
     a, b, op = P.build(gpu_ctx, prob, dense)
