@@ -1428,15 +1428,16 @@ void btod_contract2::build_ozaki(int slices) {
 
     // Tile shape: each shape's product launch is timed once on this plan's
     // own work list (after one prepare and split) and the fastest kept.
-    // Launches that would accumulate into C are never run for this: then
-    // the shape reading the fewest operand bytes (tiles x chunks x
-    // fragments per chunk) is taken.
+    // Launches that would accumulate into C are never run for this, nor
+    // for the plans of a memory-bounded perform (planned before any device
+    // work): then the shape reading the fewest operand bytes (tiles x chunks
+    // x fragments per chunk) is taken.
     bool all_zero = true;
     for(size_t b = 0; b < pl.cblocks.size(); b++) all_zero = all_zero && pl.cblocks[b].zero;
     int best = dev::OZ_TILE_16x64;
     if(m_oz_tile >= 0) {
         best = m_oz_tile;
-    } else if(all_zero && pl.events && !pl.cblocks.empty()) {
+    } else if(all_zero && !pl.batch && pl.events && !pl.cblocks.empty()) {
         hipStream_t s = ctx.stream();
         ctx.activate();
         if(pl.n_prep_chunks)
@@ -1544,9 +1545,8 @@ void btod_contract2::run(btensor &c, int mode) {
         return;
     }
     if(mode == MODE_OZAKI) {
-        int sl = m_oz_slices > 0 ? m_oz_slices : 7;
-        if(pl.oz_slices != sl || pl.oz_tile_set != m_oz_tile) build_ozaki(sl);
-        m_stats.ozaki_slices = sl;
+        ensure_ozaki();
+        m_stats.ozaki_slices = pl.oz_slices;
     }
     // The grouped perform runs as a HIP graph (prepare permute, grouped
     // GEMM, write-back permute: kernel nodes only), captured once per plan.
@@ -1918,15 +1918,24 @@ size_t btod_contract2::footprint_estimate() {
     return n * sizeof(double) + (size_t(1) << 20);
 }
 
-bool btod_contract2::try_plan(btensor &c, bool add, double d, bool measure) {
+bool btod_contract2::try_plan(btensor &c, bool add, double d, bool measure, int mode) {
+    // the plan (and in split-precision mode its split form) within the
+    // context's memory budget, or false with nothing allocated
     try {
-        build_plan(c, add, d, measure);
+        if(!plan_valid(c, add, d, mode)) build_plan(c, add, d, measure);
+        if(mode == MODE_OZAKI) ensure_ozaki();
         return true;
     } catch(const lt_error &e) {
         if(e.code != LT_ERR_NOMEM) throw;
         m_plan.reset();           // frees what the plan took
         return false;
     }
+}
+
+void btod_contract2::ensure_ozaki() {
+    contract_plan &pl = *m_plan;
+    const int sl = m_oz_slices > 0 ? m_oz_slices : 7;
+    if(pl.oz_slices != sl || pl.oz_tile_set != m_oz_tile) build_ozaki(sl);
 }
 
 void btod_contract2::run_batched(btensor &c, bool add, double d, int mode) {
@@ -1948,7 +1957,7 @@ void btod_contract2::run_batched(btensor &c, bool add, double d, int mode) {
         bool ok = true;
         for(size_t k = 0; k < parts.size() && ok; k++) {
             m_sched_subset = &parts[k];
-            ok = try_plan(c, add, d, false);
+            ok = try_plan(c, add, d, false, mode);
             m_sched_subset = 0;
             m_plan.reset();
         }
@@ -1958,6 +1967,8 @@ void btod_contract2::run_batched(btensor &c, bool add, double d, int mode) {
         c.zero_all();
         c.set_symmetry(m_symc, m_uid);
     }
+    // split-precision bounds of every batch's C blocks
+    std::vector<oz_bound> bounds;
     for(size_t k = 0; k < parts.size(); k++) {
         m_sched_subset = &parts[k];
         try {
@@ -1972,8 +1983,19 @@ void btod_contract2::run_batched(btensor &c, bool add, double d, int mode) {
             m_plan.reset();
             throw;
         }
+        if(mode == MODE_OZAKI)
+            bounds.insert(bounds.end(), m_oz_bounds.begin(), m_oz_bounds.end());
         m_sched_subset = 0;
         m_plan.reset();
+    }
+    if(mode == MODE_OZAKI) {
+        m_oz_bounds = bounds;
+        double bmax = 0.0, cmax = 0.0;
+        for(size_t i = 0; i < bounds.size(); i++) {
+            bmax = std::max(bmax, bounds[i].bound);
+            cmax = std::max(cmax, bounds[i].cmax);
+        }
+        m_stats.ozaki_max_rel_err = cmax > 0.0 ? bmax / cmax : bmax;
     }
     m_stats.n_batches = parts.size();
 }
@@ -1983,7 +2005,10 @@ void btod_contract2::perform(btensor &c, int mode) {
     check_block_sizes();
     m_a.ctx().require_device("btod_contract2::perform");
     lt_require(c.bis() == m_symc.bis, LT_ERR_DIMS, "result tensor has a different block index space");
-    if(!plan_valid(c, false, 1.0, mode) && !try_plan(c, false, 1.0)) {
+    // (no timed kernel choice for a split-precision perform: its products
+    // run in their own kernels, and nothing is launched before it is known
+    // whether the plan fits)
+    if(!try_plan(c, false, 1.0, mode != MODE_OZAKI, mode)) {
         run_batched(c, false, 1.0, mode);
         return;
     }
@@ -2018,7 +2043,7 @@ void btod_contract2::perform_add(btensor &c, double d, int mode) {
         add_into(c, *m_addtmp, id, d);
         return;
     }
-    if(!plan_valid(c, true, d, mode) && !try_plan(c, true, d)) {
+    if(!try_plan(c, true, d, mode != MODE_OZAKI, mode)) {
         run_batched(c, true, d, mode);
         return;
     }

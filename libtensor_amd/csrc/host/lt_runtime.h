@@ -434,7 +434,10 @@ private:
 
     //! Builds the plan; false (and no plan) if it does not fit the device
     //! memory budget.
-    bool try_plan(btensor &c, bool add, double d, bool measure = true);
+    bool try_plan(btensor &c, bool add, double d, bool measure = true, int mode = 1);
+    //! The split-precision form of the plan for the current slice count and
+    //! tile setting (built when missing).
+    void ensure_ozaki();
     //! Perform in batches of C blocks whose plans fit the budget.
     void run_batched(btensor &c, bool add, double d, int mode);
 

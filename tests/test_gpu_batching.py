@@ -106,3 +106,50 @@ def test_pp_ladder_in_batches():
         assert nb >= 4
     finally:
         ctx.close()
+
+
+@pytest.mark.parametrize("name", ["ladder_spin", "ring_small"])
+def test_split_precision_in_batches(name):
+    """A split-precision perform under a budget its plan and slices do not fit
+    runs in batches of C blocks: the same bits as the one-plan perform (every
+    C element gets the same exact slice products and the same FP64
+    recombination), and the stated bounds of every batch's C blocks."""
+    _say("batching, split precision", name)
+    ctx = lt.Context(0)
+    cap0 = ctx.mem()[1]
+    try:
+        prob, dense, meta, result = G.load(name)
+        a, b, op = P.build(ctx, prob, dense)
+        op.set_ozaki(7)
+        c = P.make_result_tensor(ctx, prob)
+        op.perform(c, lt.LT_MODE_OZAKI)
+        ref = c.to_dense()
+        full = sorted(op.ozaki_bounds())
+        stated = op.stats()["ozaki_max_rel_err"]
+        assert O.rel_err(ref, result) <= stated
+        used_plan = ctx.mem()[0]
+        op.close()
+        used = ctx.mem()[0]
+        plan_bytes = used_plan - used
+        op2 = P.build_op(ctx, prob, a, b)
+        op2.set_ozaki(7)
+        for f in FRACS:
+            ctx.set_cap(used + int(plan_bytes * f))
+            c.zero()
+            try:
+                op2.perform(c, lt.LT_MODE_OZAKI)
+            except lt.LtError as e:
+                assert e.code == -5, e
+                assert np.all(c.to_dense() == 0.0)     # refused before device work
+                continue
+            nb = op2.stats()["n_batches"]
+            assert nb > 1, nb
+            assert np.array_equal(c.to_dense(), ref)
+            assert sorted(op2.ozaki_bounds()) == full
+            assert op2.stats()["ozaki_max_rel_err"] == stated
+            _say("  budget %.2f of the plan: %d batches" % (f, nb))
+            return
+        pytest.fail("no budget fits a batch")
+    finally:
+        ctx.set_cap(cap0)
+        ctx.close()
