@@ -290,6 +290,15 @@ int lt_btod_contract2_set_tile(lt_btod_contract2 *op, int cfg);
  * No counterpart in the reference (gen_bto_contract2_batch_impl.h:273-276
  * runs the blocks as thread-pool tasks). */
 int lt_btod_contract2_set_graph(lt_btod_contract2 *op, int on);
+/* FP64 performs of this operation return once their work is enqueued on the
+ * context's stream (on != 0) or when it is done (on == 0, the default).
+ * Every later operation on the context is ordered after an enqueued perform:
+ * performs, block operations, tensor uploads (they wait for it) and reads of
+ * results (they wait for it and then copy); device times in lt_stats wait
+ * for it when read.  Split-precision performs always wait (they read their
+ * stated bounds).  No counterpart in the reference (its perform runs the
+ * contraction on the calling threads, gen_bto_contract2_impl.h:42-308). */
+int lt_btod_contract2_set_async(lt_btod_contract2 *op, int on);
 /* Split-precision mode (LT_MODE_OZAKI): number of int8 slices per operand,
  * 1..8 (0 = 7).  No counterpart in the reference (all of its arithmetic is
  * FP64 BLAS, linalg_cblas_level3.C:10-60). */

@@ -432,6 +432,11 @@ class btod_contract2(object):
     def set_graph(self, on):
         call("lt_btod_contract2_set_graph", self._h, 1 if on else 0)
 
+    def set_async(self, on):
+        """FP64 performs return once enqueued on the context's stream (later
+        operations and reads on the context are ordered after them)."""
+        call("lt_btod_contract2_set_async", self._h, 1 if on else 0)
+
     def set_ozaki(self, slices):
         call("lt_btod_contract2_set_ozaki", self._h, int(slices))
 

@@ -141,6 +141,7 @@ _SIGS = {
     "lt_btod_contract2_stats": [vp, ctypes.POINTER(LtStats)],
     "lt_btod_contract2_set_tile": [vp, ctypes.c_int],
     "lt_btod_contract2_set_graph": [vp, ctypes.c_int],
+"lt_btod_contract2_set_async": [vp, ctypes.c_int],
     "lt_btod_contract2_set_ozaki": [vp, ctypes.c_int],
     "lt_btod_contract2_set_ozaki_tile": [vp, ctypes.c_int],
     "lt_btod_contract3_create": [ctypes.c_size_t] * 5 + [c_size_p] * 6 + [vp, vp, vp,

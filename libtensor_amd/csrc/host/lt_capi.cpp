@@ -1344,6 +1344,13 @@ int lt_btod_contract2_set_graph(lt_btod_contract2 *op, int on) {
     });
 }
 
+int lt_btod_contract2_set_async(lt_btod_contract2 *op, int on) {
+    return guard([&]() {
+        need(op, "op");
+        op->op->set_async(on != 0);
+    });
+}
+
 int lt_btod_contract2_set_ozaki(lt_btod_contract2 *op, int slices) {
     return guard([&]() {
         need(op, "op");

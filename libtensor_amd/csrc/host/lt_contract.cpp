@@ -291,6 +291,11 @@ struct contract_plan {
         d_oz_tiles(0), n_oz_tiles(0), n_oz_items_mut(0), d_oz_bound(0), d_oz_opmax(0),
         n_oz_ops(0) { }
     ~contract_plan() {
+        // an asynchronous perform of this plan may still run
+        if(ctx && ctx->is_device()) {
+            ctx->activate();
+            (void)hipStreamSynchronize(ctx->stream());
+        }
         if(gexec) hipGraphExecDestroy(gexec);
         if(graph) hipGraphDestroy(graph);
         if(ctx && scratch) ctx->free(scratch);
@@ -320,7 +325,8 @@ uint64_t next_op_uid() {
 btod_contract2::btod_contract2(const contraction2 &c, btensor &a, double ka,
     btensor &b, double kb, double kc) :
     m_contr(c), m_a(a), m_b(b), m_ka(ka), m_kb(kb), m_kc(kc), m_uid(next_op_uid()),
-    m_tile_cfg(-2), m_tuned_cfg(-1), m_tuned_order(-1), m_tuned_va(0), m_tuned_vb(0), m_use_graph(true), m_oz_slices(0), m_oz_tile(-1),
+    m_tile_cfg(-2), m_tuned_cfg(-1), m_tuned_order(-1), m_tuned_va(0), m_tuned_vb(0), m_use_graph(true), m_async(false), m_times_pending(false), m_times_graph(false),
+    m_oz_slices(0), m_oz_tile(-1),
     m_add_uid(0), m_add_ver(0), m_add_direct(false), m_sched_subset(0),
     m_use_graph_saved(true) {
 
@@ -350,8 +356,28 @@ void btod_contract2::clst(const index_t &ic, contr_list &out) {
 }
 
 void btod_contract2::set_tile_cfg(int cfg) {
-    if(cfg != m_tile_cfg) m_plan.reset();
+    if(cfg != m_tile_cfg) drop_plan();
     m_tile_cfg = cfg;
+}
+
+void btod_contract2::drop_plan() {
+    m_times_pending = false;
+    m_plan.reset();
+}
+
+void btod_contract2::finish_times() {
+    if(!m_times_pending) return;
+    m_times_pending = false;
+    if(!m_plan) return;
+    contract_plan &pl = *m_plan;
+    m_a.ctx().activate();
+    hip_check(hipEventSynchronize(pl.ev[3]), "hipEventSynchronize");
+    float ms = 0.f, msg = 0.f;
+    hip_check(hipEventElapsedTime(&ms, pl.ev[0], pl.ev[3]), "hipEventElapsedTime");
+    if(!m_times_graph)
+        hip_check(hipEventElapsedTime(&msg, pl.ev[1], pl.ev[2]), "hipEventElapsedTime");
+    m_stats.t_gpu_ms = ms;
+    m_stats.t_gemm_ms = msg;
 }
 
 bool btod_contract2::plan_valid(btensor &c, bool add, double d, int) const {
@@ -364,6 +390,7 @@ bool btod_contract2::plan_valid(btensor &c, bool add, double d, int) const {
 // ---------------------------------------------------------------------------
 
 void btod_contract2::build_plan(btensor &c, bool add, double dscale, bool measure) {
+    m_times_pending = false;
     double t0 = now_ms();
     m_stats.n_plans++;
     context &ctx = m_a.ctx();
@@ -1094,7 +1121,7 @@ void btod_contract2::build_plan(btensor &c, bool add, double dscale, bool measur
         }
         put(o_sp, pl.spairs.data(), pl.spairs.size() * sizeof(dev::stack_pair));
     }
-    ctx.upload(pl.lists.mem, host.data(), sz);
+    ctx.upload_new(pl.lists.mem, host.data(), sz);
     char *base = (char*)pl.lists.mem;
     pl.d_prep = (dev::perm_desc*)(base + o_prep);
     pl.d_wback = (dev::perm_desc*)(base + o_wb);
@@ -1124,7 +1151,7 @@ void btod_contract2::build_plan(btensor &c, bool add, double dscale, bool measur
         std::vector<char> th(tsz, 0);
         if(nt) std::memcpy(&th[0], pl.tiles_sorted.data(), nt);
         if(nb) std::memcpy(&th[align256(nt)], pl.tiles_by_block.data(), nb);
-        ctx.upload(pl.tiles.mem, th.data(), tsz);
+        ctx.upload_new(pl.tiles.mem, th.data(), tsz);
         pl.d_tiles_sorted = (dev::gemm_tile*)pl.tiles.mem;
         pl.d_tiles_block = (dev::gemm_tile*)((char*)pl.tiles.mem + align256(nt));
         pl.tile_cfg = k;
@@ -1157,7 +1184,7 @@ void btod_contract2::build_plan(btensor &c, bool add, double dscale, bool measur
             for(size_t i = 0; i < vt.size(); i++) pl.vtiles.push_back(vt[i].second);
             size_t nb = pl.vtiles.size() * sizeof(dev::var_tile);
             pl.stk_tiles.reset(ctx, nb ? nb : 256);
-            if(nb) ctx.upload(pl.stk_tiles.mem, pl.vtiles.data(), nb);
+            if(nb) ctx.upload_new(pl.stk_tiles.mem, pl.vtiles.data(), nb);
             pl.d_vtiles = (dev::var_tile*)pl.stk_tiles.mem;
         } else if(dev::is_stacked(k)) {
             // stack tiles, longest (largest k sum) first
@@ -1211,7 +1238,7 @@ void btod_contract2::build_plan(btensor &c, bool add, double dscale, bool measur
             }
             size_t nb = pl.stiles.size() * sizeof(dev::gemm_tile);
             pl.stk_tiles.reset(ctx, nb ? nb : 256);
-            if(nb) ctx.upload(pl.stk_tiles.mem, pl.stiles.data(), nb);
+            if(nb) ctx.upload_new(pl.stk_tiles.mem, pl.stiles.data(), nb);
             pl.d_stiles = (dev::gemm_tile*)pl.stk_tiles.mem;
         }
     };
@@ -1409,7 +1436,7 @@ void btod_contract2::build_ozaki(int slices) {
         if(!tl[k].empty())
             std::memcpy(&host[o_t[k]], tl[k].data(), tl[k].size() * sizeof(dev::gemm_tile));
     pl.oz_lists.reset(ctx, host.size());
-    ctx.upload(pl.oz_lists.mem, host.data(), host.size());
+    ctx.upload_new(pl.oz_lists.mem, host.data(), host.size());
     char *base = (char*)pl.oz_lists.mem;
     pl.d_oz_split = (dev::oz_split_desc*)(base + o_d);
     pl.d_oz_items = (uint32_t*)(base + o_i);
@@ -1585,15 +1612,13 @@ void btod_contract2::run(btensor &c, int mode) {
         enqueue(mode, s, true, launches);
     }
     hip_check(hipEventRecord(pl.ev[3], s), "hipEventRecord");
-    hip_check(hipStreamSynchronize(s), "hipStreamSynchronize");
-    float ms = 0.f, msg = 0.f;
-    hip_check(hipEventElapsedTime(&ms, pl.ev[0], pl.ev[3]), "hipEventElapsedTime");
-    if(!graph)
-        hip_check(hipEventElapsedTime(&msg, pl.ev[1], pl.ev[2]), "hipEventElapsedTime");
-    m_stats.t_gpu_ms = ms;
-    m_stats.t_gemm_ms = msg;
     m_stats.n_launches = launches;
     m_stats.graph = graph ? 1 : 0;
+    m_times_pending = true;
+    m_times_graph = graph;
+    // an asynchronous perform returns here (its times are read when asked
+    // for); a split-precision perform reads its stated bounds, so it waits
+    if(!m_async || mode == MODE_OZAKI) finish_times();
     if(mode == MODE_OZAKI) m_oz_bounds.clear();
     if(mode == MODE_OZAKI && !pl.c_out.empty()) {
         // the stated error of this perform: per C block the bound and the
@@ -1927,7 +1952,7 @@ bool btod_contract2::try_plan(btensor &c, bool add, double d, bool measure, int 
         return true;
     } catch(const lt_error &e) {
         if(e.code != LT_ERR_NOMEM) throw;
-        m_plan.reset();           // frees what the plan took
+        drop_plan();           // frees what the plan took
         return false;
     }
 }
@@ -1959,7 +1984,7 @@ void btod_contract2::run_batched(btensor &c, bool add, double d, int mode) {
             m_sched_subset = &parts[k];
             ok = try_plan(c, add, d, false, mode);
             m_sched_subset = 0;
-            m_plan.reset();
+            drop_plan();
         }
         if(ok) break;
     }
@@ -1980,13 +2005,13 @@ void btod_contract2::run_batched(btensor &c, bool add, double d, int mode) {
         } catch(...) {
             m_sched_subset = 0;
             m_use_graph = m_use_graph_saved;
-            m_plan.reset();
+            drop_plan();
             throw;
         }
         if(mode == MODE_OZAKI)
             bounds.insert(bounds.end(), m_oz_bounds.begin(), m_oz_bounds.end());
         m_sched_subset = 0;
-        m_plan.reset();
+        drop_plan();
     }
     if(mode == MODE_OZAKI) {
         m_oz_bounds = bounds;

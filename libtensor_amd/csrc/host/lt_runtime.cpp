@@ -18,7 +18,8 @@ void hip_check(hipError_t e, const char *what) {
 // ---------------------------------------------------------------------------
 
 context::context(int device, size_t hbm_cap) :
-    m_device(device), m_stream(0), m_xfer(0), m_cap(hbm_cap), m_used(0), m_nthreads(1) {
+    m_device(device), m_stream(0), m_xfer(0), m_xfer_ev(0), m_cap(hbm_cap), m_used(0),
+    m_nthreads(1) {
 
     unsigned hc = std::thread::hardware_concurrency();
     m_nthreads = hc ? (hc > 16 ? 16 : hc) : 1;
@@ -37,6 +38,7 @@ context::context(int device, size_t hbm_cap) :
     // stream by another thread; every upload completes before the host
     // launches anything that reads it
     hip_check(hipStreamCreateWithFlags(&m_xfer, hipStreamNonBlocking), "hipStreamCreate");
+    hip_check(hipEventCreateWithFlags(&m_xfer_ev, hipEventDisableTiming), "hipEventCreate");
     if(m_cap == 0) {
         size_t fr = 0, tot = 0;
         hip_check(hipMemGetInfo(&fr, &tot), "hipMemGetInfo");
@@ -62,6 +64,7 @@ context::~context() {
     m_launch_cache.clear();      // their memory goes with the loop below
     for(std::map<void*, size_t>::iterator i = m_sizes.begin(); i != m_sizes.end(); ++i)
         hipFree(i->first);
+    if(m_xfer_ev) hipEventDestroy(m_xfer_ev);
     if(m_stream) hipStreamDestroy(m_stream);
     if(m_xfer) hipStreamDestroy(m_xfer);
 }
@@ -147,6 +150,17 @@ void context::free(void *p) {
 }
 
 void context::upload(void *dst, const void *src, size_t bytes) {
+    if(m_device < 0) { std::memcpy(dst, src, bytes); return; }
+    activate();
+    std::lock_guard<std::mutex> lk(m_xfer_mtx);
+    hip_check(hipEventRecord(m_xfer_ev, m_stream), "hipEventRecord");
+    hip_check(hipStreamWaitEvent(m_xfer, m_xfer_ev, 0), "hipStreamWaitEvent");
+    hip_check(hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, m_xfer),
+        "hipMemcpy H2D");
+    hip_check(hipStreamSynchronize(m_xfer), "hipStreamSynchronize");
+}
+
+void context::upload_new(void *dst, const void *src, size_t bytes) {
     if(m_device < 0) { std::memcpy(dst, src, bytes); return; }
     activate();
     hip_check(hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, m_xfer),

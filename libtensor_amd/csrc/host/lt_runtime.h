@@ -44,7 +44,16 @@ public:
     //! LT_ERR_NOMEM; memory in use is not released).
     void set_cap(size_t cap) { std::lock_guard<std::mutex> lk(m_mtx); m_cap = cap; }
 
+    //! Host-to-device copy, ordered after every operation already enqueued
+    //! on the compute stream (which may still read or write dst), complete
+    //! on return.
     void upload(void *dst, const void *src, size_t bytes);
+    //! Host-to-device copy into memory no enqueued operation uses (allocated
+    //! by the caller since): not ordered after the compute stream, so a
+    //! host thread that plans does not wait for another thread's kernels.
+    void upload_new(void *dst, const void *src, size_t bytes);
+    //! Device-to-host copy on the compute stream (after everything enqueued
+    //! there), complete on return.
     void download(void *dst, const void *src, size_t bytes);
     void sync();
     //! Makes this context's device current on the calling thread.
@@ -86,6 +95,8 @@ private:
     int m_device;
     hipStream_t m_stream;
     hipStream_t m_xfer;   //!< host-to-device copies (default priority)
+    hipEvent_t m_xfer_ev; //!< orders ordered uploads after the compute stream
+    std::mutex m_xfer_mtx;
     std::mutex m_cache_mtx;
     std::map<std::string, std::shared_ptr<cached_launch> > m_launch_cache;
     std::vector<std::string> m_launch_order;
@@ -393,7 +404,9 @@ public:
     //! and B block plus the scheduled C blocks (admission estimate).
     size_t footprint_estimate();
 
-    const exec_stats &stats() const { return m_stats; }
+    //! Statistics of the last perform (device times wait for an
+    //! asynchronous perform to end).
+    const exec_stats &stats() { finish_times(); return m_stats; }
     //! Stated error of the last split-precision perform, per C block.
     struct oz_bound { size_t aidx; double bound, cmax; };
     const std::vector<oz_bound> &ozaki_bounds() const { return m_oz_bounds; }
@@ -402,6 +415,10 @@ public:
     void set_tile_cfg(int cfg);
     //! Grouped performs run as a captured HIP graph (default) or eagerly.
     void set_graph(bool on) { m_use_graph = on; }
+    //! on: FP64 performs return once their work is enqueued on the context's
+    //! stream (later operations on the context are ordered after it; reads
+    //! of the result synchronize); off (default): they return when done.
+    void set_async(bool on) { m_async = on; }
     void set_ozaki(int slices) { m_oz_slices = slices; }
     //! split-precision tile shape (dev::oz_tile_cfg), -1 = measured per plan
     void set_ozaki_tile(int tile) { m_oz_tile = tile; }
@@ -420,6 +437,13 @@ private:
     int m_tuned_order;      //!< and its grouped-launch tile order
     uint64_t m_tuned_va, m_tuned_vb;   //!< operand versions it was found for
     bool m_use_graph;
+    bool m_async;
+    //! an asynchronous perform's device times are read from the plan's
+    //! events when the statistics are asked for
+    bool m_times_pending, m_times_graph;
+    void finish_times();
+    //! Drops the plan (its device memory; its pending times are lost).
+    void drop_plan();
     int m_oz_slices, m_oz_tile;
     std::unique_ptr<contract_plan> m_plan;
     std::vector<oz_bound> m_oz_bounds;
