@@ -24,6 +24,7 @@
 
 #include <map>
 #include <tuple>
+#include <type_traits>
 
 #include <chrono>
 #include <cmath>
@@ -302,6 +303,19 @@ struct contract_plan {
         if(events) for(int i = 0; i < 4; i++) hipEventDestroy(ev[i]);
     }
 };
+
+//! The grouped GEMM launch of configuration k over a tile list of its own
+//! (n tiles at d_list: the autotuner's sample of a large plan).
+hipError_t launch_grouped_list(const contract_plan &pl, int k, const void *d_list, uint32_t n,
+    hipStream_t s) {
+    if(dev::is_var(k))
+        return dev::launch_gemm_stacked_var(k, pl.d_stacks, pl.d_spairs, pl.d_arows, pl.d_crows,
+            (const dev::var_tile*)d_list, n, s);
+    if(dev::is_stacked(k))
+        return dev::launch_gemm_stacked(k, pl.d_stacks, pl.d_spairs, pl.d_arows, pl.d_crows,
+            (const dev::gemm_tile*)d_list, n, s);
+    return dev::launch_gemm(k, pl.d_cb, pl.d_pairs, (const dev::gemm_tile*)d_list, n, n, s);
+}
 
 //! The grouped GEMM launch of a plan in configuration k.
 hipError_t launch_grouped(const contract_plan &pl, int k, hipStream_t s) {
@@ -1267,23 +1281,65 @@ void btod_contract2::build_plan(btensor &c, bool add, double dscale, bool measur
             return pl.n_prep_chunks ? dev::launch_permute(pl.d_prep, pl.d_prep_cd,
                 pl.d_prep_co, pl.n_prep_chunks, s) : hipSuccess;
         };
+        // A large plan is timed on a sample of each candidate's launch list:
+        // every stride-th group of nx consecutive tiles (a group keeps its
+        // XCD placement), about kTuneFlops of work.  The prepare permute is
+        // timed whole, the sampled launch scaled by the list's cost over the
+        // sample's (cost of a tile: the k sum of its block or stack times its
+        // fragment count).
+        const double kTuneFlops = 2e11;
+        const size_t stride = flops > 2.0 * kTuneFlops ? size_t(flops / kTuneFlops) : 1;
+        dev_lists tune_list;
+        auto take = [&](const auto &list, auto cost, double &scale) -> uint32_t {
+            typedef typename std::decay<decltype(list[0])>::type tile_t;
+            std::vector<tile_t> sub;
+            double tot = 0.0, part = 0.0;
+            for(size_t i = 0; i < list.size(); i++) {
+                const double c = cost(list[i]);
+                tot += c;
+                if((i / nx) % stride == 0) { sub.push_back(list[i]); part += c; }
+            }
+            scale = part > 0.0 ? tot / part : 1.0;
+            const size_t nb = sub.size() * sizeof(tile_t);
+            tune_list.reset(ctx, nb ? nb : 256);
+            if(nb) ctx.upload_new(tune_list.mem, sub.data(), nb);
+            return (uint32_t)sub.size();
+        };
+        auto ksum_of = [](const std::vector<double> &ks, uint32_t b) {
+            return b < ks.size() ? ks[b] : 0.0; };
+        auto sample = [&](int k, double &scale) -> uint32_t {
+            if(dev::is_var(k))
+                return take(pl.vtiles, [&](const dev::var_tile &t) {
+                    return ksum_of(pl.stack_ksum, t.stack) * t.fm * t.fn; }, scale);
+            if(dev::is_stacked(k))
+                return take(pl.stiles, [&](const dev::gemm_tile &t) {
+                    return ksum_of(pl.stack_ksum, t.cblock); }, scale);
+            return take(pl.tiles_sorted, [&](const dev::gemm_tile &t) {
+                return ksum_of(cb_ksum, t.cblock); }, scale);
+        };
         float best = 1e30f;
         int bestk = cfg, besto = 0;
         for(size_t ci = 0; ci < 2 * candidates.size(); ci++) {
             int k = candidates[ci / 2], o = int(ci % 2);
             upload_tiles(k, o);
-            uint32_t nt = (uint32_t)pl.tiles_sorted.size();
+            double scale = 1.0;
+            const uint32_t ntune = stride > 1 ? sample(k, scale) : 0;
             // one timed launch; a short one (first-launch overheads are
             // then not negligible) is timed again
             float ms = 0.f;
             for(int rep = 0; rep < 2; rep++) {
+                float mp = 0.f, mg = 0.f;
                 hip_check(hipEventRecord(pl.ev[0], s), "hipEventRecord");
                 hip_check(prep_for(k), "autotune (prepare)");
-                hip_check(launch_grouped(pl, k, s), "autotune");
                 hip_check(hipEventRecord(pl.ev[1], s), "hipEventRecord");
+                hip_check(stride > 1 ? launch_grouped_list(pl, k, tune_list.mem, ntune, s) :
+                    launch_grouped(pl, k, s), "autotune");
+                hip_check(hipEventRecord(pl.ev[2], s), "hipEventRecord");
                 hip_check(hipStreamSynchronize(s), "hipStreamSynchronize");
-                hip_check(hipEventElapsedTime(&ms, pl.ev[0], pl.ev[1]), "hipEventElapsedTime");
-                if(ms > 20.f) break;
+                hip_check(hipEventElapsedTime(&mp, pl.ev[0], pl.ev[1]), "hipEventElapsedTime");
+                hip_check(hipEventElapsedTime(&mg, pl.ev[1], pl.ev[2]), "hipEventElapsedTime");
+                ms = mp + float(mg * scale);
+                if(mp + mg > 20.f) break;
             }
             if(o == 0 || ms < m_stats.tune_ms[k]) m_stats.tune_ms[k] = ms;
             if(ms < best) { best = ms; bestk = k; besto = o; }
