@@ -160,15 +160,17 @@ void server::planner_loop() {
     }
 }
 
-// Executor.  Ready requests below kLargeFlops form batches of up to
-// max_batch (one grouped launch); a larger request runs as a batch of its
-// own whose GEMM tiles are launched in slices of about kSliceFlops, and
-// between two slices every request that became ready meanwhile runs as a
-// complete batch (its prepare, products and write-back), so a small request
-// does not wait for a large one to end.  A request that fails its checks
-// fails alone; the others of its batch complete.
+// Executor.  Ready requests of at most kSmallFlops form batches of up to
+// max_batch (one grouped launch); a medium request (up to kLargeFlops, a
+// few milliseconds) runs as a batch of its own; a large one runs alone with
+// its GEMM tiles launched in slices of about kSliceFlops.  Between two
+// slices the small requests that became ready meanwhile run as one complete
+// batch (prepare, products and write-back), then the oldest ready medium
+// request, so neither waits for a large request to end.  A request that
+// fails its checks fails alone; the others of its batch complete.
 namespace {
-const double kLargeFlops = 2e10;
+const double kSmallFlops = 2e10;
+const double kLargeFlops = 2e11;
 const double kSliceFlops = 4e10;
 }
 
@@ -179,7 +181,7 @@ bool server::take_small(std::vector<uint64_t> &ids, std::vector<batch_item> &ite
     for(std::deque<uint64_t>::iterator i = m_ready.begin();
         i != m_ready.end() && ids.size() < m_max_batch;) {
         request &r = m_reqs[*i];
-        const bool large = r.op->flops() > kLargeFlops;
+        const bool large = r.op->flops() > kSmallFlops;     // medium or large: alone
         if(large && (small_only || !ids.empty())) { ++i; continue; }
         r.state = RUNNING;
         r.t_start = t;
@@ -214,27 +216,49 @@ void server::complete(const std::vector<uint64_t> &ids, const std::vector<batch_
     m_cv.notify_all();
 }
 
+bool server::take_medium(std::vector<uint64_t> &ids, std::vector<batch_item> &items) {
+    // called with m_mtx held
+    for(std::deque<uint64_t>::iterator i = m_ready.begin(); i != m_ready.end(); ++i) {
+        request &r = m_reqs[*i];
+        const double f = r.op->flops();
+        if(f <= kSmallFlops || f > kLargeFlops) continue;
+        r.state = RUNNING;
+        r.t_start = now_ms();
+        ids.push_back(*i);
+        batch_item it;
+        it.op = r.op; it.c = r.c; it.add = r.add; it.d = r.d;
+        items.push_back(it);
+        m_ready.erase(i);
+        return true;
+    }
+    return false;
+}
+
 void server::run_small(dev_buffer &side) {
-    std::vector<uint64_t> sids;
-    std::vector<batch_item> sitems;
-    {
-        std::lock_guard<std::mutex> lk(m_mtx);
-        take_small(sids, sitems, true);
+    // the ready small requests as one batch, then the oldest ready medium one
+    for(int pass = 0; pass < 2; pass++) {
+        std::vector<uint64_t> sids;
+        std::vector<batch_item> sitems;
+        {
+            std::lock_guard<std::mutex> lk(m_mtx);
+            if(pass == 0) take_small(sids, sitems, true);
+            else take_medium(sids, sitems);
+        }
+        if(sids.empty()) continue;
+        int sst = LT_OK;
+        std::string smsg;
+        try {
+            batch_run srun(sitems, side);
+            srun.finish();
+        } catch(const lt_error &e) {
+            sst = e.code;
+            smsg = e.what();
+        } catch(const std::exception &e) {
+            sst = LT_ERR_INTERNAL;
+            smsg = e.what();
+        }
+        complete(sids, sitems, sst, smsg);
     }
-    if(sids.empty()) return;
-    int sst = LT_OK;
-    std::string smsg;
-    try {
-        batch_run srun(sitems, side);
-        srun.finish();
-    } catch(const lt_error &e) {
-        sst = e.code;
-        smsg = e.what();
-    } catch(const std::exception &e) {
-        sst = LT_ERR_INTERNAL;
-        smsg = e.what();
-    }
-    complete(sids, sitems, sst, smsg);
 }
 
 void server::run_large(batch_item &it, dev_buffer &side) {

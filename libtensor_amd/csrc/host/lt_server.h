@@ -73,12 +73,17 @@ private:
     void planner_loop();
     void executor_loop();
     //! Moves ready requests into a batch (small ones up to max_batch, or
-    //! one large request alone; small_only: small ones only).  m_mtx held.
+    //! one medium or large request alone; small_only: small ones only).
+    //! m_mtx held.
     bool take_small(std::vector<uint64_t> &ids, std::vector<batch_item> &items,
         bool small_only);
     void run_batch(std::vector<uint64_t> &ids, std::vector<batch_item> &items,
         dev_buffer &merged, dev_buffer &side);
-    //! The ready small requests as one batch (between two slices).
+    //! The oldest ready medium request (between the small and the large
+    //! ones in size), alone.  m_mtx held.
+    bool take_medium(std::vector<uint64_t> &ids, std::vector<batch_item> &items);
+    //! The ready small requests as one batch, then the oldest ready medium
+    //! request (between two slices of a large one).
     void run_small(dev_buffer &side);
     //! A large request in slices, small batches between the slices.
     void run_large(batch_item &it, dev_buffer &side);
