@@ -249,6 +249,11 @@ struct contract_plan {
     dev::stack_crow *d_crows;
     dev::gemm_tile *d_stiles;
     dev::var_tile *d_vtiles;
+    // C blocks reading the same B operands (same order and layouts) share a
+    // group; n_xcd: XCDs of the device
+    std::vector<size_t> cb_group;
+    size_t n_groups;
+    int n_xcd;
     dev::perm_desc *d_prep, *d_wback;
     uint32_t *d_prep_cd, *d_prep_co, *d_wb_cd, *d_wb_co;
     uint32_t n_prep_chunks, n_wb_chunks;
@@ -287,6 +292,7 @@ struct contract_plan {
         d_cb(0), d_pairs(0), d_tiles_sorted(0), d_tiles_block(0), scratch(0),
         scratch_elems(0), ctx(0), events(false), graph(0), gexec(0), graph_launches(0),
         d_stacks(0), d_spairs(0), d_arows(0), d_crows(0), d_stiles(0), d_vtiles(0),
+        n_groups(0), n_xcd(8),
         n_prep_st_chunks(0), d_prep_st(0), d_pst_cd(0), d_pst_co(0),
         oz_slices(0), oz_tile(0), oz_tile_set(-1), d_oz_split(0), d_oz_items(0), n_oz_items(0), d_oz_pairs(0),
         d_oz_tiles(0), n_oz_tiles(0), n_oz_items_mut(0), d_oz_bound(0), d_oz_opmax(0),
@@ -953,6 +959,9 @@ void btod_contract2::build_plan(btensor &c, bool add, double dscale, bool measur
             cb_group[b] = i->second;
         }
     }
+    pl.cb_group = cb_group;
+    pl.n_groups = ngroups;
+    pl.n_xcd = nx;
     const uint32_t null_cb = (uint32_t)pl.cblocks.size();
     {
         dev::gemm_cblock z;
@@ -1457,19 +1466,83 @@ void btod_contract2::build_ozaki(int slices) {
         o.d = pl.pairs[p].d;
         o.ia = (uint32_t)pa[p]; o.ib = (uint32_t)pb[p];
     }
-    // C tiles of every product-kernel shape
+    // C tiles of every product-kernel shape.  A workgroup runs four tiles
+    // (one per wave) and workgroups are dealt to the XCDs round-robin, each
+    // XCD with its own L2.  The C blocks of a group (same B operands) go to
+    // one XCD, groups longest first onto the least loaded XCD; a group's
+    // tiles run column tile by column tile, so the row tiles that read the
+    // same B slice panels run together.  Workgroup w takes four tiles of XCD
+    // w % nx's queue, or of the longest remaining queue once its own is
+    // empty.  The order changes no result: each tile is one wave's sums.
     std::vector<dev::gemm_tile> tl[dev::OZ_TILE_COUNT];
-    for(int k = 0; k < dev::OZ_TILE_COUNT; k++) {
-        uint32_t bm, bn;
-        dev::oz_tile_shape(k, bm, bn);
-        for(size_t b = 0; b < pl.cblocks.size(); b++) {
-            const dev::gemm_cblock &cb = pl.cblocks[b];
-            uint32_t tm = (cb.m + bm - 1) / bm, tn = (cb.n + bn - 1) / bn;
-            lt_require(tm < 65536 && tn < 65536, LT_ERR_PARAM, "C block too large");
-            for(uint32_t i = 0; i < tm; i++) for(uint32_t j = 0; j < tn; j++) {
-                dev::gemm_tile t;
-                t.cblock = (uint32_t)b; t.tm = (uint16_t)i; t.tn = (uint16_t)j;
-                tl[k].push_back(t);
+    {
+        const size_t nb = pl.cblocks.size();
+        std::vector<size_t> grp(nb);
+        size_t ng = pl.n_groups;
+        if(pl.cb_group.size() == nb) grp = pl.cb_group;
+        else {
+            std::iota(grp.begin(), grp.end(), size_t(0));
+            ng = nb;
+        }
+        const size_t nx = size_t(std::max(1, pl.n_xcd));
+        std::vector<double> cb_cost(nb, 0.0);       // 64-k chunks of a tile
+        std::vector<std::vector<size_t> > gblocks(ng);
+        for(size_t b = 0; b < nb; b++) {
+            for(uint32_t p = pl.cblocks[b].pair_begin; p < pl.cblocks[b].pair_end; p++)
+                cb_cost[b] += double((pl.pairs[p].k + 63) / 64);
+            gblocks[grp[b]].push_back(b);
+        }
+        for(int k = 0; k < dev::OZ_TILE_COUNT; k++) {
+            uint32_t bm, bn;
+            dev::oz_tile_shape(k, bm, bn);
+            std::vector<std::vector<dev::gemm_tile> > gt(ng);
+            std::vector<double> gc(ng, 0.0);
+            for(size_t g = 0; g < ng; g++) {
+                uint32_t maxtn = 0;
+                for(size_t i = 0; i < gblocks[g].size(); i++) {
+                    const dev::gemm_cblock &cb = pl.cblocks[gblocks[g][i]];
+                    const uint32_t tm = (cb.m + bm - 1) / bm, tn = (cb.n + bn - 1) / bn;
+                    lt_require(tm < 65536 && tn < 65536, LT_ERR_PARAM, "C block too large");
+                    maxtn = std::max(maxtn, tn);
+                }
+                for(uint32_t j = 0; j < maxtn; j++)
+                    for(size_t i = 0; i < gblocks[g].size(); i++) {
+                        const size_t b = gblocks[g][i];
+                        const dev::gemm_cblock &cb = pl.cblocks[b];
+                        const uint32_t tm = (cb.m + bm - 1) / bm, tn = (cb.n + bn - 1) / bn;
+                        if(j >= tn) continue;
+                        for(uint32_t r = 0; r < tm; r++) {
+                            dev::gemm_tile t;
+                            t.cblock = (uint32_t)b; t.tm = (uint16_t)r; t.tn = (uint16_t)j;
+                            gt[g].push_back(t);
+                            gc[g] += cb_cost[b];
+                        }
+                    }
+            }
+            std::vector<size_t> go(ng);
+            std::iota(go.begin(), go.end(), size_t(0));
+            std::stable_sort(go.begin(), go.end(), [&](size_t x, size_t y) { return gc[x] > gc[y]; });
+            std::vector<double> load(nx, 0.0);
+            std::vector<std::vector<dev::gemm_tile> > q(nx);
+            size_t total = 0;
+            for(size_t i = 0; i < ng; i++) {
+                const size_t x = size_t(std::min_element(load.begin(), load.end()) - load.begin());
+                q[x].insert(q[x].end(), gt[go[i]].begin(), gt[go[i]].end());
+                load[x] += gc[go[i]];
+                total += gt[go[i]].size();
+            }
+            std::vector<size_t> qp(nx, 0);
+            tl[k].reserve(total);
+            for(size_t w = 0; tl[k].size() < total; w++) {
+                for(int j = 0; j < 4 && tl[k].size() < total; j++) {
+                    size_t x = w % nx;
+                    if(qp[x] >= q[x].size()) {
+                        size_t left = 0;
+                        for(size_t y = 0; y < nx; y++)
+                            if(q[y].size() - qp[y] > left) { left = q[y].size() - qp[y]; x = y; }
+                    }
+                    tl[k].push_back(q[x][qp[x]++]);
+                }
             }
         }
     }
