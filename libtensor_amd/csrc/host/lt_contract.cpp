@@ -282,6 +282,12 @@ struct contract_plan {
     dev::gemm_tile *d_oz_tiles;
     uint32_t n_oz_tiles;
     uint32_t n_oz_items_mut;          //!< split items of operands split every perform
+    // prepare of the split-precision performs: the copies of operands of
+    // mutable tensors only (the others are prepared once, with their split)
+    bool oz_prep_own;
+    dev::perm_desc *d_oz_prep;
+    uint32_t *d_oz_prep_cd, *d_oz_prep_co;
+    uint32_t n_oz_prep_chunks;
     double *d_oz_bound;               //!< stated error per C block (bound, max |C|)
     int32_t *d_oz_opmax;              //!< largest exponent per split operand
     uint32_t n_oz_ops;
@@ -295,7 +301,8 @@ struct contract_plan {
         n_groups(0), n_xcd(8),
         n_prep_st_chunks(0), d_prep_st(0), d_pst_cd(0), d_pst_co(0),
         oz_slices(0), oz_tile(0), oz_tile_set(-1), d_oz_split(0), d_oz_items(0), n_oz_items(0), d_oz_pairs(0),
-        d_oz_tiles(0), n_oz_tiles(0), n_oz_items_mut(0), d_oz_bound(0), d_oz_opmax(0),
+        d_oz_tiles(0), n_oz_tiles(0), n_oz_items_mut(0), oz_prep_own(false), d_oz_prep(0),
+        d_oz_prep_cd(0), d_oz_prep_co(0), n_oz_prep_chunks(0), d_oz_bound(0), d_oz_opmax(0),
         n_oz_ops(0) { }
     ~contract_plan() {
         // an asynchronous perform of this plan may still run
@@ -1381,7 +1388,33 @@ void btod_contract2::build_ozaki(int slices) {
     context &ctx = m_a.ctx();
     pl.oz_lists.release();
     pl.oz_mem.release();
+    pl.oz_prep_own = false;
     pl.oz_slices = slices;
+
+    // Prepared copies (one descriptor each, each in its own scratch region)
+    // of blocks of an immutable tensor never change: they are prepared and
+    // split once, with this plan's split form; the performs prepare the
+    // others only.
+    auto immutable_src = [&](const double *p) {
+        return (m_a.is_immutable() && m_a.owns(p)) || (m_b.is_immutable() && m_b.owns(p));
+    };
+    std::vector<std::pair<const double*, size_t> > inv_copies;    // (dst, elements), sorted
+    std::vector<dev::perm_desc> prep_mut, prep_inv;
+    for(size_t i = 0; i < pl.prep.size(); i++) {
+        if(immutable_src(pl.prep[i].src)) {
+            prep_inv.push_back(pl.prep[i]);
+            inv_copies.push_back(std::make_pair((const double*)pl.prep[i].dst,
+                size_t(pl.prep[i].total)));
+        } else prep_mut.push_back(pl.prep[i]);
+    }
+    std::sort(inv_copies.begin(), inv_copies.end());
+    auto in_invariant_copy = [&](const double *p) {
+        std::vector<std::pair<const double*, size_t> >::const_iterator i = std::upper_bound(
+            inv_copies.begin(), inv_copies.end(), std::make_pair(p, ~size_t(0)));
+        if(i == inv_copies.begin()) return false;
+        --i;
+        return p >= i->first && p < i->first + i->second;
+    };
 
     // one split per distinct operand matrix: A-role operands scaled per row
     // of C, B-role operands per column of C
@@ -1404,9 +1437,9 @@ void btod_contract2::build_ozaki(int slices) {
         o.rs = kmajor ? 1 : ld;
         o.ks = kmajor ? ld : 1;
         o.off = bytes; o.eoff = nexp;
-        // an operand read in place from an immutable tensor is split once
-        // per plan, the others on every perform
-        o.cached = (m_a.is_immutable() && m_a.owns(src)) || (m_b.is_immutable() && m_b.owns(src));
+        // an operand read in place from an immutable tensor, or an invariant
+        // copy, is split once per plan, the others on every perform
+        o.cached = immutable_src(src) || in_invariant_copy(src);
         bytes += size_t(slices) * ((rows + 15) & ~15u) * o.kp;
         bytes = (bytes + 255) & ~size_t(255);
         nexp += rows;
@@ -1557,7 +1590,22 @@ void btod_contract2::build_ozaki(int slices) {
         o_t[k] = sz;
         sz = align256(sz + tl[k].size() * sizeof(dev::gemm_tile));
     }
+    std::vector<uint32_t> pm_cd, pm_co, pi_cd, pi_co;
+    make_chunks(prep_mut, pm_cd, pm_co);
+    make_chunks(prep_inv, pi_cd, pi_co);
+    size_t o_pm = sz; sz = align256(sz + prep_mut.size() * sizeof(dev::perm_desc));
+    size_t o_pmcd = sz; sz = align256(sz + pm_cd.size() * 4);
+    size_t o_pmco = sz; sz = align256(sz + pm_co.size() * 4);
+    size_t o_pi = sz; sz = align256(sz + prep_inv.size() * sizeof(dev::perm_desc));
+    size_t o_picd = sz; sz = align256(sz + pi_cd.size() * 4);
+    size_t o_pico = sz; sz = align256(sz + pi_co.size() * 4);
     std::vector<char> host(sz ? sz : 256, 0);
+    if(!prep_mut.empty()) std::memcpy(&host[o_pm], prep_mut.data(), prep_mut.size() * sizeof(dev::perm_desc));
+    if(!pm_cd.empty()) std::memcpy(&host[o_pmcd], pm_cd.data(), pm_cd.size() * 4);
+    if(!pm_co.empty()) std::memcpy(&host[o_pmco], pm_co.data(), pm_co.size() * 4);
+    if(!prep_inv.empty()) std::memcpy(&host[o_pi], prep_inv.data(), prep_inv.size() * sizeof(dev::perm_desc));
+    if(!pi_cd.empty()) std::memcpy(&host[o_picd], pi_cd.data(), pi_cd.size() * 4);
+    if(!pi_co.empty()) std::memcpy(&host[o_pico], pi_co.data(), pi_co.size() * 4);
     if(!descs.empty()) std::memcpy(&host[o_d], descs.data(), descs.size() * sizeof(dev::oz_split_desc));
     if(!items.empty()) std::memcpy(&host[o_i], items.data(), items.size() * 4);
     if(!ozp.empty()) std::memcpy(&host[o_p], ozp.data(), ozp.size() * sizeof(dev::oz_pair));
@@ -1575,9 +1623,19 @@ void btod_contract2::build_ozaki(int slices) {
     pl.d_oz_opmax = (int32_t*)(base + o_om);
     pl.n_oz_ops = (uint32_t)ops.size();
     pl.n_oz_items_mut = n_mut;
+    pl.d_oz_prep = (dev::perm_desc*)(base + o_pm);
+    pl.d_oz_prep_cd = (uint32_t*)(base + o_pmcd);
+    pl.d_oz_prep_co = (uint32_t*)(base + o_pmco);
+    pl.n_oz_prep_chunks = (uint32_t)pm_cd.size();
+    pl.oz_prep_own = true;
     if(pl.n_oz_items > n_mut) {
-        // operands of immutable tensors: split now, once for this plan
+        // operands of immutable tensors (in place or invariant copies,
+        // prepared first): split now, once for this plan
         ctx.activate();
+        if(!pi_cd.empty())
+            hip_check(dev::launch_permute((const dev::perm_desc*)(base + o_pi),
+                (const uint32_t*)(base + o_picd), (const uint32_t*)(base + o_pico),
+                (uint32_t)pi_cd.size(), ctx.stream()), "permute (prepare, immutable operands)");
         hip_check(dev::launch_oz_split(pl.d_oz_split, pl.d_oz_items + 2 * n_mut,
             pl.n_oz_items - n_mut, ctx.stream()), "split (ozaki, immutable operands)");
     }
@@ -1639,6 +1697,12 @@ void btod_contract2::enqueue(int mode, hipStream_t s, bool timed, size_t &launch
         if(pl.n_prep_st_chunks) {
             hip_check(dev::launch_permute(pl.d_prep_st, pl.d_pst_cd, pl.d_pst_co,
                 pl.n_prep_st_chunks, s), "permute (prepare)");
+            launches++;
+        }
+    } else if(mode == MODE_OZAKI && pl.oz_prep_own) {
+        if(pl.n_oz_prep_chunks) {
+            hip_check(dev::launch_permute(pl.d_oz_prep, pl.d_oz_prep_cd, pl.d_oz_prep_co,
+                pl.n_oz_prep_chunks, s), "permute (prepare)");
             launches++;
         }
     } else if(pl.n_prep_chunks) {

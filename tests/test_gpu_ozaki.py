@@ -189,6 +189,32 @@ def test_immutable_operand_split_once(gpu_ctx):
         lt.btod_copy(b2, b2)            # B := P(B) writes into B
 
 
+@pytest.mark.parametrize("which", ["A", "B"])
+@pytest.mark.parametrize("name", G.names())
+def test_immutable_operand_copies_prepared_once(gpu_ctx, name, which):
+    """Copies of blocks of an immutable operand (permuted or unfolded for the
+    contraction) are prepared and split once per plan: repeated
+    split-precision performs, with an FP64 perform of the same operation in
+    between, equal the perform with mutable operands bit for bit."""
+    _say("immutable", name, which)
+    prob, dense, meta, result = G.load(name)
+    a, b, op = P.build(gpu_ctx, prob, dense)
+    op.set_ozaki(5)
+    c = P.make_result_tensor(gpu_ctx, prob)
+    op.perform(c, lt.LT_MODE_OZAKI)
+    ref = c.to_dense()
+    a2, b2, op2 = P.build(gpu_ctx, prob, dense)
+    (a2 if which == "A" else b2).set_immutable()
+    op2.set_ozaki(5)
+    c2 = P.make_result_tensor(gpu_ctx, prob)
+    for i in range(3):
+        op2.perform(c2, lt.LT_MODE_OZAKI)
+        assert np.array_equal(c2.to_dense(), ref), i
+        if i == 1:
+            op2.perform(c2, lt.LT_MODE_GROUPED)
+            assert O.rel_err(c2.to_dense(), result) < 1e-12
+
+
 def test_nonfinite_input_gives_nan(gpu_ctx):
     """A NaN in an operand makes the products of its row NaN (as in FP64)
     and the stated bound infinite, instead of finite wrong values."""
