@@ -1724,13 +1724,21 @@ void btod_contract2::enqueue(int mode, hipStream_t s, bool timed, size_t &launch
             (uint32_t)pl.c_out.size(), pl.oz_slices, pl.d_oz_bound, s), "bound (ozaki)");
         launches += 2;
     } else if(mode == MODE_PER_BLOCK) {
+        // one launch per C block; the blocks are independent, so their
+        // launches go round-robin to side streams forked from s and joined
+        // back into it (libtensor computes C blocks in parallel too)
+        context &ctx = m_a.ctx();
+        std::lock_guard<std::mutex> lk(ctx.side_lock());
+        ctx.side_fork(s);
+        int j = 0;
         for(size_t b = 0; b + 1 < pl.block_tile_begin.size(); b++) {
             uint32_t t0 = pl.block_tile_begin[b], t1 = pl.block_tile_begin[b + 1];
             if(t1 == t0) continue;
             hip_check(dev::launch_gemm(pl.tile_cfg, pl.d_cb, pl.d_pairs,
-                pl.d_tiles_block + t0, t1 - t0, t1 - t0, s), "gemm (per block)");
+                pl.d_tiles_block + t0, t1 - t0, t1 - t0, ctx.side(j++)), "gemm (per block)");
             launches++;
         }
+        ctx.side_join(s);
     } else {
         hip_check(launch_grouped(pl, pl.tile_cfg, s), "gemm (grouped)");
         launches++;

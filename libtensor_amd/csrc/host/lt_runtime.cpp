@@ -65,8 +65,36 @@ context::~context() {
     for(std::map<void*, size_t>::iterator i = m_sizes.begin(); i != m_sizes.end(); ++i)
         hipFree(i->first);
     if(m_xfer_ev) hipEventDestroy(m_xfer_ev);
+    for(int i = 0; i < kSideStreams; i++) {
+        if(m_side[i]) hipStreamSynchronize(m_side[i]);
+        if(m_side_ev[i]) hipEventDestroy(m_side_ev[i]);
+        if(m_side[i]) hipStreamDestroy(m_side[i]);
+    }
+    if(m_fork_ev) hipEventDestroy(m_fork_ev);
     if(m_stream) hipStreamDestroy(m_stream);
     if(m_xfer) hipStreamDestroy(m_xfer);
+}
+
+void context::side_fork(hipStream_t s) {
+    require_device("side streams");
+    if(!m_fork_ev) {
+        for(int i = 0; i < kSideStreams; i++) {
+            hip_check(hipStreamCreateWithFlags(&m_side[i], hipStreamNonBlocking), "hipStreamCreate");
+            hip_check(hipEventCreateWithFlags(&m_side_ev[i], hipEventDisableTiming),
+                "hipEventCreate");
+        }
+        hip_check(hipEventCreateWithFlags(&m_fork_ev, hipEventDisableTiming), "hipEventCreate");
+    }
+    hip_check(hipEventRecord(m_fork_ev, s), "hipEventRecord");
+    for(int i = 0; i < kSideStreams; i++)
+        hip_check(hipStreamWaitEvent(m_side[i], m_fork_ev, 0), "hipStreamWaitEvent");
+}
+
+void context::side_join(hipStream_t s) {
+    for(int i = 0; i < kSideStreams; i++) {
+        hip_check(hipEventRecord(m_side_ev[i], m_side[i]), "hipEventRecord");
+        hip_check(hipStreamWaitEvent(s, m_side_ev[i], 0), "hipStreamWaitEvent");
+    }
 }
 
 void context::activate() const {

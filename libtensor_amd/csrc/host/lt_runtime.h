@@ -91,6 +91,16 @@ public:
     void keep_launch(const std::string &key, std::shared_ptr<cached_launch> l);
     static const size_t kLaunchCache = 32;
 
+    //! Side streams (default priority, created on first use) for independent
+    //! launches: side_fork makes them wait for the work enqueued on s so
+    //! far, side_join makes s wait for the work enqueued on them since.
+    //! Hold side_lock() from the fork to the join.
+    static const int kSideStreams = 4;
+    void side_fork(hipStream_t s);
+    void side_join(hipStream_t s);
+    hipStream_t side(int i) const { return m_side[i % kSideStreams]; }
+    std::mutex &side_lock() { return m_side_mtx; }
+
 private:
     int m_device;
     hipStream_t m_stream;
@@ -109,6 +119,10 @@ private:
     void *m_stage = nullptr;
     size_t m_stage_bytes = 0;
     std::mutex m_stage_mtx;
+    hipStream_t m_side[kSideStreams] = {};
+    hipEvent_t m_side_ev[kSideStreams] = {};
+    hipEvent_t m_fork_ev = nullptr;
+    std::mutex m_side_mtx;
 };
 
 //! Slab allocator for the blocks of one tensor (few hipMallocs, stable
