@@ -102,15 +102,26 @@ contraction2 make_contraction(size_t N, size_t M, size_t K, const size_t *permc,
 class scratch_arena {
 public:
     explicit scratch_arena(context &ctx, char *region = 0, size_t cap = 0) :
-        m_ctx(ctx), m_region(region), m_cap(cap), m_off(0) { }
+        m_ctx(ctx), m_region(region), m_cap(cap), m_off(0), m_measure(false) { }
+    //! A measuring arena: take() hands out placeholder addresses and only
+    //! adds up the bytes (used() after the takes of a layout).
+    struct measuring { };
+    scratch_arena(context &ctx, measuring) :
+        m_ctx(ctx), m_region(0), m_cap(0), m_off(0), m_measure(true) { }
     ~scratch_arena() {
-        if(!m_region) for(size_t i = 0; i < m_taken.size(); i++) m_ctx.free(m_taken[i]);
+        if(!m_region && !m_measure) for(size_t i = 0; i < m_taken.size(); i++) m_ctx.free(m_taken[i]);
     }
     scratch_arena(const scratch_arena&) = delete;
     scratch_arena &operator=(const scratch_arena&) = delete;
+    size_t used() const { return m_off; }
     //! A 256-byte aligned buffer of `bytes` bytes.
     char *take(size_t bytes) {
         bytes = (std::max<size_t>(bytes, 8) + 255) & ~size_t(255);
+        if(m_measure) {
+            char *p = (char*)(uintptr_t(256) + m_off);
+            m_off += bytes;
+            return p;
+        }
         if(!m_region) {
             char *p = (char*)m_ctx.alloc(bytes);
             m_taken.push_back(p);
@@ -128,6 +139,7 @@ private:
     context &m_ctx;
     char *m_region;
     size_t m_cap, m_off;
+    bool m_measure;
     std::vector<void*> m_taken;
 };
 
@@ -154,7 +166,7 @@ struct permute_job {
         mem = ar.take(host.size());
     }
     void upload(context &ctx) { if(nchunks) ctx.upload(mem, host.data(), host.size()); }
-    void launch(context &ctx) {
+    void launch(context &ctx) const {
         if(!nchunks) return;
         hip_check(dev::launch_permute((dev::perm_desc*)mem, (uint32_t*)(mem + ocd),
             (uint32_t*)(mem + oco), (uint32_t)nchunks, ctx.stream()), "permute");
@@ -172,24 +184,65 @@ void dense_permute_in(context &ctx, const double *src, const dims_t &dsrc,
     ctx.sync();
 }
 
-//! tod_contract2 with every temporary (aligned operand copies, the aligned
-//! result, descriptors and tile list) taken from `ar` before the first
-//! device operation.
-void dense_contract2_in(context &cx, const contraction2 &contr, const dims_t &da,
-    const double *a, const dims_t &db, const double *b, double d, bool zero, double *c,
-    scratch_arena &ar) {
+//! One tod_contract2 laid out on the host: its operand copies and aligned
+//! result taken from `tmp`, its descriptor and tile lists from `dsc` (host
+//! images kept for the upload), all before any device operation.
+struct contract_job {
+    bool skip;                  //!< d == 0, C kept
+    bool fill;                  //!< d == 0, C := 0
+    double *c;
+    size_t cn;
+    permute_job ja, jb, jc;
+    char *mem;                  //!< GEMM block, pair and tile lists
+    std::vector<char> host;
+    uint32_t ntiles;
+    contract_job() : skip(false), fill(false), c(0), cn(0), mem(0), ntiles(0) { }
+    //! The uploads of the descriptor lists (each ordered after the stream).
+    void upload(context &cx) {
+        ja.upload(cx);
+        jb.upload(cx);
+        jc.upload(cx);
+        if(ntiles) cx.upload(mem, host.data(), host.size());
+    }
+    //! Every blob of the descriptor lists into a host image of the region
+    //! at `base` (one upload for many jobs).
+    void stage(char *image, const char *base) const {
+        const permute_job *js[3] = { &ja, &jb, &jc };
+        for(int i = 0; i < 3; i++)
+            if(js[i]->nchunks)
+                std::memcpy(image + (js[i]->mem - base), js[i]->host.data(), js[i]->host.size());
+        if(ntiles) std::memcpy(image + (mem - base), host.data(), host.size());
+    }
+    //! The launches, in stream order (no synchronization).
+    void launch(context &cx) const {
+        if(skip) return;
+        if(fill) {
+            hip_check(dev::launch_fill(c, cn, 0.0, cx.stream()), "fill");
+            return;
+        }
+        ja.launch(cx);
+        jb.launch(cx);
+        hip_check(dev::launch_gemm(dev::TILE_64x64, (dev::gemm_cblock*)mem,
+            (dev::gemm_pair*)(mem + 512), (dev::gemm_tile*)(mem + 1024),
+            ntiles, ntiles, cx.stream()), "gemm");
+        jc.launch(cx);
+    }
+};
+
+void contract_prepare(const contraction2 &contr, const dims_t &da, const double *a,
+    const dims_t &db, const double *b, double d, bool zero, double *c,
+    scratch_arena &tmp, scratch_arena &dsc, contract_job &j) {
     dims_t dc = contract2_dims(contr, da, db);
     // the GEMM kernels address operands with 32-bit byte offsets
     const size_t lim = size_t(1) << 29;
     lt_require(da.size() < lim && db.size() < lim && dc.size() < lim, LT_ERR_PARAM,
         "tod_contract2: a tensor of 2^29 or more elements (32-bit byte offsets in the "
         "GEMM kernels)");
+    j.c = c;
+    j.cn = dc.size();
     if(d == 0.0) {
-        if(zero) {
-            cx.activate();
-            hip_check(dev::launch_fill(c, dc.size(), 0.0, cx.stream()), "fill");
-            cx.sync();
-        }
+        j.fill = zero;
+        j.skip = !zero;
         return;
     }
     // align as tod_contract2::perform (tod_contract2_impl.h:124-135),
@@ -204,24 +257,22 @@ void dense_contract2_in(context &cx, const contraction2 &contr, const dims_t &da
     db2.permute(al.permb);
     dc2.permute(al.permc);
 
-    // host side: every buffer and descriptor
     const double *pa = a, *pb = b;
-    permute_job ja, jb, jc;
     if(!al.perma.is_identity()) {
-        double *t = (double*)ar.take(da.size() * 8);
-        ja.prepare(a, da, tensor_tr(al.perma, scalar_tr()), t, false, ar);
+        double *t = (double*)tmp.take(da.size() * 8);
+        j.ja.prepare(a, da, tensor_tr(al.perma, scalar_tr()), t, false, dsc);
         pa = t;
     }
     if(!al.permb.is_identity()) {
-        double *t = (double*)ar.take(db.size() * 8);
-        jb.prepare(b, db, tensor_tr(al.permb, scalar_tr()), t, false, ar);
+        double *t = (double*)tmp.take(db.size() * 8);
+        j.jb.prepare(b, db, tensor_tr(al.permb, scalar_tr()), t, false, dsc);
         pb = t;
     }
     double *pc = c;
     bool cperm = !al.permc.is_identity();
     if(cperm) {
-        pc = (double*)ar.take(dc.size() * 8);
-        jc.prepare(pc, dc2, tensor_tr(al.permc.inverse(), scalar_tr()), c, !zero, ar);
+        pc = (double*)tmp.take(dc.size() * 8);
+        j.jc.prepare(pc, dc2, tensor_tr(al.permc.inverse(), scalar_tr()), c, !zero, dsc);
     }
     gemm_form g = aligned_gemm_form(ca, da2, db2);
     dev::gemm_cblock cb;
@@ -250,29 +301,32 @@ void dense_contract2_in(context &cx, const contraction2 &contr, const dims_t &da
     dev::tile_shape(dev::TILE_64x64, bm, bn);
     std::vector<dev::gemm_tile> tiles;
     for(uint32_t i = 0; i < (cb.m + bm - 1) / bm; i++)
-        for(uint32_t j = 0; j < (cb.n + bn - 1) / bn; j++) {
-            dev::gemm_tile t; t.cblock = 0; t.tm = (uint16_t)i; t.tn = (uint16_t)j;
+        for(uint32_t jj = 0; jj < (cb.n + bn - 1) / bn; jj++) {
+            dev::gemm_tile t; t.cblock = 0; t.tm = (uint16_t)i; t.tn = (uint16_t)jj;
             tiles.push_back(t);
         }
     size_t bytes = 512 + 512 + tiles.size() * sizeof(dev::gemm_tile);
-    char *mem = ar.take(bytes);
-    std::vector<char> host(bytes, 0);
-    std::memcpy(&host[0], &cb, sizeof(cb));
-    std::memcpy(&host[512], &gp, sizeof(gp));
-    std::memcpy(&host[1024], tiles.data(), tiles.size() * sizeof(dev::gemm_tile));
+    j.mem = dsc.take(bytes);
+    j.host.assign(bytes, 0);
+    std::memcpy(&j.host[0], &cb, sizeof(cb));
+    std::memcpy(&j.host[512], &gp, sizeof(gp));
+    std::memcpy(&j.host[1024], tiles.data(), tiles.size() * sizeof(dev::gemm_tile));
+    j.ntiles = (uint32_t)tiles.size();
+}
 
+//! tod_contract2 with every temporary (aligned operand copies, the aligned
+//! result, descriptors and tile list) taken from `ar` before the first
+//! device operation.
+void dense_contract2_in(context &cx, const contraction2 &contr, const dims_t &da,
+    const double *a, const dims_t &db, const double *b, double d, bool zero, double *c,
+    scratch_arena &ar) {
+    contract_job j;
+    contract_prepare(contr, da, a, db, b, d, zero, c, ar, ar, j);
+    if(j.skip) return;
     // device side, in stream order
-    ja.upload(cx);
-    jb.upload(cx);
-    jc.upload(cx);
-    cx.upload(mem, host.data(), bytes);
+    j.upload(cx);
     cx.activate();
-    ja.launch(cx);
-    jb.launch(cx);
-    hip_check(dev::launch_gemm(dev::TILE_64x64, (dev::gemm_cblock*)mem,
-        (dev::gemm_pair*)(mem + 512), (dev::gemm_tile*)(mem + 1024),
-        (uint32_t)tiles.size(), (uint32_t)tiles.size(), cx.stream()), "gemm");
-    jc.launch(cx);
+    j.launch(cx);
     cx.sync();
 }
 
@@ -290,6 +344,45 @@ void dense_contract2(context &cx, const contraction2 &contr, const dims_t &da, c
     const dims_t &db, const double *b, double d, bool zero, double *c) {
     scratch_arena ar(cx);
     dense_contract2_in(cx, contr, da, a, db, b, d, zero, c, ar);
+}
+
+void dense_contract2_many(context &cx, const std::vector<dense_contract_item> &items,
+    double *c) {
+    if(items.empty()) return;
+    // every temporary and descriptor of all items in one allocation: the
+    // layout is measured first (same takes, placeholder addresses), then
+    // made in the allocation; the descriptors go up in one upload, the
+    // launches follow in item order, one synchronization at the end
+    size_t tbytes = 0, dbytes = 0;
+    {
+        scratch_arena mt(cx, scratch_arena::measuring()), md(cx, scratch_arena::measuring());
+        for(size_t i = 0; i < items.size(); i++) {
+            contract_job j;
+            contract_prepare(items[i].contr, items[i].da, items[i].a, items[i].db, items[i].b,
+                items[i].d, items[i].zero, c, mt, md, j);
+        }
+        tbytes = mt.used();
+        dbytes = md.used();
+    }
+    char *region = (char*)cx.alloc(std::max<size_t>(tbytes + dbytes, 256));
+    try {
+        scratch_arena at(cx, region, tbytes), ad(cx, region + tbytes, dbytes);
+        std::vector<contract_job> jobs(items.size());
+        for(size_t i = 0; i < items.size(); i++)
+            contract_prepare(items[i].contr, items[i].da, items[i].a, items[i].db, items[i].b,
+                items[i].d, items[i].zero, c, at, ad, jobs[i]);
+        std::vector<char> image(dbytes, 0);
+        for(size_t i = 0; i < jobs.size(); i++) jobs[i].stage(image.data(), region + tbytes);
+        if(dbytes) cx.upload(region + tbytes, image.data(), dbytes);
+        cx.activate();
+        for(size_t i = 0; i < jobs.size(); i++) jobs[i].launch(cx);
+        cx.sync();
+    } catch(...) {
+        cx.sync();
+        cx.free(region);
+        throw;
+    }
+    cx.free(region);
 }
 }
 

@@ -2291,35 +2291,63 @@ void btod_contract2::compute_block(bool zero, const index_t &ic, const tensor_tr
     check_block_sizes();
     context &ctx = m_a.ctx();
     ctx.require_device("btod_contract2::compute_block");
-    dims_t bida = m_a.bidims(), bidb = m_b.bidims();
-    block_list bla(bida, m_a.nonzero_blocks()), blb(bidb, m_b.nonzero_blocks());
-    block_list blax(bida), blbx(bidb);
-    unfold_block_list(m_a.sym(), bla, blax);
-    unfold_block_list(m_b.sym(), blb, blbx);
-    contr_list cl;
-    build_clst_full(m_contr, m_a.sym(), m_b.sym(), blax, blbx,
-        m_symc.bis.block_index_dims(), ic, false, cl);
     dims_t dc = m_symc.bis.block_dims(ic);
     lt_require(trc.perm.n == dc.n, LT_ERR_PARAM, "block transformation order");
-    double *sum = (double*)ctx.alloc(std::max<size_t>(dc.size(), 1) * sizeof(double));
-    try {
-        bool first = true;
+    // the block's contraction terms: built once per operand structure (a
+    // direct tensor asks for the same blocks again and again)
+    const size_t key = abs_index(ic, m_symc.bis.block_index_dims());
+    std::vector<dense_contract_item> items;
+    bool known = false;
+    {
+        std::lock_guard<std::mutex> lk(m_cb_mtx);
+        if(m_cb_va != m_a.version() || m_cb_vb != m_b.version()) {
+            m_cb_terms.clear();
+            m_cb_va = m_a.version();
+            m_cb_vb = m_b.version();
+        }
+        std::map<size_t, std::vector<dense_contract_item> >::const_iterator i =
+            m_cb_terms.find(key);
+        if(i != m_cb_terms.end()) { items = i->second; known = true; }
+    }
+    if(!known) {
+        const uint64_t va = m_a.version(), vb = m_b.version();
+        dims_t bida = m_a.bidims(), bidb = m_b.bidims();
+        block_list bla(bida, m_a.nonzero_blocks()), blb(bidb, m_b.nonzero_blocks());
+        block_list blax(bida), blbx(bidb);
+        unfold_block_list(m_a.sym(), bla, blax);
+        unfold_block_list(m_b.sym(), blb, blbx);
+        contr_list cl;
+        build_clst_full(m_contr, m_a.sym(), m_b.sym(), blax, blbx,
+            m_symc.bis.block_index_dims(), ic, false, cl);
         for(contr_list::const_iterator p = cl.begin(); p != cl.end(); ++p) {
             double d = p->tra.st.c * m_ka * p->trb.st.c * m_kb * m_kc;
             if(d == 0.0) continue;
-            dims_t da = m_a.bis().block_dims(get_index(p->acia, bida));
-            dims_t db = m_b.bis().block_dims(get_index(p->acib, bidb));
-            contraction2 cc(m_contr);
-            cc.permute_a(p->tra.perm.inverse());
-            cc.permute_b(p->trb.perm.inverse());
-            dense_contract2(ctx, cc, da, m_a.block_ptr(p->acia), db, m_b.block_ptr(p->acib),
-                d, first, sum);
-            first = false;
+            dense_contract_item it;
+            it.da = m_a.bis().block_dims(get_index(p->acia, bida));
+            it.db = m_b.bis().block_dims(get_index(p->acib, bidb));
+            it.contr = m_contr;
+            it.contr.permute_a(p->tra.perm.inverse());
+            it.contr.permute_b(p->trb.perm.inverse());
+            it.a = m_a.block_ptr(p->acia);
+            it.b = m_b.block_ptr(p->acib);
+            it.d = d;
+            it.zero = items.empty();
+            items.push_back(it);
         }
-        if(first) {
+        std::lock_guard<std::mutex> lk(m_cb_mtx);
+        if(m_cb_va == va && m_cb_vb == vb) {
+            if(m_cb_terms.size() >= 65536) m_cb_terms.clear();
+            m_cb_terms[key] = items;
+        }
+    }
+    double *sum = (double*)ctx.alloc(std::max<size_t>(dc.size(), 1) * sizeof(double));
+    try {
+        // the terms in list order, one allocation, upload and
+        // synchronization for all of them
+        if(items.empty()) {
             ctx.activate();
             hip_check(dev::launch_fill(sum, dc.size(), 0.0, ctx.stream()), "fill");
-        }
+        } else dense_contract2_many(ctx, items, sum);
         dense_permute(ctx, sum, dc, trc, blk, !zero);
     } catch(...) {
         ctx.free(sum);

@@ -302,6 +302,18 @@ void dense_permute(context &ctx, const double *src, const dims_t &dsrc,
     const tensor_tr &tr, double *dst, bool accumulate);
 void dense_contract2(context &cx, const contraction2 &contr, const dims_t &da, const double *a,
     const dims_t &db, const double *b, double d, bool zero, double *c);
+//! One term of dense_contract2_many: C = (zero) or += d contract(A, B).
+struct dense_contract_item {
+    contraction2 contr;
+    dims_t da, db;
+    const double *a, *b;
+    double d;
+    bool zero;
+};
+//! The terms into one C in order, as that many dense_contract2 calls, with
+//! one allocation, one upload and one synchronization for all of them.
+void dense_contract2_many(context &cx, const std::vector<dense_contract_item> &items,
+    double *c);
 
 struct contract_plan;
 class btod_contract2;
@@ -469,6 +481,12 @@ private:
     //! The batch of the schedule the next plan is built for (null: all).
     const std::vector<size_t> *m_sched_subset;
     bool m_use_graph_saved;
+    //! compute_block: the contraction terms of each C block asked for (by
+    //! absolute block index), valid for the operand structure versions
+    //! m_cb_va / m_cb_vb
+    std::mutex m_cb_mtx;
+    uint64_t m_cb_va = 0, m_cb_vb = 0;
+    std::map<size_t, std::vector<dense_contract_item> > m_cb_terms;
 
     //! Builds the plan; false (and no plan) if it does not fit the device
     //! memory budget.

@@ -313,6 +313,38 @@ def test_compute_block_device(gpu_ctx, name):
         out2.free()
 
 
+def test_compute_block_sees_structure_changes(gpu_ctx):
+    """compute_block keeps each block's contraction terms between calls and
+    drops them when an operand's non-zero blocks change: a repeated call
+    gives the same bits, the block is zero once A's blocks are zeroed, and
+    the first result comes back when they are set again."""
+    prob, dense, meta, result = G.load("ladder_spin")
+    a, b, op = P.build(gpu_ctx, prob, dense)
+    c = P.make_result_tensor(gpu_ctx, prob)
+    bi = None
+    for x in meta["schedule"]:
+        idx = c.index(int(x))
+        st, dims = c.block_start(idx), c.block_dims(idx)
+        if np.any(result[tuple(slice(s0, s0 + n) for s0, n in zip(st, dims))]):
+            bi = int(x)
+            break
+    assert bi is not None
+    dims = tuple(c.block_dims(c.index(bi)))
+    out = lt.device_array(gpu_ctx, dims)
+    op.compute_block(bi, out)
+    first = out.download()
+    assert np.any(first)
+    op.compute_block(bi, out)
+    assert np.array_equal(out.download(), first)
+    a.zero()
+    op.compute_block(bi, out)
+    assert not np.any(out.download())
+    a.from_dense(dense["A"])
+    op.compute_block(bi, out)
+    assert np.array_equal(out.download(), first)
+    out.free()
+
+
 def test_async_performs_are_ordered(gpu_ctx):
     """Performs that return once enqueued (set_async): a tensor upload after
     one waits for it, a second operation's perform sees the new data, reads
