@@ -238,9 +238,10 @@ hipError_t launch_oz_bound(const gemm_cblock *cb, const gemm_pair *pairs, const 
     const int32_t *opmax, uint32_t ncb, int slices, double *out, hipStream_t s);
 //! Largest exponent of every split operand (opmax[i] over the rows of
 //! descs[i]; kOzNonFinite when a row holds a NaN or an infinity), read by
-//! the bound kernel.
+//! the bound kernel; also zeroes the max |C| slots bound_out[2 b + 1] of
+//! the ncb C blocks that the bound kernel fills next.
 hipError_t launch_oz_opmax(const oz_split_desc *descs, uint32_t nops, int32_t *opmax,
-    hipStream_t s);
+    uint32_t ncb, double *bound_out, hipStream_t s);
 
 //! Fill n doubles with v.
 hipError_t launch_fill(double *p, size_t n, double v, hipStream_t s);

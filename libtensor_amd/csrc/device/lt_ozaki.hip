@@ -16,12 +16,15 @@
 // H. Ootomo, K. Ozaki, R. Yokota, "DGEMM on integer matrix multiplication
 // unit", IJHPCA (2024).
 #include <hip/hip_runtime.h>
+#include <algorithm>
 #include "lt_device.h"
 
 namespace lta {
 namespace dev {
 
 typedef int int4_t __attribute__((ext_vector_type(4)));
+
+static const uint32_t kOzBoundSlices = 8;
 
 void oz_tile_shape(int tile, uint32_t &bm, uint32_t &bn) {
     switch(tile) {
@@ -227,7 +230,10 @@ oz_gemm_kernel(const gemm_cblock *__restrict__ cbs,
 // NaN or an infinity holds kOzNonFinite, larger than any finite exponent).
 __global__ void __launch_bounds__(256)
 oz_opmax_kernel(const oz_split_desc *__restrict__ descs, uint32_t nops,
-    int32_t *__restrict__ opmax) {
+    int32_t *__restrict__ opmax, uint32_t ncb, double *__restrict__ bound_out) {
+    // the max |C| slots of the bound kernel (which runs next) start at zero
+    for(uint32_t b = blockIdx.x * 256 + threadIdx.x; b < ncb; b += gridDim.x * 256)
+        bound_out[2 * b + 1] = 0.0;
     const uint32_t i = blockIdx.x * 4 + (threadIdx.x >> 6);
     if(i >= nops) return;             // whole wave: no barrier in this kernel
     const int lane = threadIdx.x & 63;
@@ -239,12 +245,16 @@ oz_opmax_kernel(const oz_split_desc *__restrict__ descs, uint32_t nops,
     if(lane == 0) opmax[i] = m;
 }
 
-// Stated error of every C block of a split-precision product: one
-// workgroup per C block; bound = sum over the block's pairs (in list order)
-// of |d| K (s + 2) 2^(max_i e_i + max_j f_j - 7 s) (the per-element bound
-// above with the largest row and column exponents of the pair's operands),
-// and the largest |C| of the block as the product kernel wrote it (NaN
-// propagates).  out[2 b] = bound, out[2 b + 1] = max |C|.
+// Stated error of every C block of a split-precision product: workgroups
+// (C block b, row slice y of gridDim.y); slice 0 states the bound = sum over
+// the block's pairs (in list order) of |d| K (s + 2) 2^(max_i e_i + max_j
+// f_j - 7 s) (the per-element bound above with the largest row and column
+// exponents of the pair's operands), and every slice folds the largest |C|
+// of its rows (as the product kernel wrote them) into the block's slot with
+// an integer atomic max: |C| >= 0, and non-negative doubles order as their
+// bit patterns, with infinity and then NaN above every finite value, so
+// NaN propagates and the result does not depend on the order.
+// out[2 b] = bound, out[2 b + 1] = max |C| (zeroed by oz_opmax_kernel).
 __global__ void __launch_bounds__(256)
 oz_bound_kernel(const gemm_cblock *__restrict__ cbs, const gemm_pair *__restrict__ pairs,
     const oz_pair *__restrict__ ozp, const int32_t *__restrict__ opmax, int slices,
@@ -254,10 +264,11 @@ oz_bound_kernel(const gemm_cblock *__restrict__ cbs, const gemm_pair *__restrict
     __shared__ double rt[256];
     const gemm_cblock cb = cbs[blockIdx.x];
     const int tid = threadIdx.x;
+    const uint32_t ns = gridDim.y, sl = blockIdx.y;
     // the terms of up to 256 pairs at a time, one per thread, then added by
-    // thread 0 in list order
+    // thread 0 in list order (slice 0 only: uniform over the workgroup)
     double bound = 0.0;
-    for(uint32_t p0 = cb.pair_begin; p0 < cb.pair_end; p0 += 256) {
+    for(uint32_t p0 = cb.pair_begin; sl == 0 && p0 < cb.pair_end; p0 += 256) {
         const uint32_t p = p0 + tid;
         double term = 0.0;
         if(p < cb.pair_end) {
@@ -277,9 +288,10 @@ oz_bound_kernel(const gemm_cblock *__restrict__ cbs, const gemm_pair *__restrict
         }
         __syncthreads();
     }
-    // max |C|: wave w rows w, w + 4, ...; lanes columns, four loads in flight
+    // max |C|: wave w of slice y rows 4 y + w, 4 y + w + 4 ns, ...; lanes
+    // columns, four loads in flight
     double cm = 0.0;
-    for(uint32_t r = tid >> 6; r < cb.m; r += 4) {
+    for(uint32_t r = 4 * sl + (tid >> 6); r < cb.m; r += 4 * ns) {
         const double *row = cb.c + size_t(r) * cb.ldc;
         for(uint32_t x = tid & 63; x < cb.n; x += 256) {
             double v[4];
@@ -299,22 +311,27 @@ oz_bound_kernel(const gemm_cblock *__restrict__ cbs, const gemm_pair *__restrict
         __syncthreads();
     }
     if(tid == 0) {
-        out[2 * blockIdx.x] = bound;
-        out[2 * blockIdx.x + 1] = rc[0];
+        if(sl == 0) out[2 * blockIdx.x] = bound;
+        atomicMax((unsigned long long *)&out[2 * blockIdx.x + 1],
+            (unsigned long long)__double_as_longlong(rc[0]));
     }
 }
 
 hipError_t launch_oz_opmax(const oz_split_desc *descs, uint32_t nops, int32_t *opmax,
-    hipStream_t s) {
-    if(nops == 0) return hipSuccess;
-    hipLaunchKernelGGL(oz_opmax_kernel, dim3((nops + 3) / 4), dim3(256), 0, s, descs, nops, opmax);
+    uint32_t ncb, double *bound_out, hipStream_t s) {
+    if(nops == 0 && ncb == 0) return hipSuccess;
+    const uint32_t g = std::max((nops + 3) / 4, (ncb + 255) / 256);
+    hipLaunchKernelGGL(oz_opmax_kernel, dim3(g), dim3(256), 0, s, descs, nops, opmax, ncb,
+        bound_out);
     return hipGetLastError();
 }
 
 hipError_t launch_oz_bound(const gemm_cblock *cb, const gemm_pair *pairs, const oz_pair *ozp,
     const int32_t *opmax, uint32_t ncb, int slices, double *out, hipStream_t s) {
     if(ncb == 0) return hipSuccess;
-    hipLaunchKernelGGL(oz_bound_kernel, dim3(ncb), dim3(256), 0, s, cb, pairs, ozp, opmax,
+    // rows of a C block in up to kOzBoundSlices workgroups
+    hipLaunchKernelGGL(oz_bound_kernel, dim3(ncb, kOzBoundSlices), dim3(256), 0, s, cb, pairs,
+        ozp, opmax,
         slices, out);
     return hipGetLastError();
 }

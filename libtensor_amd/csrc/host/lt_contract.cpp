@@ -1718,7 +1718,8 @@ void btod_contract2::enqueue(int mode, hipStream_t s, bool timed, size_t &launch
             pl.oz_slices, pl.oz_tile, s), "gemm (ozaki)");
         launches += 2;
         if(timed) hip_check(hipEventRecord(pl.ev[2], s), "hipEventRecord");
-        hip_check(dev::launch_oz_opmax(pl.d_oz_split, pl.n_oz_ops, pl.d_oz_opmax, s),
+        hip_check(dev::launch_oz_opmax(pl.d_oz_split, pl.n_oz_ops, pl.d_oz_opmax,
+            (uint32_t)pl.c_out.size(), pl.d_oz_bound, s),
             "exponents (ozaki)");
         hip_check(dev::launch_oz_bound(pl.d_cb, pl.d_pairs, pl.d_oz_pairs, pl.d_oz_opmax,
             (uint32_t)pl.c_out.size(), pl.oz_slices, pl.d_oz_bound, s), "bound (ozaki)");
